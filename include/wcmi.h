@@ -1,0 +1,265 @@
+/*
+ * wcmi.h - C ABI of libwcmi.so, the MI355X (gfx950) implementation of the
+ * witwin-channel deterministic / path propagation hot path.
+ *
+ * Conventions
+ *  - Every pointer argument is a device pointer (HBM) unless the name ends in
+ *    `_host`. The caller owns and allocates all buffers (PyTorch tensors on
+ *    the Python side); the library never allocates device memory except in
+ *    wc_scene_* helpers that are documented to do so.
+ *  - `stream` is a hipStream_t passed as void*. Work is enqueued
+ *    asynchronously on it; no entry point synchronizes unless stated.
+ *  - Return value 0 = success. Non-zero = failure; wc_last_error() returns a
+ *    thread-local message. Host-side validation failures return 1, HIP
+ *    launch errors 2.
+ *  - Vectors are float32 xyz triples, complex numbers are interleaved
+ *    (re, im) float32 pairs (the memory layout of torch.complex64).
+ *
+ * Each entry point names the reference operator it replaces (file:line in
+ * the witwin-channel source tree).
+ */
+#ifndef WCMI_H
+#define WCMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ identity */
+const char* wc_last_error(void);
+void wc_clear_error(void);
+int wc_abi_version(void);
+const char* wc_build_target(void);
+
+/* ------------------------------------------------------------ scene / LBVH
+ * Replaces the RayD scene creation behind _channel.rayd_scene_create
+ * (native/channel/binding/rayd.cpp:367, native/channel/rayd/scene.cpp:19-57):
+ * builds a linear BVH (Morton codes, LDS radix sort, Karras hierarchy,
+ * bottom-up boxes) over the triangles.
+ *   nodes_out:        wc_bvh_node_count(face_count) x 64 B
+ *   sorted_tris_out:  face_count x 48 B (BVH leaf order)
+ *   face_tris_out:    face_count x 48 B (original face order)
+ *   workspace:        wc_bvh_workspace_bytes(face_count) bytes
+ */
+size_t wc_bvh_workspace_bytes(int face_count);
+int wc_bvh_node_count(int face_count);
+int wc_bvh_build(const float* vertices, const int* faces, const int* face_group, const int* face_material,
+                 int face_count, void* nodes_out, void* sorted_tris_out, void* face_tris_out, void* workspace,
+                 size_t workspace_bytes, void* stream);
+
+/* Scene view consumed by the path kernels. Group planes are float64 (unit
+ * normal xyz, offset). Material arrays hold the first-layer scalar view;
+ * layer arrays hold the CSR stacks (reference scene/compiler.py:131-261).
+ * Edge records are 80-byte structs laid out as in wc_edge. */
+typedef struct wc_edge {
+    float ax, ay, az;
+    float dx, dy, dz;
+    float length;
+    float n;
+    float t0x, t0y, t0z;
+    float n0x, n0y, n0z;
+    int face0, face1;
+    int group0, group1;
+    int material0, material1;
+} wc_edge;
+
+typedef struct wc_scene {
+    const void* nodes;
+    const void* tris;
+    int tri_count;
+    int node_count;
+    const void* faces;
+    const int* face_group;
+    const int* face_material;
+    int face_count;
+    const double* group_plane;
+    const int* group_face_offset;
+    const int* group_faces;
+    const int* group_material;
+    int group_count;
+    const float* mat_eps;
+    const float* mat_sigma;
+    const float* mat_mu;
+    const float* mat_gain;
+    const float* mat_thickness;
+    const int* mat_layer_offset;
+    const int* mat_layer_count;
+    const float* layer_thickness;
+    const float* layer_eps;
+    const float* layer_sigma;
+    const float* layer_mu;
+    int material_count;
+    const wc_edge* edges;
+    int edge_count;
+} wc_scene;
+
+/* Transmitters, receiver cells and candidate reflection chains of one solve.
+ * Cells of transmitter t are [cell_offset[t], cell_offset[t+1]); with
+ * rx_shared = 1 every transmitter uses rx_pos[cell - cell_offset[t]], else
+ * rx_pos[cell]. rx_pol is [1,3] (rx_pol_stride 0) or one per position.
+ * Chains of (transmitter t, depth d) are rows chain_offset[t*5+d-1] ..
+ * chain_offset[t*5+d] of chain_groups [K,5] / chain_images [K,5,3] (float64). */
+typedef struct wc_solve {
+    const float* tx_pos;
+    const float* tx_pol;
+    const float* tx_power;
+    int tx_count;
+    const float* rx_pos;
+    const float* rx_pol;
+    int rx_pol_stride;
+    const int* cell_offset;
+    int rx_shared;
+    int total_cells;
+    const int* chain_offset;
+    const int* chain_groups;
+    const double* chain_images;
+    float frequency_hz;
+    int components;  /* bit 0 los, 1 reflection, 2 diffraction, 3 transmission */
+    int max_depth;
+    int coherent;
+} wc_solve;
+
+/* ------------------------------------------------------------ radiomap
+ * Fused deterministic radiomap: per (transmitter, cell) discovery of every
+ * line-of-sight, reflection chain, diffraction edge and transmission path,
+ * field evaluation, and coherent accumulation in float64 registers. No path
+ * rows are materialized. Replaces the chain evaluate_enumerated_paths ->
+ * enumerated_canonical_compact -> evaluate_path_fields ->
+ * deterministic_accumulate_flat
+ * (witwin/channel/propagation/enumerated.py:549-837,
+ *  native/channel/kernels/deterministic.cu:860-970).
+ * Outputs (cells = total_cells):
+ *   power_total [cells] f32, field_total [cells] c64,
+ *   component_power [4, cells] f32 (slots los, reflection, diffraction,
+ *   transmission), component_field [4, cells] c64,
+ *   diffraction_vector [cells, 3] c64 (may be NULL),
+ *   path_count [cells] i32 (may be NULL),
+ *   topology [cells, topology_capacity] i32 path descriptors for the
+ *   fixed-topology adjoint (may be NULL); failure [1] i32 gets bit 1 when a
+ *   cell holds more paths than topology_capacity.
+ * use_tiles: 1 = persistent launch over wave-aligned tiles of 64 cells from a
+ * global work counter (tile_counter [1] i32, zeroed by the call). */
+typedef struct wc_radiomap_out {
+    float* power_total;
+    float* field_total;
+    float* component_power;
+    float* component_field;
+    float* diffraction_vector;
+    int* path_count;
+    int* topology;
+    int topology_capacity;
+    int* failure;
+} wc_radiomap_out;
+
+int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
+                        const int* tiles, int tile_count, int* tile_counter, void* stream);
+
+/* ------------------------------------------------------------ path rows
+ * Explicit path export (deterministic export_paths, path.solve,
+ * consumer.evaluate). Pass 1 counts the valid paths of every cell; the
+ * caller scans the counts in canonical (rx-major, tx-minor) pair order into
+ * row_offset [cells]; pass 2 writes each cell's rows at row_offset[cell] in
+ * discovery order; wc_paths_canonical_order then sorts every pair segment by
+ * (depth, component, primitive sequence, primitive, edge), stable, marks the
+ * first strictly-shortest row of each identity group and returns the
+ * permutation. Replaces the reference topology stages + canonical compact
+ * (native/channel/kernels/evaluated_paths.cu:513-965, paths.cu:1808-2007). */
+int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream);
+
+typedef struct wc_path_rows {
+    int width; /* sequence width (max depth, >= 1) */
+    int* tx_id;
+    int* rx_id;
+    int* depth;
+    int* component_id;
+    int* primitive_id;
+    int* edge_id;
+    int* material_id;
+    float* path_length_m;
+    float* delay_s;
+    float* path_gain;
+    int* primitive_sequence;      /* [K, width] */
+    int* material_sequence;       /* [K, width] */
+    float* interaction_positions; /* [K, width, 3] */
+    float* interaction_normals;   /* [K, width, 3] */
+    float* path_field;            /* [K] c64 */
+    float* coefficient;           /* [K] c64 */
+    float* field_xyz;             /* [K, 3] c64 */
+    float* arrival_direction;     /* [K, 3] */
+    float* departure_direction;   /* [K, 3] */
+} wc_path_rows;
+
+int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const int* row_offset, const wc_path_rows* rows,
+                  void* stream);
+
+int wc_paths_canonical_order(const wc_path_rows* rows, const int* pair_offset, int pair_count, int* permutation,
+                             unsigned char* keep, void* stream);
+
+/* ------------------------------------------------------------ row field operators
+ * Per-row field kernels with the reference facade signatures
+ * (witwin/channel/kernels/fields.py field_free_space :110,
+ *  field_project_complex3 :149, field_reflection_sequence :180;
+ *  native/channel/kernels/fields.cu:36-204). Outputs per row: field_vector
+ * [N,3] c64, coefficient [N] c64, path_field [N] c64, path_gain [N],
+ * path_length [N], delay [N], direction [N,3]. */
+int wc_field_free_space(int count, const float* source, const float* target, const float* tx_power,
+                        const float* tx_polarization, const float* rx_polarization, float frequency_hz,
+                        float* field_vector, float* coefficient, float* path_field, float* path_gain,
+                        float* path_length, float* delay, float* direction, void* stream);
+
+int wc_field_project_complex3(int count, const float* field_vector, const float* direction,
+                              const float* rx_polarization, float* coefficient, float* path_gain, void* stream);
+
+int wc_field_reflection_sequence(int count, int depth, const float* source, const float* target,
+                                 const float* interaction_positions, const float* interaction_normals,
+                                 const float* tx_power, const float* tx_polarization, const float* rx_polarization,
+                                 const float* eps_r, const float* sigma_e, const float* mu_r, const float* gain,
+                                 const float* thickness, float frequency_hz, float* field_vector, float* coefficient,
+                                 float* path_field, float* path_gain, float* path_length, float* delay,
+                                 float* direction, void* stream);
+
+/* ------------------------------------------------------------ accumulation
+ * Deterministic flat accumulation (reference deterministic.cu:829-970,
+ * witwin/channel/kernels/deterministic.py:23-102): rows are ordered by cell
+ * with a stable radix sort and each cell is reduced in row order in float64,
+ * so results are bit-reproducible. Slot map: component 0/1/2 -> 0/1/2,
+ * 5 -> 3, 6 -> 4, 3/4/7 -> 5. Outputs: component_power [6, T*R],
+ * component_field [6, T*R] c64, power_total [T*R], field_total [T*R] c64.
+ * workspace: wc_accumulate_workspace_bytes(count, num_tx, num_rx) bytes. */
+size_t wc_accumulate_workspace_bytes(int count, int num_tx, int num_rx);
+int wc_accumulate_flat(int count, const unsigned char* valid, const int* tx_id, const int* rx_id,
+                       const int* component_id, const float* path_gain, const float* field_real,
+                       const float* field_imag, int num_tx, int num_rx, int coherent, float* component_power,
+                       float* component_field, float* power_total, float* field_total, void* workspace,
+                       size_t workspace_bytes, void* stream);
+
+/* ------------------------------------------------------------ visibility
+ * Segment visibility (reference kernels/geometry.py rayd_visibility_forward
+ * :380 -> RayD). visible[i] = 1 when segment source[i] -> target[i] crosses
+ * no triangle (end points excluded by a 1e-4 m clearance). */
+int wc_visibility(const wc_scene* scene, int count, const float* source, const float* target,
+                  unsigned char* visible, void* stream);
+
+/* ------------------------------------------------------------ discovery
+ * Transmitter-launched specular discovery rays (reference
+ * native/channel/rayd/reflection.cpp:9-21, witwin/channel/interactions/
+ * reflection.py:541-569): ray_count directions on a Fibonacci sphere from
+ * each transmitter, traced with closest hits and specular continuation up to
+ * max_depth bounces. chains [tx_count, ray_count, max_depth] i32 receives the
+ * coplanar group hit at each bounce (-1 after the ray escapes). */
+int wc_discover_chains(const wc_scene* scene, const float* tx_pos, int tx_count, int ray_count, int max_depth,
+                       int* chains, void* stream);
+
+/* Image points of candidate chains: images[k, j] = source of the chain's
+ * transmitter mirrored successively across groups 0..j (float64). */
+int wc_chain_images(const wc_scene* scene, const float* tx_pos, const int* chain_tx, const int* chain_groups,
+                    const int* chain_depth, int chain_count, double* images, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WCMI_H */

@@ -1,0 +1,288 @@
+"""CPU tests: the FP64 oracle against golden vectors and closed forms, the host
+compile logic, and the C ABI of libwcmi.so (load + exported symbols only)."""
+
+from __future__ import annotations
+
+import cmath
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from witwin.channel.propagation.engine import exhaustive_sequences, is_exhaustive, make_tiles
+from witwin.channel.scene.compiler import build_edges, compile as compile_scene, coplanar_face_groups
+from witwin.channel import workloads
+from witwin.core import Mesh, PhysicalMaterial, Scene, Structure
+
+ROOT = Path(__file__).resolve().parents[1]
+EPS0 = 8.8541878128e-12
+C0 = 299792458.0
+
+
+# ----------------------------------------------------------------- oracle pins
+
+def test_slab_known_answer():
+    """Reference tests/kernels/test_field_transport.py:116-159 (normal incidence,
+    3.5 GHz, eps 4, sigma 0.025, gain 0.8, t 0.12 m, L 3 m)."""
+    f = 3.5e9
+    omega = 2 * math.pi * f
+    lam = C0 / f
+    eta = complex(4.0, -0.025 / (omega * EPS0))
+    root = cmath.sqrt(eta)
+    r0 = (1 - root) / (1 + root)
+    ph = cmath.exp(-2j * 2 * math.pi * 0.12 / lam * root)
+    slab = 0.8 * r0 * (1 - ph) / (1 - r0 * r0 * ph)
+    expected = lam / (4 * math.pi * 3.0) * cmath.exp(-2j * math.pi * 3.0 / lam) * slab
+    row = oracle.reflection_sequence([0, 0, 1.0], [0, 0, 2.0], [[0, 0, 0.0]], [[0, 0, 1.0]], 1.0, [1, 0, 0],
+                                     [1, 0, 0], [4.0], [0.025], [1.0], [0.8], [0.12], f)
+    assert abs(row["coefficient"] - expected) <= 1e-8 * abs(expected)
+
+
+def test_slab_limits():
+    # thick lossy slab -> half-space Fresnel; PEC -> -1 / +1; zero thickness -> 0
+    te, tm = oracle.slab_fresnel(0.6, 5.31, 0.0326, 1.0, 1.0, 5.0, 3.5e9)
+    eta = complex(5.31, -0.0326 / (2 * math.pi * 3.5e9 * EPS0))
+    root = cmath.sqrt(eta - (1 - 0.36))
+    assert abs(te - (0.6 - root) / (0.6 + root)) < 1e-10
+    assert abs(tm - (eta * 0.6 - root) / (eta * 0.6 + root)) < 1e-10
+    te, tm = oracle.slab_fresnel(0.3, 1.0, 1e9, 1.0, 1.0, 0.1, 3e9)
+    assert abs(te + 1) < 1e-4 and abs(tm - 1) < 1e-4
+    te, tm = oracle.slab_fresnel(0.3, 4.0, 0.01, 1.0, 1.0, 0.0, 3e9)
+    assert abs(te) < 1e-15 and abs(tm) < 1e-15
+    # Brewster null of the TM interface coefficient (lossless, thick limit via large loss-free phase average)
+    eps = 4.0
+    ct_b = math.cos(math.atan(math.sqrt(eps)))
+    eta = complex(eps, 0.0)
+    root = cmath.sqrt(eta - (1 - ct_b * ct_b))
+    assert abs((eta * ct_b - root) / (eta * ct_b + root)) < 1e-12
+
+
+def test_free_space_closed_form():
+    f = 3.0e9
+    row = oracle.free_space([0, 0, 0], [10.0, 0, 0], 4.0, [0, 0, 1], [0, 0, 1], f)
+    lam = C0 / f
+    expected = lam / (4 * math.pi * 10.0) * cmath.exp(-2j * math.pi * 10.0 / lam)
+    assert abs(row["coefficient"] - expected) < 1e-12 * abs(expected) + 1e-18
+    assert abs(row["path_field"] - 2.0 * expected) < 1e-12
+    # dipole pattern: p_rx.p_tx - (p_tx.k)(p_rx.k) (reference tests/support/reference_channel.py:63-77)
+    row = oracle.free_space([0, 0, 0], [3.0, 4.0, 1.0], 1.0, [0, 0, 1], [0, 0, 1], f)
+    k = np.array([3.0, 4.0, 1.0]) / math.sqrt(26.0)
+    dip = (1 - k[2] * k[2]) ** 2
+    assert abs(row["path_gain"] - (lam / (4 * math.pi * math.sqrt(26.0))) ** 2 * dip) < 1e-12 * row["path_gain"]
+
+
+def test_fresnel_integrals_match_scipy():
+    from scipy.special import fresnel as sp_fresnel
+
+    for x in np.concatenate([np.linspace(-6, 6, 241), [0.01, 1.49, 1.51, 3.99, 4.01, 10, 40]]):
+        s_ref, c_ref = sp_fresnel(x)
+        c, s = oracle.fresnel(float(x))
+        assert abs(c - c_ref) < 2e-9 and abs(s - s_ref) < 2e-9, x
+
+
+def test_transition_function_limits():
+    # F(X) -> 1 for large X; small-X form F ~ sqrt(pi X) e^{j pi/4}
+    big = oracle.transition(200.0)
+    assert abs(big - (1 + 0.5j / 200.0)) < 1e-4
+    small = oracle.transition(1e-6)
+    assert abs(small - math.sqrt(math.pi * 1e-6) * cmath.exp(1j * math.pi / 4)) < 1e-5
+
+
+def _utd_scalar_field(n, phi, phi_p, k, rho, rho_p, r0=-1.0, rn=-1.0):
+    """2-D (normal incidence) soft diffracted field of a unit cylindrical-like
+    source with the UTD coefficient; used only for continuity checks."""
+    L = rho * rho_p / (rho + rho_p)
+    ds, _ = oracle.utd(n, phi, phi_p, 1.0, k, L, r0s=r0, rns=rn)
+    spread = math.sqrt(rho_p / (rho * (rho + rho_p)))
+    return ds * spread * cmath.exp(-1j * k * (rho + rho_p)) / rho_p
+
+
+@pytest.mark.parametrize("n", [1.5, 2.0])
+def test_utd_compensates_incident_shadow_boundary(n):
+    """Across the incident shadow boundary phi = pi + phi' the diffracted field
+    jumps by exactly the (removed) incident field: GO + diffraction is continuous.
+    A sign error in any cot term breaks this."""
+    k = 2 * math.pi / 0.1
+    rho_p, rho = 7.0, 5.0
+    phi_p = 0.6
+    isb = math.pi + phi_p
+    d = 1e-7
+    lit = _utd_scalar_field(n, isb - d, phi_p, k, rho, rho_p)
+    shadow = _utd_scalar_field(n, isb + d, phi_p, k, rho, rho_p)
+    incident = cmath.exp(-1j * k * (rho + rho_p)) / (rho + rho_p) / rho_p * rho_p  # unit amplitude at distance
+    # GO field at the boundary: incident wave at distance rho + rho_p (spherical spreading ~ 1/(rho+rho_p))
+    go = cmath.exp(-1j * k * (rho + rho_p)) / (rho + rho_p)
+    jump = shadow - lit
+    assert abs(jump - go) < 2e-3 * abs(go), (jump, go)
+    del incident
+
+
+def test_utd_compensates_reflection_shadow_boundary():
+    k = 2 * math.pi / 0.1
+    rho_p, rho = 7.0, 5.0
+    phi_p = 0.6
+    n = 1.5
+    rsb = math.pi - phi_p
+    d = 1e-7
+    before = _utd_scalar_field(n, rsb - d, phi_p, k, rho, rho_p, r0=-1.0)
+    after = _utd_scalar_field(n, rsb + d, phi_p, k, rho, rho_p, r0=-1.0)
+    go = -1.0 * cmath.exp(-1j * k * (rho + rho_p)) / (rho + rho_p)  # reflected (soft: R = -1)
+    assert abs((after - before) - go) < 2e-3 * abs(go)
+
+
+def test_stack_transmission_identities():
+    f = 5e9
+    # vacuum layer is transparent
+    t = oracle.stack_t(0.7, [0.3], [1.0], [0.0], [1.0], f, 0)
+    assert abs(abs(t) - 1.0) < 1e-12
+    # lossless single layer: |r|^2 + |t|^2 = 1 (same medium both sides); compare with slab r
+    for pol in (0, 1):
+        t = oracle.stack_t(0.8, [0.125], [4.0], [0.0], [1.0], f, pol)
+        te, tm = oracle.slab_fresnel(0.8, 4.0, 0.0, 1.0, 1.0, 0.125, f)
+        r = te if pol == 0 else tm
+        assert abs(abs(r) ** 2 + abs(t) ** 2 - 1.0) < 1e-10
+    # two identical layers equal one layer of double thickness
+    t2 = oracle.stack_t(0.5, [0.1, 0.1], [3.0, 3.0], [0.01, 0.01], [1.0, 1.0], f, 0)
+    t1 = oracle.stack_t(0.5, [0.2], [3.0], [0.01], [1.0], f, 0)
+    assert abs(t2 - t1) < 1e-12
+
+
+def _two_wall_scene():
+    def wall(v, f, mat, sid):
+        return Structure(Mesh(torch.tensor(v), torch.tensor(f)), mat, surface_id=sid)
+
+    wx = wall([[2.0, -1.0, 0.0], [2.0, 3.0, 0.0], [2.0, -1.0, 2.0], [2.0, 3.0, 2.0]], [[0, 1, 2], [1, 3, 2]],
+              PhysicalMaterial(eps_r=3.0, sigma_e=0.005), 10)
+    wy = wall([[0.0, 2.0, 0.0], [3.0, 2.0, 0.0], [0.0, 2.0, 2.0], [3.0, 2.0, 2.0]], [[0, 2, 1], [1, 2, 3]],
+              PhysicalMaterial(eps_r=4.0, sigma_e=0.01), 11)
+    return Scene(structures=[wx, wy], endpoints=[workloads.transmitter((0.0, 0.0, 1.0)),
+                                                 workloads.receiver((0.0, 1.0, 1.0))])
+
+
+def _exhaustive_chains(groups, depth):
+    chains, depths = [], []
+    for d in range(1, depth + 1):
+        s = exhaustive_sequences(groups, d)
+        pad = np.full((s.shape[0], 5), -1, np.int32)
+        pad[:, :d] = s
+        chains.append(pad)
+        depths.append(np.full(s.shape[0], d, np.int32))
+    return np.concatenate(chains), np.concatenate(depths)
+
+
+def test_oracle_multibounce_golden_vectors():
+    """Reference tests/deterministic/test_reflection_multibounce.py:179-215."""
+    c = compile_scene(_two_wall_scene(), reference_frequency_hz=3e9)
+    chains, depths = _exhaustive_chains(c.group_count, 2)
+    paths = oracle.Oracle(c).pair_paths(tx=(0, 0, 1), rx=(0, 1, 1), frequency=3e9, components=2, max_depth=2,
+                                        chains=chains, chain_depth=depths)
+    assert [p.depth for p in paths] == [1, 1, 2]
+    assert [list(p.prim)[:2] for p in paths] == [[0, -1], [2, -1], [1, 2]]
+    # golden lengths are quoted to 10 digits; tolerance as in the reference test
+    np.testing.assert_allclose([p.length for p in paths], [4.123105526, 3.0, 5.0], rtol=1e-5, atol=1e-6)
+    assert abs(paths[0].length - math.sqrt(17.0)) < 1e-12
+
+
+def test_oracle_transmission_vacuum_wall_equals_los():
+    """Reference tests/deterministic/test_transmission.py:88-120: a vacuum wall
+    (eps_r = 1, sigma = 0) leaves the line-of-sight field unchanged."""
+    vac = PhysicalMaterial(eps_r=1.0, sigma_e=0.0, thickness_m=0.3)
+    v = np.array([[2.0, -4, -4], [2.0, 4, -4], [2.0, -4, 4], [2.0, 4, 4]], np.float32)
+    wall = Structure(Mesh(torch.from_numpy(v), torch.tensor([[0, 1, 2], [1, 3, 2]])), vac, surface_id=1)
+    scene = Scene(structures=[wall], endpoints=[workloads.transmitter((0.0, 0.0, 0.0)),
+                                                workloads.receiver((5.0, 1.0, 0.5))])
+    c = compile_scene(scene, reference_frequency_hz=3e9)
+    o = oracle.Oracle(c)
+    paths = o.pair_paths(tx=(0, 0, 0), rx=(5, 1, 0.5), frequency=3e9, components=9, max_depth=1,
+                         chains=np.zeros((0, 5), np.int32), chain_depth=np.zeros(0, np.int32))
+    assert [p.component for p in paths] == [5]
+    ref = oracle.free_space([0, 0, 0], [5, 1, 0.5], 1.0, [0, 0, 1], [0, 0, 1], 3e9)
+    # the vacuum wall's stack phase exp(-j k d cos) cancels the carrier shortening
+    assert abs(complex(*paths[0].coefficient) - ref["coefficient"]) < 1e-4 * abs(ref["coefficient"])
+
+
+def test_oracle_canonical_sort_contract():
+    """Reference sort-order vector (test_reflection_multibounce.py:25-80): rows
+    with sequences [[4,2],[4,1],[3,9]] sort to [[3,9],[4,1],[4,2]]."""
+    seqs = np.array([[4, 2], [4, 1], [3, 9]])
+    order = sorted(range(3), key=lambda i: tuple(seqs[i]))
+    assert seqs[order].tolist() == [[3, 9], [4, 1], [4, 2]]
+
+
+# ----------------------------------------------------------------- host compile
+
+def test_coplanar_groups_golden():
+    """Reference test_reflection_multibounce.py:83-117: same surface id, two
+    planes -> groups [0, 0, 1], representatives [0, 2]."""
+    pts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    faces = np.array([[0, 1, 2], [1, 3, 2], [4, 5, 6]])
+    tri_a = pts[faces[:, 0]]
+    n = np.cross(pts[faces[:, 1]] - tri_a, pts[faces[:, 2]] - tri_a)
+    g = coplanar_face_groups(tri_a, n, np.full(3, 7))
+    assert g["group_count"] == 2
+    assert g["face_group_id"].tolist() == [0, 0, 1]
+    assert g["representative_faces"].tolist() == [0, 2]
+
+
+def test_box_edges_are_convex_wedges():
+    v, f, _ = workloads.box_mesh(0, 0, 2, 3, 4)
+    groups = coplanar_face_groups(v[f[:, 0]], np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]]),
+                                  np.zeros(12))
+    edges = build_edges(v, f, groups["face_group_id"], np.zeros(12, np.int32))
+    assert len(edges) == 12
+    np.testing.assert_allclose(edges["n"], 1.5, atol=1e-6)
+    # t0 is perpendicular to the edge and n0 points away from the second face
+    d = np.stack([edges["dx"], edges["dy"], edges["dz"]], 1)
+    t0 = np.stack([edges["t0x"], edges["t0y"], edges["t0z"]], 1)
+    assert np.abs((d * t0).sum(1)).max() < 1e-6
+
+
+def test_exhaustive_planning_thresholds():
+    assert is_exhaustive(4096, 1) and not is_exhaustive(4097, 1)
+    assert is_exhaustive(316, 2) and not is_exhaustive(317, 2)
+    s = exhaustive_sequences(3, 3)
+    assert s.shape == (3 * 2 * 2, 3)
+    assert (s[:, 1:] != s[:, :-1]).all()
+    assert s.tolist() == sorted(s.tolist())
+
+
+def test_tiles_never_straddle_transmitters():
+    offsets = np.array([0, 100, 101, 300])
+    tiles = make_tiles(offsets)
+    for t, base in tiles:
+        assert offsets[t] <= base < offsets[t + 1]
+    assert sum(min(64, offsets[t + 1] - b) for t, b in tiles) == 300
+
+
+def test_urban_canyon_compiles():
+    c = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(4, 4)), reference_frequency_hz=28e9)
+    assert 760 <= c.face_count <= 780
+    assert c.group_count == 64 * 6 + 1
+    assert c.materials.count == 3
+
+
+# ----------------------------------------------------------------- C ABI
+
+def test_library_exports_every_declared_symbol():
+    from witwin.channel import _native
+
+    lib = _native.load()
+    header = (ROOT / "include" / "wcmi.h").read_text()
+    declared = set(re.findall(r"^\s*(?:[\w\*]+\s+)+\**(wc_\w+)\(", header, flags=re.M))
+    assert declared, "no declarations parsed"
+    for name in sorted(declared):
+        assert hasattr(lib, name), f"libwcmi.so does not export {name}"
+    assert lib.wc_abi_version() == 1
+    assert lib.wc_build_target() == b"gfx950"
+
+
+def test_native_errors_raise_without_gpu_work():
+    from witwin.channel import _native
+
+    with pytest.raises(RuntimeError, match="count must be non-negative"):
+        _native.call("wc_field_free_space", -1, *([None] * 5), 1.0, *([None] * 7), None)

@@ -1,0 +1,345 @@
+"""GPU parity of the HIP hot path against the FP64 oracle (run on an MI355X).
+
+Gates (stated per test): path / cell power within 0.01 dB, coefficient phase
+within 1e-3 rad, bit-exact topology (component, depth, primitive sequence,
+edge id) and per-pair segmentation. Cells whose coherent sum is a deep
+interference null are reported separately: there the relative error of
+|sum F|^2 is the per-path error amplified by sum|F|^2 / |sum F|^2.
+"""
+
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gpu_helpers import DB_TOL, PHASE_TOL, db_error, phase_error, run_gpu, run_oracle, tx_chains
+from witwin.channel import _native, workloads
+from witwin.channel.propagation import engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# ----------------------------------------------------------------- row operators
+
+def test_free_space_rows_match_oracle(device):
+    _say("case: free_space rows, 512 random links, 28 GHz")
+    rng = np.random.default_rng(11)
+    n = 512
+    src = rng.uniform(-200, 200, (n, 3)).astype(np.float32)
+    tgt = rng.uniform(-200, 200, (n, 3)).astype(np.float32)
+    txp = rng.normal(size=(n, 3)).astype(np.float32)
+    rxp = rng.normal(size=(n, 3)).astype(np.float32)
+    power = rng.uniform(0.5, 4.0, n).astype(np.float32)
+    t = lambda a: torch.as_tensor(a).to(device).contiguous()  # noqa: E731
+    outs = {k: torch.empty(s, dtype=d, device=device) for k, s, d in (
+        ("fv", (n, 3), torch.complex64), ("coef", (n,), torch.complex64), ("pf", (n,), torch.complex64),
+        ("gain", (n,), torch.float32), ("len", (n,), torch.float32), ("delay", (n,), torch.float32),
+        ("dir", (n, 3), torch.float32))}
+    args = [t(src), t(tgt), t(power), t(txp), t(rxp)]
+    _native.call("wc_field_free_space", n, *[_native.ptr(a) for a in args], 28.0e9,
+                 *[_native.ptr(outs[k]) for k in ("fv", "coef", "pf", "gain", "len", "delay", "dir")],
+                 _native.stream_handle(device))
+    torch.cuda.synchronize()
+    coef = outs["coef"].cpu().numpy()
+    gain = outs["gain"].cpu().numpy()
+    for i in range(n):
+        ref = oracle.free_space(src[i], tgt[i], power[i], txp[i], rxp[i], 28.0e9)
+        assert abs(coef[i] - ref["coefficient"]) <= 1e-4 * abs(ref["coefficient"]) + 1e-12
+        if abs(ref["coefficient"]) > 1e-9:
+            assert phase_error(coef[i], ref["coefficient"]) <= PHASE_TOL
+        assert db_error(gain[i:i + 1], np.array([ref["path_gain"]]))[0] <= DB_TOL
+
+
+def test_reflection_sequence_known_answer_and_oracle(device):
+    _say("case: reflection_sequence rows, slab known answer + 256 random depth-2 rows")
+    # reference known answer (tests/kernels/test_field_transport.py:116-159)
+    f = 3.5e9
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32)).to(device).contiguous()  # noqa: E731
+
+    def run(src, tgt, pos, nrm, txp, rxp, eps, sig, mu, gain, thick, power, freq):
+        n, d = pos.shape[0], pos.shape[1]
+        outs = [torch.empty(s, dtype=dt, device=device) for s, dt in (
+            ((n, 3), torch.complex64), ((n,), torch.complex64), ((n,), torch.complex64), ((n,), torch.float32),
+            ((n,), torch.float32), ((n,), torch.float32), ((n, 3), torch.float32))]
+        ins = [t(src), t(tgt), t(pos), t(nrm), t(power), t(txp), t(rxp), t(eps), t(sig), t(mu), t(gain), t(thick)]
+        _native.call("wc_field_reflection_sequence", n, d, *[_native.ptr(a) for a in ins], freq,
+                     *[_native.ptr(o) for o in outs], _native.stream_handle(device))
+        torch.cuda.synchronize()
+        return outs
+
+    outs = run(np.array([[0, 0, 1.0]]), np.array([[0, 0, 2.0]]), np.zeros((1, 1, 3)), np.array([[[0, 0, 1.0]]]),
+               np.array([[1.0, 0, 0]]), np.array([[1.0, 0, 0]]), [[4.0]], [[0.025]], [[1.0]], [[0.8]], [[0.12]],
+               [1.0], f)
+    eps0 = 8.8541878128e-12
+    omega = 2 * math.pi * f
+    lam = 299792458.0 / f
+    eta = complex(4.0, -0.025 / (omega * eps0))
+    root = np.sqrt(eta)
+    r0 = (1 - root) / (1 + root)
+    ph = np.exp(-2j * 2 * math.pi * 0.12 / lam * root)
+    expected = lam / (4 * math.pi * 3.0) * np.exp(-2j * math.pi * 3.0 / lam) * 0.8 * r0 * (1 - ph) / (1 - r0 * r0 * ph)
+    got = outs[1].cpu().numpy()[0]
+    assert abs(got - expected) <= 1e-4 * abs(expected)
+    assert phase_error(got, expected) <= PHASE_TOL
+
+    rng = np.random.default_rng(5)
+    n = 256
+    src = rng.uniform(-20, 20, (n, 3))
+    tgt = rng.uniform(-20, 20, (n, 3))
+    pos = rng.uniform(-20, 20, (n, 2, 3))
+    nrm = rng.normal(size=(n, 2, 3))
+    txp = rng.normal(size=(n, 3))
+    rxp = rng.normal(size=(n, 3))
+    eps = rng.uniform(2, 8, (n, 2))
+    sig = rng.uniform(0, 0.5, (n, 2))
+    mu = np.ones((n, 2))
+    gain = rng.uniform(0.5, 1, (n, 2))
+    thick = rng.uniform(0.05, 0.3, (n, 2))
+    power = rng.uniform(0.5, 2, n)
+    outs = run(src, tgt, pos, nrm, txp, rxp, eps, sig, mu, gain, thick, power, 28e9)
+    coef = outs[1].cpu().numpy()
+    for i in range(n):
+        ref = oracle.reflection_sequence(np.float32(src[i]), np.float32(tgt[i]), np.float32(pos[i]),
+                                         np.float32(nrm[i]), np.float32(power[i]), np.float32(txp[i]),
+                                         np.float32(rxp[i]), np.float32(eps[i]), np.float32(sig[i]),
+                                         np.float32(mu[i]), np.float32(gain[i]), np.float32(thick[i]), 28e9)
+        scale = abs(ref["coefficient"])
+        if scale < 1e-12:
+            continue
+        assert abs(coef[i] - ref["coefficient"]) <= 2e-4 * scale, i
+        assert phase_error(coef[i], ref["coefficient"]) <= PHASE_TOL, i
+
+
+# ----------------------------------------------------------------- BVH
+
+def test_bvh_visibility_matches_brute_force(device):
+    _say("case: LBVH visibility, urban canyon seed 2, 4096 random segments")
+    scene = workloads.urban_canyon_scene(seed=2, shape=(2, 2))
+    run = run_gpu(scene, {"los"}, 0, 28e9, device)
+    rng = np.random.default_rng(7)
+    n = 4096
+    a = rng.uniform(-10, 390, (n, 3)).astype(np.float32)
+    b = rng.uniform(-10, 390, (n, 3)).astype(np.float32)
+    a[:, 2] = rng.uniform(0.5, 45, n)
+    b[:, 2] = rng.uniform(0.5, 45, n)
+    ta, tb = torch.as_tensor(a).to(device), torch.as_tensor(b).to(device)
+    vis = torch.empty(n, dtype=torch.uint8, device=device)
+    _native.call("wc_visibility", run.dscene.ref(), n, _native.ptr(ta), _native.ptr(tb), _native.ptr(vis),
+                 _native.stream_handle(device))
+    torch.cuda.synchronize()
+    got = vis.cpu().numpy().astype(bool)
+    tri = run.compiled.vertices[run.compiled.faces].astype(np.float64)
+    v0, e1, e2 = tri[:, 0], tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]
+    mismatch = 0
+    for i in range(n):
+        o = a[i].astype(np.float64)
+        d = b[i].astype(np.float64) - o
+        length = np.linalg.norm(d)
+        p = np.cross(d, e2)
+        det = (e1 * p).sum(1)
+        ok = np.abs(det) > 1e-300
+        inv = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)
+        tv = o - v0
+        u = (tv * p).sum(1) * inv
+        q = np.cross(tv, e1)
+        v = (d * q).sum(1) * inv
+        tt = (e2 * q).sum(1) * inv
+        eps = 1e-4 / length
+        hit = ok & (u >= 0) & (u <= 1) & (v >= 0) & (u + v <= 1) & (tt > eps) & (tt < 1 - eps)
+        if hit.any() == got[i]:
+            mismatch += 1
+    _say(f"visibility mismatches: {mismatch} / {n}")
+    assert mismatch == 0
+
+
+# ----------------------------------------------------------------- radiomaps
+
+def test_two_ray_radiomap_matches_oracle(device):
+    _say("case: two-ray link map 64x64, 3.5 GHz, los + ground reflection")
+    scene = workloads.two_ray_scene()
+    run = run_gpu(scene, {"los", "reflection"}, 1, 3.5e9, device)
+    cells = run.inputs.total_cells
+    idx = np.arange(cells)
+    ref = run_oracle(run, 0, idx)
+    rec = ref["record"]
+    p_gpu = run.out.power_total.cpu().numpy()
+    f_gpu = run.out.field_total.cpu().numpy()
+    counts = run.out.path_count.cpu().numpy()
+    assert np.array_equal(counts, ref["counts"]), "path counts differ"
+    assert (counts == 2).all()
+    err = db_error(p_gpu, rec[:, 0])
+    _say(f"two-ray: max dB error {err.max():.2e}")
+    assert err.max() <= DB_TOL
+    fref = rec[:, 1] + 1j * rec[:, 2]
+    assert phase_error(f_gpu, fref).max() <= PHASE_TOL
+    for slot in (0, 1):
+        cp = run.out.component_power[slot].cpu().numpy()
+        assert db_error(cp, rec[:, 3 + slot]).max() <= DB_TOL
+
+
+def _canyon(device, shape, spacing, components=("los", "reflection", "diffraction"), depth=3):
+    scene = workloads.urban_canyon_scene(seed=2, shape=shape, spacing=spacing)
+    return run_gpu(scene, set(components), depth, 28e9, device)
+
+
+def test_urban_canyon_radiomap_matches_oracle(device):
+    _say("case: urban canyon seed 2, 64x64 grid at 6 m, depth 3, los+refl+diff, 28 GHz; 160 sampled cells")
+    run = _canyon(device, (64, 64), 6.0)
+    cells = run.inputs.total_cells
+    rng = np.random.default_rng(1)
+    idx = np.sort(rng.choice(cells, 160, replace=False))
+    ref = run_oracle(run, 0, idx, paths_cap=512, perturb=2e-4)
+    rec = ref["record"]
+    amb = ref["ambiguous"].astype(bool)
+    counts = run.out.path_count.cpu().numpy()[idx]
+    p_gpu = run.out.power_total.cpu().numpy()[idx]
+    stable = ~amb
+    _say(f"ambiguous cells: {amb.sum()} / {len(idx)}")
+    assert np.array_equal(counts[stable], ref["counts"][stable])
+    incoherent = np.array([sum(p.gain for p in cell) for cell in ref["paths"]])
+    cond = rec[:, 0] / np.maximum(incoherent, 1e-300)
+    err = db_error(p_gpu, rec[:, 0])
+    well = stable & (cond > 1e-3) & (rec[:, 0] > 1e-25)
+    _say(f"canyon: max dB error (well-conditioned, {well.sum()} cells) {err[well].max():.3e}; "
+         f"all stable {err[stable].max():.3e}")
+    assert err[well].max() <= DB_TOL
+
+
+def test_urban_canyon_export_topology_bit_exact(device):
+    _say("case: urban canyon seed 2, 24x24 grid at 15 m, explicit rows vs oracle topology")
+    run = _canyon(device, (24, 24), 15.0)
+    rows = engine.export_rows(run.dscene, run.inputs, width=run.depth)
+    cells = run.inputs.total_cells
+    idx = np.arange(cells)
+    ref = run_oracle(run, 0, idx, paths_cap=512, perturb=2e-4)
+    amb = ref["ambiguous"].astype(bool)
+    rx_id = rows["rx_id"].cpu().numpy()
+    comp = rows["component_id"].cpu().numpy()
+    depth = rows["depth"].cpu().numpy()
+    seq = rows["primitive_sequence"].cpu().numpy()
+    edge = rows["edge_id"].cpu().numpy()
+    gain = rows["path_gain"].cpu().numpy()
+    length = rows["path_length_m"].cpu().numpy()
+    # rows are canonical: rx ascending
+    assert np.all(np.diff(rx_id) >= 0)
+    starts = np.searchsorted(rx_id, idx, side="left")
+    ends = np.searchsorted(rx_id, idx, side="right")
+    checked = 0
+    worst_db = 0.0
+    for i in idx:
+        if amb[i]:
+            continue
+        ref_paths = ref["paths"][i]
+        sl = slice(starts[i], ends[i])
+        assert ends[i] - starts[i] == len(ref_paths), f"cell {i}: {ends[i] - starts[i]} vs {len(ref_paths)}"
+        for j, p in enumerate(ref_paths):
+            k = starts[i] + j
+            assert comp[k] == p.component and depth[k] == p.depth, f"cell {i} row {j}"
+            if p.component == 2:
+                assert edge[k] == p.edge
+            else:
+                for w in range(run.depth):
+                    assert seq[k, w] == (p.prim[w] if w < p.depth else -1), f"cell {i} row {j} slot {w}"
+            assert abs(length[k] - p.length) <= 1e-5 * p.length + 1e-6
+            if p.gain > 1e-25:
+                worst_db = max(worst_db, float(db_error(gain[k:k + 1], np.array([p.gain]))[0]))
+        checked += 1
+        del sl
+    _say(f"topology checked on {checked} cells (ambiguous skipped: {amb.sum()}), worst path dB {worst_db:.2e}")
+    assert checked >= 0.9 * cells
+    assert worst_db <= DB_TOL
+
+
+def test_discovery_rays_match_oracle(device):
+    _say("case: discovery rays, urban canyon seed 2, 4096 of 262144 rays, depth 3")
+    scene = workloads.urban_canyon_scene(seed=2, shape=(2, 2))
+    run = run_gpu(scene, {"los"}, 0, 28e9, device)
+    tx = torch.as_tensor(run.compiled.endpoints.tx_position).to(device)
+    chains = engine.discover(run.dscene, tx, 3, 262_144)[0]
+    orc = oracle.Oracle(run.compiled)
+    rays = np.arange(0, 262_144, 64)
+    ref = orc.discover(run.compiled.endpoints.tx_position[0], 262_144, 3, rays)
+    same = (chains[rays] == ref).all(axis=1)
+    _say(f"discovery: {same.sum()} / {len(rays)} rays identical")
+    assert same.mean() >= 0.995
+
+
+def test_multibounce_golden_vectors(device):
+    _say("case: two-wall multibounce golden vectors (reference test_reflection_multibounce.py:126-215)")
+    from witwin.channel.deterministic import Config, solve
+    from witwin.core import Mesh, PhysicalMaterial, Scene, Structure
+
+    def wall(v, f, mat, sid):
+        return Structure(Mesh(torch.tensor(v), torch.tensor(f)), mat, surface_id=sid)
+
+    wx = wall([[2.0, -1.0, 0.0], [2.0, 3.0, 0.0], [2.0, -1.0, 2.0], [2.0, 3.0, 2.0]], [[0, 1, 2], [1, 3, 2]],
+              PhysicalMaterial(eps_r=3.0, sigma_e=0.005), 10)
+    wy = wall([[0.0, 2.0, 0.0], [3.0, 2.0, 0.0], [0.0, 2.0, 2.0], [3.0, 2.0, 2.0]], [[0, 2, 1], [1, 2, 3]],
+              PhysicalMaterial(eps_r=4.0, sigma_e=0.01), 11)
+    scene = Scene(structures=[wx, wy], endpoints=[workloads.transmitter((0.0, 0.0, 1.0)),
+                                                  workloads.receiver((0.0, 1.0, 1.0))])
+    result = solve(scene, Config(components={"reflection"}, max_depth=2, coherent=True, export_paths=True),
+                   reference_frequency_hz=3.0e9)
+    p = result.paths
+    assert p.depth.cpu().tolist() == [1, 1, 2]
+    assert p.primitive_sequence.cpu().tolist() == [[0, -1], [2, -1], [1, 2]]
+    np.testing.assert_allclose(p.path_length_m.cpu().numpy(), [4.123105526, 3.0, 5.0], rtol=1e-5, atol=1e-6)
+    torch.testing.assert_close(result.path_gain.reshape(-1), result.field.abs().square().reshape(-1), rtol=2e-4,
+                               atol=1e-10)
+
+
+def test_accumulate_flat_matches_float64(device):
+    _say("case: deterministic accumulate_flat, 20000 random rows onto 7x300 cells")
+    rng = np.random.default_rng(3)
+    n, t, r = 20000, 7, 300
+    valid = rng.uniform(size=n) < 0.9
+    tx = rng.integers(0, t, n).astype(np.int32)
+    rx = rng.integers(0, r, n).astype(np.int32)
+    comp = rng.choice([0, 1, 2, 5, 6, 3, 9], n).astype(np.int32)
+    gain = rng.uniform(0, 1e-6, n).astype(np.float32)
+    fre = rng.normal(size=n).astype(np.float32) * 1e-3
+    fim = rng.normal(size=n).astype(np.float32) * 1e-3
+    dt = lambda a: torch.as_tensor(a).to(device)  # noqa: E731
+    cells = t * r
+    cp = torch.empty((6, cells), dtype=torch.float32, device=device)
+    cf = torch.empty((6, cells), dtype=torch.complex64, device=device)
+    pt = torch.empty(cells, dtype=torch.float32, device=device)
+    ft = torch.empty(cells, dtype=torch.complex64, device=device)
+    ws = int(_native.call("wc_accumulate_workspace_bytes", n, t, r))
+    work = torch.empty(ws, dtype=torch.uint8, device=device)
+    args = [dt(valid.astype(np.uint8)), dt(tx), dt(rx), dt(comp), dt(gain), dt(fre), dt(fim)]
+    for coherent in (1, 0):
+        _native.call("wc_accumulate_flat", n, *[_native.ptr(a) for a in args], t, r, coherent, _native.ptr(cp),
+                     _native.ptr(cf), _native.ptr(pt), _native.ptr(ft), _native.ptr(work), ws,
+                     _native.stream_handle(device))
+        torch.cuda.synchronize()
+        slot_map = {0: 0, 1: 1, 2: 2, 5: 3, 6: 4, 3: 5}
+        fs = np.zeros((6, cells), np.complex128)
+        ps = np.zeros((6, cells))
+        for i in range(n):
+            if not valid[i] or comp[i] not in slot_map:
+                continue
+            c = tx[i] * r + rx[i]
+            s = slot_map[int(comp[i])]
+            fs[s, c] += complex(float(fre[i]), float(fim[i]))
+            ps[s, c] += float(gain[i])
+        if coherent:
+            comp_p = np.abs(fs) ** 2
+            comp_p[4] = ps[4]
+            tot_f = fs[[0, 1, 2, 3, 5]].sum(0)
+            tot_p = np.abs(tot_f) ** 2 + ps[4]
+        else:
+            comp_p = ps
+            tot_p = ps.sum(0)
+        np.testing.assert_allclose(cp.cpu().numpy(), comp_p, rtol=2e-6, atol=1e-30)
+        np.testing.assert_allclose(pt.cpu().numpy(), tot_p, rtol=2e-6, atol=1e-30)

@@ -1,0 +1,16 @@
+// Error reporting shared by every C-ABI entry point: a non-zero status plus a
+// thread-local message readable through wc_last_error().
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstddef>
+
+int wc_fail(const char* message);
+int wc_fail_hip(const char* where, hipError_t err);
+
+namespace wc {
+int radix_sort_pairs(unsigned int* keys, int* values, unsigned int* keys_tmp, int* values_tmp, int* hist, int n,
+                     int key_bits, hipStream_t stream);
+size_t radix_sort_hist_ints(int n);
+}  // namespace wc

@@ -1,0 +1,628 @@
+// Fused radiomap, path export and discovery kernels behind the C ABI.
+#include "wc_api_internal.h"
+#include "wc_paths.h"
+#include "../../include/wcmi.h"
+
+namespace wc {
+
+static_assert(sizeof(SceneView) == sizeof(wc_scene), "wc_scene must mirror SceneView");
+static_assert(sizeof(Edge) == sizeof(wc_edge), "wc_edge must mirror Edge");
+
+__device__ __forceinline__ int slot_of(int component) {
+    switch (component) {
+        case kCidLos: return 0;
+        case kCidRefl: return 1;
+        case kCidDiff: return 2;
+        case kCidTrans: return 3;
+        default: return -1;
+    }
+}
+
+// Walks every candidate of one (transmitter, cell) pair in canonical order
+// and hands each valid path to the sink.
+template <class Sink>
+__device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, Sink& sink) {
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
+    const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
+    const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
+    const V3<float> rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
+    const float freq = q.frequency_hz;
+    const double k_d = (double)freq * kTwoPi / kC0;
+    const float k = (float)k_d;
+    const int comps = q.components;
+    const int max_depth = q.max_depth;
+
+    // --- straight segment: line of sight and ordered wall hits
+    Hit hits[kMaxHits];
+    int nh = 0;
+    const V3<float> seg = rx - tx;
+    const V3<double> segd = to_d(rx) - to_d(tx);
+    const double dist_d = norm(segd);
+    const V3<float> dir = safe_normalize(seg, v3c<float>(0.f, 0.f, 1.f));
+    if (comps & (kCompLos | kCompTrans)) {
+        const float len = (float)dist_d;
+        if (len > 2.0f * kSegmentBias) {
+            const float eps = kSegmentBias / len;
+            const RayPre r = ray_setup(tx.x, tx.y, tx.z, seg.x, seg.y, seg.z);
+            if (comps & kCompTrans) {
+                nh = gather_hits(s, r, eps, 1.0f - eps, max_depth, hits);
+            } else {
+                nh = occluded(s, r, eps, 1.0f - eps, -1, -1) ? 1 : 0;
+            }
+        }
+    }
+    if ((comps & kCompLos) && nh == 0) {
+        PathRecord rec;
+        rec.component = kCidLos;
+        rec.depth = 0;
+        rec.candidate = -1;
+        rec.edge = -1;
+        rec.length_d = dist_d;
+        rec.length = (float)dist_d;
+        const C3<float> v = los_field(dir, tx_pol, k, k_d, dist_d);
+        finish_record(rec, v, dir, rx_pol, sqrt_power);
+        rec.dep[0] = dir.x; rec.dep[1] = dir.y; rec.dep[2] = dir.z;
+        sink.path(rec);
+    }
+    for (int d = 1; d <= max_depth; ++d) {
+        if (comps & kCompRefl) {
+            const int c0 = q.chain_offset[t * kMaxDepth + d - 1];
+            const int c1 = q.chain_offset[t * kMaxDepth + d];
+            for (int c = c0; c < c1; ++c) {
+                const int* groups = q.chain_groups + c * kMaxDepth;
+                const double* images = q.chain_images + c * kMaxDepth * 3;
+                V3<double> hd[kMaxDepth];
+                int faces[kMaxDepth];
+                if (!solve_chain(s, d, groups, images, tx, rx, hd, faces)) continue;
+                PathRecord rec;
+                rec.component = kCidRefl;
+                rec.depth = d;
+                rec.candidate = c;
+                rec.edge = -1;
+                V3<float> hf[kMaxDepth], nf[kMaxDepth];
+                for (int b = 0; b < d; ++b) {
+                    hf[b] = to_f(hd[b]);
+                    double off;
+                    nf[b] = to_f(group_normal(s, groups[b], off));
+                    rec.prim[b] = faces[b];
+                    rec.mat[b] = s.face_material[faces[b]];
+                    rec.pos[b][0] = hf[b].x; rec.pos[b][1] = hf[b].y; rec.pos[b][2] = hf[b].z;
+                    rec.nrm[b][0] = nf[b].x; rec.nrm[b][1] = nf[b].y; rec.nrm[b][2] = nf[b].z;
+                }
+                const V3<double> img{images[3 * (d - 1)], images[3 * (d - 1) + 1], images[3 * (d - 1) + 2]};
+                rec.length_d = norm(to_d(rx) - img);
+                rec.length = (float)rec.length_d;
+                V3<float> arrival, departure;
+                const C3<float> v = reflection_field(s, d, tx, rx, hf, nf, rec.mat, tx_pol, freq, k_d,
+                                                     rec.length_d, arrival, departure);
+                finish_record(rec, v, arrival, rx_pol, sqrt_power);
+                rec.dep[0] = departure.x; rec.dep[1] = departure.y; rec.dep[2] = departure.z;
+                sink.path(rec);
+            }
+        }
+        if (d == 1 && (comps & kCompDiff)) {
+            for (int ei = 0; ei < s.edge_count; ++ei) {
+                const Edge e = s.edges[ei];
+                DiffGeometry g;
+                if (!diffraction_geometry(e, tx, rx, g)) continue;
+                // visibility at the clamped point of the finite edge
+                const double tc = fmin(fmax(g.tstar, 0.0), (double)e.length);
+                const V3<float> qc{e.ax + e.dx * (float)tc, e.ay + e.dy * (float)tc, e.az + e.dz * (float)tc};
+                bool vis = true;
+                {
+                    const V3<float> a = qc - tx;
+                    const float len = norm(a);
+                    if (len > 2.0f * kSegmentBias) {
+                        const float eps = kSegmentBias / len;
+                        const RayPre r = ray_setup(tx.x, tx.y, tx.z, a.x, a.y, a.z);
+                        vis = !occluded(s, r, eps, 1.0f - eps, e.group0, e.group1);
+                    }
+                }
+                if (vis) {
+                    const V3<float> b = rx - qc;
+                    const float len = norm(b);
+                    if (len > 2.0f * kSegmentBias) {
+                        const float eps = kSegmentBias / len;
+                        const RayPre r = ray_setup(qc.x, qc.y, qc.z, b.x, b.y, b.z);
+                        vis = !occluded(s, r, eps, 1.0f - eps, e.group0, e.group1);
+                    }
+                }
+                if (!vis) continue;
+                PathRecord rec;
+                rec.component = kCidDiff;
+                rec.depth = 1;
+                rec.candidate = ei;
+                rec.edge = ei;
+                rec.prim[0] = ei;
+                rec.mat[0] = e.material0;
+                rec.pos[0][0] = (float)g.q.x; rec.pos[0][1] = (float)g.q.y; rec.pos[0][2] = (float)g.q.z;
+                rec.nrm[0][0] = e.n0x; rec.nrm[0][1] = e.n0y; rec.nrm[0][2] = e.n0z;
+                rec.length_d = g.s + g.sp;
+                rec.length = (float)rec.length_d;
+                V3<float> arrival, departure;
+                const C3<float> v = diffraction_field(s, e, g, tx, rx, tx_pol, freq, k_d, arrival, departure);
+                finish_record(rec, v, arrival, rx_pol, sqrt_power);
+                if (!(rec.gain >= kGainFloor)) continue;
+                rec.dep[0] = departure.x; rec.dep[1] = departure.y; rec.dep[2] = departure.z;
+                sink.path(rec);
+            }
+        }
+        if ((comps & kCompTrans) && nh == d) {
+            PathRecord rec;
+            rec.component = kCidTrans;
+            rec.depth = d;
+            rec.candidate = -1;
+            rec.edge = -1;
+            V3<float> normals[kMaxHits];
+            const C3<float> v = transmission_field(s, d, hits, dir, tx_pol, freq, k_d, dist_d, rec.mat, normals);
+            for (int w = 0; w < d; ++w) {
+                rec.prim[w] = hits[w].face;
+                const float tt = hits[w].t;
+                rec.pos[w][0] = tx.x + seg.x * tt; rec.pos[w][1] = tx.y + seg.y * tt; rec.pos[w][2] = tx.z + seg.z * tt;
+                rec.nrm[w][0] = normals[w].x; rec.nrm[w][1] = normals[w].y; rec.nrm[w][2] = normals[w].z;
+            }
+            rec.length_d = dist_d;
+            rec.length = (float)dist_d;
+            finish_record(rec, v, dir, rx_pol, sqrt_power);
+            rec.dep[0] = dir.x; rec.dep[1] = dir.y; rec.dep[2] = dir.z;
+            sink.path(rec);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ sinks
+struct AccumSink {
+    double s_re = 0.0, s_im = 0.0, p_inc = 0.0;
+    double c_re[4] = {0, 0, 0, 0}, c_im[4] = {0, 0, 0, 0}, c_pow[4] = {0, 0, 0, 0};
+    double dv[6] = {0, 0, 0, 0, 0, 0};
+    int count = 0;
+    int* topo = nullptr;
+    int cap = 0;
+    float sqrt_power = 1.0f;
+    __device__ void path(const PathRecord& r) {
+        const int slot = slot_of(r.component);
+        const double re = r.path_field.re, im = r.path_field.im;
+        c_re[slot] += re;
+        c_im[slot] += im;
+        c_pow[slot] += (double)r.gain;
+        s_re += re;
+        s_im += im;
+        p_inc += (double)r.gain;
+        if (r.component == kCidDiff) {
+            dv[0] += (double)r.field.x.re * sqrt_power; dv[1] += (double)r.field.x.im * sqrt_power;
+            dv[2] += (double)r.field.y.re * sqrt_power; dv[3] += (double)r.field.y.im * sqrt_power;
+            dv[4] += (double)r.field.z.re * sqrt_power; dv[5] += (double)r.field.z.im * sqrt_power;
+        }
+        if (topo && count < cap) topo[count] = (r.component << 24) | (r.candidate & 0xFFFFFF);
+        ++count;
+    }
+};
+
+struct CountSink {
+    int count = 0;
+    __device__ void path(const PathRecord&) { ++count; }
+};
+
+struct EmitSink {
+    const wc_path_rows* rows;
+    int row;
+    int tx, rx;
+    __device__ void path(const PathRecord& r) {
+        const int k = row++;
+        const int w = rows->width;
+        rows->tx_id[k] = tx;
+        rows->rx_id[k] = rx;
+        rows->depth[k] = r.depth;
+        rows->component_id[k] = r.component;
+        rows->edge_id[k] = r.edge;
+        rows->path_length_m[k] = r.length;
+        rows->delay_s[k] = (float)(r.length_d / kC0);
+        rows->path_gain[k] = r.gain;
+        const bool has_seq = r.component != kCidLos;
+        rows->primitive_id[k] = has_seq ? r.prim[0] : -1;
+        rows->material_id[k] = has_seq ? r.mat[0] : -1;
+        for (int j = 0; j < w; ++j) {
+            const bool on = has_seq && j < r.depth;
+            rows->primitive_sequence[k * w + j] = on ? r.prim[j] : -1;
+            rows->material_sequence[k * w + j] = on ? r.mat[j] : -1;
+            for (int c = 0; c < 3; ++c) {
+                rows->interaction_positions[(k * w + j) * 3 + c] = on ? r.pos[j][c] : 0.0f;
+                rows->interaction_normals[(k * w + j) * 3 + c] = on ? r.nrm[j][c] : 0.0f;
+            }
+        }
+        rows->path_field[2 * k] = r.path_field.re;
+        rows->path_field[2 * k + 1] = r.path_field.im;
+        rows->coefficient[2 * k] = r.coefficient.re;
+        rows->coefficient[2 * k + 1] = r.coefficient.im;
+        float* fx = rows->field_xyz + 6 * k;
+        fx[0] = r.field.x.re; fx[1] = r.field.x.im;
+        fx[2] = r.field.y.re; fx[3] = r.field.y.im;
+        fx[4] = r.field.z.re; fx[5] = r.field.z.im;
+        for (int c = 0; c < 3; ++c) {
+            rows->arrival_direction[3 * k + c] = r.arr[c];
+            rows->departure_direction[3 * k + c] = r.dep[c];
+        }
+    }
+};
+
+__device__ __forceinline__ int tx_of_cell(const wc_solve& q, int cell) {
+    int lo = 0, hi = q.tx_count - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (q.cell_offset[mid] <= cell) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void write_cell(const wc_solve& q, const wc_radiomap_out& o, int cell,
+                                           const AccumSink& a) {
+    const int cells = q.total_cells;
+    double pt, fre, fim;
+    if (q.coherent) {
+        fre = a.s_re;
+        fim = a.s_im;
+        pt = fre * fre + fim * fim;
+    } else {
+        pt = a.p_inc;
+        fre = sqrt(fmax(pt, 0.0));
+        fim = 0.0;
+    }
+    o.power_total[cell] = (float)pt;
+    o.field_total[2 * cell] = (float)fre;
+    o.field_total[2 * cell + 1] = (float)fim;
+    for (int sl = 0; sl < 4; ++sl) {
+        const double cp = q.coherent ? a.c_re[sl] * a.c_re[sl] + a.c_im[sl] * a.c_im[sl] : a.c_pow[sl];
+        o.component_power[sl * cells + cell] = (float)cp;
+        o.component_field[2 * (sl * cells + cell)] = (float)a.c_re[sl];
+        o.component_field[2 * (sl * cells + cell) + 1] = (float)a.c_im[sl];
+    }
+    if (o.diffraction_vector) {
+        for (int j = 0; j < 6; ++j) o.diffraction_vector[6 * cell + j] = (float)a.dv[j];
+    }
+    if (o.path_count) o.path_count[cell] = a.count;
+    if (o.topology && a.count > o.topology_capacity && o.failure) atomicOr(o.failure, 2);
+}
+
+__device__ __forceinline__ void radiomap_cell(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o,
+                                              int t, int cell) {
+    AccumSink a;
+    a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    if (o.topology) {
+        a.topo = o.topology + (size_t)cell * o.topology_capacity;
+        a.cap = o.topology_capacity;
+    }
+    eval_cell(s, q, t, cell, a);
+    write_cell(q, o, cell, a);
+}
+
+__global__ void __launch_bounds__(256) radiomap_kernel(SceneView s, wc_solve q, wc_radiomap_out o) {
+    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
+        const int t = tx_of_cell(q, cell);
+        radiomap_cell(s, q, o, t, cell);
+    }
+}
+
+// Persistent variant: wavefronts pull wave-aligned tiles (tx, first cell) of
+// 64 cells from a global counter, so a wavefront never mixes transmitters and
+// uneven receiver sets balance across the machine.
+__global__ void __launch_bounds__(256) radiomap_tiles_kernel(SceneView s, wc_solve q, wc_radiomap_out o,
+                                                             const int* __restrict__ tiles, int tile_count,
+                                                             int* __restrict__ counter) {
+    const int lane = threadIdx.x & 63;
+    while (true) {
+        int tile = 0;
+        if (lane == 0) tile = atomicAdd(counter, 1);
+        tile = __shfl(tile, 0);
+        if (tile >= tile_count) break;
+        const int t = tiles[2 * tile];
+        const int cell = tiles[2 * tile + 1] + lane;
+        if (cell < q.cell_offset[t + 1]) radiomap_cell(s, q, o, t, cell);
+    }
+}
+
+__global__ void __launch_bounds__(256) count_kernel(SceneView s, wc_solve q, int* __restrict__ out) {
+    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
+        const int t = tx_of_cell(q, cell);
+        CountSink c;
+        eval_cell(s, q, t, cell, c);
+        out[cell] = c.count;
+    }
+}
+
+__global__ void __launch_bounds__(256) emit_kernel(SceneView s, wc_solve q, const int* __restrict__ row_offset,
+                                                   wc_path_rows rows) {
+    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
+        const int t = tx_of_cell(q, cell);
+        EmitSink e;
+        e.rows = &rows;
+        e.row = row_offset[cell];
+        e.tx = t;
+        e.rx = cell - q.cell_offset[t];
+        eval_cell(s, q, t, cell, e);
+    }
+}
+
+// Canonical order inside each pair segment: (depth, component, sequence
+// columns 0..w-1, primitive, edge), stable; then the first strictly-shortest
+// row of each identity group (tx, rx, per-slot event type and object id) is kept
+// (reference native/channel/kernels/evaluated_paths.cu:126-310, 654-700).
+__device__ __forceinline__ bool row_less(const wc_path_rows& r, int a, int b) {
+    if (r.depth[a] != r.depth[b]) return r.depth[a] < r.depth[b];
+    if (r.component_id[a] != r.component_id[b]) return r.component_id[a] < r.component_id[b];
+    for (int j = 0; j < r.width; ++j) {
+        const int x = r.primitive_sequence[a * r.width + j], y = r.primitive_sequence[b * r.width + j];
+        if (x != y) return x < y;
+    }
+    if (r.primitive_id[a] != r.primitive_id[b]) return r.primitive_id[a] < r.primitive_id[b];
+    if (r.edge_id[a] != r.edge_id[b]) return r.edge_id[a] < r.edge_id[b];
+    return false;
+}
+
+__device__ __forceinline__ int event_type(int component, int depth, int slot) {
+    if (component == 1 && slot < depth) return 1;
+    if (component == 2 && depth > 0 && slot == 0) return 2;
+    if (component == 5 && slot < depth) return 4;
+    if (component == 6 && depth > 0 && slot == 0) return 8;
+    return 0;
+}
+
+__device__ __forceinline__ bool same_identity(const wc_path_rows& r, int a, int b) {
+    for (int j = 0; j < r.width; ++j) {
+        const int ta = event_type(r.component_id[a], r.depth[a], j);
+        const int tb = event_type(r.component_id[b], r.depth[b], j);
+        if (ta != tb) return false;
+        if (ta == 0) continue;
+        const int oa = (r.component_id[a] == 2 && j == 0) ? r.edge_id[a] : r.primitive_sequence[a * r.width + j];
+        const int ob = (r.component_id[b] == 2 && j == 0) ? r.edge_id[b] : r.primitive_sequence[b * r.width + j];
+        if (oa != ob) return false;
+    }
+    return true;
+}
+
+__global__ void canonical_kernel(wc_path_rows rows, const int* __restrict__ pair_offset, int pair_count,
+                                 int* __restrict__ perm, unsigned char* __restrict__ keep) {
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < pair_count; p += blockDim.x * gridDim.x) {
+        const int begin = pair_offset[p], end = pair_offset[p + 1];
+        for (int i = begin; i < end; ++i) perm[i] = i;
+        for (int i = begin + 1; i < end; ++i) {
+            const int v = perm[i];
+            int j = i;
+            while (j > begin && row_less(rows, v, perm[j - 1])) {
+                perm[j] = perm[j - 1];
+                --j;
+            }
+            perm[j] = v;
+        }
+        int g = begin;
+        while (g < end) {
+            int h = g + 1;
+            while (h < end && same_identity(rows, perm[g], perm[h])) ++h;
+            float best = rows.path_length_m[perm[g]];
+            for (int i = g + 1; i < h; ++i) best = fminf(best, rows.path_length_m[perm[i]]);
+            bool taken = false;
+            for (int i = g; i < h; ++i) {
+                const bool win = !taken && rows.path_length_m[perm[i]] == best;
+                keep[i] = win ? 1 : 0;
+                taken = taken || win;
+            }
+            g = h;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ discovery
+__global__ void discover_kernel(SceneView s, const float* __restrict__ tx_pos, int tx_count, int ray_count,
+                                int max_depth, int* __restrict__ chains) {
+    const long long total = (long long)tx_count * ray_count;
+    for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+         idx += (long long)blockDim.x * gridDim.x) {
+        const int t = (int)(idx / ray_count);
+        const int i = (int)(idx - (long long)t * ray_count);
+        // Fibonacci sphere direction i of ray_count
+        const double golden = kPi * (3.0 - sqrt(5.0));
+        const double z = 1.0 - (2.0 * i + 1.0) / (double)ray_count;
+        const double rad = sqrt(fmax(0.0, 1.0 - z * z));
+        const double th = golden * (double)i;
+        V3<float> d{(float)(rad * cos(th)), (float)(rad * sin(th)), (float)z};
+        V3<float> o = ld3(tx_pos + 3 * t);
+        int* out = chains + idx * max_depth;
+        int ignore = -1;
+        int b = 0;
+        for (; b < max_depth; ++b) {
+            const RayPre r = ray_setup(o.x, o.y, o.z, d.x, d.y, d.z);
+            const Hit h = closest_hit(s, r, 1.0e-4f, 1.0e30f, ignore);
+            if (h.t < 0.0f) break;
+            out[b] = h.group;
+            double off;
+            const V3<float> n = to_f(group_normal(s, h.group, off));
+            o = o + d * h.t;
+            d = d - n * (2.0f * dot(d, n));
+            ignore = h.group;
+        }
+        for (; b < max_depth; ++b) out[b] = -1;
+    }
+}
+
+__global__ void chain_images_kernel(SceneView s, const float* __restrict__ tx_pos, const int* __restrict__ chain_tx,
+                                    const int* __restrict__ chain_groups, const int* __restrict__ chain_depth,
+                                    int chain_count, double* __restrict__ images) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < chain_count; c += blockDim.x * gridDim.x) {
+        const int t = chain_tx[c];
+        V3<double> p{(double)tx_pos[3 * t], (double)tx_pos[3 * t + 1], (double)tx_pos[3 * t + 2]};
+        const int d = chain_depth[c];
+        for (int j = 0; j < kMaxDepth; ++j) {
+            if (j < d) {
+                double off;
+                const V3<double> n = group_normal(s, chain_groups[c * kMaxDepth + j], off);
+                p = p - n * (2.0 * (dot(p, n) - off));
+            }
+            images[(c * kMaxDepth + j) * 3 + 0] = p.x;
+            images[(c * kMaxDepth + j) * 3 + 1] = p.y;
+            images[(c * kMaxDepth + j) * 3 + 2] = p.z;
+        }
+    }
+}
+
+__global__ void visibility_kernel(SceneView s, int count, const float* __restrict__ src, const float* __restrict__ dst,
+                                  unsigned char* __restrict__ visible) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += blockDim.x * gridDim.x) {
+        const V3<float> a = ld3(src + 3 * i), b = ld3(dst + 3 * i);
+        const V3<float> seg = b - a;
+        const float len = norm(seg);
+        bool vis = true;
+        if (len > 2.0f * kSegmentBias) {
+            const float eps = kSegmentBias / len;
+            const RayPre r = ray_setup(a.x, a.y, a.z, seg.x, seg.y, seg.z);
+            vis = !occluded(s, r, eps, 1.0f - eps, -1, -1);
+        }
+        visible[i] = vis ? 1 : 0;
+    }
+}
+
+__global__ void zero_int_kernel(int* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
+static int blocks_for(long long n, int threads) {
+    long long b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > 1048576) b = 1048576;
+    return (int)b;
+}
+
+static int check_scene(const wc_scene* scene) {
+    if (!scene) return wc_fail("scene is null");
+    if (scene->tri_count > 0 && (!scene->nodes || !scene->tris)) return wc_fail("scene BVH pointers are null");
+    if (scene->group_count > 0 && (!scene->group_plane || !scene->group_face_offset || !scene->group_faces))
+        return wc_fail("scene group tables are null");
+    if (scene->edge_count > 0 && !scene->edges) return wc_fail("scene edge table is null");
+    return 0;
+}
+
+static int check_solve(const wc_solve* q) {
+    if (!q) return wc_fail("solve is null");
+    if (q->tx_count <= 0 && q->total_cells > 0) return wc_fail("solve has cells but no transmitters");
+    if (q->max_depth < 0 || q->max_depth > kMaxDepth) return wc_fail("max_depth must be in [0, 5]");
+    if (!(q->frequency_hz > 0.0f)) return wc_fail("frequency_hz must be positive");
+    if (q->total_cells > 0 && (!q->tx_pos || !q->tx_pol || !q->tx_power || !q->rx_pos || !q->rx_pol ||
+                               !q->cell_offset))
+        return wc_fail("solve endpoint pointers are null");
+    if ((q->components & kCompRefl) && q->max_depth > 0 && (!q->chain_offset || !q->chain_groups || !q->chain_images))
+        return wc_fail("reflection requested without chain tables");
+    return 0;
+}
+
+}  // namespace wc
+
+using namespace wc;
+
+extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
+                                   const int* tiles, int tile_count, int* tile_counter, void* stream) {
+    if (int rc = check_scene(scene)) return rc;
+    if (int rc = check_solve(solve)) return rc;
+    if (!out || !out->power_total || !out->field_total || !out->component_power || !out->component_field)
+        return wc_fail("wc_radiomap_forward: output pointers are null");
+    if (out->topology && out->topology_capacity <= 0) return wc_fail("topology_capacity must be positive");
+    if (solve->total_cells == 0) return 0;
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    hipStream_t st = (hipStream_t)stream;
+    if (tiles) {
+        if (!tile_counter || tile_count <= 0) return wc_fail("tiled launch needs tile_counter and tile_count");
+        // a fill kernel rather than a memset node keeps the launch graph-capturable
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 1);
+        int device = 0;
+        (void)hipGetDevice(&device);
+        int cus = 256;
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        const int waves_total = tile_count;
+        int blocks = std::min((waves_total + 3) / 4, cus * 8);
+        blocks = std::max(blocks, 1);
+        hipLaunchKernelGGL(radiomap_tiles_kernel, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles, tile_count,
+                           tile_counter);
+    } else {
+        hipLaunchKernelGGL(radiomap_kernel, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0, st, s, *solve,
+                           *out);
+    }
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_forward", err);
+}
+
+extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream) {
+    if (int rc = check_scene(scene)) return rc;
+    if (int rc = check_solve(solve)) return rc;
+    if (!path_count && solve->total_cells > 0) return wc_fail("wc_paths_count: path_count is null");
+    if (solve->total_cells == 0) return 0;
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    hipLaunchKernelGGL(count_kernel, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0, (hipStream_t)stream, s,
+                       *solve, path_count);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_paths_count", err);
+}
+
+extern "C" int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const int* row_offset,
+                             const wc_path_rows* rows, void* stream) {
+    if (int rc = check_scene(scene)) return rc;
+    if (int rc = check_solve(solve)) return rc;
+    if (!rows || !row_offset) return wc_fail("wc_paths_emit: rows / row_offset are null");
+    if (rows->width < 1 || rows->width < solve->max_depth) return wc_fail("wc_paths_emit: width must be >= max_depth");
+    if (solve->total_cells == 0) return 0;
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    hipLaunchKernelGGL(emit_kernel, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0, (hipStream_t)stream, s,
+                       *solve, row_offset, *rows);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_paths_emit", err);
+}
+
+extern "C" int wc_paths_canonical_order(const wc_path_rows* rows, const int* pair_offset, int pair_count,
+                                        int* permutation, unsigned char* keep, void* stream) {
+    if (!rows || !pair_offset || !permutation || !keep) return wc_fail("wc_paths_canonical_order: null argument");
+    if (pair_count <= 0) return 0;
+    hipLaunchKernelGGL(canonical_kernel, dim3(blocks_for(pair_count, 256)), dim3(256), 0, (hipStream_t)stream, *rows,
+                       pair_offset, pair_count, permutation, keep);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_paths_canonical_order", err);
+}
+
+extern "C" int wc_visibility(const wc_scene* scene, int count, const float* source, const float* target,
+                             unsigned char* visible, void* stream) {
+    if (int rc = check_scene(scene)) return rc;
+    if (count < 0) return wc_fail("wc_visibility: count must be non-negative");
+    if (count == 0) return 0;
+    if (!source || !target || !visible) return wc_fail("wc_visibility: null argument");
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    hipLaunchKernelGGL(visibility_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, s, count,
+                       source, target, visible);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_visibility", err);
+}
+
+extern "C" int wc_discover_chains(const wc_scene* scene, const float* tx_pos, int tx_count, int ray_count,
+                                  int max_depth, int* chains, void* stream) {
+    if (int rc = check_scene(scene)) return rc;
+    if (tx_count < 0 || ray_count < 0) return wc_fail("wc_discover_chains: counts must be non-negative");
+    if (max_depth < 1 || max_depth > kMaxDepth) return wc_fail("wc_discover_chains: max_depth must be in [1, 5]");
+    const long long total = (long long)tx_count * ray_count;
+    if (total == 0) return 0;
+    if (!tx_pos || !chains) return wc_fail("wc_discover_chains: null argument");
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    hipLaunchKernelGGL(discover_kernel, dim3(blocks_for(total, 256)), dim3(256), 0, (hipStream_t)stream, s, tx_pos,
+                       tx_count, ray_count, max_depth, chains);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_discover_chains", err);
+}
+
+extern "C" int wc_chain_images(const wc_scene* scene, const float* tx_pos, const int* chain_tx,
+                               const int* chain_groups, const int* chain_depth, int chain_count, double* images,
+                               void* stream) {
+    if (int rc = check_scene(scene)) return rc;
+    if (chain_count < 0) return wc_fail("wc_chain_images: chain_count must be non-negative");
+    if (chain_count == 0) return 0;
+    if (!tx_pos || !chain_tx || !chain_groups || !chain_depth || !images) return wc_fail("wc_chain_images: null argument");
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    hipLaunchKernelGGL(chain_images_kernel, dim3(blocks_for(chain_count, 256)), dim3(256), 0, (hipStream_t)stream, s,
+                       tx_pos, chain_tx, chain_groups, chain_depth, chain_count, images);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_chain_images", err);
+}

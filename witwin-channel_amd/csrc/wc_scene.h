@@ -1,0 +1,80 @@
+// Device-resident scene layout shared by the BVH builder, the tracers and the
+// path kernels. All views are plain pointers into HBM owned by the caller
+// (torch tensors on the Python side); the library allocates only the BVH.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace wc {
+
+constexpr int kMaxDepth = 5;          // public reflection/transmission depth cap
+constexpr int kMaxHits = 8;           // ordered wall hits gathered on a straight segment
+constexpr int kStackSize = 64;        // BVH traversal stack (LBVH depth for <= 2^32 leaves with margin)
+
+// Triangle in BVH (Morton) order for intersection: three float4 rows,
+// v0.w = group id, v1.w = original face id, v2.w = material id (bit casts).
+struct Tri {
+    float4 v0, v1, v2;
+};
+
+// Binary LBVH node with both child boxes inline (one 64 B load tests two children).
+// child >= 0: internal node index; child < 0: leaf holding sorted triangle ~child.
+struct Node {
+    float4 a;  // c0.lo.x c0.lo.y c0.lo.z c0.hi.x
+    float4 b;  // c0.hi.y c0.hi.z c1.lo.x c1.lo.y
+    float4 c;  // c1.lo.z c1.hi.x c1.hi.y c1.hi.z
+    int4 d;    // child0 child1 - -
+};
+
+// Wedge (two faces) or half-plane (one face) diffraction edge. Angles are
+// measured in the plane normal to `dir`, from `t0` (into face 0) towards the
+// exterior side given by `n0`; the exterior spans [0, n*pi].
+struct Edge {
+    float ax, ay, az;      // start point
+    float dx, dy, dz;      // unit direction
+    float length;          // edge length (m)
+    float n;               // exterior wedge index n (2 for a half-plane)
+    float t0x, t0y, t0z;   // unit in-plane direction into face 0, perpendicular to dir
+    float n0x, n0y, n0z;   // face 0 normal, pointing into the exterior
+    int face0, face1;      // original face ids (face1 = -1 for a half-plane)
+    int group0, group1;    // coplanar groups of the faces (-1 when absent)
+    int material0, material1;
+};
+
+struct SceneView {
+    // BVH over triangles in sorted order
+    const Node* nodes;
+    const Tri* tris;
+    int tri_count;
+    int node_count;
+    // original face order
+    const Tri* faces;        // same packing as Tri, indexed by face id
+    const int* face_group;   // face -> coplanar group
+    const int* face_material;
+    int face_count;
+    // coplanar reflection groups
+    const double* group_plane;  // [G,4] unit normal xyz + offset (n.x = d), double precision
+    const int* group_face_offset;  // [G+1]
+    const int* group_faces;        // faces of each group, ascending
+    const int* group_material;     // material of the group's representative face
+    int group_count;
+    // materials (scalar first-layer view + CSR layer stacks)
+    const float* mat_eps;
+    const float* mat_sigma;
+    const float* mat_mu;
+    const float* mat_gain;
+    const float* mat_thickness;
+    const int* mat_layer_offset;
+    const int* mat_layer_count;
+    const float* layer_thickness;
+    const float* layer_eps;
+    const float* layer_sigma;
+    const float* layer_mu;
+    int material_count;
+    // diffraction edges
+    const Edge* edges;
+    int edge_count;
+};
+
+}  // namespace wc

@@ -1,0 +1,259 @@
+// BVH traversal for segment visibility, ordered wall-hit gathering and
+// closest-hit discovery rays. Triangle test: watertight ray/triangle
+// intersection (Woop, Benthin, Wald, JCGT 2(1), 2013) with the double-precision
+// fallback for zero edge functions, so a segment through a shared edge hits
+// exactly one of the two triangles and never slips between them.
+#pragma once
+
+#include "wc_scene.h"
+#include "wc_math.h"
+
+namespace wc {
+
+__device__ __forceinline__ int bits_i(float f) { return __float_as_int(f); }
+
+struct RayPre {
+    float ox, oy, oz;
+    float dx, dy, dz;
+    float ix, iy, iz;   // reciprocal direction for slab tests
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ float comp(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ __forceinline__ RayPre ray_setup(float ox, float oy, float oz, float dx, float dy, float dz) {
+    RayPre r;
+    r.ox = ox; r.oy = oy; r.oz = oz;
+    r.dx = dx; r.dy = dy; r.dz = dz;
+    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    int kz = 0;
+    if (ay > ax) kz = 1;
+    if (az > fmaxf(ax, ay)) kz = 2;
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    if (comp(dx, dy, dz, kz) < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    const float dkz = comp(dx, dy, dz, kz);
+    r.Sx = comp(dx, dy, dz, kx) / dkz;
+    r.Sy = comp(dx, dy, dz, ky) / dkz;
+    r.Sz = 1.0f / dkz;
+    return r;
+}
+
+// Watertight test; returns the hit parameter t in (tmin, tmax) or -1.
+__device__ __forceinline__ float tri_hit(const RayPre& r, const Tri& tri, float tmin, float tmax) {
+    const float Ax0 = tri.v0.x - r.ox, Ay0 = tri.v0.y - r.oy, Az0 = tri.v0.z - r.oz;
+    const float Bx0 = tri.v1.x - r.ox, By0 = tri.v1.y - r.oy, Bz0 = tri.v1.z - r.oz;
+    const float Cx0 = tri.v2.x - r.ox, Cy0 = tri.v2.y - r.oy, Cz0 = tri.v2.z - r.oz;
+    const float Akz = comp(Ax0, Ay0, Az0, r.kz), Bkz = comp(Bx0, By0, Bz0, r.kz), Ckz = comp(Cx0, Cy0, Cz0, r.kz);
+    const float Ax = comp(Ax0, Ay0, Az0, r.kx) - r.Sx * Akz;
+    const float Ay = comp(Ax0, Ay0, Az0, r.ky) - r.Sy * Akz;
+    const float Bx = comp(Bx0, By0, Bz0, r.kx) - r.Sx * Bkz;
+    const float By = comp(Bx0, By0, Bz0, r.ky) - r.Sy * Bkz;
+    const float Cx = comp(Cx0, Cy0, Cz0, r.kx) - r.Sx * Ckz;
+    const float Cy = comp(Cx0, Cy0, Cz0, r.ky) - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return -1.0f;
+    const float det = U + V + W;
+    if (det == 0.0f) return -1.0f;
+    const float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float t = T / det;
+    if (!(t > tmin && t < tmax)) return -1.0f;
+    return t;
+}
+
+__device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
+                                     float tmin, float tmax, float& tnear) {
+    const float t0x = (lx - r.ox) * r.ix, t1x = (hx - r.ox) * r.ix;
+    const float t0y = (ly - r.oy) * r.iy, t1y = (hy - r.oy) * r.iy;
+    const float t0z = (lz - r.oz) * r.iz, t1z = (hz - r.oz) * r.iz;
+    const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    tnear = tn;
+    return tn <= tf;
+}
+
+// Any hit on the segment o + t*d, t in (tmin, tmax), skipping triangles of
+// groups ig0/ig1 (the surfaces the segment starts or ends on).
+__device__ inline bool occluded(const SceneView& s, const RayPre& r, float tmin, float tmax, int ig0, int ig1) {
+    if (s.tri_count == 0) return false;
+    int stack[kStackSize];
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const Node nd = s.nodes[node];
+        float tn0, tn1;
+        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, tmax, tn0);
+        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, tmax, tn1);
+        int next = -1;
+        int c0 = nd.d.x, c1 = nd.d.y;
+        if (h0) {
+            if (c0 < 0) {
+                const Tri tri = s.tris[~c0];
+                const int g = bits_i(tri.v0.w);
+                if (g != ig0 && g != ig1 && tri_hit(r, tri, tmin, tmax) >= 0.0f) return true;
+            } else {
+                next = c0;
+            }
+        }
+        if (h1) {
+            if (c1 < 0) {
+                const Tri tri = s.tris[~c1];
+                const int g = bits_i(tri.v0.w);
+                if (g != ig0 && g != ig1 && tri_hit(r, tri, tmin, tmax) >= 0.0f) return true;
+            } else {
+                if (next >= 0) {
+                    if (sp < kStackSize) stack[sp++] = c1;
+                } else {
+                    next = c1;
+                }
+            }
+        }
+        if (next < 0) {
+            if (sp == 0) break;
+            next = stack[--sp];
+        }
+        node = next;
+    }
+    return false;
+}
+
+struct Hit {
+    float t;
+    int face;
+    int group;
+};
+
+// Closest hit in (tmin, tmax), skipping group `ignore`; t < 0 when nothing is hit.
+__device__ inline Hit closest_hit(const SceneView& s, const RayPre& r, float tmin, float tmax, int ignore) {
+    Hit best{-1.0f, -1, -1};
+    if (s.tri_count == 0) return best;
+    int stack[kStackSize];
+    int sp = 0;
+    int node = 0;
+    float tbest = tmax;
+    while (true) {
+        const Node nd = s.nodes[node];
+        float tn0, tn1;
+        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, tbest, tn0);
+        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, tbest, tn1);
+        int c0 = nd.d.x, c1 = nd.d.y;
+        int near_child = -1, far_child = -1;
+        if (h0) {
+            if (c0 < 0) {
+                const Tri tri = s.tris[~c0];
+                const int g = bits_i(tri.v0.w);
+                if (g != ignore) {
+                    const float t = tri_hit(r, tri, tmin, tbest);
+                    if (t >= 0.0f) { tbest = t; best = {t, bits_i(tri.v1.w), g}; }
+                }
+            } else {
+                near_child = c0;
+            }
+        }
+        if (h1) {
+            if (c1 < 0) {
+                const Tri tri = s.tris[~c1];
+                const int g = bits_i(tri.v0.w);
+                if (g != ignore) {
+                    const float t = tri_hit(r, tri, tmin, tbest);
+                    if (t >= 0.0f) { tbest = t; best = {t, bits_i(tri.v1.w), g}; }
+                }
+            } else {
+                if (near_child < 0) {
+                    near_child = c1;
+                } else {
+                    // visit the nearer box first
+                    if (tn1 < tn0) { far_child = near_child; near_child = c1; } else { far_child = c1; }
+                }
+            }
+        }
+        if (far_child >= 0 && sp < kStackSize) stack[sp++] = far_child;
+        if (near_child < 0) {
+            if (sp == 0) break;
+            near_child = stack[--sp];
+        }
+        node = near_child;
+    }
+    return best;
+}
+
+// All hits on the segment in (tmin, tmax), ordered by t, at most one per
+// coplanar group (a segment crossing a wall through the shared diagonal of
+// its two triangles counts once). Returns the number of distinct hits, or
+// cap + 1 when there are more than `cap`.
+__device__ inline int gather_hits(const SceneView& s, const RayPre& r, float tmin, float tmax, int cap,
+                                  Hit* hits) {
+    int count = 0;
+    if (s.tri_count == 0) return 0;
+    int stack[kStackSize];
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        const Node nd = s.nodes[node];
+        float tn0, tn1;
+        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, tmax, tn0);
+        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, tmax, tn1);
+        int next = -1;
+        const int cs[2] = {nd.d.x, nd.d.y};
+        const bool hs[2] = {h0, h1};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!hs[k]) continue;
+            const int c = cs[k];
+            if (c < 0) {
+                const Tri tri = s.tris[~c];
+                const float t = tri_hit(r, tri, tmin, tmax);
+                if (t >= 0.0f) {
+                    const int g = bits_i(tri.v0.w);
+                    const int f = bits_i(tri.v1.w);
+                    bool dup = false;
+                    for (int i = 0; i < count && i < cap; ++i) {
+                        if (hits[i].group == g) {
+                            dup = true;
+                            if (f < hits[i].face) { hits[i].face = f; hits[i].t = t; }
+                        }
+                    }
+                    if (!dup) {
+                        if (count < cap) {
+                            // insertion by t
+                            int j = count;
+                            while (j > 0 && hits[j - 1].t > t) { hits[j] = hits[j - 1]; --j; }
+                            hits[j] = {t, f, g};
+                        }
+                        ++count;
+                        if (count > cap) return cap + 1;
+                    }
+                }
+            } else {
+                if (next < 0) next = c;
+                else if (sp < kStackSize) stack[sp++] = c;
+            }
+        }
+        if (next < 0) {
+            if (sp == 0) break;
+            next = stack[--sp];
+        }
+        node = next;
+    }
+    // keep ascending order after face-id replacements
+    for (int i = 1; i < count; ++i) {
+        Hit h = hits[i];
+        int j = i;
+        while (j > 0 && hits[j - 1].t > h.t) { hits[j] = hits[j - 1]; --j; }
+        hits[j] = h;
+    }
+    return count;
+}
+
+}  // namespace wc

@@ -1,0 +1,182 @@
+"""ctypes binding of libwcmi.so, the gfx950 HIP library behind the hot path.
+
+The library exports a plain C ABI (declared in include/wcmi.h at the repository
+root). This module mirrors its structs and wraps each entry point so a non-zero
+status raises ``RuntimeError`` with ``wc_last_error()``. There is no fallback:
+if the library is missing or fails to load, every solver raises.
+(Reference loader contract: witwin/channel/runtime.py:552-579 — missing native
+symbols raise instead of falling back.)
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+from typing import Any
+
+import torch
+
+_LIB_PATH = Path(__file__).resolve().parent / "libwcmi.so"
+_lib: ctypes.CDLL | None = None
+
+c_int = ctypes.c_int
+c_float = ctypes.c_float
+c_size_t = ctypes.c_size_t
+c_void_p = ctypes.c_void_p
+c_char_p = ctypes.c_char_p
+c_double_p = ctypes.POINTER(ctypes.c_double)
+
+
+class NativeLibraryError(RuntimeError):
+    """libwcmi.so is missing or does not export a required symbol."""
+
+
+class WcEdge(ctypes.Structure):
+    _fields_ = [(name, ctypes.c_float) for name in (
+        "ax", "ay", "az", "dx", "dy", "dz", "length", "n", "t0x", "t0y", "t0z", "n0x", "n0y", "n0z")] + [
+        (name, ctypes.c_int) for name in ("face0", "face1", "group0", "group1", "material0", "material1")]
+
+
+class WcScene(ctypes.Structure):
+    _fields_ = [
+        ("nodes", c_void_p), ("tris", c_void_p), ("tri_count", c_int), ("node_count", c_int),
+        ("faces", c_void_p), ("face_group", c_void_p), ("face_material", c_void_p), ("face_count", c_int),
+        ("group_plane", c_void_p), ("group_face_offset", c_void_p), ("group_faces", c_void_p),
+        ("group_material", c_void_p), ("group_count", c_int),
+        ("mat_eps", c_void_p), ("mat_sigma", c_void_p), ("mat_mu", c_void_p), ("mat_gain", c_void_p),
+        ("mat_thickness", c_void_p), ("mat_layer_offset", c_void_p), ("mat_layer_count", c_void_p),
+        ("layer_thickness", c_void_p), ("layer_eps", c_void_p), ("layer_sigma", c_void_p),
+        ("layer_mu", c_void_p), ("material_count", c_int),
+        ("edges", c_void_p), ("edge_count", c_int),
+    ]
+
+
+class WcSolve(ctypes.Structure):
+    _fields_ = [
+        ("tx_pos", c_void_p), ("tx_pol", c_void_p), ("tx_power", c_void_p), ("tx_count", c_int),
+        ("rx_pos", c_void_p), ("rx_pol", c_void_p), ("rx_pol_stride", c_int), ("cell_offset", c_void_p),
+        ("rx_shared", c_int), ("total_cells", c_int),
+        ("chain_offset", c_void_p), ("chain_groups", c_void_p), ("chain_images", c_void_p),
+        ("frequency_hz", c_float), ("components", c_int), ("max_depth", c_int), ("coherent", c_int),
+    ]
+
+
+class WcRadiomapOut(ctypes.Structure):
+    _fields_ = [
+        ("power_total", c_void_p), ("field_total", c_void_p), ("component_power", c_void_p),
+        ("component_field", c_void_p), ("diffraction_vector", c_void_p), ("path_count", c_void_p),
+        ("topology", c_void_p), ("topology_capacity", c_int), ("failure", c_void_p),
+    ]
+
+
+class WcPathRows(ctypes.Structure):
+    _fields_ = [("width", c_int)] + [(name, c_void_p) for name in (
+        "tx_id", "rx_id", "depth", "component_id", "primitive_id", "edge_id", "material_id", "path_length_m",
+        "delay_s", "path_gain", "primitive_sequence", "material_sequence", "interaction_positions",
+        "interaction_normals", "path_field", "coefficient", "field_xyz", "arrival_direction",
+        "departure_direction")]
+
+
+class WcAdjoint(ctypes.Structure):
+    _fields_ = [
+        ("grad_power_total", c_void_p), ("grad_field_total", c_void_p), ("grad_component_power", c_void_p),
+        ("grad_component_field", c_void_p), ("grad_diffraction_power", c_void_p), ("field_total", c_void_p),
+        ("component_field", c_void_p), ("diffraction_vector", c_void_p), ("topology", c_void_p),
+        ("path_count", c_void_p), ("topology_capacity", c_int),
+        ("grad_eps", c_void_p), ("grad_sigma", c_void_p), ("grad_frequency", c_void_p),
+        ("grad_tx_pos", c_void_p), ("partials", c_void_p), ("partial_count", c_int),
+    ]
+
+
+_SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
+    "wc_last_error": (c_char_p, []),
+    "wc_clear_error": (None, []),
+    "wc_abi_version": (c_int, []),
+    "wc_build_target": (c_char_p, []),
+    "wc_bvh_workspace_bytes": (c_size_t, [c_int]),
+    "wc_bvh_node_count": (c_int, [c_int]),
+    "wc_bvh_build": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
+                             c_void_p, c_size_t, c_void_p]),
+    "wc_radiomap_forward": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
+                                    ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_void_p]),
+    "wc_paths_count": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p]),
+    "wc_paths_emit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p,
+                              ctypes.POINTER(WcPathRows), c_void_p]),
+    "wc_paths_canonical_order": (c_int, [ctypes.POINTER(WcPathRows), c_void_p, c_int, c_void_p, c_void_p,
+                                         c_void_p]),
+    "wc_field_free_space": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+                                    c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                    c_void_p]),
+    "wc_field_project_complex3": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "wc_field_reflection_sequence": (c_int, [c_int, c_int] + [c_void_p] * 12 + [c_float] + [c_void_p] * 8),
+    "wc_accumulate_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "wc_accumulate_flat": (c_int, [c_int] + [c_void_p] * 7 + [c_int, c_int, c_int] + [c_void_p] * 5
+                           + [c_size_t, c_void_p]),
+    "wc_visibility": (c_int, [ctypes.POINTER(WcScene), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "wc_discover_chains": (c_int, [ctypes.POINTER(WcScene), c_void_p, c_int, c_int, c_int, c_void_p,
+                                   c_void_p]),
+    "wc_chain_images": (c_int, [ctypes.POINTER(WcScene), c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                                c_void_p, c_void_p]),
+}
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Load libwcmi.so once; raise NativeLibraryError when it is absent."""
+
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("WCMI_LIBRARY", str(_LIB_PATH)))
+    if not path.exists():
+        raise NativeLibraryError(
+            f"libwcmi.so not found at {path}; build it with "
+            "`python witwin-channel_amd/build_native.py` (there is no CPU fallback)"
+        )
+    lib = ctypes.CDLL(str(path))
+    for name, (restype, argtypes) in _SIGNATURES.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            raise NativeLibraryError(f"libwcmi.so does not export {name}")
+        fn.restype = restype
+        fn.argtypes = argtypes
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    return sorted(_SIGNATURES)
+
+
+def call(name: str, *args: Any) -> Any:
+    lib = load()
+    fn = getattr(lib, name)
+    status = fn(*args)
+    if fn.restype is c_int and name not in ("wc_abi_version", "wc_bvh_node_count", "wc_vjp_partial_count") \
+            and status != 0:
+        message = lib.wc_last_error().decode("utf-8", "replace")
+        raise RuntimeError(f"{name} failed: {message}")
+    return status
+
+
+def ptr(tensor: torch.Tensor | None) -> int | None:
+    if tensor is None:
+        return None
+    if not tensor.is_contiguous():
+        raise ValueError("native calls require contiguous tensors")
+    return tensor.data_ptr() if tensor.numel() else None
+
+
+def stream_handle(device: torch.device | None = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise RuntimeError("witwin.channel requires a ROCm GPU (torch.cuda.is_available() is False)")
+    load()
+    return torch.device("cuda", torch.cuda.current_device())

@@ -1,0 +1,340 @@
+"""Deterministic RF solver: ``Config``, ``PathTable``, ``Result``, ``solve``.
+
+Drop-in surface of witwin/channel/deterministic.py:105-971. A solve compiles
+the scene (cached per scene object), plans reflection chains per transmitter
+and runs the fused gfx950 radiomap kernel, which discovers every path of every
+(transmitter, cell) pair, evaluates its field and sums the cell coherently in
+float64 registers — no per-path rows are materialized unless
+``export_paths=True``. ``ad_mode="vjp"`` attaches the fixed-topology adjoint
+(:mod:`witwin.channel.kernels.deterministic`).
+
+Result layout follows the reference: ``path_gain``/``field`` are
+``[T, cols, rows]`` for a single ``ReceiverGrid`` (the flat receiver index is
+``row * cols + col``, reference propagation/geometry.py:72-89) and ``[T, R]``
+for point receivers; ``component_power`` holds los/reflection/diffraction plus
+transmission when requested; for a single grid the diffraction power is the
+vector-coherent sum over edges (reference deterministic.py:870-882).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+import torch
+
+from witwin.channel import _native
+from witwin.channel.components import (
+    AD_MODES as _VALID_AD_MODES,
+    DEFAULT_COMPONENTS as _DEFAULT_COMPONENTS,
+    DEPTH_CAPPED_COMPONENTS,
+    component_bits,
+    component_max_depth,
+    validate_max_depth,
+    validated_components,
+)
+from witwin.channel.constants import PHASE_CONVENTION
+from witwin.channel.propagation import engine
+from witwin.channel.scene.compiler import CompiledScene, compile as compile_scene, device_scene
+
+_VALID_SORT_KEYS = frozenset({"receiver_transmitter_depth_component"})
+_VALID_MAX_PATHS_SCOPES = frozenset({"global", "per_pair"})
+_BASE_COMPONENTS = ("los", "reflection", "diffraction")
+_SLOTS = {"los": 0, "reflection": 1, "diffraction": 2, "transmission": 3}
+
+
+@dataclass(frozen=True, slots=True)
+class Config:
+    max_depth: int = 1
+    max_diffraction_order: int = 1
+    components: frozenset[str] | set[str] | tuple[str, ...] | list[str] = _DEFAULT_COMPONENTS
+    coherent: bool = True
+    return_field: bool = True
+    export_paths: bool = False
+    max_paths: int | None = None
+    max_paths_scope: str = "global"
+    sort_key: str = "receiver_transmitter_depth_component"
+    diagnostics: bool = False
+    ad_mode: str = "none"
+    coupled_paths: bool = False
+    coupled_candidate_limit: int = 1_000_000
+    scattering_samples_per_m2: float = 8.0
+    scattering_max_paths_per_pair: int = 4096
+    scattering_power_threshold: float = 0.0
+    scattering_coherent: bool = False
+    scattering_chain_max_depth: int = 0
+    scattering_chain_samples_per_m2: float = 2.0
+    scattering_chain_max_rows: int = 256
+    isb_boundary_taper: bool = False
+    isb_boundary_taper_width: float = 0.5
+
+    def __post_init__(self) -> None:
+        validate_max_depth(self.max_depth)
+        if not (0.0 < self.isb_boundary_taper_width <= 4.0):
+            raise ValueError("isb_boundary_taper_width must be in (0, 4]")
+        if self.max_diffraction_order < 0:
+            raise ValueError("max_diffraction_order must be 0 or 1")
+        if self.max_diffraction_order > 1:
+            raise RuntimeError("max_diffraction_order above 1 is not supported yet")
+        components = validated_components(
+            self.components, error_message="components must be a non-empty subset of {valid}")
+        if self.scattering_coherent and "scattering" not in components:
+            raise RuntimeError("scattering_coherent=True requires the 'scattering' component")
+        if self.max_depth > 5 and components & DEPTH_CAPPED_COMPONENTS:
+            raise RuntimeError("deterministic reflection/transmission currently support max_depth <= 5")
+        if self.coupled_paths:
+            if self.max_depth < 2:
+                raise RuntimeError("coupled reflection-diffraction paths require max_depth >= 2")
+            if not {"reflection", "diffraction"}.issubset(components):
+                raise RuntimeError("coupled paths require both reflection and diffraction components")
+        if self.coupled_candidate_limit <= 0:
+            raise ValueError("coupled_candidate_limit must be positive")
+        if self.max_paths is not None and self.max_paths <= 0:
+            raise ValueError("max_paths must be positive when set")
+        if self.max_paths_scope not in _VALID_MAX_PATHS_SCOPES:
+            raise ValueError("max_paths_scope must be 'global' or 'per_pair'")
+        if self.sort_key not in _VALID_SORT_KEYS:
+            raise ValueError(f"sort_key must be one of {sorted(_VALID_SORT_KEYS)}")
+        if self.ad_mode not in _VALID_AD_MODES:
+            raise RuntimeError(f"deterministic ad_mode must be one of {sorted(_VALID_AD_MODES)}")
+        object.__setattr__(self, "components", components)
+
+
+@dataclass(frozen=True, slots=True)
+class PathTable:
+    valid: torch.Tensor
+    tx_id: torch.Tensor
+    rx_id: torch.Tensor
+    depth: torch.Tensor
+    component_id: torch.Tensor
+    primitive_id: torch.Tensor
+    edge_id: torch.Tensor
+    path_length_m: torch.Tensor
+    delay_s: torch.Tensor
+    path_gain: torch.Tensor
+    interaction_position: torch.Tensor
+    interaction_normal: torch.Tensor
+    material_id: torch.Tensor
+    primitive_sequence: torch.Tensor
+    material_sequence: torch.Tensor
+    interaction_positions: torch.Tensor
+    interaction_normals: torch.Tensor
+    field_real: torch.Tensor
+    field_imag: torch.Tensor
+    coefficient: torch.Tensor
+    field_xyz: torch.Tensor
+    field_direction: torch.Tensor
+    phase_rad: torch.Tensor
+    interaction_count: torch.Tensor
+
+
+@dataclass(frozen=True, slots=True)
+class Result:
+    path_gain: torch.Tensor
+    field: torch.Tensor
+    component_power: dict[str, torch.Tensor]
+    component_fields: dict[str, torch.Tensor]
+    paths: PathTable | None
+    metadata: dict[str, Any]
+    diagnostics: dict[str, Any] | None = None
+
+
+def apply_layout(values: torch.Tensor, compiled: CompiledScene) -> torch.Tensor:
+    ep = compiled.endpoints
+    if ep.layout_kind == "grid":
+        rows, cols = ep.grid_shape  # type: ignore[misc]
+        return values.reshape(values.shape[0], rows, cols).transpose(1, 2).contiguous()
+    return values.contiguous()
+
+
+def phase_from_field(field: torch.Tensor) -> torch.Tensor:
+    """(-arg F) mod 2 pi (reference deterministic.cu:390-406)."""
+    phase = -torch.atan2(field.imag, field.real)
+    phase = torch.remainder(phase, 2.0 * np.pi)
+    return phase.to(torch.float32)
+
+
+def select_max_paths(rows: dict[str, Any], max_paths: int | None, scope: str, num_tx: int) -> torch.Tensor:
+    """Row mask keeping the first ``max_paths`` rows globally or per pair, in
+    canonical order (reference evaluated_paths.cu:355-384)."""
+    count = int(rows["tx_id"].shape[0])
+    device = rows["tx_id"].device
+    if max_paths is None:
+        return torch.ones(count, dtype=torch.bool, device=device)
+    if scope == "global":
+        return torch.arange(count, device=device) < max_paths
+    pair = rows["rx_id"].long() * num_tx + rows["tx_id"].long()
+    _, inverse, counts = torch.unique_consecutive(pair, return_inverse=True, return_counts=True)
+    starts = torch.cumsum(counts, 0) - counts
+    rank = torch.arange(count, device=device) - starts[inverse]
+    return rank < max_paths
+
+
+def build_path_table(rows: dict[str, Any], *, include_fields: bool = True) -> PathTable:
+    path_field = rows["path_field"] if include_fields else torch.zeros_like(rows["path_field"])
+    count = int(rows["tx_id"].shape[0])
+    device = rows["tx_id"].device
+    width = rows["primitive_sequence"].shape[1] if count else 1
+    return PathTable(
+        valid=torch.ones(count, dtype=torch.bool, device=device),
+        tx_id=rows["tx_id"], rx_id=rows["rx_id"], depth=rows["depth"], component_id=rows["component_id"],
+        primitive_id=rows["primitive_id"], edge_id=rows["edge_id"], path_length_m=rows["path_length_m"],
+        delay_s=rows["delay_s"], path_gain=rows["path_gain"],
+        interaction_position=rows["interaction_positions"][:, 0, :].contiguous() if count else
+        torch.zeros((0, 3), device=device),
+        interaction_normal=rows["interaction_normals"][:, 0, :].contiguous() if count else
+        torch.zeros((0, 3), device=device),
+        material_id=rows["material_id"], primitive_sequence=rows["primitive_sequence"].reshape(count, width),
+        material_sequence=rows["material_sequence"].reshape(count, width),
+        interaction_positions=rows["interaction_positions"].reshape(count, width, 3),
+        interaction_normals=rows["interaction_normals"].reshape(count, width, 3),
+        field_real=path_field.real.contiguous(), field_imag=path_field.imag.contiguous(),
+        coefficient=rows["coefficient"], field_xyz=rows["field_xyz"], field_direction=rows["arrival_direction"],
+        phase_rad=phase_from_field(path_field), interaction_count=rows["depth"],
+    )
+
+
+def _validate(config: Config) -> None:
+    unsupported = set(config.components) - {"los", "reflection", "diffraction", "transmission"}
+    if unsupported:
+        raise NotImplementedError(f"components {sorted(unsupported)} are outside this build's hot path")
+    if config.coupled_paths:
+        raise NotImplementedError("coupled reflection-diffraction paths are outside this build's hot path")
+    if config.isb_boundary_taper:
+        raise NotImplementedError("isb_boundary_taper is outside this build's hot path")
+    if "reflection" in config.components and config.max_depth < 1:
+        raise RuntimeError("deterministic reflection requires max_depth >= 1")
+    if "diffraction" in config.components:
+        if config.max_depth < 1:
+            raise RuntimeError("deterministic diffraction requires max_depth >= 1")
+        if config.max_diffraction_order < 1:
+            raise RuntimeError("deterministic diffraction requires max_diffraction_order >= 1")
+
+
+def _metadata(config: Config, path_count: int, counts: dict[str, int], launches: int) -> dict[str, Any]:
+    status = {name: ("enabled" if name in config.components else "not_requested")
+              for name in ("los", "reflection", "diffraction", "transmission", "scattering")}
+    if "transmission" in config.components and counts.get("transmission", 0) == 0:
+        status["transmission"] = "enabled_no_paths"
+    meta: dict[str, Any] = {
+        "max_depth": config.max_depth,
+        "max_diffraction_order": config.max_diffraction_order,
+        "coherent": config.coherent,
+        "return_field": config.return_field,
+        "export_paths": config.export_paths,
+        "max_paths": config.max_paths,
+        "max_paths_scope": config.max_paths_scope,
+        "sort_key": config.sort_key,
+        "accumulation_strategy": "coherent" if config.coherent else "incoherent",
+        "components": status,
+        "counts": {"path_count": path_count, "valid_path_count": path_count, "components": counts},
+        "capability": {"rayd_native": True, "path_native": True, "cuda_available": True,
+                       "optix_available": False},
+        "kernel": {"primitive": "deterministic_solver", "forward_launch_count": launches,
+                   "accumulation_strategy": "per_cell_float64_registers", "scheduling_strategy": "fused_hip",
+                   "ad_status": config.ad_mode},
+        "field_abi": "complex3_v1",
+        "phase_convention": dict(PHASE_CONVENTION),
+        "coefficient_semantics": "unit_excitation_dimensionless_receiver_projection",
+        "component_max_depth": component_max_depth(config.components, chain_depth=config.max_depth,
+                                                   single_bounce_depth=1),
+    }
+    if "transmission" in config.components:
+        meta["transmission"] = {"thin_sheet_straight_path_approximation": True, "group_delay": "geometric"}
+    return meta
+
+
+def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: torch.Tensor | None = None,
+                    material_params: Any = None) -> Result:
+    device = _native.require_gpu()
+    _validate(config)
+    ep = compiled.endpoints
+    if len(ep.rx_position) == 0 or len(ep.tx_position) == 0:
+        raise RuntimeError("deterministic solve requires at least one transmitter and one receiver")
+    dscene = device_scene(compiled, device)
+    bits = component_bits(config.components)
+    depth = config.max_depth if bits & (2 | 4 | 8) else 0
+    num_tx = int(ep.tx_position.shape[0])
+    num_rx = int(ep.rx_position.shape[0])
+    inputs = engine.solve_inputs(compiled, dscene, components=bits, max_depth=depth, coherent=config.coherent,
+                                 frequency_hz=compiled.frequency_hz, tx_pos=tx_position)
+    if config.ad_mode == "vjp":
+        from witwin.channel.kernels.deterministic import radiomap_with_vjp
+
+        out, tensors = radiomap_with_vjp(compiled, dscene, inputs, material_params=material_params,
+                                         tx_position=tx_position)
+        power_total, field_total, comp_power, diff_power = tensors
+    else:
+        out = engine.radiomap_forward(dscene, inputs)
+        power_total, field_total, comp_power = out.power_total, out.field_total, out.component_power
+        diff_power = out.diffraction_vector.abs().square().sum(dim=-1)
+    power_total = power_total.reshape(num_tx, num_rx)
+    field_total = field_total.reshape(num_tx, num_rx)
+    names = list(_BASE_COMPONENTS) + (["transmission"] if "transmission" in config.components else [])
+    component_power = {name: comp_power[_SLOTS[name]].reshape(num_tx, num_rx) for name in names}
+    component_fields = {name: out.component_field[_SLOTS[name]].reshape(num_tx, num_rx) for name in names}
+    if ep.layout_kind == "grid" and "diffraction" in config.components:
+        exact = diff_power.reshape(num_tx, num_rx)
+        if not config.coherent:
+            power_total = power_total - component_power["diffraction"] + exact
+        component_power["diffraction"] = exact
+    counts_t = out.path_count.reshape(num_tx, num_rx)
+    path_count = int(counts_t.sum().item())
+    paths = None
+    counts = {"los": 0, "reflection": 0, "diffraction": 0}
+    if config.export_paths:
+        rows = engine.export_rows(dscene, inputs, width=max(depth, 1))
+        keep = select_max_paths(rows, config.max_paths, config.max_paths_scope, num_tx)
+        rows = {k: v[keep] if isinstance(v, torch.Tensor) else v for k, v in rows.items()}
+        paths = build_path_table(rows, include_fields=config.return_field)
+        path_count = int(paths.tx_id.shape[0])
+        cid = paths.component_id
+        for name, c in (("los", 0), ("reflection", 1), ("diffraction", 2), ("transmission", 5)):
+            counts[name] = int((cid == c).sum().item())
+    metadata = _metadata(config, path_count, counts, launches=2 if config.export_paths else 1)
+    path_gain = apply_layout(power_total, compiled)
+    field = apply_layout(field_total, compiled) if config.return_field else \
+        torch.empty((0,), device=device, dtype=torch.complex64)
+    comp_p = {k: apply_layout(v, compiled) for k, v in component_power.items()}
+    comp_f = {k: (apply_layout(v, compiled) if config.return_field else
+                  torch.empty((0,), device=device, dtype=torch.complex64)) for k, v in component_fields.items()}
+    diagnostics = None
+    if config.diagnostics:
+        diagnostics = {
+            "path_gain_shape": tuple(path_gain.shape), "field_shape": tuple(field.shape),
+            "path_count": path_count, "component_counts": counts, "coherent": config.coherent,
+            "accumulation_mode": "coherent" if config.coherent else "incoherent",
+            "native_launch_count": metadata["kernel"]["forward_launch_count"],
+            "diffraction_accumulation": ("exact_vector_coherent_paths" if ep.layout_kind == "grid"
+                                         else "scalar_path_accumulation"),
+            "path_planning": {"max_paths": config.max_paths, "max_paths_scope": config.max_paths_scope,
+                              "candidate_count": inputs.chains.count, "truncated": False},
+        }
+    if bool((out.failure != 0).any().item()):
+        raise RuntimeError("deterministic solve exceeded a native capacity (failure bits "
+                           f"{int(out.failure.item())})")
+    return Result(path_gain=path_gain, field=field, component_power=comp_p, component_fields=comp_f, paths=paths,
+                  metadata=metadata, diagnostics=diagnostics)
+
+
+_COMPILE_CACHE: dict[tuple[int, float], CompiledScene] = {}
+
+
+def compile_cached(scene: Any, reference_frequency_hz: Any) -> CompiledScene:
+    freq = float(reference_frequency_hz.detach()) if isinstance(reference_frequency_hz, torch.Tensor) \
+        else float(reference_frequency_hz)
+    key = (id(scene), freq)
+    cached = _COMPILE_CACHE.get(key)
+    if cached is None or cached.cache.get("scene_ref") is not scene:
+        cached = compile_scene(scene, reference_frequency_hz=freq)
+        cached.cache["scene_ref"] = scene
+        _COMPILE_CACHE[key] = cached
+    return cached
+
+
+def solve(scene, config: Config, *, reference_frequency_hz) -> Result:  # type: ignore[no-untyped-def]
+    """Run the deterministic propagation pipeline."""
+
+    compiled = compile_scene(scene, reference_frequency_hz=reference_frequency_hz)
+    return _solve_pipeline(compiled, config)

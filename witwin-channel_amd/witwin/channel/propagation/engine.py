@@ -1,0 +1,339 @@
+"""Host orchestration of the MI355X propagation engine.
+
+This is the counterpart of the reference stage driver
+``propagation.enumerated.evaluate_enumerated_paths``
+(witwin/channel/propagation/enumerated.py:549-837). The reference runs one
+Python loop per transmitter x receiver chunk x depth, materializes every
+candidate row, compacts, sorts and only then evaluates fields. Here the
+whole (transmitter, cell) product is one fused kernel launch:
+
+1. candidate chains per transmitter (exhaustive face-group sequences when
+   their count is small, transmitter-launched discovery rays otherwise; the
+   thresholds are the reference's: reflection.py:30-34, 120, 186-205),
+2. double-precision image points of every chain (one small kernel),
+3. ``wc_radiomap_forward`` (fused discovery + field + float64 coherent sum per
+   cell), or ``wc_paths_count`` / ``wc_paths_emit`` /
+   ``wc_paths_canonical_order`` when explicit path rows are requested.
+
+Host<->device synchronization happens once per row export (the row count),
+like the reference's single count read (docs/dev/deterministic/README.md:30-36).
+"""
+
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass
+from typing import Any
+
+import numpy as np
+import torch
+
+from witwin.channel import _native
+from witwin.channel.scene.compiler import CompiledScene, DeviceScene
+
+MAX_DEPTH = 5
+EXHAUSTIVE_GROUP_LIMIT = 4096        # depth-1 exhaustive when groups <= this
+EXHAUSTIVE_SEQUENCE_LIMIT = 100_000  # depth >= 2 exhaustive when G (G-1)^(d-1) <= this
+DISCOVERY_RAY_COUNT = 262_144
+TILE = 64
+
+
+@dataclass
+class ChainPlan:
+    offset: torch.Tensor        # [T*5+1] int32
+    groups: torch.Tensor        # [K,5] int32
+    images: torch.Tensor        # [K,5,3] float64
+    depth: torch.Tensor         # [K] int32
+    tx: torch.Tensor            # [K] int32
+    groups_host: np.ndarray
+    offset_host: np.ndarray
+    discovered_depths: tuple[int, ...]
+
+    @property
+    def count(self) -> int:
+        return int(self.groups_host.shape[0])
+
+
+def exhaustive_sequences(group_count: int, depth: int) -> np.ndarray:
+    """All adjacent-distinct group sequences of one depth, lexicographic."""
+    if group_count <= 0:
+        return np.zeros((0, depth), np.int32)
+    seqs = np.arange(group_count, dtype=np.int32)[:, None]
+    for _ in range(depth - 1):
+        nxt = np.arange(group_count, dtype=np.int32)
+        a = np.repeat(seqs, group_count, axis=0)
+        b = np.tile(nxt, seqs.shape[0])[:, None]
+        keep = a[:, -1] != b[:, 0]
+        seqs = np.concatenate([a[keep], b[keep]], axis=1)
+    return seqs
+
+
+def is_exhaustive(group_count: int, depth: int) -> bool:
+    if depth == 1:
+        return group_count <= EXHAUSTIVE_GROUP_LIMIT
+    return group_count * (group_count - 1) ** (depth - 1) <= EXHAUSTIVE_SEQUENCE_LIMIT
+
+
+def discover(dscene: DeviceScene, tx_pos: torch.Tensor, max_depth: int, ray_count: int) -> np.ndarray:
+    """[T, ray_count, max_depth] int32 group hit per bounce (-1 = escaped)."""
+    t = int(tx_pos.shape[0])
+    chains = torch.empty((t, ray_count, max_depth), dtype=torch.int32, device=tx_pos.device)
+    _native.call("wc_discover_chains", dscene.ref(), _native.ptr(tx_pos), t, ray_count, max_depth,
+                 _native.ptr(chains), _native.stream_handle(tx_pos.device))
+    return chains.cpu().numpy()
+
+
+def unique_prefixes(rows: np.ndarray, depth: int) -> np.ndarray:
+    pref = rows[:, :depth]
+    pref = pref[(pref >= 0).all(axis=1)]
+    if pref.shape[0] == 0:
+        return np.zeros((0, depth), np.int32)
+    return np.unique(pref, axis=0).astype(np.int32)
+
+
+def plan_chains(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tensor, max_depth: int, *,
+                reflection: bool, ray_count: int = DISCOVERY_RAY_COUNT,
+                discovered: np.ndarray | None = None) -> ChainPlan:
+    """Candidate reflection chains of every transmitter (sorted by depth, then
+    lexicographically by group sequence)."""
+    device = tx_pos.device
+    t = int(tx_pos.shape[0])
+    g = compiled.group_count
+    depths = range(1, max_depth + 1) if reflection and g > 0 else range(0)
+    needs_rays = [d for d in depths if not is_exhaustive(g, d)]
+    if needs_rays and discovered is None:
+        discovered = discover(dscene, tx_pos, max(needs_rays), ray_count)
+    blocks = []
+    offsets = np.zeros(t * MAX_DEPTH + 1, dtype=np.int32)
+    cursor = 0
+    exhaustive_cache: dict[int, np.ndarray] = {}
+    for ti in range(t):
+        for d in range(1, MAX_DEPTH + 1):
+            seqs = np.zeros((0, d), np.int32)
+            if d in depths:
+                if d in needs_rays:
+                    seqs = unique_prefixes(discovered[ti], d)
+                else:
+                    if d not in exhaustive_cache:
+                        exhaustive_cache[d] = exhaustive_sequences(g, d)
+                    seqs = exhaustive_cache[d]
+            padded = np.full((seqs.shape[0], MAX_DEPTH), -1, dtype=np.int32)
+            padded[:, :d] = seqs
+            blocks.append((padded, ti, d))
+            cursor += seqs.shape[0]
+            offsets[ti * MAX_DEPTH + d] = cursor
+    groups = np.concatenate([b[0] for b in blocks], axis=0) if blocks else np.zeros((0, MAX_DEPTH), np.int32)
+    chain_tx = np.concatenate([np.full(b[0].shape[0], b[1], np.int32) for b in blocks]) if blocks else \
+        np.zeros(0, np.int32)
+    chain_depth = np.concatenate([np.full(b[0].shape[0], b[2], np.int32) for b in blocks]) if blocks else \
+        np.zeros(0, np.int32)
+    k = groups.shape[0]
+    groups_t = torch.as_tensor(groups).to(device)
+    tx_t = torch.as_tensor(chain_tx).to(device)
+    depth_t = torch.as_tensor(chain_depth).to(device)
+    images = torch.zeros((max(k, 1), MAX_DEPTH, 3), dtype=torch.float64, device=device)
+    if k:
+        _native.call("wc_chain_images", dscene.ref(), _native.ptr(tx_pos), _native.ptr(tx_t), _native.ptr(groups_t),
+                     _native.ptr(depth_t), k, _native.ptr(images), _native.stream_handle(device))
+    return ChainPlan(
+        offset=torch.as_tensor(offsets).to(device), groups=groups_t if k else torch.full((1, MAX_DEPTH), -1,
+                                                                                         dtype=torch.int32,
+                                                                                         device=device),
+        images=images, depth=depth_t, tx=tx_t, groups_host=groups, offset_host=offsets,
+        discovered_depths=tuple(needs_rays),
+    )
+
+
+@dataclass
+class SolveInputs:
+    """Device-resident endpoint tables of one solve and the wc_solve view."""
+
+    tx_pos: torch.Tensor
+    tx_pol: torch.Tensor
+    tx_power: torch.Tensor
+    rx_pos: torch.Tensor
+    rx_pol: torch.Tensor
+    cell_offset: torch.Tensor
+    rx_shared: bool
+    chains: ChainPlan
+    view: _native.WcSolve
+    tiles: torch.Tensor
+    tile_count: int
+    rx_count: np.ndarray   # receivers per transmitter (host)
+
+    @property
+    def total_cells(self) -> int:
+        return int(self.view.total_cells)
+
+
+def make_tiles(cell_offset: np.ndarray) -> np.ndarray:
+    """Wave-aligned work tiles (tx, first cell) of 64 cells that never straddle
+    two transmitters."""
+    tiles = []
+    for t in range(cell_offset.shape[0] - 1):
+        for base in range(int(cell_offset[t]), int(cell_offset[t + 1]), TILE):
+            tiles.append((t, base))
+    return np.asarray(tiles, dtype=np.int32).reshape(-1, 2)
+
+
+def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: int, max_depth: int,
+                 coherent: bool, frequency_hz: float, tx_pos: np.ndarray | torch.Tensor | None = None,
+                 tx_pol: np.ndarray | None = None, tx_power: np.ndarray | None = None,
+                 rx_pos: np.ndarray | None = None, rx_pol: np.ndarray | None = None,
+                 rx_count: np.ndarray | None = None, ray_count: int = DISCOVERY_RAY_COUNT,
+                 chains: ChainPlan | None = None) -> SolveInputs:
+    """Upload endpoint tables. Without ``rx_count`` every transmitter shares
+    all receivers; with it, receivers are the concatenation of per-transmitter
+    sets of the given sizes (uneven batched solves)."""
+    device = dscene.device
+    ep = compiled.endpoints
+    tx_pos = ep.tx_position if tx_pos is None else tx_pos
+    if isinstance(tx_pos, torch.Tensor):
+        tx_pos_t = tx_pos.detach().to(device=device, dtype=torch.float32).contiguous()
+    else:
+        tx_pos_t = torch.as_tensor(np.ascontiguousarray(tx_pos, dtype=np.float32)).to(device)
+    t = int(tx_pos_t.shape[0])
+    tx_pol = ep.tx_polarization if tx_pol is None else tx_pol
+    tx_power = ep.tx_power if tx_power is None else tx_power
+    rx_pos = ep.rx_position if rx_pos is None else rx_pos
+    rx_pol = ep.rx_polarization if rx_pol is None else rx_pol
+    f32 = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32)).to(device)  # noqa: E731
+    if rx_count is None:
+        r = int(rx_pos.shape[0])
+        offsets = np.arange(t + 1, dtype=np.int64) * r
+        shared = True
+        counts = np.full(t, r, dtype=np.int64)
+    else:
+        counts = np.asarray(rx_count, dtype=np.int64)
+        offsets = np.concatenate([[0], np.cumsum(counts)])
+        shared = False
+    if offsets[-1] >= 2**31:
+        raise RuntimeError("transmitter x receiver cell count exceeds int32 indexing")
+    if chains is None:
+        chains = plan_chains(compiled, dscene, tx_pos_t, max_depth, reflection=bool(components & 2),
+                             ray_count=ray_count)
+    cell_offset = torch.as_tensor(offsets.astype(np.int32)).to(device)
+    tiles_np = make_tiles(offsets)
+    tiles = torch.as_tensor(tiles_np if tiles_np.size else np.zeros((1, 2), np.int32)).to(device)
+    inputs = SolveInputs(tx_pos_t, f32(tx_pol).reshape(-1, 3), f32(tx_power).reshape(-1), f32(rx_pos).reshape(-1, 3),
+                         f32(rx_pol).reshape(-1, 3), cell_offset, shared, chains, None, tiles,  # type: ignore[arg-type]
+                         int(tiles_np.shape[0]), counts)
+    p = _native.ptr
+    inputs.view = _native.WcSolve(
+        tx_pos=p(inputs.tx_pos), tx_pol=p(inputs.tx_pol), tx_power=p(inputs.tx_power), tx_count=t,
+        rx_pos=p(inputs.rx_pos), rx_pol=p(inputs.rx_pol), rx_pol_stride=1, cell_offset=p(cell_offset),
+        rx_shared=1 if shared else 0, total_cells=int(offsets[-1]), chain_offset=p(chains.offset),
+        chain_groups=p(chains.groups), chain_images=p(chains.images), frequency_hz=float(frequency_hz),
+        components=int(components), max_depth=int(max_depth), coherent=1 if coherent else 0,
+    )
+    return inputs
+
+
+@dataclass
+class RadiomapOutputs:
+    power_total: torch.Tensor        # [cells] f32
+    field_total: torch.Tensor        # [cells] c64
+    component_power: torch.Tensor    # [4, cells] f32
+    component_field: torch.Tensor    # [4, cells] c64
+    diffraction_vector: torch.Tensor  # [cells, 3] c64
+    path_count: torch.Tensor         # [cells] i32
+    topology: torch.Tensor | None
+    topology_capacity: int
+    failure: torch.Tensor
+
+
+def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capacity: int = 0,
+                     tiled: bool = True, out: RadiomapOutputs | None = None) -> RadiomapOutputs:
+    device = dscene.device
+    cells = inputs.total_cells
+    if out is None:
+        out = RadiomapOutputs(
+            power_total=torch.empty(cells, dtype=torch.float32, device=device),
+            field_total=torch.empty(cells, dtype=torch.complex64, device=device),
+            component_power=torch.empty((4, cells), dtype=torch.float32, device=device),
+            component_field=torch.empty((4, cells), dtype=torch.complex64, device=device),
+            diffraction_vector=torch.empty((cells, 3), dtype=torch.complex64, device=device),
+            path_count=torch.empty(cells, dtype=torch.int32, device=device),
+            topology=(torch.empty((cells, topology_capacity), dtype=torch.int32, device=device)
+                      if topology_capacity > 0 else None),
+            topology_capacity=topology_capacity,
+            failure=torch.zeros(1, dtype=torch.int32, device=device),
+        )
+    p = _native.ptr
+    view = _native.WcRadiomapOut(
+        power_total=p(out.power_total), field_total=p(out.field_total), component_power=p(out.component_power),
+        component_field=p(out.component_field), diffraction_vector=p(out.diffraction_vector),
+        path_count=p(out.path_count), topology=p(out.topology), topology_capacity=out.topology_capacity,
+        failure=p(out.failure),
+    )
+    counter = inputs.__dict__.setdefault("_tile_counter", torch.zeros(1, dtype=torch.int32, device=device))
+    import ctypes
+    _native.call("wc_radiomap_forward", dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(view),
+                 p(inputs.tiles) if tiled else None, inputs.tile_count if tiled else 0,
+                 p(counter) if tiled else None, _native.stream_handle(device))
+    return out
+
+
+ROW_FIELDS_INT = ("tx_id", "rx_id", "depth", "component_id", "primitive_id", "edge_id", "material_id")
+ROW_FIELDS_FLOAT = ("path_length_m", "delay_s", "path_gain")
+
+
+def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int) -> dict[str, Any]:
+    """Explicit path rows in canonical order: pairs rx-major, tx-minor; inside a
+    pair (depth, component, primitive sequence, primitive, edge)."""
+    import ctypes
+
+    device = dscene.device
+    cells = inputs.total_cells
+    t = int(inputs.view.tx_count)
+    stream = _native.stream_handle(device)
+    counts = torch.empty(max(cells, 1), dtype=torch.int32, device=device)
+    _native.call("wc_paths_count", dscene.ref(), ctypes.byref(inputs.view), _native.ptr(counts), stream)
+    counts_h = counts[:cells].cpu().numpy().astype(np.int64)
+    offsets_cell = np.concatenate([[0], np.cumsum(inputs.rx_count)]).astype(np.int64)
+    # canonical pair order: rx-major then tx
+    cell_tx = np.repeat(np.arange(t), inputs.rx_count)
+    cell_rx = np.arange(cells) - offsets_cell[cell_tx]
+    canon = np.lexsort((cell_tx, cell_rx))          # cells sorted by (rx, tx)
+    canon_counts = counts_h[canon]
+    starts = np.zeros(cells, dtype=np.int64)
+    starts[canon] = np.cumsum(canon_counts) - canon_counts
+    total = int(counts_h.sum())
+    w = max(int(width), 1)
+    rows: dict[str, torch.Tensor] = {}
+    k = max(total, 1)
+    for name in ROW_FIELDS_INT:
+        rows[name] = torch.empty(k, dtype=torch.int32, device=device)
+    for name in ROW_FIELDS_FLOAT:
+        rows[name] = torch.empty(k, dtype=torch.float32, device=device)
+    rows["primitive_sequence"] = torch.empty((k, w), dtype=torch.int32, device=device)
+    rows["material_sequence"] = torch.empty((k, w), dtype=torch.int32, device=device)
+    rows["interaction_positions"] = torch.empty((k, w, 3), dtype=torch.float32, device=device)
+    rows["interaction_normals"] = torch.empty((k, w, 3), dtype=torch.float32, device=device)
+    rows["path_field"] = torch.empty(k, dtype=torch.complex64, device=device)
+    rows["coefficient"] = torch.empty(k, dtype=torch.complex64, device=device)
+    rows["field_xyz"] = torch.empty((k, 3), dtype=torch.complex64, device=device)
+    rows["arrival_direction"] = torch.empty((k, 3), dtype=torch.float32, device=device)
+    rows["departure_direction"] = torch.empty((k, 3), dtype=torch.float32, device=device)
+    p = _native.ptr
+    view = _native.WcPathRows(width=w, **{name: p(rows[name]) for name in (
+        *ROW_FIELDS_INT, *ROW_FIELDS_FLOAT, "primitive_sequence", "material_sequence", "interaction_positions",
+        "interaction_normals", "path_field", "coefficient", "field_xyz", "arrival_direction",
+        "departure_direction")})
+    row_offset = torch.as_tensor(starts.astype(np.int32)).to(device)
+    if total:
+        _native.call("wc_paths_emit", dscene.ref(), ctypes.byref(inputs.view), p(row_offset), ctypes.byref(view),
+                     stream)
+    pair_offset_h = np.concatenate([[0], np.cumsum(canon_counts)]).astype(np.int32)
+    pair_offset = torch.as_tensor(pair_offset_h).to(device)
+    perm = torch.empty(k, dtype=torch.int32, device=device)
+    keep = torch.empty(k, dtype=torch.uint8, device=device)
+    if total:
+        _native.call("wc_paths_canonical_order", ctypes.byref(view), p(pair_offset), cells, p(perm), p(keep), stream)
+    perm = perm[:total].long()
+    keep_b = keep[:total].bool()
+    out = {name: value[:total][perm][keep_b] if name not in ("primitive_sequence",) else value[:total][perm][keep_b]
+           for name, value in rows.items()}
+    out["pair_counts_canonical"] = canon_counts
+    out["cell_order_canonical"] = canon
+    return out

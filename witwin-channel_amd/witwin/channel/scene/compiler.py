@@ -1,0 +1,473 @@
+"""Scene compile: geometry, coplanar reflection groups, wedge edges, materials.
+
+Produces a :class:`CompiledScene` whose host arrays (numpy) describe the
+triangle soup, the coplanar face groups used by the image method, the
+diffraction edge table and the material ABI (first-layer scalar view plus CSR
+layer stacks), and whose :class:`DeviceScene` holds the same tables in HBM
+together with the LBVH built by ``wc_bvh_build``.
+
+Reference: witwin/channel/scene/compiler.py:131-322 (stores), :967-1150
+(material records), native/channel/kernels/paths.cu:3142-3300 (coplanar face
+groups keyed by surface id and the plane quantized at 1e-4),
+witwin/channel/scene/resources.py:169-245 (scene + BVH creation).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Any
+
+import numpy as np
+import torch
+
+from witwin.channel.constants import EPS0
+from witwin.core import AntennaState, PhysicalMaterial, ReceiverGrid, Scene
+
+DIELECTRIC_MODEL_ID = 1
+PEC_MODEL_ID = 2
+PEC_EFFECTIVE_SIGMA_E = 1.0e9
+PLANE_GROUP_QUANTIZATION = 1.0e-4
+# vertex positions are merged at this resolution when building the edge table
+EDGE_WELD_QUANTIZATION = 1.0e-5
+# two faces whose interior angle is within this of pi are coplanar: no wedge
+COPLANAR_ANGLE_TOLERANCE = 1.0e-4
+
+
+@dataclass
+class MaterialTable:
+    eps_r: np.ndarray
+    sigma_e: np.ndarray
+    mu_r: np.ndarray
+    gain: np.ndarray
+    thickness_m: np.ndarray
+    model_id: np.ndarray
+    layer_offset: np.ndarray
+    layer_count: np.ndarray
+    layer_thickness_m: np.ndarray
+    layer_eps_r: np.ndarray
+    layer_sigma_e: np.ndarray
+    layer_mu_r: np.ndarray
+
+    @property
+    def count(self) -> int:
+        return int(self.eps_r.shape[0])
+
+
+@dataclass
+class Endpoints:
+    tx_position: np.ndarray      # [T,3] f32
+    tx_polarization: np.ndarray  # [T,3] f32
+    tx_power: np.ndarray         # [T] f32
+    rx_position: np.ndarray      # [R,3] f32
+    rx_polarization: np.ndarray  # [R,3] f32
+    layout_kind: str             # "grid" (single receiver grid) or "point"
+    grid_shape: tuple[int, int] | None
+
+
+@dataclass
+class CompiledScene:
+    vertices: np.ndarray
+    faces: np.ndarray
+    face_material: np.ndarray
+    face_surface: np.ndarray
+    face_group: np.ndarray
+    group_plane: np.ndarray
+    group_face_offset: np.ndarray
+    group_faces: np.ndarray
+    group_material: np.ndarray
+    representative_faces: np.ndarray
+    materials: MaterialTable
+    edges: np.ndarray            # structured array with the wc_edge layout
+    frequency_hz: float
+    endpoints: Endpoints
+    edge_policy: str = "all"
+    device: "DeviceScene | None" = None
+    cache: dict[str, Any] = field(default_factory=dict)
+
+    @property
+    def face_count(self) -> int:
+        return int(self.faces.shape[0])
+
+    @property
+    def group_count(self) -> int:
+        return int(self.group_plane.shape[0])
+
+    @property
+    def edge_count(self) -> int:
+        return int(self.edges.shape[0])
+
+    def face_vertices(self) -> np.ndarray:
+        """[F, 9] float32: the three vertices of each face, original order."""
+        return self.vertices[self.faces].reshape(-1, 9).astype(np.float32)
+
+
+EDGE_DTYPE = np.dtype(
+    [(name, np.float32) for name in (
+        "ax", "ay", "az", "dx", "dy", "dz", "length", "n", "t0x", "t0y", "t0z", "n0x", "n0y", "n0z")]
+    + [(name, np.int32) for name in ("face0", "face1", "group0", "group1", "material0", "material1")]
+)
+
+
+# ---------------------------------------------------------------- materials
+
+def _sigma_from(eps_r: Any, sigma_e: float, frequency_hz: float) -> tuple[float, float]:
+    """Real permittivity and total conductivity; a complex eps_r carries its
+    loss as sigma = -Im(eps_r) * omega * eps0 (reference compiler.py:955-965)."""
+    if isinstance(eps_r, complex):
+        extra = -eps_r.imag * (2.0 * math.pi * frequency_hz * EPS0)
+        return float(eps_r.real), float(sigma_e) + extra
+    return float(eps_r), float(sigma_e)
+
+
+def compile_materials(materials: list[PhysicalMaterial], frequency_hz: float) -> MaterialTable:
+    rows = []
+    layers = []
+    offsets = []
+    counts = []
+    for material in materials:
+        model = PEC_MODEL_ID if material.conductor_model == "perfect" else DIELECTRIC_MODEL_ID
+        stack = material.layer_rows()
+        offsets.append(len(layers))
+        counts.append(len(stack))
+        for layer in stack:
+            eps, sigma = _sigma_from(layer.eps_r, layer.sigma_e, frequency_hz)
+            if model == PEC_MODEL_ID:
+                sigma = max(sigma, PEC_EFFECTIVE_SIGMA_E)
+            layers.append((float(layer.thickness_m), eps, sigma, float(layer.mu_r)))
+        first = layers[offsets[-1]]
+        rows.append((first[1], first[2], first[3], float(material.gain), first[0], model))
+    if not rows:
+        rows.append((1.0, 0.0, 1.0, 1.0, 0.0, DIELECTRIC_MODEL_ID))
+        offsets.append(0)
+        counts.append(1)
+        layers.append((0.0, 1.0, 0.0, 1.0))
+    arr = np.asarray(rows, dtype=np.float64)
+    lay = np.asarray(layers, dtype=np.float64)
+    return MaterialTable(
+        eps_r=arr[:, 0].astype(np.float32),
+        sigma_e=arr[:, 1].astype(np.float32),
+        mu_r=arr[:, 2].astype(np.float32),
+        gain=arr[:, 3].astype(np.float32),
+        thickness_m=arr[:, 4].astype(np.float32),
+        model_id=arr[:, 5].astype(np.int32),
+        layer_offset=np.asarray(offsets, dtype=np.int32),
+        layer_count=np.asarray(counts, dtype=np.int32),
+        layer_thickness_m=lay[:, 0].astype(np.float32),
+        layer_eps_r=lay[:, 1].astype(np.float32),
+        layer_sigma_e=lay[:, 2].astype(np.float32),
+        layer_mu_r=lay[:, 3].astype(np.float32),
+    )
+
+
+# ---------------------------------------------------------------- coplanar groups
+
+def coplanar_face_groups(tri_a: np.ndarray, normals: np.ndarray, surface_ids: np.ndarray,
+                         quantization: float = PLANE_GROUP_QUANTIZATION) -> dict[str, Any]:
+    """Group faces by (surface id, quantized canonical plane).
+
+    The normal is flipped so its largest-magnitude component is positive; the
+    key is (surface_id, round(n*s), round(offset*s)) with s = 1/quantization,
+    and group ids follow the lexicographic order of the keys (faces with equal
+    keys keep ascending face order). Mirrors the reference key construction of
+    native/channel/kernels/paths.cu:3142-3190 and its sort-based assignment.
+    """
+    tri_a = np.asarray(tri_a, dtype=np.float32)
+    normals = np.asarray(normals, dtype=np.float32)
+    count = tri_a.shape[0]
+    if count == 0:
+        return {"face_group_id": np.zeros(0, np.int32), "representative_faces": np.zeros(0, np.int64),
+                "group_count": 0}
+    nrm = np.sqrt((normals.astype(np.float32) ** 2).sum(axis=1, dtype=np.float32))
+    inv = np.where(nrm > 1.0e-6, np.float32(1.0) / np.maximum(nrm, np.float32(1e-30)), np.float32(0.0))
+    n = normals * inv[:, None]
+    major = np.argmax(np.abs(n), axis=1)
+    sign = np.where(n[np.arange(count), major] < 0.0, np.float32(-1.0), np.float32(1.0))
+    n = n * sign[:, None]
+    offset = (n * tri_a).sum(axis=1, dtype=np.float32)
+    scale = np.float32(1.0 / quantization)
+    keys = np.stack([
+        np.asarray(surface_ids, dtype=np.int64),
+        np.rint(n[:, 0] * scale).astype(np.int64),
+        np.rint(n[:, 1] * scale).astype(np.int64),
+        np.rint(n[:, 2] * scale).astype(np.int64),
+        np.rint(offset * scale).astype(np.int64),
+    ], axis=1)
+    order = np.lexsort(keys.T[::-1])  # stable, first column most significant
+    sorted_keys = keys[order]
+    starts = np.ones(count, dtype=bool)
+    starts[1:] = np.any(sorted_keys[1:] != sorted_keys[:-1], axis=1)
+    group_of_sorted = np.cumsum(starts) - 1
+    face_group = np.empty(count, dtype=np.int32)
+    face_group[order] = group_of_sorted.astype(np.int32)
+    representative = order[starts].astype(np.int64)
+    return {"face_group_id": face_group, "representative_faces": representative,
+            "group_count": int(starts.sum())}
+
+
+def _group_tables(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray, representative: np.ndarray,
+                  face_material: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    g_count = representative.shape[0]
+    v64 = vertices.astype(np.float64)
+    planes = np.zeros((g_count, 4), dtype=np.float64)
+    for g, f in enumerate(representative):
+        a, b, c = v64[faces[f]]
+        n = np.cross(b - a, c - a)
+        n = n / max(np.linalg.norm(n), 1e-300)
+        k = int(np.argmax(np.abs(n)))
+        if n[k] < 0.0:
+            n = -n
+        planes[g, :3] = n
+        planes[g, 3] = float(np.dot(n, a))
+    order = np.argsort(face_group, kind="stable")
+    counts = np.bincount(face_group, minlength=g_count)
+    offsets = np.zeros(g_count + 1, dtype=np.int32)
+    offsets[1:] = np.cumsum(counts)
+    group_material = face_material[representative].astype(np.int32) if g_count else np.zeros(0, np.int32)
+    return planes, offsets, order.astype(np.int32), group_material
+
+
+# ---------------------------------------------------------------- edges
+
+def build_edges(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray, face_material: np.ndarray,
+                policy: str = "all") -> np.ndarray:
+    """Diffraction edge table.
+
+    Every triangle edge is keyed by its welded end points. An edge shared by
+    two faces of different, non-coplanar planes is a wedge whose exterior is the
+    side the lower-id face's winding normal points to; an edge of exactly one
+    face is a half-plane (n = 2). Edges between coplanar faces and edges with
+    more than two faces are not diffracting. Ids follow first appearance
+    (faces ascending, local edges v0v1, v1v2, v2v0). ``policy="vertical_only"``
+    keeps edges with |dz|/length > 0.7.
+    """
+    v = vertices.astype(np.float64)
+    q = np.rint(v / EDGE_WELD_QUANTIZATION).astype(np.int64)
+    table: dict[tuple, list] = {}
+    order: list[tuple] = []
+    for f in range(faces.shape[0]):
+        idx = faces[f]
+        for k in range(3):
+            i0, i1, i2 = int(idx[k]), int(idx[(k + 1) % 3]), int(idx[(k + 2) % 3])
+            a, b = tuple(q[i0]), tuple(q[i1])
+            key = (a, b) if a <= b else (b, a)
+            if key not in table:
+                table[key] = []
+                order.append(key)
+            table[key].append((f, i0, i1, i2))
+    rows = []
+    for key in order:
+        entries = table[key]
+        if len(entries) > 2:
+            continue
+        f0, i0, i1, i2 = entries[0]
+        A, B = v[i0], v[i1]
+        d = B - A
+        length = float(np.linalg.norm(d))
+        if length < 1e-9:
+            continue
+        d = d / length
+        if policy == "vertical_only" and abs(d[2]) <= 0.7:
+            continue
+
+        def in_plane_dir(third: np.ndarray) -> np.ndarray:
+            w = third - A
+            w = w - d * np.dot(w, d)
+            return w / max(np.linalg.norm(w), 1e-300)
+
+        a0, b0, c0 = v[faces[f0]]
+        n0 = np.cross(b0 - a0, c0 - a0)
+        n0 = n0 / max(np.linalg.norm(n0), 1e-300)
+        t0 = in_plane_dir(v[i2])
+        # n0 must be perpendicular to the edge; remove numerical drift
+        n0 = n0 - d * np.dot(n0, d)
+        n0 = n0 / max(np.linalg.norm(n0), 1e-300)
+        if len(entries) == 2:
+            f1, j0, j1, j2 = entries[1]
+            if face_group[f1] == face_group[f0]:
+                continue
+            t1 = in_plane_dir(v[j2])
+            alpha = math.atan2(-float(np.dot(n0, t1)), float(np.dot(t0, t1)))
+            if alpha < 0.0:
+                alpha += 2.0 * math.pi
+            if abs(alpha - math.pi) < COPLANAR_ANGLE_TOLERANCE or alpha < 1e-6:
+                continue
+            wedge_n = 2.0 - alpha / math.pi
+            face1 = f1
+            group1 = int(face_group[f1])
+            material1 = int(face_material[f1])
+        else:
+            wedge_n = 2.0
+            face1 = -1
+            group1 = -1
+            material1 = -1
+        rows.append((A[0], A[1], A[2], d[0], d[1], d[2], length, wedge_n, t0[0], t0[1], t0[2],
+                     n0[0], n0[1], n0[2], f0, face1, int(face_group[f0]), group1, int(face_material[f0]),
+                     material1))
+    return np.array(rows, dtype=EDGE_DTYPE)
+
+
+# ---------------------------------------------------------------- endpoints
+
+def _pol(endpoint: AntennaState) -> np.ndarray:
+    if endpoint.polarization is None:
+        return np.array([0.0, 0.0, 1.0], dtype=np.float32)
+    return endpoint.polarization.detach().cpu().numpy().astype(np.float32)
+
+
+def grid_points(grid: ReceiverGrid) -> np.ndarray:
+    """p = origin + x_axis*row*s0 + y_axis*col*s1, flat index row*cols+col, float32
+    arithmetic as the reference grid kernel (native/channel/kernels/los.cu:391-420)."""
+    rows, cols = grid.shape
+    o = grid.origin.numpy().astype(np.float32)
+    xa = grid.x_axis.numpy().astype(np.float32)
+    ya = grid.y_axis.numpy().astype(np.float32)
+    r = np.arange(rows, dtype=np.float32)[:, None] * np.float32(grid.spacing[0])
+    c = np.arange(cols, dtype=np.float32)[None, :] * np.float32(grid.spacing[1])
+    pts = np.empty((rows, cols, 3), dtype=np.float32)
+    for k in range(3):
+        pts[..., k] = o[k] + xa[k] * r + ya[k] * c
+    return pts.reshape(-1, 3)
+
+
+def compile_endpoints(scene: Scene) -> Endpoints:
+    txs = scene.transmitters
+    rxs = scene.receivers
+    tx_pos = np.array([t.position.detach().cpu().numpy().reshape(3) for t in txs], dtype=np.float32).reshape(-1, 3)
+    tx_pol = np.array([_pol(t) for t in txs], dtype=np.float32).reshape(-1, 3)
+    tx_pow = np.array([1.0 if t.power_w is None else float(t.power_w) for t in txs], dtype=np.float32)
+    positions = []
+    pols = []
+    for rx in rxs:
+        pts = grid_points(rx) if isinstance(rx, ReceiverGrid) else rx.position.detach().cpu().numpy().reshape(-1, 3)
+        positions.append(pts.astype(np.float32))
+        pols.append(np.broadcast_to(_pol(rx), pts.shape).astype(np.float32))
+    rx_pos = np.concatenate(positions, axis=0) if positions else np.zeros((0, 3), np.float32)
+    rx_pol = np.concatenate(pols, axis=0) if pols else np.zeros((0, 3), np.float32)
+    if len(rxs) == 1 and isinstance(rxs[0], ReceiverGrid):
+        kind, shape = "grid", rxs[0].shape
+    else:
+        kind, shape = "point", None
+    return Endpoints(tx_pos, tx_pol, tx_pow, rx_pos, rx_pol, kind, shape)
+
+
+# ---------------------------------------------------------------- compile
+
+def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # noqa: A001 - reference name
+    frequency = float(reference_frequency_hz.detach()) if isinstance(reference_frequency_hz, torch.Tensor) \
+        else float(reference_frequency_hz)
+    if not frequency > 0.0:
+        raise ValueError("reference_frequency_hz must be positive")
+    vert_chunks, face_chunks, mat_ids, surf_ids = [], [], [], []
+    unique: list[PhysicalMaterial] = []
+    lookup: dict[int, int] = {}
+    base = 0
+    for index, structure in enumerate(scene.structures):
+        mesh = structure.geometry
+        v = mesh.vertices.detach().cpu().numpy().astype(np.float32)
+        f = mesh.faces.detach().cpu().numpy().astype(np.int64)
+        mat = structure.material
+        if id(mat) not in lookup:
+            lookup[id(mat)] = len(unique)
+            unique.append(mat)
+        vert_chunks.append(v)
+        face_chunks.append(f + base)
+        mat_ids.append(np.full(f.shape[0], lookup[id(mat)], dtype=np.int32))
+        sid = structure.surface_id if structure.surface_id is not None else index
+        surf_ids.append(np.full(f.shape[0], int(sid), dtype=np.int64))
+        base += v.shape[0]
+    vertices = np.concatenate(vert_chunks, axis=0) if vert_chunks else np.zeros((0, 3), np.float32)
+    faces = np.concatenate(face_chunks, axis=0).astype(np.int32) if face_chunks else np.zeros((0, 3), np.int32)
+    face_material = np.concatenate(mat_ids) if mat_ids else np.zeros(0, np.int32)
+    face_surface = np.concatenate(surf_ids) if surf_ids else np.zeros(0, np.int64)
+    materials = compile_materials(unique, frequency)
+    if faces.shape[0]:
+        tri = vertices[faces]
+        normals = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]).astype(np.float32)
+        groups = coplanar_face_groups(tri[:, 0], normals, face_surface)
+    else:
+        groups = coplanar_face_groups(np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0))
+    face_group = groups["face_group_id"]
+    representative = groups["representative_faces"]
+    planes, offsets, members, group_material = _group_tables(vertices, faces, face_group, representative,
+                                                             face_material)
+    policy = getattr(scene.metadata.get("imported_edge_policy"), "edge_selection_mode", "all") \
+        if scene.metadata else "all"
+    edges = build_edges(vertices, faces, face_group, face_material, policy)
+    return CompiledScene(
+        vertices=vertices, faces=faces, face_material=face_material, face_surface=face_surface,
+        face_group=face_group, group_plane=planes, group_face_offset=offsets, group_faces=members,
+        group_material=group_material, representative_faces=representative, materials=materials,
+        edges=edges, frequency_hz=frequency, endpoints=compile_endpoints(scene), edge_policy=policy,
+    )
+
+
+# ---------------------------------------------------------------- device scene
+
+class DeviceScene:
+    """HBM copy of a compiled scene plus its LBVH, and the wc_scene view."""
+
+    def __init__(self, compiled: CompiledScene, device: torch.device) -> None:
+        from witwin.channel import _native
+
+        self.device = device
+        t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a)).to(device=device, dtype=dt).contiguous()  # noqa: E731
+        mats = compiled.materials
+        self.vertices = t(compiled.vertices, torch.float32)
+        self.faces = t(compiled.faces, torch.int32)
+        self.face_group = t(compiled.face_group, torch.int32)
+        self.face_material = t(compiled.face_material, torch.int32)
+        self.group_plane = t(compiled.group_plane, torch.float64)
+        self.group_face_offset = t(compiled.group_face_offset, torch.int32)
+        self.group_faces = t(compiled.group_faces, torch.int32)
+        self.group_material = t(compiled.group_material, torch.int32)
+        self.mat_eps = t(mats.eps_r, torch.float32)
+        self.mat_sigma = t(mats.sigma_e, torch.float32)
+        self.mat_mu = t(mats.mu_r, torch.float32)
+        self.mat_gain = t(mats.gain, torch.float32)
+        self.mat_thickness = t(mats.thickness_m, torch.float32)
+        self.layer_offset = t(mats.layer_offset, torch.int32)
+        self.layer_count = t(mats.layer_count, torch.int32)
+        self.layer_thickness = t(mats.layer_thickness_m, torch.float32)
+        self.layer_eps = t(mats.layer_eps_r, torch.float32)
+        self.layer_sigma = t(mats.layer_sigma_e, torch.float32)
+        self.layer_mu = t(mats.layer_mu_r, torch.float32)
+        edge_bytes = compiled.edges.view(np.uint8).reshape(-1) if compiled.edge_count else np.zeros(0, np.uint8)
+        self.edges = torch.as_tensor(edge_bytes.copy()).to(device)
+        f = compiled.face_count
+        self.node_count = int(_native.call("wc_bvh_node_count", f))
+        self.nodes = torch.zeros(self.node_count * 64, dtype=torch.uint8, device=device)
+        self.sorted_tris = torch.zeros(max(f, 1) * 48, dtype=torch.uint8, device=device)
+        self.face_tris = torch.zeros(max(f, 1) * 48, dtype=torch.uint8, device=device)
+        if f:
+            ws_bytes = int(_native.call("wc_bvh_workspace_bytes", f))
+            workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+            _native.call("wc_bvh_build", _native.ptr(self.vertices), _native.ptr(self.faces),
+                         _native.ptr(self.face_group), _native.ptr(self.face_material), f,
+                         _native.ptr(self.nodes), _native.ptr(self.sorted_tris), _native.ptr(self.face_tris),
+                         _native.ptr(workspace), ws_bytes, _native.stream_handle(device))
+            torch.cuda.current_stream(device).synchronize()
+            del workspace
+        p = _native.ptr
+        self.view = _native.WcScene(
+            nodes=p(self.nodes), tris=p(self.sorted_tris), tri_count=f, node_count=self.node_count,
+            faces=p(self.face_tris), face_group=p(self.face_group), face_material=p(self.face_material),
+            face_count=f, group_plane=p(self.group_plane), group_face_offset=p(self.group_face_offset),
+            group_faces=p(self.group_faces), group_material=p(self.group_material),
+            group_count=compiled.group_count, mat_eps=p(self.mat_eps), mat_sigma=p(self.mat_sigma),
+            mat_mu=p(self.mat_mu), mat_gain=p(self.mat_gain), mat_thickness=p(self.mat_thickness),
+            mat_layer_offset=p(self.layer_offset), mat_layer_count=p(self.layer_count),
+            layer_thickness=p(self.layer_thickness), layer_eps=p(self.layer_eps), layer_sigma=p(self.layer_sigma),
+            layer_mu=p(self.layer_mu), material_count=mats.count, edges=p(self.edges),
+            edge_count=compiled.edge_count,
+        )
+
+    def ref(self) -> "ctypes._Pointer":
+        return ctypes.byref(self.view)
+
+
+def device_scene(compiled: CompiledScene, device: torch.device) -> DeviceScene:
+    if compiled.device is None or compiled.device.device != device:
+        compiled.device = DeviceScene(compiled, device)
+    return compiled.device

@@ -1,0 +1,158 @@
+"""Seeded synthetic workloads used by the parity tests and the benchmark.
+
+* ``two_ray_scene`` — one isotropic transmitter at (0, 0, 10) m over a flat
+  ground (eps_r 5.31, sigma 0.0326 S/m, 5 m slab so the slab coefficient equals
+  the half-space one to < 1e-11), 3.5 GHz, 64 x 64 receiver grid at z = 1.5 m
+  with 1 m spacing centred on the transmitter.
+* ``urban_canyon_scene`` — seeded 8 x 8 Manhattan grid of box buildings
+  (30 x 30 m footprint, 15 m streets, heights U[10, 40] m, 12 triangles each)
+  on a ground quad, concrete walls (eps_r 5.31, sigma 0.0326) with a seeded
+  25 % of wall faces glass (eps_r 6.27, sigma 0.0043), 0.2 m thick; one
+  transmitter at a street intersection, 25 m high; a receiver grid at
+  z = 1.5 m covering the block (256 x 256 at 2 m by default).
+* ``batched_transmitters`` — seeded transmitter positions over the streets
+  with log-uniform receiver-set sizes.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from witwin.core import (
+    AntennaState,
+    MaterialLayer,
+    Mesh,
+    PhysicalMaterial,
+    ReceiverGrid,
+    Scene,
+    Structure,
+    new_antenna_id,
+)
+
+CONCRETE = dict(eps_r=5.31, sigma_e=0.0326)
+GLASS = dict(eps_r=6.27, sigma_e=0.0043)
+
+# box faces as quads (outward winding): -x, +x, -y, +y, bottom, top
+_BOX_QUADS = (
+    ((0, 4, 6, 2), "wall"),   # x = x0
+    ((1, 3, 7, 5), "wall"),   # x = x1
+    ((0, 1, 5, 4), "wall"),   # y = y0
+    ((2, 6, 7, 3), "wall"),   # y = y1
+    ((0, 2, 3, 1), "floor"),  # z = 0
+    ((4, 5, 7, 6), "roof"),   # z = h
+)
+
+
+def box_mesh(x0: float, y0: float, x1: float, y1: float, h: float) -> tuple[np.ndarray, np.ndarray, list[str]]:
+    """12-triangle box with outward-facing winding; returns vertices, faces and
+    the role of each face pair."""
+    v = np.array([[x, y, z] for z in (0.0, h) for y in (y0, y1) for x in (x0, x1)], dtype=np.float32)
+    faces, roles = [], []
+    for (a, b, c, d), role in _BOX_QUADS:
+        faces.append((a, b, c))
+        faces.append((a, c, d))
+        roles.extend([role, role])
+    return v, np.asarray(faces, dtype=np.int64), roles
+
+
+def ground_structure(x0: float, y0: float, x1: float, y1: float, material: PhysicalMaterial,
+                     surface_id: int = 0) -> Structure:
+    v = np.array([[x0, y0, 0.0], [x1, y0, 0.0], [x1, y1, 0.0], [x0, y1, 0.0]], dtype=np.float32)
+    f = np.array([[0, 1, 2], [0, 2, 3]], dtype=np.int64)
+    return Structure(Mesh(torch.from_numpy(v), torch.from_numpy(f)), material, name="ground", surface_id=surface_id)
+
+
+def transmitter(position, power_w: float = 1.0, polarization=None) -> AntennaState:
+    return AntennaState(new_antenna_id(), "tx", torch.tensor(position, dtype=torch.float32),
+                        polarization=polarization, power_w=power_w)
+
+
+def receiver(position, polarization=None) -> AntennaState:
+    return AntennaState(new_antenna_id(), "rx", torch.tensor(position, dtype=torch.float32), polarization=polarization)
+
+
+def grid(origin, shape, spacing) -> ReceiverGrid:
+    return ReceiverGrid(new_antenna_id(), torch.tensor(origin, dtype=torch.float32),
+                        torch.tensor([1.0, 0.0, 0.0]), torch.tensor([0.0, 1.0, 0.0]), shape, spacing)
+
+
+def two_ray_scene(shape: tuple[int, int] = (64, 64), spacing: float = 1.0, extent: float = 1.0e4) -> Scene:
+    ground = PhysicalMaterial(eps_r=5.31, sigma_e=0.0326, thickness_m=5.0, name="ground")
+    rows, cols = shape
+    origin = (-(rows - 1) * spacing / 2.0, -(cols - 1) * spacing / 2.0, 1.5)
+    return Scene(structures=[ground_structure(-extent, -extent, extent, extent, ground)],
+                 endpoints=[transmitter((0.0, 0.0, 10.0)), grid(origin, shape, spacing)])
+
+
+def manhattan_structures(seed: int = 2, blocks: int = 8, footprint: float = 30.0, street: float = 15.0,
+                         height_range: tuple[float, float] = (10.0, 40.0), glass_fraction: float = 0.25,
+                         thickness: float = 0.2) -> tuple[list[Structure], float]:
+    """Box buildings on a regular street grid; returns (structures, pitch)."""
+    rng = np.random.default_rng(seed)
+    concrete = PhysicalMaterial(thickness_m=thickness, name="concrete", **CONCRETE)
+    glass = PhysicalMaterial(thickness_m=thickness, name="glass", **GLASS)
+    pitch = footprint + street
+    structures = []
+    sid = 1
+    for i in range(blocks):
+        for j in range(blocks):
+            x0 = street + i * pitch
+            y0 = street + j * pitch
+            h = float(rng.uniform(*height_range))
+            v, f, roles = box_mesh(x0, y0, x0 + footprint, y0 + footprint, h)
+            glass_faces = [k for k in range(0, 8, 2) if rng.uniform() < glass_fraction]
+            # split walls by material: one structure per material so each face keeps its surface
+            concrete_faces, glass_faces_list = [], []
+            for k in range(0, 12, 2):
+                target = glass_faces_list if (roles[k] == "wall" and k in glass_faces) else concrete_faces
+                target.extend([f[k], f[k + 1]])
+            for faces, mat in ((concrete_faces, concrete), (glass_faces_list, glass)):
+                if not faces:
+                    continue
+                structures.append(Structure(Mesh(torch.from_numpy(v), torch.from_numpy(np.asarray(faces))), mat,
+                                            name=f"box-{i}-{j}", surface_id=sid))
+                sid += 1
+    return structures, pitch
+
+
+def urban_canyon_scene(seed: int = 2, shape: tuple[int, int] = (256, 256), spacing: float = 2.0,
+                       tx_position=(22.5, 22.5, 25.0)) -> Scene:
+    structures, pitch = manhattan_structures(seed)
+    extent = 15.0 + 8 * pitch
+    ground = PhysicalMaterial(thickness_m=0.2, name="ground-concrete", **CONCRETE)
+    structures.insert(0, ground_structure(-5.0, -5.0, extent + 5.0, extent + 5.0, ground))
+    # grid centred on the block; origin offset by a fraction of the spacing so receivers do not sit
+    # exactly on wall planes
+    rows, cols = shape
+    span_r, span_c = (rows - 1) * spacing, (cols - 1) * spacing
+    origin = ((extent - span_r) / 2.0 + 0.173, (extent - span_c) / 2.0 + 0.291, 1.5)
+    return Scene(structures=structures, endpoints=[transmitter(tx_position), grid(origin, shape, spacing)])
+
+
+def street_points(rng: np.random.Generator, count: int, street: float = 15.0, pitch: float = 45.0,
+                  blocks: int = 8, z: float | tuple[float, float] = 1.5) -> np.ndarray:
+    """Uniform points on the street network (outside every footprint)."""
+    extent = street + blocks * pitch
+    pts = []
+    while len(pts) < count:
+        x, y = rng.uniform(0.0, extent, size=2)
+        in_x = (x % pitch) >= street
+        in_y = (y % pitch) >= street
+        if in_x and in_y and x < extent and y < extent:
+            continue  # inside a building footprint
+        zz = rng.uniform(*z) if isinstance(z, tuple) else z
+        pts.append((x, y, zz))
+    return np.asarray(pts, dtype=np.float32)
+
+
+def batched_transmitters(seed: int = 3, tx_count: int = 64, min_rx: int = 16, max_rx: int = 4096):
+    """Transmitters over the streets (z U[3, 30]) and log-uniform receiver-set
+    sizes; returns (tx_positions [T,3], rx_counts [T], rx_positions [sum,3])."""
+    rng = np.random.default_rng(seed)
+    tx = street_points(rng, tx_count, z=(3.0, 30.0))
+    sizes = np.floor(np.exp(rng.uniform(math.log(min_rx), math.log(max_rx), size=tx_count))).astype(np.int64)
+    rx = np.concatenate([street_points(rng, int(n)) for n in sizes], axis=0)
+    return tx, sizes, rx
