@@ -162,7 +162,7 @@ int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_r
  * consumer.evaluate). Pass 1 counts the valid paths of every cell; the
  * caller scans the counts in canonical (rx-major, tx-minor) pair order into
  * row_offset [cells]; pass 2 writes each cell's rows at row_offset[cell] in
- * discovery order; wc_paths_canonical_order then sorts every pair segment by
+ * discovery order, never more than the path_count [cells] of pass 1; wc_paths_canonical_order then sorts every pair segment by
  * (depth, component, primitive sequence, primitive, edge), stable, marks the
  * first strictly-shortest row of each identity group and returns the
  * permutation. Replaces the reference topology stages + canonical compact
@@ -192,8 +192,8 @@ typedef struct wc_path_rows {
     float* departure_direction;   /* [K, 3] */
 } wc_path_rows;
 
-int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const int* row_offset, const wc_path_rows* rows,
-                  void* stream);
+int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const int* row_offset, const int* path_count,
+                  const wc_path_rows* rows, void* stream);
 
 int wc_paths_canonical_order(const wc_path_rows* rows, const int* pair_offset, int pair_count, int* permutation,
                              unsigned char* keep, void* stream);

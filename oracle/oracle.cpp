@@ -760,11 +760,15 @@ void oracle_radiomap(const Scene* sp, const Solve* base, const double* rx_pos, i
         if (counts) counts[i] = c;
         if (paths) for (int j = 0; j < c && j < cap; ++j) paths[(size_t)i * cap + j] = buf[j];
         if (ambiguous) {
+            // perturb the receiver and the transmitter separately, both signs: a
+            // decision on a leg that ends at a fixed point (a clamped edge vertex)
+            // only moves with the transmitter
             int amb = 0;
-            for (int sgn = -1; sgn <= 1 && !amb; sgn += 2) {
+            for (int probe = 0; probe < 4 && !amb; ++probe) {
                 Solve qp = q;
-                const double dlt = sgn * perturb / std::sqrt(3.0);
-                qp.rx[0] += dlt; qp.rx[1] += dlt; qp.rx[2] += dlt;
+                const double dlt = (probe % 2 ? -1.0 : 1.0) * perturb / std::sqrt(3.0);
+                double* p = probe < 2 ? qp.rx : qp.tx;
+                p[0] += dlt; p[1] += dlt * 0.7; p[2] += dlt * 1.3;
                 std::vector<OraclePath> b2(buf.size() + 64);
                 int c2 = oracle_pair_paths(sp, &qp, b2.data(), (int)b2.size());
                 if (c2 > (int)b2.size()) { b2.resize(c2); c2 = oracle_pair_paths(sp, &qp, b2.data(), c2); }

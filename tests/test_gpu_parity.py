@@ -116,7 +116,8 @@ def test_reflection_sequence_known_answer_and_oracle(device):
         scale = abs(ref["coefficient"])
         if scale < 1e-12:
             continue
-        assert abs(coef[i] - ref["coefficient"]) <= 2e-4 * scale, i
+        # contract gates: |20 log10 |c/c_ref|| <= 0.01 dB and phase <= 1e-3 rad
+        assert abs(20.0 * math.log10(abs(coef[i]) / scale)) <= DB_TOL, i
         assert phase_error(coef[i], ref["coefficient"]) <= PHASE_TOL, i
 
 
@@ -205,7 +206,23 @@ def test_urban_canyon_radiomap_matches_oracle(device):
     p_gpu = run.out.power_total.cpu().numpy()[idx]
     stable = ~amb
     _say(f"ambiguous cells: {amb.sum()} / {len(idx)}")
-    assert np.array_equal(counts[stable], ref["counts"][stable])
+    bad = np.nonzero(stable & (counts != ref["counts"]))[0]
+    if bad.size:
+        rows = engine.export_rows(run.dscene, run.inputs, width=run.depth)
+        rx_id = rows["rx_id"].cpu().numpy()
+        for b in bad[:6]:
+            cell = int(idx[b])
+            sel = np.nonzero(rx_id == cell)[0]
+            _say(f"cell {cell} rx={run.compiled.endpoints.rx_position[cell]}: gpu {counts[b]} oracle "
+                 f"{ref['counts'][b]}")
+            for k in sel:
+                _say(f"  gpu comp={int(rows['component_id'][k])} d={int(rows['depth'][k])} "
+                     f"seq={rows['primitive_sequence'][k].tolist()} edge={int(rows['edge_id'][k])} "
+                     f"gain={float(rows['path_gain'][k]):.3e} len={float(rows['path_length_m'][k]):.6f}")
+            for p in ref["paths"][b]:
+                _say(f"  orc comp={p.component} d={p.depth} seq={list(p.prim)[:run.depth]} edge={p.edge} "
+                     f"gain={p.gain:.3e} len={p.length:.6f}")
+    assert bad.size == 0
     incoherent = np.array([sum(p.gain for p in cell) for cell in ref["paths"]])
     cond = rec[:, 0] / np.maximum(incoherent, 1e-300)
     err = db_error(p_gpu, rec[:, 0])

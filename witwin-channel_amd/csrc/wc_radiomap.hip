@@ -208,8 +208,10 @@ struct CountSink {
 struct EmitSink {
     const wc_path_rows* rows;
     int row;
+    int row_end;  // rows reserved for this cell by the count pass; never written past
     int tx, rx;
     __device__ void path(const PathRecord& r) {
+        if (row >= row_end) return;
         const int k = row++;
         const int w = rows->width;
         rows->tx_id[k] = tx;
@@ -332,12 +334,13 @@ __global__ void __launch_bounds__(256) count_kernel(SceneView s, wc_solve q, int
 }
 
 __global__ void __launch_bounds__(256) emit_kernel(SceneView s, wc_solve q, const int* __restrict__ row_offset,
-                                                   wc_path_rows rows) {
+                                                   const int* __restrict__ path_count, wc_path_rows rows) {
     for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
         const int t = tx_of_cell(q, cell);
         EmitSink e;
         e.rows = &rows;
         e.row = row_offset[cell];
+        e.row_end = e.row + path_count[cell];
         e.tx = t;
         e.rx = cell - q.cell_offset[t];
         eval_cell(s, q, t, cell, e);
@@ -562,15 +565,15 @@ extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int*
 }
 
 extern "C" int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const int* row_offset,
-                             const wc_path_rows* rows, void* stream) {
+                             const int* path_count, const wc_path_rows* rows, void* stream) {
     if (int rc = check_scene(scene)) return rc;
     if (int rc = check_solve(solve)) return rc;
-    if (!rows || !row_offset) return wc_fail("wc_paths_emit: rows / row_offset are null");
+    if (!rows || !row_offset || !path_count) return wc_fail("wc_paths_emit: rows / row_offset / path_count are null");
     if (rows->width < 1 || rows->width < solve->max_depth) return wc_fail("wc_paths_emit: width must be >= max_depth");
     if (solve->total_cells == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     hipLaunchKernelGGL(emit_kernel, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0, (hipStream_t)stream, s,
-                       *solve, row_offset, *rows);
+                       *solve, row_offset, path_count, *rows);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_paths_emit", err);
 }

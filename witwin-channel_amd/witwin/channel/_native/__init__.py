@@ -101,7 +101,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_radiomap_forward": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                     ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_void_p]),
     "wc_paths_count": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p]),
-    "wc_paths_emit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p,
+    "wc_paths_emit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p,
                               ctypes.POINTER(WcPathRows), c_void_p]),
     "wc_paths_canonical_order": (c_int, [ctypes.POINTER(WcPathRows), c_void_p, c_int, c_void_p, c_void_p,
                                          c_void_p]),

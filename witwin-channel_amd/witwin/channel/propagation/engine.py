@@ -322,8 +322,8 @@ def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int) -> dict
         "departure_direction")})
     row_offset = torch.as_tensor(starts.astype(np.int32)).to(device)
     if total:
-        _native.call("wc_paths_emit", dscene.ref(), ctypes.byref(inputs.view), p(row_offset), ctypes.byref(view),
-                     stream)
+        _native.call("wc_paths_emit", dscene.ref(), ctypes.byref(inputs.view), p(row_offset), p(counts),
+                     ctypes.byref(view), stream)
     pair_offset_h = np.concatenate([[0], np.cumsum(canon_counts)]).astype(np.int32)
     pair_offset = torch.as_tensor(pair_offset_h).to(device)
     perm = torch.empty(k, dtype=torch.int32, device=device)
@@ -332,8 +332,7 @@ def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int) -> dict
         _native.call("wc_paths_canonical_order", ctypes.byref(view), p(pair_offset), cells, p(perm), p(keep), stream)
     perm = perm[:total].long()
     keep_b = keep[:total].bool()
-    out = {name: value[:total][perm][keep_b] if name not in ("primitive_sequence",) else value[:total][perm][keep_b]
-           for name, value in rows.items()}
+    out = {name: value[:total][perm][keep_b] for name, value in rows.items()}
     out["pair_counts_canonical"] = canon_counts
     out["cell_order_canonical"] = canon
     return out
