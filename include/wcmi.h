@@ -157,6 +157,40 @@ typedef struct wc_radiomap_out {
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
 
+/* ------------------------------------------------------------ radiomap adjoint
+ * Fixed-topology VJP of wc_radiomap_forward. Replaces the reference adjoint
+ * chain _DeterministicAccumulateFlatAdFunction.backward ->
+ * _Field*AdFunction.backward -> RayD EPC backward
+ * (witwin/channel/kernels/deterministic.py:186-367, kernels/fields.py:1087-1827,
+ * native/channel/kernels/deterministic.cu:977-1117, fields.cu:1237,1873).
+ * Instead of tapes it recomputes every path stored in `topology` (from the
+ * forward call) with forward-mode dual numbers over the path's transmitter
+ * position, frequency and per-interaction material parameters, contracts
+ * with the cell adjoint, and writes one partial gradient vector per tile:
+ * partials [tile_count, wc_vjp_partial_stride(M)] =
+ *   [dL/deps_r[0..M), dL/dsigma[0..M), dL/df, dL/dtx_x, dL/dtx_y, dL/dtx_z]
+ * where the transmitter terms belong to the tile's transmitter. The upstream
+ * gradients may be NULL (zero). Materials are the first-layer view (M <= 16). */
+typedef struct wc_adjoint {
+    const float* grad_power_total;       /* [cells] */
+    const float* grad_field_total;       /* [cells] c64 */
+    const float* grad_component_power;   /* [4, cells] */
+    const float* grad_component_field;   /* [4, cells] c64 */
+    const float* grad_diffraction_power; /* [cells]: of sum_axis |diffraction_vector|^2 */
+    const float* power_total;            /* forward outputs */
+    const float* field_total;
+    const float* component_field;
+    const float* diffraction_vector;
+    const int* topology;
+    const int* path_count;
+    int topology_capacity;
+    float* partials;
+} wc_adjoint;
+
+int wc_vjp_partial_stride(int material_count);
+int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint, const int* tiles,
+                    int tile_count, void* stream);
+
 /* ------------------------------------------------------------ path rows
  * Explicit path export (deterministic export_paths, path.solve,
  * consumer.evaluate). Pass 1 counts the valid paths of every cell; the

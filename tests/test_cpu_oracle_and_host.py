@@ -281,6 +281,17 @@ def test_library_exports_every_declared_symbol():
     assert lib.wc_build_target() == b"gfx950"
 
 
+def test_value_returning_entry_points():
+    from witwin.channel import _native
+
+    assert _native.call("wc_vjp_partial_stride", 3) == 10
+    assert _native.call("wc_bvh_node_count", 5) == 4
+    assert _native.call("wc_bvh_node_count", 1) == 1
+    assert _native.call("wc_abi_version") == 1
+    assert _native.call("wc_bvh_workspace_bytes", 100) > 0
+    assert _native.call("wc_accumulate_workspace_bytes", 10, 2, 3) > 0
+
+
 def test_native_errors_raise_without_gpu_work():
     from witwin.channel import _native
 

@@ -17,6 +17,11 @@ namespace wc {
 enum ComponentBits : int { kCompLos = 1, kCompRefl = 2, kCompDiff = 4, kCompTrans = 8 };
 enum ComponentId : int { kCidLos = 0, kCidRefl = 1, kCidDiff = 2, kCidTrans = 5 };
 
+// accumulation slot of a component id: los 0, reflection 1, diffraction 2, transmission 3
+__device__ __forceinline__ int slot_of_component(int component) {
+    return component == kCidLos ? 0 : component == kCidRefl ? 1 : component == kCidDiff ? 2 : 3;
+}
+
 // minimum clearance of a visibility segment from its end points (m)
 constexpr float kSegmentBias = 1.0e-4f;
 // paths below this power are not emitted (diffraction existence floor)

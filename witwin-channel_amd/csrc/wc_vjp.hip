@@ -1,0 +1,334 @@
+// Fixed-topology adjoint of the fused radiomap.
+//
+// For every cell the forward pass stored its path descriptors (component,
+// candidate). The adjoint recomputes each stored path from the transmitter
+// position, the frequency and its per-interaction material parameters with
+// dual numbers (no visibility traversal: the topology is frozen; only a
+// transmission path re-gathers its walls), contracts the path tangents with
+// the cell adjoint of the coherent sum
+//   dL/dF_p = gP * 2 S + gS + gP_slot * 2 C_slot + gC_slot   (coherent)
+// and reduces the per-lane gradients over each 64-cell tile with a fixed
+// butterfly, writing one partial vector per tile. Summing the partials over
+// tiles (in tile order) gives dL/d(eps_r, sigma) per material, dL/df and
+// dL/d(tx position) per transmitter.
+#include "wc_api_internal.h"
+#include "wc_tangent.h"
+#include "../../include/wcmi.h"
+
+namespace wc {
+namespace {
+
+constexpr int kTangents = 8;
+constexpr int kMaxMaterials = 16;
+using D8 = Dual<kTangents>;
+
+__device__ __forceinline__ Cx<float> ld_c(const float* p, size_t i) { return {p[2 * i], p[2 * i + 1]}; }
+
+struct CellAdjoint {
+    float a_re[4], a_im[4];   // coherent per-slot field adjoint
+    float g_inc[4];           // incoherent: dL/dgain of each slot (gP + gP_slot)
+    float g_f_inc;            // incoherent: gF_re / sqrt(P)
+    Cx<float> b[3];           // adjoint of the diffraction vector sum (per axis)
+    bool coherent;
+};
+
+__device__ CellAdjoint cell_adjoint(const wc_adjoint& g, const wc_solve& q, int cell) {
+    const int cells = q.total_cells;
+    CellAdjoint a;
+    a.coherent = q.coherent != 0;
+    const float gp = g.grad_power_total ? g.grad_power_total[cell] : 0.0f;
+    const Cx<float> gf = g.grad_field_total ? ld_c(g.grad_field_total, cell) : Cx<float>{0.f, 0.f};
+    const Cx<float> S = ld_c(g.field_total, cell);
+    for (int s = 0; s < 4; ++s) {
+        const float gps = g.grad_component_power ? g.grad_component_power[(size_t)s * cells + cell] : 0.0f;
+        const Cx<float> gcf =
+            g.grad_component_field ? ld_c(g.grad_component_field, (size_t)s * cells + cell) : Cx<float>{0.f, 0.f};
+        const Cx<float> C = ld_c(g.component_field, (size_t)s * cells + cell);
+        a.a_re[s] = gp * 2.0f * S.re + gf.re + gps * 2.0f * C.re + gcf.re;
+        a.a_im[s] = gp * 2.0f * S.im + gf.im + gps * 2.0f * C.im + gcf.im;
+        a.g_inc[s] = gp + gps;
+    }
+    const float p = g.power_total ? g.power_total[cell] : 0.0f;
+    a.g_f_inc = p > 0.0f ? gf.re / sqrtf(p) : 0.0f;
+    const float gd = g.grad_diffraction_power ? g.grad_diffraction_power[cell] : 0.0f;
+    for (int j = 0; j < 3; ++j) {
+        const Cx<float> v = g.diffraction_vector ? ld_c(g.diffraction_vector, (size_t)3 * cell + j)
+                                                 : Cx<float>{0.f, 0.f};
+        a.b[j] = {gd * 2.0f * v.re, gd * 2.0f * v.im};
+    }
+    return a;
+}
+
+struct LaneGrad {
+    float eps[kMaxMaterials];
+    float sig[kMaxMaterials];
+    float freq;
+    float tx[3];
+};
+
+__device__ __forceinline__ void add_material(LaneGrad& g, int m, float de, float ds) {
+#pragma unroll
+    for (int i = 0; i < kMaxMaterials; ++i) {
+        if (i == m) {
+            g.eps[i] += de;
+            g.sig[i] += ds;
+        }
+    }
+}
+
+// Local parameter slots of one path: 0..2 tx xyz, 3 frequency, 4+2k eps of
+// interaction k, 5+2k sigma of interaction k.
+struct PathParams {
+    int count;               // 4 + 2 * interactions
+    int material[kMaxDepth];
+    float eps[kMaxDepth];
+    float sig[kMaxDepth];
+};
+
+__device__ __forceinline__ D8 seeded(float v, int slot, int pass) {
+    const int local = slot - pass * kTangents;
+    return seed<kTangents>(v, (local >= 0 && local < kTangents) ? local : -1);
+}
+
+// d(contribution)/d(slot) for the seeded slots of one pass
+__device__ __forceinline__ void contract(const C3<D8>& field, const V3<D8>& arrival, const V3<float>& rx_pol,
+                                         float sqrt_power, int slot_id, const CellAdjoint& adj, bool is_diff,
+                                         float out[kTangents]) {
+    const Cx<D8> coef = project_receiver(field, arrival, lift3<D8>(rx_pol));
+    float ar, ai;
+    if (adj.coherent) {
+        ar = adj.a_re[slot_id];
+        ai = adj.a_im[slot_id];
+    } else {
+        // gain = |F|^2 with F = coef * sqrt_power
+        const float fr = coef.re.v * sqrt_power, fi = coef.im.v * sqrt_power;
+        ar = adj.g_inc[slot_id] * 2.0f * fr + adj.g_f_inc * fr;
+        ai = adj.g_inc[slot_id] * 2.0f * fi + adj.g_f_inc * fi;
+    }
+#pragma unroll
+    for (int i = 0; i < kTangents; ++i) {
+        float v = (ar * coef.re.d[i] + ai * coef.im.d[i]) * sqrt_power;
+        if (is_diff) {
+            const Cx<D8> comps[3] = {field.x, field.y, field.z};
+            for (int j = 0; j < 3; ++j)
+                v += (adj.b[j].re * comps[j].re.d[i] + adj.b[j].im * comps[j].im.d[i]) * sqrt_power;
+        }
+        out[i] = v;
+    }
+}
+
+__device__ __forceinline__ void scatter(LaneGrad& g, const PathParams& pp, int pass, const float d[kTangents]) {
+#pragma unroll
+    for (int i = 0; i < kTangents; ++i) {
+        const int slot = pass * kTangents + i;
+        if (slot >= pp.count) break;
+        if (slot < 3) {
+            g.tx[slot] += d[i];
+        } else if (slot == 3) {
+            g.freq += d[i];
+        } else {
+            const int k = (slot - 4) >> 1;
+            if ((slot - 4) & 1) add_material(g, pp.material[k], 0.0f, d[i]);
+            else add_material(g, pp.material[k], d[i], 0.0f);
+        }
+    }
+}
+
+__device__ void adjoint_cell(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell,
+                             LaneGrad& g) {
+    const CellAdjoint adj = cell_adjoint(ga, q, cell);
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
+    const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
+    const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
+    const V3<float> rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
+    const float freq = q.frequency_hz;
+    const double k_d = (double)freq * kTwoPi / kC0;
+    const int count = min(ga.path_count[cell], ga.topology_capacity);
+    const int* topo = ga.topology + (size_t)cell * ga.topology_capacity;
+    Hit walls[kMaxHits];
+    int wall_count = -1;
+    for (int i = 0; i < count; ++i) {
+        const int desc = topo[i];
+        const int comp = desc >> 24;
+        int cand = desc & 0xFFFFFF;
+        if (cand == 0xFFFFFF) cand = -1;
+        PathParams pp;
+        pp.count = 4;
+        int groups[kMaxDepth];
+        int faces[kMaxDepth];
+        int depth = 0;
+        double length_d = 0.0;
+        if (comp == kCidRefl) {
+            const int* cg = q.chain_groups + cand * kMaxDepth;
+            const double* img = q.chain_images + cand * kMaxDepth * 3;
+            while (depth < kMaxDepth && cg[depth] >= 0) {
+                groups[depth] = cg[depth];
+                ++depth;
+            }
+            // interaction faces of the frozen chain (image back-projection only)
+            V3<double> next = to_d(rx);
+            for (int k = depth - 1; k >= 0; --k) {
+                double off;
+                const V3<double> n = group_normal(s, groups[k], off);
+                const V3<double> im{img[3 * k], img[3 * k + 1], img[3 * k + 2]};
+                const double si = dot(n, im) - off, sn = dot(n, next) - off;
+                const V3<double> h = im + (next - im) * (si / (si - sn));
+                int f = face_containing(s, groups[k], to_f(h), to_f(n));
+                faces[k] = f >= 0 ? f : s.group_faces[s.group_face_offset[groups[k]]];
+                next = h;
+            }
+            const V3<double> last{img[3 * (depth - 1)], img[3 * (depth - 1) + 1], img[3 * (depth - 1) + 2]};
+            length_d = norm(to_d(rx) - last);
+            for (int k = 0; k < depth; ++k) {
+                const int m = s.face_material[faces[k]];
+                pp.material[k] = m;
+                pp.eps[k] = s.mat_eps[m];
+                pp.sig[k] = s.mat_sigma[m];
+            }
+            pp.count = 4 + 2 * depth;
+        } else if (comp == kCidDiff) {
+            const Edge e = s.edges[cand];
+            DiffGeometry dg;
+            diffraction_geometry(e, tx, rx, dg);
+            length_d = dg.s + dg.sp;
+            const int mn = e.material1 >= 0 ? e.material1 : e.material0;
+            pp.material[0] = e.material0;
+            pp.material[1] = mn;
+            pp.eps[0] = s.mat_eps[e.material0];
+            pp.sig[0] = s.mat_sigma[e.material0];
+            pp.eps[1] = s.mat_eps[mn];
+            pp.sig[1] = s.mat_sigma[mn];
+            pp.count = 8;
+        } else if (comp == kCidTrans) {
+            if (wall_count < 0) {
+                const V3<float> seg = rx - tx;
+                const float len = norm(seg);
+                wall_count = 0;
+                if (len > 2.0f * kSegmentBias) {
+                    const float eps = kSegmentBias / len;
+                    const RayPre r = ray_setup(tx.x, tx.y, tx.z, seg.x, seg.y, seg.z);
+                    wall_count = gather_hits(s, r, eps, 1.0f - eps, q.max_depth, walls);
+                }
+            }
+            depth = min(wall_count, q.max_depth);
+            double carrier = norm(to_d(rx) - to_d(tx));
+            for (int w = 0; w < depth; ++w) {
+                double off;
+                const V3<double> nd = group_normal(s, walls[w].group, off);
+                const V3<double> dir = (to_d(rx) - to_d(tx)) * (1.0 / norm(to_d(rx) - to_d(tx)));
+                const double ct = fmin(fmax(fabs(dot(dir, nd)), 1e-6), 1.0);
+                const int m = s.face_material[walls[w].face];
+                double wall = 0.0;
+                for (int l = 0; l < s.mat_layer_count[m]; ++l)
+                    wall += fmax((double)s.layer_thickness[s.mat_layer_offset[m] + l], 0.0);
+                carrier -= wall * ct;
+                pp.material[w] = m;
+                pp.eps[w] = s.layer_eps[s.mat_layer_offset[m]];
+                pp.sig[w] = s.layer_sigma[s.mat_layer_offset[m]];
+            }
+            length_d = carrier;
+            pp.count = 4 + 2 * depth;
+        } else {
+            length_d = norm(to_d(rx) - to_d(tx));
+        }
+        const int slot_id = slot_of_component(comp);
+        const int passes = (pp.count + kTangents - 1) / kTangents;
+        for (int pass = 0; pass < passes; ++pass) {
+            const V3<D8> txd{seeded(tx.x, 0, pass), seeded(tx.y, 1, pass), seeded(tx.z, 2, pass)};
+            const D8 fd = seeded(freq, 3, pass);
+            D8 eps[kMaxDepth], sig[kMaxDepth];
+            const int nm = (pp.count - 4) / 2;
+            for (int k = 0; k < nm; ++k) {
+                eps[k] = seeded(pp.eps[k], 4 + 2 * k, pass);
+                sig[k] = seeded(pp.sig[k], 5 + 2 * k, pass);
+            }
+            V3<D8> arrival;
+            C3<D8> field;
+            if (comp == kCidLos) {
+                field = los_tangent(txd, rx, tx_pol, fd, k_d, length_d, arrival);
+            } else if (comp == kCidRefl) {
+                field = reflection_tangent(s, depth, groups, faces, txd, rx, tx_pol, fd, eps, sig, k_d, length_d,
+                                           arrival);
+            } else if (comp == kCidDiff) {
+                field = diffraction_tangent(s, s.edges[cand], txd, rx, tx_pol, fd, eps, sig, k_d, length_d, arrival);
+            } else {
+                field = transmission_tangent(s, depth, walls, txd, rx, tx_pol, fd, eps, sig, k_d, length_d, arrival);
+            }
+            float d[kTangents];
+            contract(field, arrival, rx_pol, sqrt_power, slot_id, adj, comp == kCidDiff, d);
+            scatter(g, pp, pass, d);
+        }
+    }
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles,
+                                                  int tile_count) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (tile >= tile_count) return;
+    const int t = tiles[2 * tile];
+    const int cell = tiles[2 * tile + 1] + lane;
+    LaneGrad g;
+#pragma unroll
+    for (int i = 0; i < kMaxMaterials; ++i) {
+        g.eps[i] = 0.0f;
+        g.sig[i] = 0.0f;
+    }
+    g.freq = 0.0f;
+    g.tx[0] = g.tx[1] = g.tx[2] = 0.0f;
+    if (cell < q.cell_offset[t + 1]) adjoint_cell(s, q, ga, t, cell, g);
+    const int m = s.material_count;
+    const int stride = 2 * m + 4;
+    float* out = ga.partials + (size_t)tile * stride;
+#pragma unroll
+    for (int i = 0; i < kMaxMaterials; ++i) {
+        const float se = wave_sum(g.eps[i]);
+        const float ss = wave_sum(g.sig[i]);
+        if (lane == 0 && i < m) {
+            out[i] = se;
+            out[m + i] = ss;
+        }
+    }
+    const float sf = wave_sum(g.freq);
+    const float sx = wave_sum(g.tx[0]), sy = wave_sum(g.tx[1]), sz = wave_sum(g.tx[2]);
+    if (lane == 0) {
+        out[2 * m] = sf;
+        out[2 * m + 1] = sx;
+        out[2 * m + 2] = sy;
+        out[2 * m + 3] = sz;
+    }
+}
+
+}  // namespace
+}  // namespace wc
+
+using namespace wc;
+
+extern "C" int wc_vjp_partial_stride(int material_count) { return 2 * material_count + 4; }
+
+extern "C" int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
+                               const int* tiles, int tile_count, void* stream) {
+    if (!scene || !solve || !adjoint) return wc_fail("wc_radiomap_vjp: null argument");
+    if (scene->material_count > kMaxMaterials)
+        return wc_fail("wc_radiomap_vjp: at most 16 materials are supported by the adjoint");
+    if (!adjoint->topology || !adjoint->path_count || adjoint->topology_capacity <= 0)
+        return wc_fail("wc_radiomap_vjp: the forward topology (descriptors and counts) is required");
+    if (!adjoint->field_total || !adjoint->component_field || !adjoint->partials)
+        return wc_fail("wc_radiomap_vjp: forward fields and the partials buffer are required");
+    if (!tiles || tile_count < 0) return wc_fail("wc_radiomap_vjp: tiles are required");
+    if (tile_count == 0) return 0;
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    const int blocks = (tile_count + 3) / 4;
+    hipLaunchKernelGGL(vjp_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve, *adjoint, tiles,
+                       tile_count);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_vjp", err);
+}

@@ -81,11 +81,9 @@ class WcPathRows(ctypes.Structure):
 class WcAdjoint(ctypes.Structure):
     _fields_ = [
         ("grad_power_total", c_void_p), ("grad_field_total", c_void_p), ("grad_component_power", c_void_p),
-        ("grad_component_field", c_void_p), ("grad_diffraction_power", c_void_p), ("field_total", c_void_p),
-        ("component_field", c_void_p), ("diffraction_vector", c_void_p), ("topology", c_void_p),
-        ("path_count", c_void_p), ("topology_capacity", c_int),
-        ("grad_eps", c_void_p), ("grad_sigma", c_void_p), ("grad_frequency", c_void_p),
-        ("grad_tx_pos", c_void_p), ("partials", c_void_p), ("partial_count", c_int),
+        ("grad_component_field", c_void_p), ("grad_diffraction_power", c_void_p), ("power_total", c_void_p),
+        ("field_total", c_void_p), ("component_field", c_void_p), ("diffraction_vector", c_void_p),
+        ("topology", c_void_p), ("path_count", c_void_p), ("topology_capacity", c_int), ("partials", c_void_p),
     ]
 
 
@@ -100,6 +98,9 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                              c_void_p, c_size_t, c_void_p]),
     "wc_radiomap_forward": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                     ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_void_p]),
+    "wc_radiomap_vjp": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
+                                c_void_p, c_int, c_void_p]),
+    "wc_vjp_partial_stride": (c_int, [c_int]),
     "wc_paths_count": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p]),
     "wc_paths_emit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p,
                               ctypes.POINTER(WcPathRows), c_void_p]),
@@ -119,6 +120,10 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_chain_images": (c_int, [ctypes.POINTER(WcScene), c_void_p, c_void_p, c_void_p, c_void_p, c_int,
                                 c_void_p, c_void_p]),
 }
+
+
+# int-returning entry points whose result is a value, not a status
+_VALUE_RETURNING = frozenset({"wc_abi_version", "wc_bvh_node_count", "wc_vjp_partial_stride"})
 
 
 def library_path() -> Path:
@@ -156,8 +161,7 @@ def call(name: str, *args: Any) -> Any:
     lib = load()
     fn = getattr(lib, name)
     status = fn(*args)
-    if fn.restype is c_int and name not in ("wc_abi_version", "wc_bvh_node_count", "wc_vjp_partial_count") \
-            and status != 0:
+    if fn.restype is c_int and name not in _VALUE_RETURNING and status != 0:
         message = lib.wc_last_error().decode("utf-8", "replace")
         raise RuntimeError(f"{name} failed: {message}")
     return status

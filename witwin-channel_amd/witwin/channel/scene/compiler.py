@@ -382,6 +382,13 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
     face_material = np.concatenate(mat_ids) if mat_ids else np.zeros(0, np.int32)
     face_surface = np.concatenate(surf_ids) if surf_ids else np.zeros(0, np.int64)
     materials = compile_materials(unique, frequency)
+    firsts = [mat.layer_rows()[0] for mat in unique]
+    ad_leaves = {
+        "eps_r": [l.eps_r if isinstance(l.eps_r, torch.Tensor) else None for l in firsts],
+        "sigma_e": [l.sigma_e if isinstance(l.sigma_e, torch.Tensor) else None for l in firsts],
+        "tx_position": [t.position for t in scene.transmitters],
+        "frequency": reference_frequency_hz if isinstance(reference_frequency_hz, torch.Tensor) else None,
+    }
     if faces.shape[0]:
         tri = vertices[faces]
         normals = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]).astype(np.float32)
@@ -395,12 +402,14 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
     policy = getattr(scene.metadata.get("imported_edge_policy"), "edge_selection_mode", "all") \
         if scene.metadata else "all"
     edges = build_edges(vertices, faces, face_group, face_material, policy)
-    return CompiledScene(
+    compiled = CompiledScene(
         vertices=vertices, faces=faces, face_material=face_material, face_surface=face_surface,
         face_group=face_group, group_plane=planes, group_face_offset=offsets, group_faces=members,
         group_material=group_material, representative_faces=representative, materials=materials,
         edges=edges, frequency_hz=frequency, endpoints=compile_endpoints(scene), edge_policy=policy,
     )
+    compiled.cache["ad_leaves"] = ad_leaves
+    return compiled
 
 
 # ---------------------------------------------------------------- device scene
