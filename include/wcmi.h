@@ -140,8 +140,11 @@ typedef struct wc_solve {
  *   topology [cells, topology_capacity] i32 path descriptors for the
  *   fixed-topology adjoint (may be NULL); failure [1] i32 gets bit 1 when a
  *   cell holds more paths than topology_capacity.
- * use_tiles: 1 = persistent launch over wave-aligned tiles of 64 cells from a
- * global work counter (tile_counter [1] i32, zeroed by the call). */
+ * tiles != NULL: persistent launches over wave-aligned tiles (tx, first cell)
+ * of 64 cells pulled from global work counters (tile_counter [2] i32, zeroed
+ * by the call with a fill kernel, so the sequence is graph-capturable).
+ * The evaluation runs as two launches, reflection/LoS/transmission then
+ * diffraction, each with its own register budget. */
 typedef struct wc_radiomap_out {
     float* power_total;
     float* field_total;
@@ -152,8 +155,11 @@ typedef struct wc_radiomap_out {
     int* topology;
     int topology_capacity;
     int* failure;
+    void* workspace; /* wc_radiomap_workspace_bytes(cells): per-cell sums carried
+                        from the reflection pass to the diffraction pass */
 } wc_radiomap_out;
 
+size_t wc_radiomap_workspace_bytes(int cells);
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
 
@@ -188,6 +194,11 @@ typedef struct wc_adjoint {
 } wc_adjoint;
 
 int wc_vjp_partial_stride(int material_count);
+/* Deterministic reduction of the tile partials (tile order, float64 sums):
+ * grad_params [2M+1] = [dL/deps_r, dL/dsigma, dL/df], grad_tx [tx_count, 3].
+ * tiles are the (transmitter, first cell) pairs of the launch. */
+int wc_vjp_reduce(const float* partials, int tile_count, int stride, const int* tiles, int tx_count,
+                  float* grad_params, float* grad_tx, void* stream);
 int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint, const int* tiles,
                     int tile_count, void* stream);
 

@@ -129,11 +129,14 @@ __device__ inline int face_containing(const SceneView& s, int g, const V3<float>
 // receiver through the images, requires each hit strictly between its image
 // and the next point and inside a face of its group, then tests every leg for
 // occlusion (legs skip the groups they start and end on). On success fills
-// hits/faces and returns true.
-__device__ inline bool solve_chain(const SceneView& s, int depth, const int* groups, const double* images,
-                                   const V3<float>& tx, const V3<float>& rx, V3<double>* hits, int* faces) {
+// hits/faces and returns true. The depth D is a compile-time constant so the
+// per-bounce arrays stay in registers.
+template <int D>
+__device__ inline bool solve_chain(const SceneView& s, const int* groups, const double* images, const V3<float>& tx,
+                                   const V3<float>& rx, V3<double>* hits, int* faces) {
     V3<double> next = to_d(rx);
-    for (int k = depth - 1; k >= 0; --k) {
+#pragma unroll
+    for (int k = D - 1; k >= 0; --k) {
         const int g = groups[k];
         double off;
         const V3<double> n = group_normal(s, g, off);
@@ -149,12 +152,13 @@ __device__ inline bool solve_chain(const SceneView& s, int depth, const int* gro
         faces[k] = f;
         next = h;
     }
-    // visibility of the d+1 legs
+    // visibility of the D+1 legs
     V3<float> a = tx;
     int ga = -1;
-    for (int k = 0; k <= depth; ++k) {
-        const V3<float> b = k < depth ? to_f(hits[k]) : rx;
-        const int gb = k < depth ? groups[k] : -1;
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+        const V3<float> b = k < D ? to_f(hits[k]) : rx;
+        const int gb = k < D ? groups[k] : -1;
         const V3<float> seg = b - a;
         const float len = norm(seg);
         if (len > 2.0f * kSegmentBias) {
@@ -171,17 +175,19 @@ __device__ inline bool solve_chain(const SceneView& s, int depth, const int* gro
 // Field of a reflection chain with the interaction points fixed: the Jones
 // transport of fields.cu:113-204 with amplitude 1/(2kL) and phase -kL, L the
 // double-precision unfolded length.
-__device__ inline C3<float> reflection_field(const SceneView& s, int depth, const V3<float>& tx,
-                                             const V3<float>& rx, const V3<float>* hits, const V3<float>* normals,
-                                             const int* mats, const V3<float>& tx_pol, float freq, double k_d,
-                                             double length_d, V3<float>& arrival, V3<float>& departure) {
+template <int D>
+__device__ inline C3<float> reflection_field(const SceneView& s, const V3<float>& tx, const V3<float>& rx,
+                                             const V3<float>* hits, const V3<float>* normals, const int* mats,
+                                             const V3<float>& tx_pol, float freq, double k_d, double length_d,
+                                             V3<float>& arrival, V3<float>& departure) {
     const V3<float> ez = v3c<float>(0.f, 0.f, 1.f);
     V3<float> prev = tx;
     V3<float> incident = safe_normalize(hits[0] - prev, ez);
     departure = incident;
     C3<float> value = c3_from_real(transverse(incident, tx_pol));
     V3<float> outgoing = incident;
-    for (int b = 0; b < depth; ++b) {
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
         incident = safe_normalize(hits[b] - prev, outgoing);
         const int m = mats[b];
         value = reflect_c3(value, incident, normals[b], s.mat_eps[m], s.mat_sigma[m], s.mat_mu[m], s.mat_gain[m],

@@ -18,9 +18,69 @@ __device__ __forceinline__ int slot_of(int component) {
     }
 }
 
+// Reflection chains of one compile-time depth D for one cell.
+template <int D, class Sink>
+__device__ __forceinline__ void reflect_depth(const SceneView& s, const wc_solve& q, int t, const V3<float>& tx,
+                                              const V3<float>& rx, const V3<float>& tx_pol, const V3<float>& rx_pol,
+                                              float sqrt_power, float freq, double k_d, Sink& sink) {
+    const int c0 = q.chain_offset[t * kMaxDepth + D - 1];
+    const int c1 = q.chain_offset[t * kMaxDepth + D];
+    for (int c = c0; c < c1; ++c) {
+        const int* groups = q.chain_groups + c * kMaxDepth;
+        const double* images = q.chain_images + c * kMaxDepth * 3;
+        V3<double> hd[D];
+        int faces[D];
+        if (!solve_chain<D>(s, groups, images, tx, rx, hd, faces)) continue;
+        PathRecord rec;
+        rec.component = kCidRefl;
+        rec.depth = D;
+        rec.candidate = c;
+        rec.edge = -1;
+        V3<float> hf[D], nf[D];
+        int mats[D];
+#pragma unroll
+        for (int b = 0; b < D; ++b) {
+            hf[b] = to_f(hd[b]);
+            double off;
+            nf[b] = to_f(group_normal(s, groups[b], off));
+            mats[b] = s.face_material[faces[b]];
+            if constexpr (Sink::kNeedsRows) {
+                rec.prim[b] = faces[b];
+                rec.mat[b] = mats[b];
+                rec.pos[b][0] = hf[b].x; rec.pos[b][1] = hf[b].y; rec.pos[b][2] = hf[b].z;
+                rec.nrm[b][0] = nf[b].x; rec.nrm[b][1] = nf[b].y; rec.nrm[b][2] = nf[b].z;
+            }
+        }
+        const V3<double> img{images[3 * (D - 1)], images[3 * (D - 1) + 1], images[3 * (D - 1) + 2]};
+        rec.length_d = norm(to_d(rx) - img);
+        rec.length = (float)rec.length_d;
+        V3<float> arrival, departure;
+        const C3<float> v = reflection_field<D>(s, tx, rx, hf, nf, mats, tx_pol, freq, k_d, rec.length_d, arrival,
+                                                departure);
+        finish_record(rec, v, arrival, rx_pol, sqrt_power);
+        rec.dep[0] = departure.x; rec.dep[1] = departure.y; rec.dep[2] = departure.z;
+        sink.path(rec);
+    }
+}
+
+__device__ __forceinline__ bool segment_clear(const SceneView& s, const V3<float>& a, const V3<float>& b, int ig0,
+                                              int ig1) {
+    const V3<float> d = b - a;
+    const float len = norm(d);
+    if (!(len > 2.0f * kSegmentBias)) return true;
+    const float eps = kSegmentBias / len;
+    const RayPre r = ray_setup(a.x, a.y, a.z, d.x, d.y, d.z);
+    return !occluded(s, r, eps, 1.0f - eps, ig0, ig1);
+}
+
+// Passes of the fused evaluation: all components (row export), the
+// non-diffraction components, or diffraction only. Splitting the radiomap
+// into two launches gives each its own register budget.
+enum EvalPass : int { kPassAll = 0, kPassMain = 1, kPassDiff = 2 };
+
 // Walks every candidate of one (transmitter, cell) pair in canonical order
 // and hands each valid path to the sink.
-template <class Sink>
+template <class Sink, int kPass>
 __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, Sink& sink) {
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
@@ -33,15 +93,16 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
     const float k = (float)k_d;
     const int comps = q.components;
     const int max_depth = q.max_depth;
+    constexpr bool kMain = kPass != kPassDiff;
+    constexpr bool kDiff = kPass != kPassMain;
 
     // --- straight segment: line of sight and ordered wall hits
     Hit hits[kMaxHits];
     int nh = 0;
     const V3<float> seg = rx - tx;
-    const V3<double> segd = to_d(rx) - to_d(tx);
-    const double dist_d = norm(segd);
+    const double dist_d = norm(to_d(rx) - to_d(tx));
     const V3<float> dir = safe_normalize(seg, v3c<float>(0.f, 0.f, 1.f));
-    if (comps & (kCompLos | kCompTrans)) {
+    if (kMain && (comps & (kCompLos | kCompTrans))) {
         const float len = (float)dist_d;
         if (len > 2.0f * kSegmentBias) {
             const float eps = kSegmentBias / len;
@@ -53,7 +114,7 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
             }
         }
     }
-    if ((comps & kCompLos) && nh == 0) {
+    if (kMain && (comps & kCompLos) && nh == 0) {
         PathRecord rec;
         rec.component = kCidLos;
         rec.depth = 0;
@@ -67,78 +128,38 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
         sink.path(rec);
     }
     for (int d = 1; d <= max_depth; ++d) {
-        if (comps & kCompRefl) {
-            const int c0 = q.chain_offset[t * kMaxDepth + d - 1];
-            const int c1 = q.chain_offset[t * kMaxDepth + d];
-            for (int c = c0; c < c1; ++c) {
-                const int* groups = q.chain_groups + c * kMaxDepth;
-                const double* images = q.chain_images + c * kMaxDepth * 3;
-                V3<double> hd[kMaxDepth];
-                int faces[kMaxDepth];
-                if (!solve_chain(s, d, groups, images, tx, rx, hd, faces)) continue;
-                PathRecord rec;
-                rec.component = kCidRefl;
-                rec.depth = d;
-                rec.candidate = c;
-                rec.edge = -1;
-                V3<float> hf[kMaxDepth], nf[kMaxDepth];
-                for (int b = 0; b < d; ++b) {
-                    hf[b] = to_f(hd[b]);
-                    double off;
-                    nf[b] = to_f(group_normal(s, groups[b], off));
-                    rec.prim[b] = faces[b];
-                    rec.mat[b] = s.face_material[faces[b]];
-                    rec.pos[b][0] = hf[b].x; rec.pos[b][1] = hf[b].y; rec.pos[b][2] = hf[b].z;
-                    rec.nrm[b][0] = nf[b].x; rec.nrm[b][1] = nf[b].y; rec.nrm[b][2] = nf[b].z;
-                }
-                const V3<double> img{images[3 * (d - 1)], images[3 * (d - 1) + 1], images[3 * (d - 1) + 2]};
-                rec.length_d = norm(to_d(rx) - img);
-                rec.length = (float)rec.length_d;
-                V3<float> arrival, departure;
-                const C3<float> v = reflection_field(s, d, tx, rx, hf, nf, rec.mat, tx_pol, freq, k_d,
-                                                     rec.length_d, arrival, departure);
-                finish_record(rec, v, arrival, rx_pol, sqrt_power);
-                rec.dep[0] = departure.x; rec.dep[1] = departure.y; rec.dep[2] = departure.z;
-                sink.path(rec);
+        if (kMain && (comps & kCompRefl)) {
+            switch (d) {
+                case 1: reflect_depth<1>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
+                case 2: reflect_depth<2>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
+                case 3: reflect_depth<3>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
+                case 4: reflect_depth<4>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
+                default: reflect_depth<5>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
             }
         }
-        if (d == 1 && (comps & kCompDiff)) {
+        if (kDiff && d == 1 && (comps & kCompDiff)) {
             for (int ei = 0; ei < s.edge_count; ++ei) {
                 const Edge e = s.edges[ei];
                 DiffGeometry g;
                 if (!diffraction_geometry(e, tx, rx, g)) continue;
-                // visibility at the clamped point of the finite edge
+                // visibility at the clamped point of the finite edge; the
+                // receiver leg is tested first (street-level receivers see
+                // fewer edges than an elevated transmitter)
                 const double tc = fmin(fmax(g.tstar, 0.0), (double)e.length);
                 const V3<float> qc{e.ax + e.dx * (float)tc, e.ay + e.dy * (float)tc, e.az + e.dz * (float)tc};
-                bool vis = true;
-                {
-                    const V3<float> a = qc - tx;
-                    const float len = norm(a);
-                    if (len > 2.0f * kSegmentBias) {
-                        const float eps = kSegmentBias / len;
-                        const RayPre r = ray_setup(tx.x, tx.y, tx.z, a.x, a.y, a.z);
-                        vis = !occluded(s, r, eps, 1.0f - eps, e.group0, e.group1);
-                    }
-                }
-                if (vis) {
-                    const V3<float> b = rx - qc;
-                    const float len = norm(b);
-                    if (len > 2.0f * kSegmentBias) {
-                        const float eps = kSegmentBias / len;
-                        const RayPre r = ray_setup(qc.x, qc.y, qc.z, b.x, b.y, b.z);
-                        vis = !occluded(s, r, eps, 1.0f - eps, e.group0, e.group1);
-                    }
-                }
-                if (!vis) continue;
+                if (!segment_clear(s, qc, rx, e.group0, e.group1)) continue;
+                if (!segment_clear(s, tx, qc, e.group0, e.group1)) continue;
                 PathRecord rec;
                 rec.component = kCidDiff;
                 rec.depth = 1;
                 rec.candidate = ei;
                 rec.edge = ei;
-                rec.prim[0] = ei;
-                rec.mat[0] = e.material0;
-                rec.pos[0][0] = (float)g.q.x; rec.pos[0][1] = (float)g.q.y; rec.pos[0][2] = (float)g.q.z;
-                rec.nrm[0][0] = e.n0x; rec.nrm[0][1] = e.n0y; rec.nrm[0][2] = e.n0z;
+                if constexpr (Sink::kNeedsRows) {
+                    rec.prim[0] = ei;
+                    rec.mat[0] = e.material0;
+                    rec.pos[0][0] = (float)g.q.x; rec.pos[0][1] = (float)g.q.y; rec.pos[0][2] = (float)g.q.z;
+                    rec.nrm[0][0] = e.n0x; rec.nrm[0][1] = e.n0y; rec.nrm[0][2] = e.n0z;
+                }
                 rec.length_d = g.s + g.sp;
                 rec.length = (float)rec.length_d;
                 V3<float> arrival, departure;
@@ -149,7 +170,7 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
                 sink.path(rec);
             }
         }
-        if ((comps & kCompTrans) && nh == d) {
+        if (kMain && (comps & kCompTrans) && nh == d) {
             PathRecord rec;
             rec.component = kCidTrans;
             rec.depth = d;
@@ -157,11 +178,14 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
             rec.edge = -1;
             V3<float> normals[kMaxHits];
             const C3<float> v = transmission_field(s, d, hits, dir, tx_pol, freq, k_d, dist_d, rec.mat, normals);
-            for (int w = 0; w < d; ++w) {
-                rec.prim[w] = hits[w].face;
-                const float tt = hits[w].t;
-                rec.pos[w][0] = tx.x + seg.x * tt; rec.pos[w][1] = tx.y + seg.y * tt; rec.pos[w][2] = tx.z + seg.z * tt;
-                rec.nrm[w][0] = normals[w].x; rec.nrm[w][1] = normals[w].y; rec.nrm[w][2] = normals[w].z;
+            if constexpr (Sink::kNeedsRows) {
+                for (int w = 0; w < d; ++w) {
+                    rec.prim[w] = hits[w].face;
+                    const float tt = hits[w].t;
+                    rec.pos[w][0] = tx.x + seg.x * tt; rec.pos[w][1] = tx.y + seg.y * tt;
+                    rec.pos[w][2] = tx.z + seg.z * tt;
+                    rec.nrm[w][0] = normals[w].x; rec.nrm[w][1] = normals[w].y; rec.nrm[w][2] = normals[w].z;
+                }
             }
             rec.length_d = dist_d;
             rec.length = (float)dist_d;
@@ -174,6 +198,7 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
 
 // ------------------------------------------------------------------ sinks
 struct AccumSink {
+    static constexpr bool kNeedsRows = false;
     double s_re = 0.0, s_im = 0.0, p_inc = 0.0;
     double c_re[4] = {0, 0, 0, 0}, c_im[4] = {0, 0, 0, 0}, c_pow[4] = {0, 0, 0, 0};
     double dv[6] = {0, 0, 0, 0, 0, 0};
@@ -201,11 +226,21 @@ struct AccumSink {
 };
 
 struct CountSink {
+    static constexpr bool kNeedsRows = false;
     int count = 0;
     __device__ void path(const PathRecord&) { ++count; }
 };
 
+// Per-cell state carried from the main pass to the diffraction pass.
+struct CellPartial {
+    double s_re, s_im, p_inc;
+    int count;
+    int pad;
+};
+static_assert(sizeof(CellPartial) == 32, "CellPartial layout");
+
 struct EmitSink {
+    static constexpr bool kNeedsRows = true;
     const wc_path_rows* rows;
     int row;
     int row_end;  // rows reserved for this cell by the count pass; never written past
@@ -258,9 +293,17 @@ __device__ __forceinline__ int tx_of_cell(const wc_solve& q, int cell) {
     return lo;
 }
 
-__device__ __forceinline__ void write_cell(const wc_solve& q, const wc_radiomap_out& o, int cell,
+__device__ __forceinline__ void write_slot(const wc_solve& q, const wc_radiomap_out& o, int cell, int sl,
                                            const AccumSink& a) {
     const int cells = q.total_cells;
+    const double cp = q.coherent ? a.c_re[sl] * a.c_re[sl] + a.c_im[sl] * a.c_im[sl] : a.c_pow[sl];
+    o.component_power[sl * cells + cell] = (float)cp;
+    o.component_field[2 * (sl * cells + cell)] = (float)a.c_re[sl];
+    o.component_field[2 * (sl * cells + cell) + 1] = (float)a.c_im[sl];
+}
+
+__device__ __forceinline__ void write_totals(const wc_solve& q, const wc_radiomap_out& o, int cell,
+                                             const AccumSink& a) {
     double pt, fre, fim;
     if (q.coherent) {
         fre = a.s_re;
@@ -274,12 +317,6 @@ __device__ __forceinline__ void write_cell(const wc_solve& q, const wc_radiomap_
     o.power_total[cell] = (float)pt;
     o.field_total[2 * cell] = (float)fre;
     o.field_total[2 * cell + 1] = (float)fim;
-    for (int sl = 0; sl < 4; ++sl) {
-        const double cp = q.coherent ? a.c_re[sl] * a.c_re[sl] + a.c_im[sl] * a.c_im[sl] : a.c_pow[sl];
-        o.component_power[sl * cells + cell] = (float)cp;
-        o.component_field[2 * (sl * cells + cell)] = (float)a.c_re[sl];
-        o.component_field[2 * (sl * cells + cell) + 1] = (float)a.c_im[sl];
-    }
     if (o.diffraction_vector) {
         for (int j = 0; j < 6; ++j) o.diffraction_vector[6 * cell + j] = (float)a.dv[j];
     }
@@ -287,28 +324,56 @@ __device__ __forceinline__ void write_cell(const wc_solve& q, const wc_radiomap_
     if (o.topology && a.count > o.topology_capacity && o.failure) atomicOr(o.failure, 2);
 }
 
+// One pass of the radiomap for one cell. kPassMain writes the slots of every
+// non-diffraction component and either the totals (no diffraction requested)
+// or the partial sums for kPassDiff, which adds the diffraction slot and
+// writes the totals.
+template <int kPass>
 __device__ __forceinline__ void radiomap_cell(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o,
                                               int t, int cell) {
     AccumSink a;
     a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    CellPartial* partial = reinterpret_cast<CellPartial*>(o.workspace);
+    if (kPass == kPassDiff) {
+        const CellPartial p = partial[cell];
+        a.s_re = p.s_re;
+        a.s_im = p.s_im;
+        a.p_inc = p.p_inc;
+        a.count = p.count;
+    }
     if (o.topology) {
         a.topo = o.topology + (size_t)cell * o.topology_capacity;
         a.cap = o.topology_capacity;
     }
-    eval_cell(s, q, t, cell, a);
-    write_cell(q, o, cell, a);
+    eval_cell<AccumSink, kPass>(s, q, t, cell, a);
+    const bool diffraction = (q.components & kCompDiff) != 0;
+    if (kPass == kPassMain) {
+        write_slot(q, o, cell, 0, a);
+        write_slot(q, o, cell, 1, a);
+        write_slot(q, o, cell, 3, a);
+        if (diffraction) {
+            partial[cell] = CellPartial{a.s_re, a.s_im, a.p_inc, a.count, 0};
+            return;
+        }
+        write_slot(q, o, cell, 2, a);
+    } else {
+        write_slot(q, o, cell, 2, a);
+    }
+    write_totals(q, o, cell, a);
 }
 
+template <int kPass>
 __global__ void __launch_bounds__(256) radiomap_kernel(SceneView s, wc_solve q, wc_radiomap_out o) {
     for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
         const int t = tx_of_cell(q, cell);
-        radiomap_cell(s, q, o, t, cell);
+        radiomap_cell<kPass>(s, q, o, t, cell);
     }
 }
 
 // Persistent variant: wavefronts pull wave-aligned tiles (tx, first cell) of
 // 64 cells from a global counter, so a wavefront never mixes transmitters and
 // uneven receiver sets balance across the machine.
+template <int kPass>
 __global__ void __launch_bounds__(256) radiomap_tiles_kernel(SceneView s, wc_solve q, wc_radiomap_out o,
                                                              const int* __restrict__ tiles, int tile_count,
                                                              int* __restrict__ counter) {
@@ -320,7 +385,7 @@ __global__ void __launch_bounds__(256) radiomap_tiles_kernel(SceneView s, wc_sol
         if (tile >= tile_count) break;
         const int t = tiles[2 * tile];
         const int cell = tiles[2 * tile + 1] + lane;
-        if (cell < q.cell_offset[t + 1]) radiomap_cell(s, q, o, t, cell);
+        if (cell < q.cell_offset[t + 1]) radiomap_cell<kPass>(s, q, o, t, cell);
     }
 }
 
@@ -328,7 +393,7 @@ __global__ void __launch_bounds__(256) count_kernel(SceneView s, wc_solve q, int
     for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
         const int t = tx_of_cell(q, cell);
         CountSink c;
-        eval_cell(s, q, t, cell, c);
+        eval_cell<CountSink, kPassAll>(s, q, t, cell, c);
         out[cell] = c.count;
     }
 }
@@ -343,7 +408,7 @@ __global__ void __launch_bounds__(256) emit_kernel(SceneView s, wc_solve q, cons
         e.row_end = e.row + path_count[cell];
         e.tx = t;
         e.rx = cell - q.cell_offset[t];
-        eval_cell(s, q, t, cell, e);
+        eval_cell<EmitSink, kPassAll>(s, q, t, cell, e);
     }
 }
 
@@ -531,26 +596,43 @@ extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve,
     if (solve->total_cells == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     hipStream_t st = (hipStream_t)stream;
+    const bool diffraction = (solve->components & kCompDiff) != 0;
+    if (diffraction && !out->workspace)
+        return wc_fail("wc_radiomap_forward: workspace (wc_radiomap_workspace_bytes) is required with diffraction");
     if (tiles) {
         if (!tile_counter || tile_count <= 0) return wc_fail("tiled launch needs tile_counter and tile_count");
-        // a fill kernel rather than a memset node keeps the launch graph-capturable
-        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 1);
+        // compute-unit count of the current device, queried once (safe during stream capture)
+        static thread_local int cached_device = -1;
+        static thread_local int cached_cus = 256;
         int device = 0;
         (void)hipGetDevice(&device);
-        int cus = 256;
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-        const int waves_total = tile_count;
-        int blocks = std::min((waves_total + 3) / 4, cus * 8);
+        if (device != cached_device) {
+            int cus_query = 256;
+            if (hipDeviceGetAttribute(&cus_query, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
+                cached_cus = cus_query;
+            cached_device = device;
+        }
+        int blocks = std::min((tile_count + 3) / 4, cached_cus * 8);
         blocks = std::max(blocks, 1);
-        hipLaunchKernelGGL(radiomap_tiles_kernel, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles, tile_count,
-                           tile_counter);
+        // a fill kernel rather than a memset node keeps the launches graph-capturable
+        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 2);
+        hipLaunchKernelGGL(radiomap_tiles_kernel<kPassMain>, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
+                           tile_count, tile_counter);
+        if (diffraction)
+            hipLaunchKernelGGL(radiomap_tiles_kernel<kPassDiff>, dim3(blocks), dim3(256), 0, st, s, *solve, *out,
+                               tiles, tile_count, tile_counter + 1);
     } else {
-        hipLaunchKernelGGL(radiomap_kernel, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0, st, s, *solve,
-                           *out);
+        hipLaunchKernelGGL(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0, st,
+                           s, *solve, *out);
+        if (diffraction)
+            hipLaunchKernelGGL(radiomap_kernel<kPassDiff>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
+                               st, s, *solve, *out);
     }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_forward", err);
 }
+
+extern "C" size_t wc_radiomap_workspace_bytes(int cells) { return (size_t)(cells > 0 ? cells : 0) * 32; }
 
 extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream) {
     if (int rc = check_scene(scene)) return rc;

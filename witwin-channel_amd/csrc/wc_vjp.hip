@@ -307,10 +307,44 @@ __global__ void __launch_bounds__(256) vjp_kernel(SceneView s, wc_solve q, wc_ad
     }
 }
 
+// Column sums of the tile partials in tile order (deterministic): thread j < 2M+1
+// sums the material and frequency columns over all tiles; the transmitter
+// columns are summed over the contiguous tile range of each transmitter.
+__global__ void vjp_reduce_kernel(const float* __restrict__ partials, int tile_count, int stride,
+                                  const int* __restrict__ tiles, int tx_count, float* __restrict__ grad_params,
+                                  float* __restrict__ grad_tx) {
+    const int m2 = stride - 3;  // 2M + 1 shared columns
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m2) {
+        double acc = 0.0;
+        for (int t = 0; t < tile_count; ++t) acc += (double)partials[(size_t)t * stride + j];
+        grad_params[j] = (float)acc;
+    } else if (j < m2 + 3 * tx_count) {
+        const int k = j - m2;
+        const int tx = k / 3, axis = k % 3;
+        double acc = 0.0;
+        for (int t = 0; t < tile_count; ++t) {
+            if (tiles[2 * t] == tx) acc += (double)partials[(size_t)t * stride + m2 + axis];
+        }
+        grad_tx[k] = (float)acc;
+    }
+}
+
 }  // namespace
 }  // namespace wc
 
 using namespace wc;
+
+extern "C" int wc_vjp_reduce(const float* partials, int tile_count, int stride, const int* tiles, int tx_count,
+                             float* grad_params, float* grad_tx, void* stream) {
+    if (stride < 4 || tile_count < 0 || tx_count < 0) return wc_fail("wc_vjp_reduce: invalid sizes");
+    if (!partials || !tiles || !grad_params || !grad_tx) return wc_fail("wc_vjp_reduce: null argument");
+    const int threads = (stride - 3) + 3 * tx_count;
+    hipLaunchKernelGGL(vjp_reduce_kernel, dim3((threads + 255) / 256), dim3(256), 0, (hipStream_t)stream, partials,
+                       tile_count, stride, tiles, tx_count, grad_params, grad_tx);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_vjp_reduce", err);
+}
 
 extern "C" int wc_vjp_partial_stride(int material_count) { return 2 * material_count + 4; }
 

@@ -66,7 +66,7 @@ class WcRadiomapOut(ctypes.Structure):
     _fields_ = [
         ("power_total", c_void_p), ("field_total", c_void_p), ("component_power", c_void_p),
         ("component_field", c_void_p), ("diffraction_vector", c_void_p), ("path_count", c_void_p),
-        ("topology", c_void_p), ("topology_capacity", c_int), ("failure", c_void_p),
+        ("topology", c_void_p), ("topology_capacity", c_int), ("failure", c_void_p), ("workspace", c_void_p),
     ]
 
 
@@ -96,11 +96,13 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_bvh_node_count": (c_int, [c_int]),
     "wc_bvh_build": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_size_t, c_void_p]),
+    "wc_radiomap_workspace_bytes": (c_size_t, [c_int]),
     "wc_radiomap_forward": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                     ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_void_p]),
     "wc_radiomap_vjp": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
                                 c_void_p, c_int, c_void_p]),
     "wc_vjp_partial_stride": (c_int, [c_int]),
+    "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "wc_paths_count": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p]),
     "wc_paths_emit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p,
                               ctypes.POINTER(WcPathRows), c_void_p]),

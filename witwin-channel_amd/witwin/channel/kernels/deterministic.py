@@ -43,8 +43,12 @@ class RadiomapFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, eps_r, sigma_e, frequency, tx_position, state: RadiomapState):  # type: ignore[override]
-        state.out = engine.radiomap_forward(state.dscene, state.inputs, topology_capacity=state.topology_capacity,
-                                            out=state.out)
+        if state.out is None:
+            state.out = engine.forward_with_topology(state.dscene, state.inputs, capacity=state.topology_capacity)
+            state.topology_capacity = state.out.topology_capacity
+        else:
+            state.out = engine.radiomap_forward(state.dscene, state.inputs,
+                                                topology_capacity=state.topology_capacity, out=state.out)
         out = state.out
         diff_power = out.diffraction_vector.abs().square().sum(dim=-1)
         ctx.state = state
@@ -80,16 +84,15 @@ class RadiomapFunction(torch.autograd.Function):
             diffraction_vector=p(out.diffraction_vector), topology=p(out.topology), path_count=p(out.path_count),
             topology_capacity=out.topology_capacity, partials=p(state.partials),
         )
+        stream = _native.stream_handle(device)
         _native.call("wc_radiomap_vjp", state.dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(view),
-                     p(inputs.tiles), tiles, _native.stream_handle(device))
-        part = state.partials
-        grad_eps = part[:, :m].sum(dim=0)
-        grad_sigma = part[:, m:2 * m].sum(dim=0)
-        grad_freq = part[:, 2 * m].sum()
-        tile_tx = inputs.tiles[:tiles, 0].long()
-        grad_tx = torch.zeros((int(inputs.view.tx_count), 3), dtype=torch.float32, device=device)
-        grad_tx.index_add_(0, tile_tx, part[:, 2 * m + 1:2 * m + 4])
-        return grad_eps, grad_sigma, grad_freq, grad_tx, None
+                     p(inputs.tiles), tiles, stream)
+        t = int(inputs.view.tx_count)
+        grad_params = torch.empty(2 * m + 1, dtype=torch.float32, device=device)
+        grad_tx = torch.empty((t, 3), dtype=torch.float32, device=device)
+        _native.call("wc_vjp_reduce", p(state.partials), tiles, stride, p(inputs.tiles), t, p(grad_params),
+                     p(grad_tx), stream)
+        return grad_params[:m], grad_params[m:2 * m], grad_params[2 * m], grad_tx, None
 
 
 def radiomap_vjp_outputs(dscene, inputs: engine.SolveInputs, *, eps_r: torch.Tensor, sigma_e: torch.Tensor,
@@ -136,3 +139,75 @@ def radiomap_with_vjp(compiled, dscene, inputs, *, material_params: Any = None, 
                                        material_count=m.count)
     power, field, comp_power, comp_field, diff_power = outs
     return state.out, (power, field, comp_power, diff_power)
+
+
+class MeanDbStep:
+    """One forward + VJP step of L = mean over lit cells of 10 log10(P_cell),
+    written with launches only (no host synchronization, no allocation after
+    the first call) so it can be captured once in a HIP graph and replayed.
+
+    dL/dP_c = 10 / (ln 10 * N * P_c) on the N cells lit in the first forward
+    (the fixed topology keeps that set). Produces ``grad_params`` =
+    [dL/deps_r (M), dL/dsigma (M), dL/df] and ``grad_tx`` [T, 3].
+    """
+
+    def __init__(self, dscene, inputs: engine.SolveInputs, material_count: int,
+                 topology_capacity: int = DEFAULT_TOPOLOGY_CAPACITY) -> None:
+        self.dscene = dscene
+        self.inputs = inputs
+        self.m = material_count
+        self.capacity = topology_capacity
+        device = dscene.device
+        self.out = engine.forward_with_topology(dscene, inputs, capacity=topology_capacity)
+        self.capacity = self.out.topology_capacity
+        power = self.out.power_total
+        lit = power > 0
+        count = max(int(lit.sum().item()), 1)
+        self.scale = torch.where(lit, torch.full_like(power, 10.0 / (2.302585092994046 * count)),
+                                 torch.zeros_like(power))
+        self.grad_power = torch.empty_like(power)
+        self.stride = int(_native.call("wc_vjp_partial_stride", material_count))
+        self.partials = torch.empty((inputs.tile_count, self.stride), dtype=torch.float32, device=device)
+        self.grad_params = torch.empty(2 * material_count + 1, dtype=torch.float32, device=device)
+        self.grad_tx = torch.empty((int(inputs.view.tx_count), 3), dtype=torch.float32, device=device)
+        self.loss = torch.empty((), dtype=torch.float32, device=device)
+        p = _native.ptr
+        self.view = _native.WcAdjoint(
+            grad_power_total=p(self.grad_power), grad_field_total=None, grad_component_power=None,
+            grad_component_field=None, grad_diffraction_power=None, power_total=p(self.out.power_total),
+            field_total=p(self.out.field_total), component_field=p(self.out.component_field),
+            diffraction_vector=p(self.out.diffraction_vector), topology=p(self.out.topology),
+            path_count=p(self.out.path_count), topology_capacity=self.out.topology_capacity,
+            partials=p(self.partials),
+        )
+        self.graph: torch.cuda.CUDAGraph | None = None
+
+    def launch(self) -> None:
+        inputs = self.inputs
+        engine.radiomap_forward(self.dscene, inputs, topology_capacity=self.capacity, out=self.out)
+        power = self.out.power_total
+        torch.div(self.scale, torch.clamp_min(power, 1.0e-38), out=self.grad_power)
+        p = _native.ptr
+        stream = _native.stream_handle(self.dscene.device)
+        _native.call("wc_radiomap_vjp", self.dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(self.view),
+                     p(inputs.tiles), inputs.tile_count, stream)
+        _native.call("wc_vjp_reduce", p(self.partials), inputs.tile_count, self.stride, p(inputs.tiles),
+                     int(inputs.view.tx_count), p(self.grad_params), p(self.grad_tx), stream)
+
+    def capture(self) -> None:
+        side = torch.cuda.Stream(self.dscene.device)
+        side.wait_stream(torch.cuda.current_stream(self.dscene.device))
+        with torch.cuda.stream(side):
+            self.launch()
+        torch.cuda.current_stream(self.dscene.device).wait_stream(side)
+        torch.cuda.synchronize(self.dscene.device)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.launch()
+        torch.cuda.synchronize(self.dscene.device)
+
+    def step(self) -> None:
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.launch()

@@ -240,6 +240,7 @@ class RadiomapOutputs:
     topology: torch.Tensor | None
     topology_capacity: int
     failure: torch.Tensor
+    workspace: torch.Tensor          # per-cell partial sums between the two radiomap passes
 
 
 def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capacity: int = 0,
@@ -258,19 +259,40 @@ def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capac
                       if topology_capacity > 0 else None),
             topology_capacity=topology_capacity,
             failure=torch.zeros(1, dtype=torch.int32, device=device),
+            workspace=torch.empty(max(int(_native.call("wc_radiomap_workspace_bytes", cells)), 32),
+                                  dtype=torch.uint8, device=device),
         )
     p = _native.ptr
     view = _native.WcRadiomapOut(
         power_total=p(out.power_total), field_total=p(out.field_total), component_power=p(out.component_power),
         component_field=p(out.component_field), diffraction_vector=p(out.diffraction_vector),
         path_count=p(out.path_count), topology=p(out.topology), topology_capacity=out.topology_capacity,
-        failure=p(out.failure),
+        failure=p(out.failure), workspace=p(out.workspace),
     )
-    counter = inputs.__dict__.setdefault("_tile_counter", torch.zeros(1, dtype=torch.int32, device=device))
+    counter = inputs.__dict__.get("_tile_counter")
+    if counter is None:
+        counter = torch.zeros(2, dtype=torch.int32, device=device)
+        inputs.__dict__["_tile_counter"] = counter
     import ctypes
     _native.call("wc_radiomap_forward", dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(view),
                  p(inputs.tiles) if tiled else None, inputs.tile_count if tiled else 0,
                  p(counter) if tiled else None, _native.stream_handle(device))
+    return out
+
+
+def forward_with_topology(dscene: DeviceScene, inputs: SolveInputs, *, capacity: int = 64,
+                          tiled: bool = True) -> RadiomapOutputs:
+    """Forward pass that stores every cell's path descriptors. When a cell
+    holds more paths than ``capacity`` (failure bit 2) the descriptor buffer is
+    regrown to the next power of two above the largest cell count and the
+    forward is rerun, so the adjoint always sees the complete topology."""
+    out = radiomap_forward(dscene, inputs, topology_capacity=capacity, tiled=tiled)
+    if int(out.failure.item()) & 2:
+        worst = int(out.path_count.max().item())
+        grown = 1 << max(worst - 1, 1).bit_length()
+        out = radiomap_forward(dscene, inputs, topology_capacity=grown, tiled=tiled)
+        if int(out.failure.item()) & 2:
+            raise RuntimeError("path descriptor capacity still exceeded after regrowth")
     return out
 
 

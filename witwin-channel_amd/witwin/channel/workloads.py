@@ -118,8 +118,14 @@ def manhattan_structures(seed: int = 2, blocks: int = 8, footprint: float = 30.0
     return structures, pitch
 
 
+# street intersection (x, y in [45, 60] with the 15 m first street) offset from
+# its centre so no transmitter->box-corner line passes exactly through another
+# corner of the regular block lattice
+URBAN_CANYON_TX = (53.231, 52.013, 25.0)
+
+
 def urban_canyon_scene(seed: int = 2, shape: tuple[int, int] = (256, 256), spacing: float = 2.0,
-                       tx_position=(22.5, 22.5, 25.0)) -> Scene:
+                       tx_position=URBAN_CANYON_TX) -> Scene:
     structures, pitch = manhattan_structures(seed)
     extent = 15.0 + 8 * pitch
     ground = PhysicalMaterial(thickness_m=0.2, name="ground-concrete", **CONCRETE)
