@@ -1,7 +1,8 @@
 """Radiomap forward+VJP throughput on one MI355X.
 
 Workload (the value line): seeded urban canyon (8 x 8 box buildings, 770
-triangles, 3 materials incl. ground), one transmitter at (22.5, 22.5, 25) m,
+triangles, 3 materials incl. ground), one transmitter at a street
+intersection (53.231, 52.013, 25) m,
 28 GHz, up to 3 specular bounces + first-order UTD diffraction + line of sight,
 256 x 256 receiver grid at 2 m, z = 1.5 m. One step = radiomap forward (fused
 discovery + fields + float64 coherent cell sums, path descriptors stored) +
