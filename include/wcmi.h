@@ -120,6 +120,8 @@ typedef struct wc_solve {
     int components;  /* bit 0 los, 1 reflection, 2 diffraction, 3 transmission */
     int max_depth;
     int coherent;
+    int lanes_per_cell; /* L: lanes sharing one cell's candidates (power of two, 1..64);
+                           tiles then hold 64 / L cells */
 } wc_solve;
 
 /* ------------------------------------------------------------ radiomap
@@ -155,11 +157,11 @@ typedef struct wc_radiomap_out {
     int* topology;
     int topology_capacity;
     int* failure;
-    void* workspace; /* wc_radiomap_workspace_bytes(cells): per-cell sums carried
+    void* workspace; /* wc_radiomap_workspace_bytes(cells, L): per-lane sums carried
                         from the reflection pass to the diffraction pass */
 } wc_radiomap_out;
 
-size_t wc_radiomap_workspace_bytes(int cells);
+size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell);
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
 

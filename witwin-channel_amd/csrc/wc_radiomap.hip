@@ -22,10 +22,11 @@ __device__ __forceinline__ int slot_of(int component) {
 template <int D, class Sink>
 __device__ __forceinline__ void reflect_depth(const SceneView& s, const wc_solve& q, int t, const V3<float>& tx,
                                               const V3<float>& rx, const V3<float>& tx_pol, const V3<float>& rx_pol,
-                                              float sqrt_power, float freq, double k_d, Sink& sink) {
+                                              float sqrt_power, float freq, double k_d, int sub, int stride,
+                                              Sink& sink) {
     const int c0 = q.chain_offset[t * kMaxDepth + D - 1];
     const int c1 = q.chain_offset[t * kMaxDepth + D];
-    for (int c = c0; c < c1; ++c) {
+    for (int c = c0 + sub; c < c1; c += stride) {
         const int* groups = q.chain_groups + c * kMaxDepth;
         const double* images = q.chain_images + c * kMaxDepth * 3;
         V3<double> hd[D];
@@ -78,10 +79,13 @@ __device__ __forceinline__ bool segment_clear(const SceneView& s, const V3<float
 // into two launches gives each its own register budget.
 enum EvalPass : int { kPassAll = 0, kPassMain = 1, kPassDiff = 2 };
 
-// Walks every candidate of one (transmitter, cell) pair in canonical order
-// and hands each valid path to the sink.
+// Walks the candidates of one (transmitter, cell) pair and hands each valid
+// path to the sink. With stride 1 (sub 0) every candidate is visited in
+// canonical order; with stride L the lane `sub` of the L lanes sharing a cell
+// visits every L-th reflection chain and diffraction edge (line of sight and
+// transmission belong to sub-lane 0).
 template <class Sink, int kPass>
-__device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, Sink& sink) {
+__device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, int sub, int stride, Sink& sink) {
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
     const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
@@ -102,7 +106,7 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
     const V3<float> seg = rx - tx;
     const double dist_d = norm(to_d(rx) - to_d(tx));
     const V3<float> dir = safe_normalize(seg, v3c<float>(0.f, 0.f, 1.f));
-    if (kMain && (comps & (kCompLos | kCompTrans))) {
+    if (kMain && sub == 0 && (comps & (kCompLos | kCompTrans))) {
         const float len = (float)dist_d;
         if (len > 2.0f * kSegmentBias) {
             const float eps = kSegmentBias / len;
@@ -114,7 +118,7 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
             }
         }
     }
-    if (kMain && (comps & kCompLos) && nh == 0) {
+    if (kMain && sub == 0 && (comps & kCompLos) && nh == 0) {
         PathRecord rec;
         rec.component = kCidLos;
         rec.depth = 0;
@@ -130,15 +134,15 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
     for (int d = 1; d <= max_depth; ++d) {
         if (kMain && (comps & kCompRefl)) {
             switch (d) {
-                case 1: reflect_depth<1>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
-                case 2: reflect_depth<2>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
-                case 3: reflect_depth<3>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
-                case 4: reflect_depth<4>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
-                default: reflect_depth<5>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sink); break;
+                case 1: reflect_depth<1>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
+                case 2: reflect_depth<2>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
+                case 3: reflect_depth<3>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
+                case 4: reflect_depth<4>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
+                default: reflect_depth<5>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
             }
         }
         if (kDiff && d == 1 && (comps & kCompDiff)) {
-            for (int ei = 0; ei < s.edge_count; ++ei) {
+            for (int ei = sub; ei < s.edge_count; ei += stride) {
                 const Edge e = s.edges[ei];
                 DiffGeometry g;
                 if (!diffraction_geometry(e, tx, rx, g)) continue;
@@ -170,7 +174,7 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
                 sink.path(rec);
             }
         }
-        if (kMain && (comps & kCompTrans) && nh == d) {
+        if (kMain && sub == 0 && (comps & kCompTrans) && nh == d) {
             PathRecord rec;
             rec.component = kCidTrans;
             rec.depth = d;
@@ -324,68 +328,120 @@ __device__ __forceinline__ void write_totals(const wc_solve& q, const wc_radioma
     if (o.topology && a.count > o.topology_capacity && o.failure) atomicOr(o.failure, 2);
 }
 
-// One pass of the radiomap for one cell. kPassMain writes the slots of every
-// non-diffraction component and either the totals (no diffraction requested)
-// or the partial sums for kPassDiff, which adds the diffraction slot and
-// writes the totals.
+// Fixed butterfly over the `width` consecutive lanes sharing a cell. Every
+// pair adds (a + b) and (b + a), which are equal in IEEE arithmetic, so all
+// lanes of the group hold the bit-identical total.
+__device__ __forceinline__ double group_sum(double v, int width) {
+    for (int off = 1; off < width; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ int group_sum_i(int v, int width) {
+    for (int off = 1; off < width; off <<= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+__device__ __forceinline__ void reduce_slot(AccumSink& a, int sl, int width) {
+    a.c_re[sl] = group_sum(a.c_re[sl], width);
+    a.c_im[sl] = group_sum(a.c_im[sl], width);
+    a.c_pow[sl] = group_sum(a.c_pow[sl], width);
+}
+
+// One pass of the radiomap for one lane of a cell's lane group (L = width
+// lanes per cell). kPassMain leaves each lane's running totals in the
+// workspace when diffraction follows; the last pass reduces the group and
+// sub-lane 0 writes the cell. Descriptor segment of a lane: topology[cell]
+// [sub * cap / L ...], terminated by -1 when not full.
 template <int kPass>
-__device__ __forceinline__ void radiomap_cell(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o,
-                                              int t, int cell) {
+__device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o,
+                                              int t, int cell, bool active, int sub, int width) {
     AccumSink a;
     a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
     CellPartial* partial = reinterpret_cast<CellPartial*>(o.workspace);
-    if (kPass == kPassDiff) {
-        const CellPartial p = partial[cell];
+    const size_t lane_slot = (size_t)cell * width + sub;
+    int cap_sub = 0;
+    if (active && kPass == kPassDiff) {
+        const CellPartial p = partial[lane_slot];
         a.s_re = p.s_re;
         a.s_im = p.s_im;
         a.p_inc = p.p_inc;
         a.count = p.count;
     }
-    if (o.topology) {
-        a.topo = o.topology + (size_t)cell * o.topology_capacity;
-        a.cap = o.topology_capacity;
+    if (active && o.topology) {
+        cap_sub = o.topology_capacity / width;
+        a.topo = o.topology + (size_t)cell * o.topology_capacity + (size_t)sub * cap_sub;
+        a.cap = cap_sub;
     }
-    eval_cell<AccumSink, kPass>(s, q, t, cell, a);
+    if (active) eval_cell<AccumSink, kPass>(s, q, t, cell, sub, width, a);
+    if (active && o.topology) {
+        if (a.count < cap_sub) a.topo[a.count] = -1;
+        if (a.count > cap_sub && o.failure) atomicOr(o.failure, 2);
+    }
     const bool diffraction = (q.components & kCompDiff) != 0;
+    if (kPass == kPassMain && diffraction) {
+        if (active) partial[lane_slot] = CellPartial{a.s_re, a.s_im, a.p_inc, a.count, 0};
+        reduce_slot(a, 0, width);
+        reduce_slot(a, 1, width);
+        reduce_slot(a, 3, width);
+        if (active && sub == 0) {
+            write_slot(q, o, cell, 0, a);
+            write_slot(q, o, cell, 1, a);
+            write_slot(q, o, cell, 3, a);
+        }
+        return;
+    }
+    a.s_re = group_sum(a.s_re, width);
+    a.s_im = group_sum(a.s_im, width);
+    a.p_inc = group_sum(a.p_inc, width);
+    for (int j = 0; j < 6; ++j) a.dv[j] = group_sum(a.dv[j], width);
+    a.count = group_sum_i(a.count, width);
+    if (kPass == kPassMain) {
+        reduce_slot(a, 0, width);
+        reduce_slot(a, 1, width);
+        reduce_slot(a, 3, width);
+    }
+    reduce_slot(a, 2, width);
+    if (!active || sub != 0) return;
     if (kPass == kPassMain) {
         write_slot(q, o, cell, 0, a);
         write_slot(q, o, cell, 1, a);
         write_slot(q, o, cell, 3, a);
-        if (diffraction) {
-            partial[cell] = CellPartial{a.s_re, a.s_im, a.p_inc, a.count, 0};
-            return;
-        }
-        write_slot(q, o, cell, 2, a);
-    } else {
-        write_slot(q, o, cell, 2, a);
     }
+    write_slot(q, o, cell, 2, a);
     write_totals(q, o, cell, a);
 }
 
 template <int kPass>
 __global__ void __launch_bounds__(256) radiomap_kernel(SceneView s, wc_solve q, wc_radiomap_out o) {
-    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
-        const int t = tx_of_cell(q, cell);
-        radiomap_cell<kPass>(s, q, o, t, cell);
+    // one lane per cell (L = 1)
+    for (int base = blockIdx.x * blockDim.x; base < q.total_cells; base += blockDim.x * gridDim.x) {
+        const int cell = base + threadIdx.x;
+        const bool active = cell < q.total_cells;
+        const int t = active ? tx_of_cell(q, cell) : 0;
+        radiomap_lane<kPass>(s, q, o, t, active ? cell : 0, active, 0, 1);
     }
 }
 
-// Persistent variant: wavefronts pull wave-aligned tiles (tx, first cell) of
-// 64 cells from a global counter, so a wavefront never mixes transmitters and
-// uneven receiver sets balance across the machine.
+// Persistent variant: wavefronts pull tiles (tx, first cell) of 64 / L cells
+// from a global counter; each cell is evaluated by L consecutive lanes that
+// split its reflection chains and diffraction edges. Tiles never mix
+// transmitters, and uneven receiver sets balance across the machine.
 template <int kPass>
 __global__ void __launch_bounds__(256) radiomap_tiles_kernel(SceneView s, wc_solve q, wc_radiomap_out o,
                                                              const int* __restrict__ tiles, int tile_count,
                                                              int* __restrict__ counter) {
     const int lane = threadIdx.x & 63;
+    const int width = q.lanes_per_cell;
+    const int sub = lane & (width - 1);
+    const int local = lane / width;
     while (true) {
         int tile = 0;
         if (lane == 0) tile = atomicAdd(counter, 1);
         tile = __shfl(tile, 0);
         if (tile >= tile_count) break;
         const int t = tiles[2 * tile];
-        const int cell = tiles[2 * tile + 1] + lane;
-        if (cell < q.cell_offset[t + 1]) radiomap_cell<kPass>(s, q, o, t, cell);
+        const int cell = tiles[2 * tile + 1] + local;
+        const bool active = cell < q.cell_offset[t + 1];
+        radiomap_lane<kPass>(s, q, o, t, active ? cell : q.cell_offset[t], active, sub, width);
     }
 }
 
@@ -393,7 +449,7 @@ __global__ void __launch_bounds__(256) count_kernel(SceneView s, wc_solve q, int
     for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < q.total_cells; cell += blockDim.x * gridDim.x) {
         const int t = tx_of_cell(q, cell);
         CountSink c;
-        eval_cell<CountSink, kPassAll>(s, q, t, cell, c);
+        eval_cell<CountSink, kPassAll>(s, q, t, cell, 0, 1, c);
         out[cell] = c.count;
     }
 }
@@ -408,7 +464,7 @@ __global__ void __launch_bounds__(256) emit_kernel(SceneView s, wc_solve q, cons
         e.row_end = e.row + path_count[cell];
         e.tx = t;
         e.rx = cell - q.cell_offset[t];
-        eval_cell<EmitSink, kPassAll>(s, q, t, cell, e);
+        eval_cell<EmitSink, kPassAll>(s, q, t, cell, 0, 1, e);
     }
 }
 
@@ -593,6 +649,12 @@ extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve,
     if (!out || !out->power_total || !out->field_total || !out->component_power || !out->component_field)
         return wc_fail("wc_radiomap_forward: output pointers are null");
     if (out->topology && out->topology_capacity <= 0) return wc_fail("topology_capacity must be positive");
+    const int width = solve->lanes_per_cell;
+    if (width != 1 && width != 2 && width != 4 && width != 8 && width != 16 && width != 32 && width != 64)
+        return wc_fail("lanes_per_cell must be a power of two in [1, 64]");
+    if (!tiles && width != 1) return wc_fail("lanes_per_cell > 1 needs the tiled launch");
+    if (out->topology && out->topology_capacity % width != 0)
+        return wc_fail("topology_capacity must be a multiple of lanes_per_cell");
     if (solve->total_cells == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     hipStream_t st = (hipStream_t)stream;
@@ -632,7 +694,9 @@ extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve,
     return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_forward", err);
 }
 
-extern "C" size_t wc_radiomap_workspace_bytes(int cells) { return (size_t)(cells > 0 ? cells : 0) * 32; }
+extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell) {
+    return (size_t)(cells > 0 ? cells : 0) * (size_t)(lanes_per_cell > 0 ? lanes_per_cell : 1) * 32;
+}
 
 extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream) {
     if (int rc = check_scene(scene)) return rc;

@@ -134,8 +134,8 @@ __device__ __forceinline__ void scatter(LaneGrad& g, const PathParams& pp, int p
     }
 }
 
-__device__ void adjoint_cell(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell,
-                             LaneGrad& g) {
+__device__ void adjoint_cell(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell, int sub,
+                             int width, LaneGrad& g) {
     const CellAdjoint adj = cell_adjoint(ga, q, cell);
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
@@ -145,8 +145,11 @@ __device__ void adjoint_cell(const SceneView& s, const wc_solve& q, const wc_adj
     const V3<float> rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
     const float freq = q.frequency_hz;
     const double k_d = (double)freq * kTwoPi / kC0;
-    const int count = min(ga.path_count[cell], ga.topology_capacity);
-    const int* topo = ga.topology + (size_t)cell * ga.topology_capacity;
+    // this lane's descriptor segment (terminated by -1 unless full)
+    const int cap_sub = ga.topology_capacity / width;
+    const int* topo = ga.topology + (size_t)cell * ga.topology_capacity + (size_t)sub * cap_sub;
+    int count = 0;
+    while (count < cap_sub && topo[count] != -1) ++count;
     Hit walls[kMaxHits];
     int wall_count = -1;
     for (int i = 0; i < count; ++i) {
@@ -274,8 +277,9 @@ __global__ void __launch_bounds__(256) vjp_kernel(SceneView s, wc_solve q, wc_ad
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (tile >= tile_count) return;
+    const int width = q.lanes_per_cell;
     const int t = tiles[2 * tile];
-    const int cell = tiles[2 * tile + 1] + lane;
+    const int cell = tiles[2 * tile + 1] + lane / width;
     LaneGrad g;
 #pragma unroll
     for (int i = 0; i < kMaxMaterials; ++i) {
@@ -284,7 +288,7 @@ __global__ void __launch_bounds__(256) vjp_kernel(SceneView s, wc_solve q, wc_ad
     }
     g.freq = 0.0f;
     g.tx[0] = g.tx[1] = g.tx[2] = 0.0f;
-    if (cell < q.cell_offset[t + 1]) adjoint_cell(s, q, ga, t, cell, g);
+    if (cell < q.cell_offset[t + 1]) adjoint_cell(s, q, ga, t, cell, lane & (width - 1), width, g);
     const int m = s.material_count;
     const int stride = 2 * m + 4;
     float* out = ga.partials + (size_t)tile * stride;
