@@ -32,9 +32,21 @@ ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "witwin-channel_amd"), str(ROOT)]
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_FWD = 48.0      # path_gain f32 + field c64 + 3 x (component power f32 + field c64)
-BYTES_PER_CELL_VJP = 4.0       # dL/dP_cell read by the adjoint
-BYTES_PER_TRIANGLE = 48.0      # scene bytes read once per launch
+# Algorithmic HBM bytes of each launch: what it must read and write at least
+# once. Per cell: receiver position (12 B) + evaluation-order entry (4 B);
+# the reflection pass writes three component slots (power f32 + field c64 =
+# 12 B each) and the 32 B float64 partial sums handed to the diffraction pass;
+# the diffraction pass reads those 32 B and writes its slot (12 B), the totals
+# (power + field, 12 B), the diffraction vector (3 x c64, 24 B) and the path
+# count (4 B); the adjoint reads the cell's dL/dP (4 B), totals (12 B), four
+# component fields (32 B), diffraction vector (24 B) and path count (4 B).
+# Each path's 4 B descriptor is written by the pass that finds it and read by
+# the adjoint. Scene tables (triangles 48 B, BVH nodes 64 B, edges 80 B, chain
+# groups + float64 images 140 B) count once per launch.
+BYTES_CELL_MAIN = 12 + 4 + 36 + 32
+BYTES_CELL_DIFF = 12 + 4 + 32 + 12 + 12 + 24 + 4
+BYTES_CELL_VJP = 12 + 4 + 4 + 12 + 32 + 24 + 4
+BYTES_PER_DESCRIPTOR = 4
 
 
 def timed(fn, steps: int, warmup: int, stream) -> tuple[float, list[float]]:
@@ -55,19 +67,65 @@ def timed(fn, steps: int, warmup: int, stream) -> tuple[float, list[float]]:
     return wall / steps * 1e3, [s.elapsed_time(e) for s, e in zip(starts, ends)]
 
 
-def kernel_time(fn, stream, reps: int = 5) -> float:
-    """Average duration (ms) of one launch sequence bracketed by HIP events."""
+def kernel_time(fn, stream, reps: int = 5, before=None) -> float:
+    """Average duration (ms) of ``fn``'s launches, bracketed by HIP events on
+    the stream they run on; ``before`` (untimed) runs ahead of each rep."""
     import torch
 
+    if before is not None:
+        before()
     fn()
     torch.cuda.synchronize()
-    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    s.record(stream)
-    for _ in range(reps):
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    for i in range(reps):
+        if before is not None:
+            before()
+        starts[i].record(stream)
         fn()
-    e.record(stream)
+        ends[i].record(stream)
     torch.cuda.synchronize()
-    return s.elapsed_time(e) / reps
+    return sum(a.elapsed_time(b) for a, b in zip(starts, ends)) / reps
+
+
+def traffic_bytes(kernel: str, workload: str):
+    """Measured HBM bytes per launch of ``kernel`` on ``workload`` from the
+    committed PMC summary (profiles/hbm_traffic.json: FETCH_SIZE + WRITE_SIZE
+    scaled by a copy kernel), or None when no summary matches."""
+    path = ROOT / "profiles" / "hbm_traffic.json"
+    if not path.exists():
+        return None
+    entry = json.loads(path.read_text()).get("kernels", {}).get(kernel)
+    if not entry or entry.get("workload") != workload:
+        return None
+    return entry.get("bytes_per_launch")
+
+
+def descriptor_split(out) -> tuple[int, int]:
+    """(reflection-pass paths, diffraction paths) stored in the topology table."""
+    import torch
+
+    cap = out.topology_capacity
+    cols = torch.arange(cap, device=out.topology.device).unsqueeze(0)
+    valid = cols < out.path_count.unsqueeze(1)
+    diff = valid & ((out.topology >> 24) == 2)
+    total = int(valid.sum().item())
+    nd = int(diff.sum().item())
+    return total - nd, nd
+
+
+def copy_calibration(nbytes: int = 256 << 20, reps: int = 3) -> int:
+    """A plain device-to-device copy of ``nbytes`` (reads and writes nbytes
+    each), the known quantity that scales the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "reps": reps}))
+    return 0
 
 
 def main() -> int:
@@ -79,10 +137,14 @@ def main() -> int:
     parser.add_argument("--tx-batch", type=int, default=64)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-batch", action="store_true")
+    parser.add_argument("--copy-calibration", action="store_true",
+                        help="only run a plain device copy of known size (HBM counter calibration)")
     args = parser.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
-        print("bench.py measures one GPU; multi-GPU runs are not part of this build "
-              "(the path shards by transmitter, see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU; runs on several GPUs are not part of this benchmark "
+              "(the radiomap shards by transmitter and receiver tile with no communication)", file=sys.stderr)
         return 2
 
     import numpy as np
@@ -119,11 +181,26 @@ def main() -> int:
     ms_batched, _ = timed(step.step, args.steps, args.warmup, stream)
     value = cells / (ms_batched * 1e-3)
 
-    # dominant kernel: the fused forward (fwd) and the adjoint, each timed alone
+    # each kernel of the step timed alone; the roofline is the longest one's
     fwd_ms = kernel_time(lambda: engine.radiomap_forward(dscene, inputs, topology_capacity=step.capacity,
                                                          out=step.out), stream)
-    traffic_fwd = cells * BYTES_PER_CELL_FWD + compiled.face_count * BYTES_PER_TRIANGLE
-    achieved = traffic_fwd / (fwd_ms * 1e-3) / 1e9
+    main_ms = kernel_time(lambda: engine.radiomap_pass(dscene, inputs, step.out, 1), stream)
+    diff_ms = kernel_time(lambda: engine.radiomap_pass(dscene, inputs, step.out, 2), stream,
+                          before=lambda: engine.radiomap_pass(dscene, inputs, step.out, 1))
+    vjp_ms = kernel_time(step.adjoint, stream)
+    reduce_ms = kernel_time(step.reduce, stream)
+    n_main, n_diff = descriptor_split(step.out)
+    scene_bytes = (compiled.face_count * 48 + int(dscene.node_count) * 64
+                   + int(dscene.view.edge_count) * 80 + int(inputs.chains.count) * (4 * 5 + 8 * 15))
+    kernels = {
+        "radiomap_main_tiles": (main_ms, cells * BYTES_CELL_MAIN + n_main * BYTES_PER_DESCRIPTOR + scene_bytes),
+        "radiomap_diff_tiles": (diff_ms, cells * BYTES_CELL_DIFF + n_diff * BYTES_PER_DESCRIPTOR + scene_bytes),
+        "vjp_kernel": (vjp_ms, cells * BYTES_CELL_VJP + (n_main + n_diff) * BYTES_PER_DESCRIPTOR + scene_bytes
+                       + inputs.tile_count * step.stride * 4),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k][0])
+    dom_ms, dom_bytes = kernels[dominant]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
 
     batch = None
     if not args.no_batch:
@@ -179,6 +256,9 @@ def main() -> int:
                "median_db_error_vs_gpu": float(np.median(err)) if err.size else 0.0,
                "p99_db_error_vs_gpu": float(np.percentile(err, 99)) if err.size else 0.0}
 
+    workload_name = (f"urban canyon 8x8 boxes, {compiled.face_count} tris, 1 Tx at street intersection "
+                     f"(53.231,52.013,25), {args.grid}x{args.grid} grid at 2 m, 28 GHz, depth 3 refl + UTD diff + "
+                     "LoS, loss mean dB over lit cells")
     line = {
         "metric": "radiomap cells/s fwd+VJP (batched vs plain, 1 GPU) at <=0.01 dB err vs FP64",
         "value": value,
@@ -193,9 +273,7 @@ def main() -> int:
         "dtype": "f32 geometry/fields/tangents, f64 phase, UTD transition and cell sums",
         "data": "synthetic seeded urban canyon (seed 2)",
         "config": {
-            "workload": f"urban canyon 8x8 boxes, {compiled.face_count} tris, 1 Tx at street intersection (53.231,52.013,25), "
-                        f"{args.grid}x{args.grid} grid at 2 m, 28 GHz, depth 3 refl + UTD diff + LoS, "
-                        "loss mean dB over lit cells",
+            "workload": workload_name,
             "pass": "forward+vjp",
             "cells": cells,
             "paths": path_count,
@@ -203,14 +281,17 @@ def main() -> int:
             "plain_ms_per_step": ms_plain,
             "plain_cells_per_s": cells / (ms_plain * 1e-3),
             "batched_over_plain": ms_plain / ms_batched,
-            "forward_kernel_ms": fwd_ms,
+            "forward_ms": fwd_ms,
+            "kernel_ms": {"radiomap_main_tiles": main_ms, "radiomap_diff_tiles": diff_ms, "vjp_kernel": vjp_ms,
+                          "vjp_reduce_kernel": reduce_ms},
+            "paths_reflection_pass": n_main,
+            "paths_diffraction": n_diff,
             "chain_planning_s": plan_s,
             "multi_tx": batch,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "radiomap_tiles_kernel", "kernel_ms": fwd_ms,
-                     "algorithmic_bytes_per_launch": traffic_fwd},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(dominant, workload_name),
+                     "kernel": dominant, "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

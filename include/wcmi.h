@@ -122,6 +122,12 @@ typedef struct wc_solve {
     int coherent;
     int lanes_per_cell; /* L: lanes sharing one cell's candidates (power of two, 1..64);
                            tiles then hold 64 / L cells */
+    const int* cell_order; /* [total_cells] or NULL (identity): evaluation position p
+                              works on cell cell_order[p]; a permutation that maps each
+                              transmitter's range [cell_offset[t], cell_offset[t+1]) onto
+                              itself. Tiles and launches run over positions, outputs stay
+                              indexed by cell; a spatially coherent order (receivers
+                              that are close together in one wavefront) cuts divergence. */
 } wc_solve;
 
 /* ------------------------------------------------------------ radiomap
@@ -164,6 +170,17 @@ typedef struct wc_radiomap_out {
 size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell);
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
+
+/* One launch of wc_radiomap_forward's sequence (tiled form only), for callers
+ * that time or schedule the passes apart: pass 1 zeroes tile_counter[0] and
+ * runs the reflection/LoS/transmission pass, pass 2 zeroes tile_counter[1]
+ * and runs the diffraction pass, which reads the per-cell sums pass 1 left in
+ * `out->workspace` (so pass 2 must follow pass 1 of the same solve).
+ * Calling pass 1 then pass 2 (when the solve has diffraction) is equivalent to
+ * wc_radiomap_forward. No reference counterpart: the reference runs its
+ * radiomap as one sequence of launches. */
+int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
+                     const int* tiles, int tile_count, int* tile_counter, int pass, void* stream);
 
 /* ------------------------------------------------------------ radiomap adjoint
  * Fixed-topology VJP of wc_radiomap_forward. Replaces the reference adjoint
@@ -244,6 +261,21 @@ int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const int* row_o
 
 int wc_paths_canonical_order(const wc_path_rows* rows, const int* pair_offset, int pair_count, int* permutation,
                              unsigned char* keep, void* stream);
+
+/* ------------------------------------------------------------ channel frequency response
+ * CFR of padded per-pair path sets, replacing PathResult.cfr
+ * (witwin/channel/path.py:646-656) and BeamformedPathResult.cfr (:824-832):
+ *   out[b, s, k] = sum_p valid[b,p] a[b,p,s] exp(-j 2 pi f_k (tau[b,p] - tau0[b]))
+ * a [pairs, max_paths, time_steps] c64, tau [pairs, max_paths] f32 (s),
+ * valid [pairs, max_paths] u8, out [pairs, time_steps, freq_count] c64.
+ * tau0[b] = min over valid paths of pair b when normalize_delays != 0, else 0
+ * (tau - tau0 rounded to float32 as the reference does). Frequencies are the
+ * list `frequencies` [freq_count] f64 (Hz) or, when it is NULL, the uniform
+ * grid f0 + k df. The phase argument f (tau - tau0) is reduced modulo one
+ * cycle in float64; path sums run in float32 in path order. */
+int wc_paths_cfr(const float* a, const float* tau, const unsigned char* valid, int pairs, int max_paths,
+                 int time_steps, const double* frequencies, double f0, double df, int freq_count,
+                 int normalize_delays, float* out, void* stream);
 
 /* ------------------------------------------------------------ row field operators
  * Per-row field kernels with the reference facade signatures

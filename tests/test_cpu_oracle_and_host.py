@@ -14,7 +14,8 @@ import pytest
 import torch
 
 import oracle
-from witwin.channel.propagation.engine import exhaustive_sequences, is_exhaustive, make_tiles
+from witwin.channel.propagation.engine import (cell_order_table, exhaustive_sequences, is_exhaustive,
+                                               locality_permutation, make_tiles)
 from witwin.channel.scene.compiler import build_edges, compile as compile_scene, coplanar_face_groups
 from witwin.channel import workloads
 from witwin.core import Mesh, PhysicalMaterial, Scene, Structure
@@ -257,6 +258,36 @@ def test_tiles_never_straddle_transmitters():
     for t, base in tiles:
         assert offsets[t] <= base < offsets[t + 1]
     assert sum(min(64, offsets[t + 1] - b) for t, b in tiles) == 300
+
+
+def test_locality_order_is_a_blocked_permutation():
+    # a 32 x 24 grid (row-major, 2 m): every 64 consecutive entries of the
+    # Z-order form an 8 x 8 block, and the order covers each cell once
+    cols, rows = 32, 24
+    pts = np.array([[2.0 * c, 2.0 * r, 1.5] for r in range(rows) for c in range(cols)])
+    perm = locality_permutation(pts)
+    assert sorted(perm.tolist()) == list(range(rows * cols))
+    for k in range(0, rows * cols, 64):
+        block = pts[perm[k:k + 64]]
+        assert np.ptp(block[:, 0]) == 14.0 and np.ptp(block[:, 1]) == 14.0
+    # ties (coincident receivers) keep input order
+    twin = locality_permutation(np.zeros((5, 3)))
+    assert twin.tolist() == [0, 1, 2, 3, 4]
+
+
+def test_cell_order_maps_each_transmitter_range_onto_itself():
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(0, 50, (40, 3))
+    shared = cell_order_table(pts, np.array([0, 40, 80, 120]), True)
+    uneven = cell_order_table(pts[:30], np.array([0, 7, 7, 30]), False)
+    for order, bounds in ((shared, [0, 40, 80, 120]), (uneven, [0, 7, 7, 30])):
+        assert order.dtype == np.int32
+        for t in range(len(bounds) - 1):
+            seg = order[bounds[t]:bounds[t + 1]]
+            assert sorted(seg.tolist()) == list(range(bounds[t], bounds[t + 1]))
+    # shared receivers: the same permutation for every transmitter
+    assert (shared[40:80] - 40 == shared[:40]).all()
+    assert (uneven[7:] - 7 == locality_permutation(pts[7:30])).all()
 
 
 def test_urban_canyon_compiles():

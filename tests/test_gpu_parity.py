@@ -232,6 +232,64 @@ def test_urban_canyon_radiomap_matches_oracle(device):
     assert err[well].max() <= DB_TOL
 
 
+def test_launch_layouts_are_bitwise_neutral(device):
+    """The evaluation order and the pass-by-pass launch change scheduling
+    only: every output must be bit-identical. Splitting a cell over 4 lanes
+    regroups its float64 sums (same paths, same count; values within 1e-6)."""
+    _say("case: urban canyon seed 2, 48x48 grid at 8 m, depth 3, los+refl+diff; layouts vs identity order")
+    scene = workloads.urban_canyon_scene(seed=2, shape=(48, 48), spacing=8.0)
+    from witwin.channel.components import component_bits
+    from witwin.channel.scene.compiler import compile as compile_scene, device_scene
+
+    compiled = compile_scene(scene, reference_frequency_hz=28e9)
+    dscene = device_scene(compiled, device)
+    bits = component_bits({"los", "reflection", "diffraction"})
+
+    def solve(**kw):
+        inputs = engine.solve_inputs(compiled, dscene, components=bits, max_depth=3, coherent=True,
+                                     frequency_hz=28e9, **kw)
+        out = engine.forward_with_topology(dscene, inputs, capacity=512)
+        torch.cuda.synchronize()
+        return inputs, out
+
+    def outputs(out):
+        return {k: getattr(out, k).cpu() for k in ("power_total", "field_total", "component_power",
+                                                     "component_field", "diffraction_vector", "path_count")}
+
+    _, base = solve(spatial_order=False)
+    assert int(base.failure.item()) == 0
+    ref = outputs(base)
+    assert int(ref["path_count"].sum()) > 0
+    _say("layout: Z-order")
+    inputs, out = solve(spatial_order=True)
+    got = outputs(out)
+    for k in ref:
+        assert torch.equal(got[k], ref[k]), ("z-order", k)
+    _say("layout: Z-order, 4 lanes per cell")
+    inputs, out = solve(spatial_order=True, lanes_per_cell=4)
+    got = outputs(out)
+    assert torch.equal(got["path_count"], ref["path_count"])
+    for k in ref:
+        if k != "path_count":
+            assert torch.allclose(got[k], ref[k], rtol=1e-6, atol=1e-30), ("lanes", k)
+    # pass 1 then pass 2 reproduces the fused forward, descriptors included
+    inputs, out = solve(spatial_order=True)
+    topo = out.topology.clone()
+    for k in ref:
+        getattr(out, k).zero_()
+    engine.radiomap_pass(dscene, inputs, out, 1)
+    engine.radiomap_pass(dscene, inputs, out, 2)
+    torch.cuda.synchronize()
+    got = outputs(out)
+    for k in ref:
+        assert torch.equal(got[k], ref[k]), ("passes", k)
+    counts = ref["path_count"].to(device)
+    valid = torch.arange(out.topology_capacity, device=device).unsqueeze(0) < counts.unsqueeze(1)
+    assert torch.equal(out.topology[valid], topo[valid])
+    with pytest.raises(RuntimeError, match="pass must be 1 or 2"):
+        engine.radiomap_pass(dscene, inputs, out, 3)
+
+
 def test_urban_canyon_export_topology_bit_exact(device):
     _say("case: urban canyon seed 2, 24x24 grid at 15 m, explicit rows vs oracle topology")
     run = _canyon(device, (24, 24), 15.0)

@@ -6,7 +6,8 @@
 #include <algorithm>
 #include <cstddef>
 
-int wc_fail(const char* message);
+// printf-style message; returns 1
+int wc_fail(const char* format, ...) __attribute__((format(printf, 1, 2)));
 int wc_fail_hip(const char* where, hipError_t err);
 
 namespace wc {

@@ -1,6 +1,7 @@
 // Library identity and error channel of the C ABI.
 #include "wc_api_internal.h"
 
+#include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -9,8 +10,17 @@ namespace {
 thread_local std::string g_last_error;
 }
 
-int wc_fail(const char* message) {
-    g_last_error = message ? message : "unknown error";
+int wc_fail(const char* format, ...) {
+    if (!format) {
+        g_last_error = "unknown error";
+        return 1;
+    }
+    char buffer[512];
+    va_list args;
+    va_start(args, format);
+    std::vsnprintf(buffer, sizeof(buffer), format, args);
+    va_end(args);
+    g_last_error = buffer;
     return 1;
 }
 

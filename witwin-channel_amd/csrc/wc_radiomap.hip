@@ -297,6 +297,11 @@ __device__ __forceinline__ int tx_of_cell(const wc_solve& q, int cell) {
     return lo;
 }
 
+// cell evaluated at launch position `pos` (identity without an order table)
+__device__ __forceinline__ int cell_at(const wc_solve& q, int pos) {
+    return q.cell_order ? q.cell_order[pos] : pos;
+}
+
 __device__ __forceinline__ void write_slot(const wc_solve& q, const wc_radiomap_out& o, int cell, int sl,
                                            const AccumSink& a) {
     const int cells = q.total_cells;
@@ -410,14 +415,23 @@ __device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve
     write_totals(q, o, cell, a);
 }
 
+// Occupancy targets (waves per SIMD) of the two radiomap passes; 1 leaves
+// the register budget to the compiler.
+#ifndef WC_MAIN_WAVES_PER_EU
+#define WC_MAIN_WAVES_PER_EU 1
+#endif
+#ifndef WC_DIFF_WAVES_PER_EU
+#define WC_DIFF_WAVES_PER_EU 1
+#endif
+
 template <int kPass>
 __global__ void __launch_bounds__(256) radiomap_kernel(SceneView s, wc_solve q, wc_radiomap_out o) {
     // one lane per cell (L = 1)
     for (int base = blockIdx.x * blockDim.x; base < q.total_cells; base += blockDim.x * gridDim.x) {
-        const int cell = base + threadIdx.x;
-        const bool active = cell < q.total_cells;
-        const int t = active ? tx_of_cell(q, cell) : 0;
-        radiomap_lane<kPass>(s, q, o, t, active ? cell : 0, active, 0, 1);
+        const int pos = base + threadIdx.x;
+        const bool active = pos < q.total_cells;
+        const int t = active ? tx_of_cell(q, pos) : 0;
+        radiomap_lane<kPass>(s, q, o, t, cell_at(q, active ? pos : 0), active, 0, 1);
     }
 }
 
@@ -426,9 +440,9 @@ __global__ void __launch_bounds__(256) radiomap_kernel(SceneView s, wc_solve q, 
 // split its reflection chains and diffraction edges. Tiles never mix
 // transmitters, and uneven receiver sets balance across the machine.
 template <int kPass>
-__global__ void __launch_bounds__(256) radiomap_tiles_kernel(SceneView s, wc_solve q, wc_radiomap_out o,
-                                                             const int* __restrict__ tiles, int tile_count,
-                                                             int* __restrict__ counter) {
+__device__ __forceinline__ void radiomap_tiles(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o,
+                                               const int* __restrict__ tiles, int tile_count,
+                                               int* __restrict__ counter) {
     const int lane = threadIdx.x & 63;
     const int width = q.lanes_per_cell;
     const int sub = lane & (width - 1);
@@ -439,10 +453,22 @@ __global__ void __launch_bounds__(256) radiomap_tiles_kernel(SceneView s, wc_sol
         tile = __shfl(tile, 0);
         if (tile >= tile_count) break;
         const int t = tiles[2 * tile];
-        const int cell = tiles[2 * tile + 1] + local;
-        const bool active = cell < q.cell_offset[t + 1];
-        radiomap_lane<kPass>(s, q, o, t, active ? cell : q.cell_offset[t], active, sub, width);
+        const int pos = tiles[2 * tile + 1] + local;
+        const bool active = pos < q.cell_offset[t + 1];
+        radiomap_lane<kPass>(s, q, o, t, cell_at(q, active ? pos : q.cell_offset[t]), active, sub, width);
     }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_MAIN_WAVES_PER_EU)))
+radiomap_main_tiles(SceneView s, wc_solve q, wc_radiomap_out o, const int* __restrict__ tiles, int tile_count,
+                    int* __restrict__ counter) {
+    radiomap_tiles<kPassMain>(s, q, o, tiles, tile_count, counter);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_DIFF_WAVES_PER_EU)))
+radiomap_diff_tiles(SceneView s, wc_solve q, wc_radiomap_out o, const int* __restrict__ tiles, int tile_count,
+                    int* __restrict__ counter) {
+    radiomap_tiles<kPassDiff>(s, q, o, tiles, tile_count, counter);
 }
 
 __global__ void __launch_bounds__(256) count_kernel(SceneView s, wc_solve q, int* __restrict__ out) {
@@ -642,12 +668,14 @@ static int check_solve(const wc_solve* q) {
 
 using namespace wc;
 
-extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
-                                   const int* tiles, int tile_count, int* tile_counter, void* stream) {
+// passes: bit 0 the reflection/LoS/transmission pass, bit 1 the diffraction pass
+static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
+                           const int* tiles, int tile_count, int* tile_counter, int passes, const char* who,
+                           void* stream) {
     if (int rc = check_scene(scene)) return rc;
     if (int rc = check_solve(solve)) return rc;
     if (!out || !out->power_total || !out->field_total || !out->component_power || !out->component_field)
-        return wc_fail("wc_radiomap_forward: output pointers are null");
+        return wc_fail("%s: output pointers are null", who);
     if (out->topology && out->topology_capacity <= 0) return wc_fail("topology_capacity must be positive");
     const int width = solve->lanes_per_cell;
     if (width != 1 && width != 2 && width != 4 && width != 8 && width != 16 && width != 32 && width != 64)
@@ -655,12 +683,13 @@ extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve,
     if (!tiles && width != 1) return wc_fail("lanes_per_cell > 1 needs the tiled launch");
     if (out->topology && out->topology_capacity % width != 0)
         return wc_fail("topology_capacity must be a multiple of lanes_per_cell");
-    if (solve->total_cells == 0) return 0;
+    const bool diffraction = (solve->components & kCompDiff) != 0;
+    if (!diffraction) passes &= 1;
+    if (solve->total_cells == 0 || passes == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     hipStream_t st = (hipStream_t)stream;
-    const bool diffraction = (solve->components & kCompDiff) != 0;
     if (diffraction && !out->workspace)
-        return wc_fail("wc_radiomap_forward: workspace (wc_radiomap_workspace_bytes) is required with diffraction");
+        return wc_fail("%s: workspace (wc_radiomap_workspace_bytes) is required with diffraction", who);
     if (tiles) {
         if (!tile_counter || tile_count <= 0) return wc_fail("tiled launch needs tile_counter and tile_count");
         // compute-unit count of the current device, queried once (safe during stream capture)
@@ -677,21 +706,40 @@ extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve,
         int blocks = std::min((tile_count + 3) / 4, cached_cus * 8);
         blocks = std::max(blocks, 1);
         // a fill kernel rather than a memset node keeps the launches graph-capturable
-        hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 2);
-        hipLaunchKernelGGL(radiomap_tiles_kernel<kPassMain>, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
-                           tile_count, tile_counter);
-        if (diffraction)
-            hipLaunchKernelGGL(radiomap_tiles_kernel<kPassDiff>, dim3(blocks), dim3(256), 0, st, s, *solve, *out,
-                               tiles, tile_count, tile_counter + 1);
+        if (passes == 3)
+            hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 2);
+        else
+            hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter + (passes == 2 ? 1 : 0), 1);
+        if (passes & 1)
+            hipLaunchKernelGGL(radiomap_main_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
+                               tile_count, tile_counter);
+        if (passes & 2)
+            hipLaunchKernelGGL(radiomap_diff_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
+                               tile_count, tile_counter + 1);
     } else {
-        hipLaunchKernelGGL(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0, st,
-                           s, *solve, *out);
-        if (diffraction)
+        if (passes & 1)
+            hipLaunchKernelGGL(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
+                               st, s, *solve, *out);
+        if (passes & 2)
             hipLaunchKernelGGL(radiomap_kernel<kPassDiff>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
                                st, s, *solve, *out);
     }
     const hipError_t err = hipGetLastError();
-    return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_forward", err);
+    return err == hipSuccess ? 0 : wc_fail_hip(who, err);
+}
+
+extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
+                                   const int* tiles, int tile_count, int* tile_counter, void* stream) {
+    return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, 3, "wc_radiomap_forward", stream);
+}
+
+extern "C" int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
+                                const int* tiles, int tile_count, int* tile_counter, int pass, void* stream) {
+    if (pass != 1 && pass != 2) return wc_fail("wc_radiomap_pass: pass must be 1 or 2 (got %d)", pass);
+    if (!tiles) return wc_fail("wc_radiomap_pass: the pass-by-pass launch needs tiles");
+    if (pass == 2 && !(solve && (solve->components & kCompDiff)))
+        return wc_fail("wc_radiomap_pass: pass 2 needs a solve with diffraction");
+    return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, pass, "wc_radiomap_pass", stream);
 }
 
 extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell) {

@@ -113,7 +113,10 @@ __device__ inline bool occluded(const SceneView& s, const RayPre& r, float tmin,
                 if (g != ig0 && g != ig1 && tri_hit(r, tri, tmin, tmax) >= 0.0f) return true;
             } else {
                 if (next >= 0) {
-                    if (sp < kStackSize) stack[sp++] = c1;
+                    // nearer box first: a blocker close to the segment's start ends the walk sooner
+                    int far_child = c1;
+                    if (tn1 < tn0) { far_child = next; next = c1; }
+                    if (sp < kStackSize) stack[sp++] = far_child;
                 } else {
                     next = c1;
                 }

@@ -18,7 +18,12 @@
 namespace wc {
 namespace {
 
-constexpr int kTangents = 8;
+// tangent directions carried per dual-number pass (a path with more local
+// parameters takes several passes)
+#ifndef WC_VJP_TANGENTS
+#define WC_VJP_TANGENTS 8
+#endif
+constexpr int kTangents = WC_VJP_TANGENTS;
 constexpr int kMaxMaterials = 16;
 using D8 = Dual<kTangents>;
 
@@ -134,8 +139,15 @@ __device__ __forceinline__ void scatter(LaneGrad& g, const PathParams& pp, int p
     }
 }
 
-__device__ void adjoint_cell(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell, int sub,
-                             int width, LaneGrad& g) {
+enum VjpPhase : int { kPhaseMain = 0, kPhaseDiff = 1 };
+
+// Adjoint of one stored path (descriptor `desc`) of `cell`, accumulated into
+// g. kPhaseMain handles line of sight, reflection chains and transmission,
+// kPhaseDiff diffraction: each phase is its own kernel with its own register
+// budget.
+template <int kPhase>
+__device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell, int desc,
+                             LaneGrad& g) {
     const CellAdjoint adj = cell_adjoint(ga, q, cell);
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
@@ -145,124 +157,116 @@ __device__ void adjoint_cell(const SceneView& s, const wc_solve& q, const wc_adj
     const V3<float> rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
     const float freq = q.frequency_hz;
     const double k_d = (double)freq * kTwoPi / kC0;
-    // this lane's descriptor segment (terminated by -1 unless full)
-    const int cap_sub = ga.topology_capacity / width;
-    const int* topo = ga.topology + (size_t)cell * ga.topology_capacity + (size_t)sub * cap_sub;
-    int count = 0;
-    while (count < cap_sub && topo[count] != -1) ++count;
+    const int comp = desc >> 24;
+    int cand = desc & 0xFFFFFF;
+    if (cand == 0xFFFFFF) cand = -1;
+    PathParams pp;
+    pp.count = 4;
+    int groups[kMaxDepth];
+    int faces[kMaxDepth];
     Hit walls[kMaxHits];
-    int wall_count = -1;
-    for (int i = 0; i < count; ++i) {
-        const int desc = topo[i];
-        const int comp = desc >> 24;
-        int cand = desc & 0xFFFFFF;
-        if (cand == 0xFFFFFF) cand = -1;
-        PathParams pp;
-        pp.count = 4;
-        int groups[kMaxDepth];
-        int faces[kMaxDepth];
-        int depth = 0;
-        double length_d = 0.0;
-        if (comp == kCidRefl) {
-            const int* cg = q.chain_groups + cand * kMaxDepth;
-            const double* img = q.chain_images + cand * kMaxDepth * 3;
-            while (depth < kMaxDepth && cg[depth] >= 0) {
-                groups[depth] = cg[depth];
-                ++depth;
-            }
-            // interaction faces of the frozen chain (image back-projection only)
-            V3<double> next = to_d(rx);
-            for (int k = depth - 1; k >= 0; --k) {
-                double off;
-                const V3<double> n = group_normal(s, groups[k], off);
-                const V3<double> im{img[3 * k], img[3 * k + 1], img[3 * k + 2]};
-                const double si = dot(n, im) - off, sn = dot(n, next) - off;
-                const V3<double> h = im + (next - im) * (si / (si - sn));
-                int f = face_containing(s, groups[k], to_f(h), to_f(n));
-                faces[k] = f >= 0 ? f : s.group_faces[s.group_face_offset[groups[k]]];
-                next = h;
-            }
-            const V3<double> last{img[3 * (depth - 1)], img[3 * (depth - 1) + 1], img[3 * (depth - 1) + 2]};
-            length_d = norm(to_d(rx) - last);
-            for (int k = 0; k < depth; ++k) {
-                const int m = s.face_material[faces[k]];
-                pp.material[k] = m;
-                pp.eps[k] = s.mat_eps[m];
-                pp.sig[k] = s.mat_sigma[m];
-            }
-            pp.count = 4 + 2 * depth;
-        } else if (comp == kCidDiff) {
-            const Edge e = s.edges[cand];
-            DiffGeometry dg;
-            diffraction_geometry(e, tx, rx, dg);
-            length_d = dg.s + dg.sp;
-            const int mn = e.material1 >= 0 ? e.material1 : e.material0;
-            pp.material[0] = e.material0;
-            pp.material[1] = mn;
-            pp.eps[0] = s.mat_eps[e.material0];
-            pp.sig[0] = s.mat_sigma[e.material0];
-            pp.eps[1] = s.mat_eps[mn];
-            pp.sig[1] = s.mat_sigma[mn];
-            pp.count = 8;
-        } else if (comp == kCidTrans) {
-            if (wall_count < 0) {
-                const V3<float> seg = rx - tx;
-                const float len = norm(seg);
-                wall_count = 0;
-                if (len > 2.0f * kSegmentBias) {
-                    const float eps = kSegmentBias / len;
-                    const RayPre r = ray_setup(tx.x, tx.y, tx.z, seg.x, seg.y, seg.z);
-                    wall_count = gather_hits(s, r, eps, 1.0f - eps, q.max_depth, walls);
-                }
-            }
-            depth = min(wall_count, q.max_depth);
-            double carrier = norm(to_d(rx) - to_d(tx));
-            for (int w = 0; w < depth; ++w) {
-                double off;
-                const V3<double> nd = group_normal(s, walls[w].group, off);
-                const V3<double> dir = (to_d(rx) - to_d(tx)) * (1.0 / norm(to_d(rx) - to_d(tx)));
-                const double ct = fmin(fmax(fabs(dot(dir, nd)), 1e-6), 1.0);
-                const int m = s.face_material[walls[w].face];
-                double wall = 0.0;
-                for (int l = 0; l < s.mat_layer_count[m]; ++l)
-                    wall += fmax((double)s.layer_thickness[s.mat_layer_offset[m] + l], 0.0);
-                carrier -= wall * ct;
-                pp.material[w] = m;
-                pp.eps[w] = s.layer_eps[s.mat_layer_offset[m]];
-                pp.sig[w] = s.layer_sigma[s.mat_layer_offset[m]];
-            }
-            length_d = carrier;
-            pp.count = 4 + 2 * depth;
-        } else {
-            length_d = norm(to_d(rx) - to_d(tx));
+    int depth = 0;
+    double length_d = 0.0;
+    if constexpr (kPhase == kPhaseDiff) {
+        const Edge e = s.edges[cand];
+        DiffGeometry dg;
+        diffraction_geometry(e, tx, rx, dg);
+        length_d = dg.s + dg.sp;
+        const int mn = e.material1 >= 0 ? e.material1 : e.material0;
+        pp.material[0] = e.material0;
+        pp.material[1] = mn;
+        pp.eps[0] = s.mat_eps[e.material0];
+        pp.sig[0] = s.mat_sigma[e.material0];
+        pp.eps[1] = s.mat_eps[mn];
+        pp.sig[1] = s.mat_sigma[mn];
+        pp.count = 8;
+    } else if (comp == kCidRefl) {
+        const int* cg = q.chain_groups + cand * kMaxDepth;
+        const double* img = q.chain_images + cand * kMaxDepth * 3;
+        while (depth < kMaxDepth && cg[depth] >= 0) {
+            groups[depth] = cg[depth];
+            ++depth;
         }
-        const int slot_id = slot_of_component(comp);
-        const int passes = (pp.count + kTangents - 1) / kTangents;
-        for (int pass = 0; pass < passes; ++pass) {
-            const V3<D8> txd{seeded(tx.x, 0, pass), seeded(tx.y, 1, pass), seeded(tx.z, 2, pass)};
-            const D8 fd = seeded(freq, 3, pass);
-            D8 eps[kMaxDepth], sig[kMaxDepth];
-            const int nm = (pp.count - 4) / 2;
-            for (int k = 0; k < nm; ++k) {
-                eps[k] = seeded(pp.eps[k], 4 + 2 * k, pass);
-                sig[k] = seeded(pp.sig[k], 5 + 2 * k, pass);
-            }
-            V3<D8> arrival;
-            C3<D8> field;
+        // interaction faces of the frozen chain (image back-projection only)
+        V3<double> next = to_d(rx);
+        for (int k = depth - 1; k >= 0; --k) {
+            double off;
+            const V3<double> n = group_normal(s, groups[k], off);
+            const V3<double> im{img[3 * k], img[3 * k + 1], img[3 * k + 2]};
+            const double si = dot(n, im) - off, sn = dot(n, next) - off;
+            const V3<double> h = im + (next - im) * (si / (si - sn));
+            int f = face_containing(s, groups[k], to_f(h), to_f(n));
+            faces[k] = f >= 0 ? f : s.group_faces[s.group_face_offset[groups[k]]];
+            next = h;
+        }
+        const V3<double> last{img[3 * (depth - 1)], img[3 * (depth - 1) + 1], img[3 * (depth - 1) + 2]};
+        length_d = norm(to_d(rx) - last);
+        for (int k = 0; k < depth; ++k) {
+            const int m = s.face_material[faces[k]];
+            pp.material[k] = m;
+            pp.eps[k] = s.mat_eps[m];
+            pp.sig[k] = s.mat_sigma[m];
+        }
+        pp.count = 4 + 2 * depth;
+    } else if (comp == kCidTrans) {
+        const V3<float> seg = rx - tx;
+        const float len = norm(seg);
+        int wall_count = 0;
+        if (len > 2.0f * kSegmentBias) {
+            const float eps = kSegmentBias / len;
+            const RayPre r = ray_setup(tx.x, tx.y, tx.z, seg.x, seg.y, seg.z);
+            wall_count = gather_hits(s, r, eps, 1.0f - eps, q.max_depth, walls);
+        }
+        depth = min(wall_count, q.max_depth);
+        double carrier = norm(to_d(rx) - to_d(tx));
+        for (int w = 0; w < depth; ++w) {
+            double off;
+            const V3<double> nd = group_normal(s, walls[w].group, off);
+            const V3<double> dir = (to_d(rx) - to_d(tx)) * (1.0 / norm(to_d(rx) - to_d(tx)));
+            const double ct = fmin(fmax(fabs(dot(dir, nd)), 1e-6), 1.0);
+            const int m = s.face_material[walls[w].face];
+            double wall = 0.0;
+            for (int l = 0; l < s.mat_layer_count[m]; ++l)
+                wall += fmax((double)s.layer_thickness[s.mat_layer_offset[m] + l], 0.0);
+            carrier -= wall * ct;
+            pp.material[w] = m;
+            pp.eps[w] = s.layer_eps[s.mat_layer_offset[m]];
+            pp.sig[w] = s.layer_sigma[s.mat_layer_offset[m]];
+        }
+        length_d = carrier;
+        pp.count = 4 + 2 * depth;
+    } else {
+        length_d = norm(to_d(rx) - to_d(tx));
+    }
+    const int slot_id = slot_of_component(comp);
+    const int passes = (pp.count + kTangents - 1) / kTangents;
+    for (int pass = 0; pass < passes; ++pass) {
+        const V3<D8> txd{seeded(tx.x, 0, pass), seeded(tx.y, 1, pass), seeded(tx.z, 2, pass)};
+        const D8 fd = seeded(freq, 3, pass);
+        D8 eps[kMaxDepth], sig[kMaxDepth];
+        const int nm = (pp.count - 4) / 2;
+        for (int k = 0; k < nm; ++k) {
+            eps[k] = seeded(pp.eps[k], 4 + 2 * k, pass);
+            sig[k] = seeded(pp.sig[k], 5 + 2 * k, pass);
+        }
+        V3<D8> arrival;
+        C3<D8> field;
+        if constexpr (kPhase == kPhaseDiff) {
+            field = diffraction_tangent(s, s.edges[cand], txd, rx, tx_pol, fd, eps, sig, k_d, length_d, arrival);
+        } else {
             if (comp == kCidLos) {
                 field = los_tangent(txd, rx, tx_pol, fd, k_d, length_d, arrival);
             } else if (comp == kCidRefl) {
                 field = reflection_tangent(s, depth, groups, faces, txd, rx, tx_pol, fd, eps, sig, k_d, length_d,
                                            arrival);
-            } else if (comp == kCidDiff) {
-                field = diffraction_tangent(s, s.edges[cand], txd, rx, tx_pol, fd, eps, sig, k_d, length_d, arrival);
             } else {
-                field = transmission_tangent(s, depth, walls, txd, rx, tx_pol, fd, eps, sig, k_d, length_d, arrival);
+                field = transmission_tangent(s, depth, walls, txd, rx, tx_pol, fd, eps, sig, k_d, length_d,
+                                             arrival);
             }
-            float d[kTangents];
-            contract(field, arrival, rx_pol, sqrt_power, slot_id, adj, comp == kCidDiff, d);
-            scatter(g, pp, pass, d);
         }
+        float d[kTangents];
+        contract(field, arrival, rx_pol, sqrt_power, slot_id, adj, kPhase == kPhaseDiff, d);
+        scatter(g, pp, pass, d);
     }
 }
 
@@ -272,14 +276,57 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-__global__ void __launch_bounds__(256) vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles,
-                                                  int tile_count) {
+__device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int n = __shfl_up(v, off);
+        if (lane >= off) v += n;
+    }
+    return v;
+}
+
+// occupancy target (waves per SIMD) of the adjoint; 1 leaves the register
+// budget to the compiler
+#ifndef WC_VJP_WAVES_PER_EU
+#define WC_VJP_WAVES_PER_EU 1
+#endif
+
+// One wavefront per tile of 64 units (unit = one lane-segment of a cell's
+// descriptors; 64 / L cells). The tile's paths of this phase are flattened
+// over its units (wave prefix sum of their counts) and dealt to the 64 lanes
+// round-robin, so a tile costs ceil(paths / 64) path evaluations per lane
+// whatever the spread of paths over its cells. Within a unit the reflection
+// pass stored its descriptors before the diffraction pass appended its own.
+// kPhaseMain writes the tile's partial vector, kPhaseDiff adds to it.
+template <int kPhase>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VJP_WAVES_PER_EU)))
+vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles, int tile_count) {
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (tile >= tile_count) return;
     const int width = q.lanes_per_cell;
     const int t = tiles[2 * tile];
-    const int cell = tiles[2 * tile + 1] + lane / width;
+    const int pos = tiles[2 * tile + 1] + lane / width;
+    const int cap_sub = ga.topology_capacity / width;
+    int cell = 0, n_main = 0, n_total = 0;
+    const int* seg = ga.topology;
+    if (pos < q.cell_offset[t + 1]) {
+        cell = q.cell_order ? q.cell_order[pos] : pos;
+        seg = ga.topology + (size_t)cell * ga.topology_capacity + (size_t)(lane & (width - 1)) * cap_sub;
+        int first_diff = -1;
+        while (n_total < cap_sub) {
+            const int d = seg[n_total];
+            if (d == -1) break;
+            if (first_diff < 0 && (d >> 24) == kCidDiff) first_diff = n_total;
+            ++n_total;
+        }
+        n_main = first_diff < 0 ? n_total : first_diff;
+    }
+    const int mine = kPhase == kPhaseMain ? n_main : n_total - n_main;
+    const int first = kPhase == kPhaseMain ? 0 : n_main;
+    const int inclusive = wave_inclusive_scan(mine, lane);
+    const int exclusive = inclusive - mine;
+    const int total = __shfl(inclusive, 63);
     LaneGrad g;
 #pragma unroll
     for (int i = 0; i < kMaxMaterials; ++i) {
@@ -288,7 +335,25 @@ __global__ void __launch_bounds__(256) vjp_kernel(SceneView s, wc_solve q, wc_ad
     }
     g.freq = 0.0f;
     g.tx[0] = g.tx[1] = g.tx[2] = 0.0f;
-    if (cell < q.cell_offset[t + 1]) adjoint_cell(s, q, ga, t, cell, lane & (width - 1), width, g);
+    for (int base = 0; base < total; base += 64) {
+        // owner unit u of path j: the last unit whose exclusive prefix is <= j
+        // (all lanes take part in the cross-lane reads)
+        const int j = base + lane;
+        int u = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const int probe = u + step;
+            if (__shfl(exclusive, probe) <= j) u = probe;
+        }
+        const int u_cell = __shfl(cell, u);
+        const int u_first = __shfl(first, u);
+        const int u_excl = __shfl(exclusive, u);
+        const int u_sub = u & (width - 1);
+        if (j < total) {
+            const int* useg = ga.topology + (size_t)u_cell * ga.topology_capacity + (size_t)u_sub * cap_sub;
+            adjoint_path<kPhase>(s, q, ga, t, u_cell, useg[u_first + (j - u_excl)], g);
+        }
+    }
     const int m = s.material_count;
     const int stride = 2 * m + 4;
     float* out = ga.partials + (size_t)tile * stride;
@@ -297,17 +362,29 @@ __global__ void __launch_bounds__(256) vjp_kernel(SceneView s, wc_solve q, wc_ad
         const float se = wave_sum(g.eps[i]);
         const float ss = wave_sum(g.sig[i]);
         if (lane == 0 && i < m) {
-            out[i] = se;
-            out[m + i] = ss;
+            if (kPhase == kPhaseMain) {
+                out[i] = se;
+                out[m + i] = ss;
+            } else {
+                out[i] += se;
+                out[m + i] += ss;
+            }
         }
     }
     const float sf = wave_sum(g.freq);
     const float sx = wave_sum(g.tx[0]), sy = wave_sum(g.tx[1]), sz = wave_sum(g.tx[2]);
     if (lane == 0) {
-        out[2 * m] = sf;
-        out[2 * m + 1] = sx;
-        out[2 * m + 2] = sy;
-        out[2 * m + 3] = sz;
+        if (kPhase == kPhaseMain) {
+            out[2 * m] = sf;
+            out[2 * m + 1] = sx;
+            out[2 * m + 2] = sy;
+            out[2 * m + 3] = sz;
+        } else {
+            out[2 * m] += sf;
+            out[2 * m + 1] += sx;
+            out[2 * m + 2] += sy;
+            out[2 * m + 3] += sz;
+        }
     }
 }
 
@@ -364,9 +441,14 @@ extern "C" int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, con
     if (!tiles || tile_count < 0) return wc_fail("wc_radiomap_vjp: tiles are required");
     if (tile_count == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    if (adjoint->topology_capacity % solve->lanes_per_cell != 0)
+        return wc_fail("wc_radiomap_vjp: topology_capacity must be a multiple of lanes_per_cell");
     const int blocks = (tile_count + 3) / 4;
-    hipLaunchKernelGGL(vjp_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve, *adjoint, tiles,
-                       tile_count);
+    hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve, *adjoint,
+                       tiles, tile_count);
+    if (solve->components & kCompDiff)
+        hipLaunchKernelGGL(vjp_kernel<kPhaseDiff>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
+                           *adjoint, tiles, tile_count);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_vjp", err);
 }

@@ -59,7 +59,7 @@ class WcSolve(ctypes.Structure):
         ("rx_shared", c_int), ("total_cells", c_int),
         ("chain_offset", c_void_p), ("chain_groups", c_void_p), ("chain_images", c_void_p),
         ("frequency_hz", c_float), ("components", c_int), ("max_depth", c_int), ("coherent", c_int),
-        ("lanes_per_cell", c_int),
+        ("lanes_per_cell", c_int), ("cell_order", c_void_p),
     ]
 
 
@@ -100,6 +100,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_radiomap_workspace_bytes": (c_size_t, [c_int, c_int]),
     "wc_radiomap_forward": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                     ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_void_p]),
+    "wc_radiomap_pass": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
+                                 ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "wc_radiomap_vjp": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
                                 c_void_p, c_int, c_void_p]),
     "wc_vjp_partial_stride": (c_int, [c_int]),
@@ -109,6 +111,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                               ctypes.POINTER(WcPathRows), c_void_p]),
     "wc_paths_canonical_order": (c_int, [ctypes.POINTER(WcPathRows), c_void_p, c_int, c_void_p, c_void_p,
                                          c_void_p]),
+    "wc_paths_cfr": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, ctypes.c_double,
+                             ctypes.c_double, c_int, c_int, c_void_p, c_void_p]),
     "wc_field_free_space": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p]),

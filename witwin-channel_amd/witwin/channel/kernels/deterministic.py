@@ -183,16 +183,25 @@ class MeanDbStep:
         self.graph: torch.cuda.CUDAGraph | None = None
 
     def launch(self) -> None:
-        inputs = self.inputs
-        engine.radiomap_forward(self.dscene, inputs, topology_capacity=self.capacity, out=self.out)
+        engine.radiomap_forward(self.dscene, self.inputs, topology_capacity=self.capacity, out=self.out)
         power = self.out.power_total
         torch.div(self.scale, torch.clamp_min(power, 1.0e-38), out=self.grad_power)
-        p = _native.ptr
-        stream = _native.stream_handle(self.dscene.device)
+        self.adjoint()
+        self.reduce()
+
+    def adjoint(self) -> None:
+        """The adjoint launch alone (per-tile partial gradients)."""
+        inputs = self.inputs
         _native.call("wc_radiomap_vjp", self.dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(self.view),
-                     p(inputs.tiles), inputs.tile_count, stream)
+                     _native.ptr(inputs.tiles), inputs.tile_count, _native.stream_handle(self.dscene.device))
+
+    def reduce(self) -> None:
+        """Deterministic column sums of the per-tile partials."""
+        inputs = self.inputs
+        p = _native.ptr
         _native.call("wc_vjp_reduce", p(self.partials), inputs.tile_count, self.stride, p(inputs.tiles),
-                     int(inputs.view.tx_count), p(self.grad_params), p(self.grad_tx), stream)
+                     int(inputs.view.tx_count), p(self.grad_params), p(self.grad_tx),
+                     _native.stream_handle(self.dscene.device))
 
     def capture(self) -> None:
         side = torch.cuda.Stream(self.dscene.device)

@@ -21,6 +21,7 @@ like the reference's single count read (docs/dev/deterministic/README.md:30-36).
 
 from __future__ import annotations
 
+import ctypes
 import itertools
 from dataclasses import dataclass
 from typing import Any
@@ -165,6 +166,7 @@ class SolveInputs:
     tiles: torch.Tensor
     tile_count: int
     rx_count: np.ndarray   # receivers per transmitter (host)
+    cell_order: torch.Tensor | None = None  # evaluation position -> cell
 
     @property
     def total_cells(self) -> int:
@@ -183,17 +185,62 @@ def make_tiles(cell_offset: np.ndarray, tile_cells: int = TILE) -> np.ndarray:
     return np.concatenate(parts, axis=0).astype(np.int32)
 
 
+def _spread_bits(v: np.ndarray) -> np.ndarray:
+    """Interleave zeros into the low 21 bits of ``v`` (bit i -> bit 2i)."""
+    v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
+    for shift, mask in ((16, 0x0000FFFF0000FFFF), (8, 0x00FF00FF00FF00FF), (4, 0x0F0F0F0F0F0F0F0F),
+                        (2, 0x3333333333333333), (1, 0x5555555555555555)):
+        v = (v | (v << np.uint64(shift))) & np.uint64(mask)
+    return v
+
+
+def locality_permutation(points: np.ndarray) -> np.ndarray:
+    """Order of ``points`` [R,3] along a Z-order curve over the ranks of their
+    x and y coordinates: a regular grid falls into 8 x 8 blocks per 64
+    consecutive entries, so the receivers of one wavefront are neighbours and
+    take the same branches (same visible chains and edges, similar traversals).
+    Ties keep input order, so the permutation is deterministic."""
+    pts = np.asarray(points, dtype=np.float64).reshape(-1, 3)
+    if pts.shape[0] == 0:
+        return np.zeros(0, dtype=np.int64)
+    _, xr = np.unique(pts[:, 0], return_inverse=True)
+    _, yr = np.unique(pts[:, 1], return_inverse=True)
+    key = _spread_bits(xr.reshape(-1)) | (_spread_bits(yr.reshape(-1)) << np.uint64(1))
+    return np.argsort(key, kind="stable")
+
+
+def cell_order_table(rx_pos: np.ndarray, offsets: np.ndarray, shared: bool) -> np.ndarray:
+    """Evaluation order of every cell (see wc_solve.cell_order): each
+    transmitter's range is permuted onto itself by ``locality_permutation`` of
+    its receivers."""
+    offsets = np.asarray(offsets, dtype=np.int64)
+    total = int(offsets[-1])
+    order = np.empty(total, dtype=np.int64)
+    if shared:
+        perm = locality_permutation(rx_pos)
+        for t in range(offsets.shape[0] - 1):
+            order[offsets[t]:offsets[t + 1]] = offsets[t] + perm
+    else:
+        pts = np.asarray(rx_pos, dtype=np.float64).reshape(-1, 3)
+        for t in range(offsets.shape[0] - 1):
+            a, b = int(offsets[t]), int(offsets[t + 1])
+            order[a:b] = a + locality_permutation(pts[a:b])
+    return order.astype(np.int32)
+
+
 def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: int, max_depth: int,
                  coherent: bool, frequency_hz: float, tx_pos: np.ndarray | torch.Tensor | None = None,
                  tx_pol: np.ndarray | None = None, tx_power: np.ndarray | None = None,
                  rx_pos: np.ndarray | None = None, rx_pol: np.ndarray | None = None,
                  rx_count: np.ndarray | None = None, ray_count: int = DISCOVERY_RAY_COUNT,
-                 chains: ChainPlan | None = None, lanes_per_cell: int = LANES_PER_CELL) -> SolveInputs:
+                 chains: ChainPlan | None = None, lanes_per_cell: int = LANES_PER_CELL,
+                 spatial_order: bool = True) -> SolveInputs:
     """Upload endpoint tables. Without ``rx_count`` every transmitter shares
     all receivers; with it, receivers are the concatenation of per-transmitter
     sets of the given sizes (uneven batched solves). ``lanes_per_cell`` lanes
     split each cell's reflection chains and diffraction edges, so a tile holds
-    64 / lanes_per_cell cells."""
+    64 / lanes_per_cell cells. ``spatial_order`` evaluates cells in the
+    Z-order of their receivers (outputs are unchanged, bit for bit)."""
     if lanes_per_cell not in (1, 2, 4, 8, 16, 32, 64):
         raise ValueError("lanes_per_cell must be a power of two in [1, 64]")
     device = dscene.device
@@ -224,11 +271,14 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
         chains = plan_chains(compiled, dscene, tx_pos_t, max_depth, reflection=bool(components & 2),
                              ray_count=ray_count)
     cell_offset = torch.as_tensor(offsets.astype(np.int32)).to(device)
+    rx_host = rx_pos.detach().cpu().numpy() if isinstance(rx_pos, torch.Tensor) else rx_pos
+    order = (torch.as_tensor(cell_order_table(rx_host, offsets, shared)).to(device)
+             if spatial_order and offsets[-1] > 0 else None)
     tiles_np = make_tiles(offsets, TILE // lanes_per_cell)
     tiles = torch.as_tensor(tiles_np if tiles_np.size else np.zeros((1, 2), np.int32)).to(device)
     inputs = SolveInputs(tx_pos_t, f32(tx_pol).reshape(-1, 3), f32(tx_power).reshape(-1), f32(rx_pos).reshape(-1, 3),
                          f32(rx_pol).reshape(-1, 3), cell_offset, shared, chains, None, tiles,  # type: ignore[arg-type]
-                         int(tiles_np.shape[0]), counts)
+                         int(tiles_np.shape[0]), counts, order)
     p = _native.ptr
     inputs.view = _native.WcSolve(
         tx_pos=p(inputs.tx_pos), tx_pol=p(inputs.tx_pol), tx_power=p(inputs.tx_power), tx_count=t,
@@ -236,7 +286,7 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
         rx_shared=1 if shared else 0, total_cells=int(offsets[-1]), chain_offset=p(chains.offset),
         chain_groups=p(chains.groups), chain_images=p(chains.images), frequency_hz=float(frequency_hz),
         components=int(components), max_depth=int(max_depth), coherent=1 if coherent else 0,
-        lanes_per_cell=int(lanes_per_cell),
+        lanes_per_cell=int(lanes_per_cell), cell_order=p(order),
     )
     return inputs
 
@@ -281,21 +331,41 @@ def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capac
                                   dtype=torch.uint8, device=device),
         )
     p = _native.ptr
-    view = _native.WcRadiomapOut(
+    view = _out_view(out)
+    counter = _tile_counter(inputs, device)
+    _native.call("wc_radiomap_forward", dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(view),
+                 p(inputs.tiles) if tiled else None, inputs.tile_count if tiled else 0,
+                 p(counter) if tiled else None, _native.stream_handle(device))
+    return out
+
+
+def _out_view(out: RadiomapOutputs) -> _native.WcRadiomapOut:
+    p = _native.ptr
+    return _native.WcRadiomapOut(
         power_total=p(out.power_total), field_total=p(out.field_total), component_power=p(out.component_power),
         component_field=p(out.component_field), diffraction_vector=p(out.diffraction_vector),
         path_count=p(out.path_count), topology=p(out.topology), topology_capacity=out.topology_capacity,
         failure=p(out.failure), workspace=p(out.workspace),
     )
+
+
+def _tile_counter(inputs: SolveInputs, device: torch.device) -> torch.Tensor:
     counter = inputs.__dict__.get("_tile_counter")
     if counter is None:
         counter = torch.zeros(2, dtype=torch.int32, device=device)
         inputs.__dict__["_tile_counter"] = counter
-    import ctypes
-    _native.call("wc_radiomap_forward", dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(view),
-                 p(inputs.tiles) if tiled else None, inputs.tile_count if tiled else 0,
-                 p(counter) if tiled else None, _native.stream_handle(device))
-    return out
+    return counter
+
+
+def radiomap_pass(dscene: DeviceScene, inputs: SolveInputs, out: RadiomapOutputs, pass_index: int) -> None:
+    """One launch of the tiled forward: pass 1 (LoS / reflection /
+    transmission) or pass 2 (diffraction, after pass 1 of the same solve into
+    the same ``out``)."""
+    p = _native.ptr
+    view = _out_view(out)
+    counter = _tile_counter(inputs, dscene.device)
+    _native.call("wc_radiomap_pass", dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(view), p(inputs.tiles),
+                 inputs.tile_count, p(counter), int(pass_index), _native.stream_handle(dscene.device))
 
 
 def forward_with_topology(dscene: DeviceScene, inputs: SolveInputs, *, capacity: int = 64,
@@ -322,7 +392,6 @@ ROW_FIELDS_FLOAT = ("path_length_m", "delay_s", "path_gain")
 def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int) -> dict[str, Any]:
     """Explicit path rows in canonical order: pairs rx-major, tx-minor; inside a
     pair (depth, component, primitive sequence, primitive, edge)."""
-    import ctypes
 
     device = dscene.device
     cells = inputs.total_cells
