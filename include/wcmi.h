@@ -151,8 +151,10 @@ typedef struct wc_solve {
  * tiles != NULL: persistent launches over wave-aligned tiles (tx, first cell)
  * of 64 cells pulled from global work counters (tile_counter [2] i32, zeroed
  * by the call with a fill kernel, so the sequence is graph-capturable).
- * The evaluation runs as two launches, reflection/LoS/transmission then
- * diffraction, each with its own register budget. */
+ * The evaluation runs as launches with their own register budgets:
+ * reflection/LoS/transmission, then (with diffraction) a visibility launch
+ * that tests both legs of every (cell, edge) candidate into bit masks, and
+ * the diffraction field launch over the visible edges. */
 typedef struct wc_radiomap_out {
     float* power_total;
     float* field_total;
@@ -163,19 +165,21 @@ typedef struct wc_radiomap_out {
     int* topology;
     int topology_capacity;
     int* failure;
-    void* workspace; /* wc_radiomap_workspace_bytes(cells, L): per-lane sums carried
-                        from the reflection pass to the diffraction pass */
+    void* workspace; /* wc_radiomap_workspace_bytes(cells, L, edge_count): per-lane
+                        sums carried from the reflection pass to the diffraction
+                        pass, then one edge-visibility bit per (cell, edge) */
 } wc_radiomap_out;
 
-size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell);
+size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count);
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
 
 /* One launch of wc_radiomap_forward's sequence (tiled form only), for callers
  * that time or schedule the passes apart: pass 1 zeroes tile_counter[0] and
  * runs the reflection/LoS/transmission pass, pass 2 zeroes tile_counter[1]
- * and runs the diffraction pass, which reads the per-cell sums pass 1 left in
- * `out->workspace` (so pass 2 must follow pass 1 of the same solve).
+ * and runs the diffraction visibility and field launches, which read the
+ * per-cell sums pass 1 left in `out->workspace` (so pass 2 must follow pass 1
+ * of the same solve).
  * Calling pass 1 then pass 2 (when the solve has diffraction) is equivalent to
  * wc_radiomap_forward. No reference counterpart: the reference runs its
  * radiomap as one sequence of launches. */

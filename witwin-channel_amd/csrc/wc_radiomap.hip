@@ -74,6 +74,23 @@ __device__ __forceinline__ bool segment_clear(const SceneView& s, const V3<float
     return !occluded(s, r, eps, 1.0f - eps, ig0, ig1);
 }
 
+// Visibility of a diffraction candidate at the stationary point clamped to
+// the finite edge; the receiver leg is tested first (street-level receivers
+// see fewer edges than an elevated transmitter).
+__device__ __forceinline__ bool edge_visible(const SceneView& s, const Edge& e, const DiffGeometry& g,
+                                             const V3<float>& tx, const V3<float>& rx) {
+    const double tc = fmin(fmax(g.tstar, 0.0), (double)e.length);
+    const V3<float> qc{e.ax + e.dx * (float)tc, e.ay + e.dy * (float)tc, e.az + e.dz * (float)tc};
+    // one traversal instance (one stack) serves both legs
+#pragma unroll 1
+    for (int leg = 0; leg < 2; ++leg) {
+        const V3<float> a = leg == 0 ? qc : tx;
+        const V3<float> b = leg == 0 ? rx : qc;
+        if (!segment_clear(s, a, b, e.group0, e.group1)) return false;
+    }
+    return true;
+}
+
 // Passes of the fused evaluation: all components (row export), the
 // non-diffraction components, or diffraction only. Splitting the radiomap
 // into two launches gives each its own register budget.
@@ -85,7 +102,8 @@ enum EvalPass : int { kPassAll = 0, kPassMain = 1, kPassDiff = 2 };
 // visits every L-th reflection chain and diffraction edge (line of sight and
 // transmission belong to sub-lane 0).
 template <class Sink, int kPass>
-__device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, int sub, int stride, Sink& sink) {
+__device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, int sub, int stride, Sink& sink,
+                          const unsigned* diff_mask = nullptr, int mask_words = 0) {
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
     const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
@@ -142,17 +160,27 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
             }
         }
         if (kDiff && d == 1 && (comps & kCompDiff)) {
-            for (int ei = sub; ei < s.edge_count; ei += stride) {
+            // with a mask (from diff_visibility_kernel) only the visible edges
+            // are visited, in ascending order; without one every edge is
+            // tested here
+            int word = 0;
+            unsigned bits = 0;
+            int ei = diff_mask ? -1 : sub - stride;
+            while (true) {
+                if (diff_mask) {
+                    while (bits == 0 && word < mask_words) bits = diff_mask[word++];
+                    if (bits == 0) break;
+                    ei = (word - 1) * 32 + __ffs(bits) - 1;
+                    bits &= bits - 1;
+                    if (ei % stride != sub) continue;
+                } else {
+                    ei += stride;
+                    if (ei >= s.edge_count) break;
+                }
                 const Edge e = s.edges[ei];
                 DiffGeometry g;
                 if (!diffraction_geometry(e, tx, rx, g)) continue;
-                // visibility at the clamped point of the finite edge; the
-                // receiver leg is tested first (street-level receivers see
-                // fewer edges than an elevated transmitter)
-                const double tc = fmin(fmax(g.tstar, 0.0), (double)e.length);
-                const V3<float> qc{e.ax + e.dx * (float)tc, e.ay + e.dy * (float)tc, e.az + e.dz * (float)tc};
-                if (!segment_clear(s, qc, rx, e.group0, e.group1)) continue;
-                if (!segment_clear(s, tx, qc, e.group0, e.group1)) continue;
+                if (!diff_mask && !edge_visible(s, e, g, tx, rx)) continue;
                 PathRecord rec;
                 rec.component = kCidDiff;
                 rec.depth = 1;
@@ -297,6 +325,18 @@ __device__ __forceinline__ int tx_of_cell(const wc_solve& q, int cell) {
     return lo;
 }
 
+// Workspace of a radiomap with diffraction: per-(cell, lane) partial sums
+// (32 B each) carried from the reflection pass, then the per-cell edge
+// visibility masks of the diffraction pass (mask_words(E) words per cell).
+__host__ __device__ __forceinline__ int mask_words(int edge_count) { return (edge_count + 31) >> 5; }
+__host__ __device__ __forceinline__ size_t partial_bytes(int cells, int lanes) {
+    return (size_t)cells * (size_t)lanes * 32;
+}
+__device__ __forceinline__ unsigned* diff_masks(const wc_radiomap_out& o, const wc_solve& q) {
+    return reinterpret_cast<unsigned*>(static_cast<char*>(o.workspace) + partial_bytes(q.total_cells,
+                                                                                        q.lanes_per_cell));
+}
+
 // cell evaluated at launch position `pos` (identity without an order table)
 __device__ __forceinline__ int cell_at(const wc_solve& q, int pos) {
     return q.cell_order ? q.cell_order[pos] : pos;
@@ -376,7 +416,15 @@ __device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve
         a.topo = o.topology + (size_t)cell * o.topology_capacity + (size_t)sub * cap_sub;
         a.cap = cap_sub;
     }
-    if (active) eval_cell<AccumSink, kPass>(s, q, t, cell, sub, width, a);
+    if (active) {
+        if (kPass == kPassDiff) {
+            const int words = mask_words(s.edge_count);
+            const unsigned* mask = diff_masks(o, q) + (size_t)cell * words;
+            eval_cell<AccumSink, kPass>(s, q, t, cell, sub, width, a, mask, words);
+        } else {
+            eval_cell<AccumSink, kPass>(s, q, t, cell, sub, width, a);
+        }
+    }
     if (active && o.topology) {
         if (a.count < cap_sub) a.topo[a.count] = -1;
         if (a.count > cap_sub && o.failure) atomicOr(o.failure, 2);
@@ -469,6 +517,42 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_DIF
 radiomap_diff_tiles(SceneView s, wc_solve q, wc_radiomap_out o, const int* __restrict__ tiles, int tile_count,
                     int* __restrict__ counter) {
     radiomap_tiles<kPassDiff>(s, q, o, tiles, tile_count, counter);
+}
+
+// Visibility of every (cell, edge) diffraction candidate: bit (ei & 31) of
+// word ei >> 5 of the cell's mask is set when the stationary point, clamped
+// to the finite edge, sees both the receiver and the transmitter. One
+// wavefront covers 64 consecutive evaluation positions x 32 edges (the edge
+// loads are wave-uniform). Nothing but geometry and traversal lives here, so
+// the kernel keeps few registers and many wavefronts per SIMD in flight to
+// hide the traversal's memory latency; the field pass then evaluates the
+// visible edges only.
+#ifndef WC_VIS_WAVES_PER_EU
+#define WC_VIS_WAVES_PER_EU 1
+#endif
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VIS_WAVES_PER_EU)))
+diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, int words, long long waves) {
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= waves) return;
+    const int chunk = (int)(wave % words);
+    const long long pos64 = (wave / words) * 64 + (threadIdx.x & 63);
+    if (pos64 >= q.total_cells) return;
+    const int pos = (int)pos64;
+    const int t = tx_of_cell(q, pos);
+    const int cell = cell_at(q, pos);
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
+    const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
+    const int e0 = chunk * 32;
+    const int e1 = min(e0 + 32, s.edge_count);
+    unsigned bits = 0;
+    for (int ei = e0; ei < e1; ++ei) {
+        const Edge e = s.edges[ei];
+        DiffGeometry g;
+        if (!diffraction_geometry(e, tx, rx, g)) continue;
+        if (edge_visible(s, e, g, tx, rx)) bits |= 1u << (ei - e0);
+    }
+    masks[(size_t)cell * words + chunk] = bits;
 }
 
 __global__ void __launch_bounds__(256) count_kernel(SceneView s, wc_solve q, int* __restrict__ out) {
@@ -668,6 +752,16 @@ static int check_solve(const wc_solve* q) {
 
 using namespace wc;
 
+static void launch_diff_visibility(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o, hipStream_t st) {
+    const int words = mask_words(s.edge_count);
+    if (words == 0) return;
+    unsigned* masks = reinterpret_cast<unsigned*>(static_cast<char*>(o.workspace) +
+                                                  partial_bytes(q.total_cells, q.lanes_per_cell));
+    const long long waves = ((long long)q.total_cells + 63) / 64 * words;
+    hipLaunchKernelGGL(diff_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q, masks, words,
+                       waves);
+}
+
 // passes: bit 0 the reflection/LoS/transmission pass, bit 1 the diffraction pass
 static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                            const int* tiles, int tile_count, int* tile_counter, int passes, const char* who,
@@ -713,16 +807,20 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
         if (passes & 1)
             hipLaunchKernelGGL(radiomap_main_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter);
-        if (passes & 2)
+        if (passes & 2) {
+            launch_diff_visibility(s, *solve, *out, st);
             hipLaunchKernelGGL(radiomap_diff_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter + 1);
+        }
     } else {
         if (passes & 1)
             hipLaunchKernelGGL(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
                                st, s, *solve, *out);
-        if (passes & 2)
+        if (passes & 2) {
+            launch_diff_visibility(s, *solve, *out, st);
             hipLaunchKernelGGL(radiomap_kernel<kPassDiff>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
                                st, s, *solve, *out);
+        }
     }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip(who, err);
@@ -742,8 +840,10 @@ extern "C" int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, co
     return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, pass, "wc_radiomap_pass", stream);
 }
 
-extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell) {
-    return (size_t)(cells > 0 ? cells : 0) * (size_t)(lanes_per_cell > 0 ? lanes_per_cell : 1) * 32;
+extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count) {
+    const int c = cells > 0 ? cells : 0;
+    return partial_bytes(c, lanes_per_cell > 0 ? lanes_per_cell : 1) +
+           (size_t)c * (size_t)mask_words(edge_count > 0 ? edge_count : 0) * 4;
 }
 
 extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream) {

@@ -327,7 +327,8 @@ def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capac
             topology_capacity=topology_capacity,
             failure=torch.zeros(1, dtype=torch.int32, device=device),
             workspace=torch.empty(max(int(_native.call("wc_radiomap_workspace_bytes", cells,
-                                                       int(inputs.view.lanes_per_cell))), 32),
+                                                       int(inputs.view.lanes_per_cell),
+                                                       int(dscene.view.edge_count))), 32),
                                   dtype=torch.uint8, device=device),
         )
     p = _native.ptr
