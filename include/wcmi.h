@@ -341,6 +341,15 @@ int wc_discover_chains(const wc_scene* scene, const float* tx_pos, int tx_count,
 int wc_chain_images(const wc_scene* scene, const float* tx_pos, const int* chain_tx, const int* chain_groups,
                     const int* chain_depth, int chain_count, double* images, void* stream);
 
+/* ------------------------------------------------------------ diagnostics
+ * Element-wise evaluation of the device special functions the UTD
+ * coefficient uses: kind 0 = Kouyoumjian-Pathak transition function F(x),
+ * kind 1 = complex Fresnel integral C(x) + j S(x); x [n] f64, out [n] c128
+ * (interleaved re, im). No reference counterpart: it exposes the kernels'
+ * own evaluation (float64, Chebyshev/asymptotic Fresnel auxiliary functions)
+ * to accuracy tests. */
+int wc_utd_special_eval(int kind, const double* x, int n, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import math
 import sys
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -161,6 +162,44 @@ def test_bvh_visibility_matches_brute_force(device):
             mismatch += 1
     _say(f"visibility mismatches: {mismatch} / {n}")
     assert mismatch == 0
+
+
+def test_utd_special_functions_match_extended_precision(device):
+    """The device transition function F(X) and Fresnel integral Fr(x) against
+    extended-precision references: F = sqrt(2 pi X) (f + j g)(sqrt(2X/pi))
+    with f, g from the series / continued fraction of gen_fresnel_aux.py, and
+    scipy.special.fresnel for Fr."""
+    _say("case: device UTD special functions, X in [1e-12, 1e4], x in [-60, 60]")
+    import importlib.util
+
+    from scipy.special import fresnel as sp_fresnel
+
+    spec = importlib.util.spec_from_file_location(
+        "gen_fresnel_aux", Path(__file__).resolve().parents[1] / "witwin-channel_amd" / "csrc" / "gen_fresnel_aux.py")
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    xs = np.concatenate([np.logspace(-12, 4, 300), [32 * math.pi * (1 - 1e-12), 32 * math.pi, 32 * math.pi * 1.001]])
+    dev_x = torch.as_tensor(xs).to(device)
+    out = torch.empty(2 * xs.size, dtype=torch.float64, device=device)
+    _native.call("wc_utd_special_eval", 0, dev_x.data_ptr(), xs.size, out.data_ptr(), _native.stream_handle(device))
+    got = out.cpu().numpy().reshape(-1, 2)
+    worst = 0.0
+    for (re, im), X in zip(got, xs):
+        f, g = gen.aux(np.sqrt(2.0 * X / np.pi))
+        r = np.sqrt(2.0 * np.pi * X)
+        ref = complex(float(r * f), float(r * g))
+        worst = max(worst, abs(complex(re, im) - ref) / abs(ref))
+    _say(f"transition function: max relative error {worst:.2e}")
+    assert worst < 1e-13
+    xf = np.concatenate([np.linspace(-60, 60, 2401), [0.0, 1e-9, -1e-9, 7.999999, 8.0, 8.000001]])
+    dev_x = torch.as_tensor(xf).to(device)
+    out = torch.empty(2 * xf.size, dtype=torch.float64, device=device)
+    _native.call("wc_utd_special_eval", 1, dev_x.data_ptr(), xf.size, out.data_ptr(), _native.stream_handle(device))
+    got = out.cpu().numpy().reshape(-1, 2)
+    s_ref, c_ref = sp_fresnel(xf)
+    err = max(np.abs(got[:, 0] - c_ref).max(), np.abs(got[:, 1] - s_ref).max())
+    _say(f"Fresnel integral: max absolute error {err:.2e}")
+    assert err < 1e-14
 
 
 # ----------------------------------------------------------------- radiomaps

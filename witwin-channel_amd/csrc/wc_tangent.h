@@ -128,11 +128,11 @@ __device__ inline C3<R> transmission_tangent(const SceneView& s, int count, cons
 // value of one cot*F term in double plus its partial derivatives in beta and kL
 __device__ inline void utd_term_partials(double n, double beta, int sgn, double kL, double& re, double& im,
                                          double& db_re, double& db_im, double& dk_re, double& dk_im) {
-    utd_term(n, beta, sgn, kL, re, im);
     const double arg = (kPi + sgn * beta) / (2.0 * n);
     const double sa = sin(arg);
     if (fabs(sa) < 1.0e-9) {
         // shadow-boundary limit n sqrt(2 pi kL) sgn(eps) e^{j pi/4}: d/dkL only
+        utd_term(n, beta, sgn, kL, re, im);
         const double f = 0.5 / fmax(kL, 1e-30);
         dk_re = re * f;
         dk_im = im * f;
@@ -149,6 +149,9 @@ __device__ inline void utd_term_partials(double n, double beta, int sgn, double 
     const double dcot = -(sgn / (2.0 * n)) / (sa * sa);
     double f_re, f_im;
     utd_transition(X, f_re, f_im);
+    // the value of utd_term, from the one transition-function evaluation
+    re = cot * f_re;
+    im = cot * f_im;
     // F'(X) = (j + 1/(2X)) F - j
     double fp_re = 0.0, fp_im = 0.0;
     if (X > 0.0) {
@@ -180,10 +183,9 @@ template <int N> struct UtdTangent<Dual<N>> {
     }
     // Fr(x) = C(x) + jS(x), dFr/dx = e^{j pi x^2 / 2}
     __device__ static Cx<D> fresnel(const D& x) {
-        double c, s;
-        fresnel_complex((double)x.v, c, s);
-        const double ph = 0.5 * kPi * (double)x.v * (double)x.v;
-        const float gr = (float)cos(ph), gi = (float)sin(ph);
+        double c, s, dr, di;
+        fresnel_complex((double)x.v, c, s, &dr, &di);
+        const float gr = (float)dr, gi = (float)di;
         Cx<D> out;
         out.re.v = (float)c;
         out.im.v = (float)s;
@@ -203,10 +205,26 @@ __device__ inline R wedge_angle_t(const V3<R>& u, const V3<float>& t0, const V3<
     return a;
 }
 
+template <typename R> __device__ __forceinline__ Cx<float> cval(const Cx<R>& a) { return {val(a.re), val(a.im)}; }
+template <typename R> __device__ __forceinline__ V3<float> vval(const V3<R>& a) {
+    return {val(a.x), val(a.y), val(a.z)};
+}
+
+// The diffracted field is linear in the four face reflection coefficients
+// (r0_s, r0_h on face 0, rn_s, rn_h on face n), the only place the face
+// materials enter. Their partials (values only) and the incidence cosines
+// handed to each face's slab coefficients let the adjoint differentiate the
+// materials without carrying material tangents through the whole path.
+struct DiffFaceTerms {
+    C3<float> d_r[4];  // d field / d r0_s, r0_h, rn_s, rn_h
+    float cos_face[2];
+};
+
 template <typename R>
 __device__ inline C3<R> diffraction_tangent(const SceneView& s, const Edge& e, const V3<R>& tx, const V3<float>& rx,
                                             const V3<float>& tx_pol, const R& freq, const R* eps, const R* sig,
-                                            double k_d, double length_d, V3<R>& arrival) {
+                                            double k_d, double length_d, V3<R>& arrival,
+                                            DiffFaceTerms* faces = nullptr) {
     const V3<float> A{e.ax, e.ay, e.az}, ed{e.dx, e.dy, e.dz};
     const V3<float> t0{e.t0x, e.t0y, e.t0z}, n0{e.n0x, e.n0y, e.n0z};
     const V3<R> AR = lift3<R>(A), EdR = lift3<R>(ed), RxR = lift3<R>(rx);
@@ -239,10 +257,12 @@ __device__ inline C3<R> diffraction_tangent(const SceneView& s, const Edge& e, c
     const int m0 = e.material0;
     const int mn = e.material1 >= 0 ? e.material1 : e.material0;
     Cx<R> r0_s, r0_h, rn_s, rn_h;
-    slab_fresnel<R>(w_abs(w_sin(phi_p)), eps[0], sig[0], lift<R>(s.mat_mu[m0]), lift<R>(s.mat_gain[m0]),
+    const R cos0 = w_abs(w_sin(phi_p));
+    const R cosn = w_abs(w_sin(lift<R>(npi) - phi));
+    slab_fresnel<R>(cos0, eps[0], sig[0], lift<R>(s.mat_mu[m0]), lift<R>(s.mat_gain[m0]),
                     lift<R>(s.mat_thickness[m0]), freq, r0_s, r0_h);
-    slab_fresnel<R>(w_abs(w_sin(lift<R>(npi) - phi)), eps[1], sig[1], lift<R>(s.mat_mu[mn]),
-                    lift<R>(s.mat_gain[mn]), lift<R>(s.mat_thickness[mn]), freq, rn_s, rn_h);
+    slab_fresnel<R>(cosn, eps[1], sig[1], lift<R>(s.mat_mu[mn]), lift<R>(s.mat_gain[mn]),
+                    lift<R>(s.mat_thickness[mn]), freq, rn_s, rn_h);
     const double nn = (double)e.n;
     const R kL = k * L;
     const Cx<R> t1 = UtdTangent<R>::term(nn, phi - phi_p, +1, kL);
@@ -265,6 +285,18 @@ __device__ inline C3<R> diffraction_tangent(const SceneView& s, const Edge& e, c
     const R ph = PhaseOf<R>::make(k, so + sp, k_d, length_d);
     const Cx<R> prop = scale(trunc, amp) * cexpj(ph);
     arrival = sov;
+    if (faces) {
+        const Cx<float> pp = cval(pre) * cval(prop);
+        const Cx<float> a3 = pp * cval(t3), a4 = pp * cval(t4);
+        const V3<float> bo = vval(b0_o), po = vval(phi_o);
+        const Cx<float> sb{-val(e_b), 0.0f}, sp_{-val(e_p), 0.0f};
+        faces->d_r[0] = c3_axis(bo, a3 * sb);
+        faces->d_r[1] = c3_axis(po, a3 * sp_);
+        faces->d_r[2] = c3_axis(bo, a4 * sb);
+        faces->d_r[3] = c3_axis(po, a4 * sp_);
+        faces->cos_face[0] = val(cos0);
+        faces->cos_face[1] = val(cosn);
+    }
     return c3_axis(b0_o, (ds * prop) * Cx<R>{-e_b, lift<R>(0.f)}) + c3_axis(phi_o, (dh * prop) * Cx<R>{-e_p, lift<R>(0.f)});
 }
 

@@ -11,155 +11,106 @@
 #pragma once
 
 #include "wc_em.h"
+#include "wc_fresnel_aux_table.h"
 
 namespace wc {
 
-// Fresnel integrals C(x), S(x) (definition C + jS = int_0^x e^{j pi t^2 / 2} dt)
-// by the power series for |x| <= 1.5 and the continued fraction of the
-// complementary error function (modified Lentz) above
-// (Numerical Recipes, 2nd ed., section 6.9).
-// For x > 1.5 the continued-fraction value h is also returned so callers can
-// form the complementary integral without cancellation.
-WC_HD void fresnel_cs(double x, double& c, double& s, bool& use_cf, double& h_re, double& h_im) {
-    const double ax = fabs(x);
-    use_cf = false;
-    h_re = 0.0;
-    h_im = 0.0;
-    if (ax < 1.0e-150) {
-        c = x;
-        s = 0.0;
+// Fresnel auxiliary functions f(x), g(x), x >= 0 (Abramowitz & Stegun
+// 7.3.5-7.3.6), through which
+//   C(x) = 1/2 + f sin(pi x^2/2) - g cos(pi x^2/2),
+//   S(x) = 1/2 - f cos(pi x^2/2) - g sin(pi x^2/2),
+//   int_x^inf e^{j pi t^2/2} dt = (g + j f) e^{j pi x^2/2}.
+// Both are smooth and positive, so they carry full relative precision where
+// C and S themselves cancel. Below x = 8: 12-term Chebyshev expansions on 32
+// intervals (wc_fresnel_aux_table.h, generated in extended precision by
+// gen_fresnel_aux.py; max relative error 1.6e-15); above: the asymptotic
+// series A&S 7.3.27-7.3.28 to nine terms (truncation below 1e-19 at x = 8).
+__device__ inline void fresnel_aux(double x, double& f, double& g) {
+    if (x < kFresnelAuxMax) {
+        const double scaled = x * (kFresnelAuxIntervals / kFresnelAuxMax);
+        int i = (int)scaled;
+        if (i > kFresnelAuxIntervals - 1) i = kFresnelAuxIntervals - 1;
+        const double t = 2.0 * (scaled - (double)i) - 1.0;
+        const double* cf = kFresnelAuxF[i];
+        const double* cg = kFresnelAuxG[i];
+        double f1 = 0.0, f2 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+        for (int j = kFresnelAuxTerms - 1; j >= 1; --j) {
+            const double nf = fma(2.0 * t, f1, cf[j] - f2);
+            const double ng = fma(2.0 * t, g1, cg[j] - g2);
+            f2 = f1;
+            f1 = nf;
+            g2 = g1;
+            g1 = ng;
+        }
+        f = fma(t, f1, cf[0] - f2);
+        g = fma(t, g1, cg[0] - g2);
         return;
     }
-    if (ax <= 1.5) {
-        double sum = 0.0, sums = 0.0, sumc = ax;
-        double sign = 1.0;
-        const double fact = 0.5 * kPi * ax * ax;
-        bool odd = true;
-        double term = ax;
-        double n = 3.0;
-        for (int k = 1; k <= 200; ++k) {
-            term *= fact / k;
-            sum += sign * term / n;
-            const double test = fabs(sum) * 1.0e-16;
-            if (odd) {
-                sign = -sign;
-                sums = sum;
-                sum = sumc;
-            } else {
-                sumc = sum;
-                sum = sums;
-            }
-            if (term < test) break;
-            odd = !odd;
-            n += 2.0;
-        }
-        c = sumc;
-        s = sums;
-    } else {
-        const double pix2 = kPi * ax * ax;
-        // complex continued fraction, b = 1 - j*pix2
-        double b_re = 1.0, b_im = -pix2;
-        double cc_re = 1.0e300, cc_im = 0.0;
-        // d = h = 1 / b
-        double den = b_re * b_re + b_im * b_im;
-        double d_re = b_re / den, d_im = -b_im / den;
-        double hr = d_re, hi = d_im;
-        double n = -1.0;
-        for (int k = 2; k <= 400; ++k) {
-            n += 2.0;
-            const double a = -n * (n + 1.0);
-            b_re += 4.0;
-            // d = 1 / (a*d + b)
-            double t_re = a * d_re + b_re, t_im = a * d_im + b_im;
-            den = t_re * t_re + t_im * t_im;
-            d_re = t_re / den;
-            d_im = -t_im / den;
-            // cc = b + a / cc
-            den = cc_re * cc_re + cc_im * cc_im;
-            const double q_re = a * cc_re / den, q_im = -a * cc_im / den;
-            cc_re = b_re + q_re;
-            cc_im = b_im + q_im;
-            // del = cc * d
-            const double del_re = cc_re * d_re - cc_im * d_im;
-            const double del_im = cc_re * d_im + cc_im * d_re;
-            const double nh_re = hr * del_re - hi * del_im;
-            const double nh_im = hr * del_im + hi * del_re;
-            hr = nh_re;
-            hi = nh_im;
-            if (fabs(del_re - 1.0) + fabs(del_im) < 1.0e-15) break;
-        }
-        // h *= (ax - j ax)
-        const double h2_re = ax * hr + ax * hi;
-        const double h2_im = ax * hi - ax * hr;
-        // cs = (0.5 + 0.5j) * (1 - e^{j pix2/2} h)
-        const double e_re = cos(0.5 * pix2), e_im = sin(0.5 * pix2);
-        const double eh_re = e_re * h2_re - e_im * h2_im;
-        const double eh_im = e_re * h2_im + e_im * h2_re;
-        const double one_re = 1.0 - eh_re, one_im = -eh_im;
-        c = 0.5 * one_re - 0.5 * one_im;
-        s = 0.5 * one_re + 0.5 * one_im;
-        use_cf = true;
-        h_re = h2_re;
-        h_im = h2_im;
+    const double z = kPi * x * x;
+    const double inv_z2 = 1.0 / (z * z);
+    double tf = 1.0, tg = 1.0, sf = 1.0, sg = 1.0;
+#pragma unroll
+    for (int m = 1; m <= 9; ++m) {
+        tf *= -(double)((4 * m - 3) * (4 * m - 1)) * inv_z2;
+        tg *= -(double)((4 * m - 1) * (4 * m + 1)) * inv_z2;
+        sf += tf;
+        sg += tg;
     }
-    if (x < 0.0) {
-        c = -c;
-        s = -s;
-    }
+    f = sf / (kPi * x);
+    g = sg / (kPi * kPi * x * x * x);
 }
 
 // Kouyoumjian-Pathak transition function
-//   F(X) = 2j sqrt(X) e^{jX} int_{sqrt X}^inf e^{-j tau^2} dtau,  X >= 0,
-// with the substitution tau = u sqrt(pi/2):
-//   int_{sqrt X}^inf e^{-j tau^2} dtau = sqrt(pi/2) * conj(int_{u0}^inf e^{j pi u^2/2} du),
-//   u0 = sqrt(2X/pi).
-WC_HD void utd_transition(double X, double& f_re, double& f_im) {
+//   F(X) = 2j sqrt(X) e^{jX} int_{sqrt X}^inf e^{-j tau^2} dtau,  X >= 0.
+// With tau = u sqrt(pi/2) and u0 = sqrt(2X/pi) (so pi u0^2/2 = X) the
+// integral is sqrt(pi/2) conj((g + j f)(u0) e^{jX}), the e^{+-jX} cancel and
+//   F(X) = sqrt(2 pi X) (f(u0) + j g(u0)),
+// which keeps full relative precision as X -> 0 (F ~ sqrt(pi X) e^{j pi/4},
+// the shadow-boundary regime) and tends to 1 + j/(2X) for large X.
+__device__ inline void utd_transition(double X, double& f_re, double& f_im) {
     if (X <= 0.0) {
         f_re = 0.0;
         f_im = 0.0;
         return;
     }
-    const double u0 = sqrt(2.0 * X / kPi);
-    double c, s, h_re, h_im;
-    bool use_cf;
-    fresnel_cs(u0, c, s, use_cf, h_re, h_im);
-    const double sx = sqrt(X);
-    const double pref = 2.0 * sx * sqrt(0.5 * kPi);
-    if (use_cf) {
-        // int_{u0}^inf e^{j pi u^2/2} = (0.5+0.5j) e^{jX} h, so
-        // F = 2j sqrt(X) sqrt(pi/2) (0.5-0.5j) conj(h)   (the e^{+-jX} cancel)
-        const double ch_re = h_re, ch_im = -h_im;
-        const double m_re = 0.5 * ch_re + 0.5 * ch_im;   // (0.5-0.5j)*conj(h)
-        const double m_im = 0.5 * ch_im - 0.5 * ch_re;
-        // multiply by j * pref
-        f_re = -pref * m_im;
-        f_im = pref * m_re;
-        return;
-    }
-    // G = conj((0.5 - C) + j(0.5 - S))
-    const double g_re = 0.5 - c;
-    const double g_im = -(0.5 - s);
-    // F = j * pref * e^{jX} * G
-    const double ex_re = cos(X), ex_im = sin(X);
-    const double t_re = ex_re * g_re - ex_im * g_im;
-    const double t_im = ex_re * g_im + ex_im * g_re;
-    f_re = -pref * t_im;
-    f_im = pref * t_re;
+    double f, g;
+    fresnel_aux(sqrt(2.0 * X / kPi), f, g);
+    const double r = sqrt(2.0 * kPi * X);
+    f_re = r * f;
+    f_im = r * g;
 }
 
-// Fr(x) = C(x) + j S(x)
-WC_HD void fresnel_complex(double x, double& re, double& im) {
-    double c, s, hr, hi;
-    bool cf;
-    fresnel_cs(x, c, s, cf, hr, hi);
+// Fr(x) = C(x) + j S(x) (absolute accuracy ~1e-16); optionally also the
+// phasor e^{j pi x^2/2} = dFr/dx. The phase is reduced as (pi/2) (x^2 mod 4)
+// with x^2 split exactly into hi + lo (fma), so large x lose no phase digits.
+__device__ inline void fresnel_complex(double x, double& re, double& im, double* d_re = nullptr,
+                                       double* d_im = nullptr) {
+    const double ax = fabs(x);
+    double f, g;
+    fresnel_aux(ax, f, g);
+    const double hi = ax * ax;
+    const double lo = fma(ax, ax, -hi);
+    double sn, cs;
+    sincos(0.5 * kPi * (fmod(hi, 4.0) + lo), &sn, &cs);
+    double c = 0.5 + f * sn - g * cs;
+    double s = 0.5 - f * cs - g * sn;
+    if (x < 0.0) {
+        c = -c;
+        s = -s;
+    }
     re = c;
     im = s;
+    if (d_re) {
+        *d_re = cs;
+        *d_im = sn;
+    }
 }
 
 // One cot * F term of the wedge coefficient. beta = phi -+ phi'; sgn selects
 // a+ (+1) or a- (-1): N is the integer nearest to (beta + sgn*pi)/(2 n pi)
 // and a(beta) = 2 cos^2((2 n pi N - beta)/2).
-WC_HD void utd_term(double n, double beta, int sgn, double kL, double& re, double& im) {
+__device__ inline void utd_term(double n, double beta, int sgn, double kL, double& re, double& im) {
     const double arg = (kPi + sgn * beta) / (2.0 * n);
     const double N = nearbyint((beta + sgn * kPi) / (2.0 * n * kPi));
     const double c = cos((2.0 * n * kPi * N - beta) * 0.5);
@@ -194,7 +145,7 @@ struct UtdCoefficients {
 // skew angle beta0 (sin_beta0 > 0), spherical-wave distance parameter L.
 // r0_s/r0_h, rn_s/rn_h are the 0-face and n-face reflection coefficients for
 // the soft (E parallel to the edge) and hard polarizations (PEC: -1 / +1).
-WC_HD UtdCoefficients utd_wedge(double n, double phi, double phi_p, double sin_beta0, double k, double L,
+__device__ inline UtdCoefficients utd_wedge(double n, double phi, double phi_p, double sin_beta0, double k, double L,
                                 Cx<float> r0_s, Cx<float> rn_s, Cx<float> r0_h, Cx<float> rn_h) {
     const double kL = k * L;
     double t1r, t1i, t2r, t2i, t3r, t3i, t4r, t4i;

@@ -122,6 +122,85 @@ __device__ __forceinline__ void contract(const C3<D8>& field, const V3<D8>& arri
     }
 }
 
+// dL/d(coefficient) of one path as (re, im) weights
+__device__ __forceinline__ void coef_weights(const Cx<float>& coef, float sqrt_power, int slot_id,
+                                             const CellAdjoint& adj, float& ar, float& ai) {
+    if (adj.coherent) {
+        ar = adj.a_re[slot_id];
+        ai = adj.a_im[slot_id];
+    } else {
+        const float fr = coef.re * sqrt_power, fi = coef.im * sqrt_power;
+        ar = adj.g_inc[slot_id] * 2.0f * fr + adj.g_f_inc * fr;
+        ai = adj.g_inc[slot_id] * 2.0f * fi + adj.g_f_inc * fi;
+    }
+}
+
+// d(contribution) along one plain (value-only) field derivative of a
+// diffraction path: receiver projection plus the diffraction-vector adjoint
+__device__ __forceinline__ float contract_plain(const C3<float>& dfield, const V3<float>& arrival,
+                                                const V3<float>& rx_pol, float sqrt_power, float ar, float ai,
+                                                const CellAdjoint& adj) {
+    const Cx<float> dc = project_receiver(dfield, arrival, rx_pol);
+    float v = ar * dc.re + ai * dc.im;
+    const Cx<float> comps[3] = {dfield.x, dfield.y, dfield.z};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v += adj.b[j].re * comps[j].re + adj.b[j].im * comps[j].im;
+    return v * sqrt_power;
+}
+
+// Diffraction adjoint: forward-mode tangents over (tx xyz, frequency) only;
+// the four face-material parameters enter through the face reflection
+// coefficients, in which the field is linear, so their derivatives are the
+// field's coefficient partials times d r / d(eps_r, sigma) of a two-tangent
+// slab evaluation.
+__device__ void diffraction_adjoint(const SceneView& s, const Edge& e, const PathParams& pp, const V3<float>& tx,
+                                    const V3<float>& rx, const V3<float>& tx_pol, const V3<float>& rx_pol,
+                                    float freq, double k_d, double length_d, float sqrt_power, int slot_id,
+                                    const CellAdjoint& adj, LaneGrad& g) {
+    using D4 = Dual<4>;
+    using D2 = Dual<2>;
+    const V3<D4> txd{seed<4>(tx.x, 0), seed<4>(tx.y, 1), seed<4>(tx.z, 2)};
+    const D4 fd = seed<4>(freq, 3);
+    const D4 eps[2] = {lift<D4>(pp.eps[0]), lift<D4>(pp.eps[1])};
+    const D4 sig[2] = {lift<D4>(pp.sig[0]), lift<D4>(pp.sig[1])};
+    V3<D4> arrival;
+    DiffFaceTerms faces;
+    const C3<D4> field = diffraction_tangent(s, e, txd, rx, tx_pol, fd, eps, sig, k_d, length_d, arrival, &faces);
+    const Cx<D4> coef = project_receiver(field, arrival, lift3<D4>(rx_pol));
+    float ar, ai;
+    coef_weights(cval(coef), sqrt_power, slot_id, adj, ar, ai);
+    float d[8];
+    const Cx<D4> comps[3] = {field.x, field.y, field.z};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float v = ar * coef.re.d[i] + ai * coef.im.d[i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v += adj.b[j].re * comps[j].re.d[i] + adj.b[j].im * comps[j].im.d[i];
+        d[i] = v * sqrt_power;
+    }
+    const V3<float> arr = vval(arrival);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int m = pp.material[k];
+        Cx<D2> rs, rh;
+        slab_fresnel<D2>(lift<D2>(faces.cos_face[k]), seed<2>(pp.eps[k], 0), seed<2>(pp.sig[k], 1),
+                         lift<D2>(s.mat_mu[m]), lift<D2>(s.mat_gain[m]), lift<D2>(s.mat_thickness[m]),
+                         lift<D2>(freq), rs, rh);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const C3<float> df = c3_scale(faces.d_r[2 * k], Cx<float>{rs.re.d[j], rs.im.d[j]}) +
+                                 c3_scale(faces.d_r[2 * k + 1], Cx<float>{rh.re.d[j], rh.im.d[j]});
+            d[4 + 2 * k + j] = contract_plain(df, arr, rx_pol, sqrt_power, ar, ai, adj);
+        }
+    }
+    g.tx[0] += d[0];
+    g.tx[1] += d[1];
+    g.tx[2] += d[2];
+    g.freq += d[3];
+    add_material(g, pp.material[0], d[4], d[5]);
+    add_material(g, pp.material[1], d[6], d[7]);
+}
+
 __device__ __forceinline__ void scatter(LaneGrad& g, const PathParams& pp, int pass, const float d[kTangents]) {
 #pragma unroll
     for (int i = 0; i < kTangents; ++i) {
@@ -239,6 +318,11 @@ __device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adj
         length_d = norm(to_d(rx) - to_d(tx));
     }
     const int slot_id = slot_of_component(comp);
+    if constexpr (kPhase == kPhaseDiff) {
+        diffraction_adjoint(s, s.edges[cand], pp, tx, rx, tx_pol, rx_pol, freq, k_d, length_d, sqrt_power, slot_id,
+                            adj, g);
+        return;
+    }
     const int passes = (pp.count + kTangents - 1) / kTangents;
     for (int pass = 0; pass < passes; ++pass) {
         const V3<D8> txd{seeded(tx.x, 0, pass), seeded(tx.y, 1, pass), seeded(tx.z, 2, pass)};
@@ -251,9 +335,7 @@ __device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adj
         }
         V3<D8> arrival;
         C3<D8> field;
-        if constexpr (kPhase == kPhaseDiff) {
-            field = diffraction_tangent(s, s.edges[cand], txd, rx, tx_pol, fd, eps, sig, k_d, length_d, arrival);
-        } else {
+        {
             if (comp == kCidLos) {
                 field = los_tangent(txd, rx, tx_pol, fd, k_d, length_d, arrival);
             } else if (comp == kCidRefl) {
@@ -265,7 +347,7 @@ __device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adj
             }
         }
         float d[kTangents];
-        contract(field, arrival, rx_pol, sqrt_power, slot_id, adj, kPhase == kPhaseDiff, d);
+        contract(field, arrival, rx_pol, sqrt_power, slot_id, adj, false, d);
         scatter(g, pp, pass, d);
     }
 }

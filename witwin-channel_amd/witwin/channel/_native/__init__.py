@@ -113,6 +113,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                          c_void_p]),
     "wc_paths_cfr": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, ctypes.c_double,
                              ctypes.c_double, c_int, c_int, c_void_p, c_void_p]),
+    "wc_utd_special_eval": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "wc_field_free_space": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p]),
