@@ -128,6 +128,9 @@ typedef struct wc_solve {
                               itself. Tiles and launches run over positions, outputs stay
                               indexed by cell; a spatially coherent order (receivers
                               that are close together in one wavefront) cuts divergence. */
+    int max_tx_chains;     /* largest reflection-chain count of one transmitter (all
+                              depths); sizes the per-cell chain masks of the tiled
+                              radiomap (0: no chain masks) */
 } wc_solve;
 
 /* ------------------------------------------------------------ radiomap
@@ -151,10 +154,13 @@ typedef struct wc_solve {
  * tiles != NULL: persistent launches over wave-aligned tiles (tx, first cell)
  * of 64 cells pulled from global work counters (tile_counter [2] i32, zeroed
  * by the call with a fill kernel, so the sequence is graph-capturable).
- * The evaluation runs as launches with their own register budgets:
- * reflection/LoS/transmission, then (with diffraction) a visibility launch
- * that tests both legs of every (cell, edge) candidate into bit masks, and
- * the diffraction field launch over the visible edges. */
+ * The evaluation runs as launches with their own register budgets (tiled
+ * form): a chain-validity launch that back-projects every (cell, reflection
+ * chain) candidate and traces its legs into bit masks, the
+ * reflection/LoS/transmission field launch over the valid chains, then (with
+ * diffraction) a visibility launch that tests both legs of every (cell, edge)
+ * candidate into bit masks and the diffraction field launch over the visible
+ * edges. */
 typedef struct wc_radiomap_out {
     float* power_total;
     float* field_total;
@@ -165,12 +171,13 @@ typedef struct wc_radiomap_out {
     int* topology;
     int topology_capacity;
     int* failure;
-    void* workspace; /* wc_radiomap_workspace_bytes(cells, L, edge_count): per-lane
-                        sums carried from the reflection pass to the diffraction
-                        pass, then one edge-visibility bit per (cell, edge) */
+    void* workspace; /* wc_radiomap_workspace_bytes(cells, L, edge_count, max_tx_chains):
+                        per-lane sums carried from the reflection pass to the
+                        diffraction pass, one edge-visibility bit per (cell, edge)
+                        and one chain-validity bit per (cell, chain) */
 } wc_radiomap_out;
 
-size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count);
+size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count, int max_tx_chains);
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
 

@@ -272,8 +272,8 @@ def test_urban_canyon_radiomap_matches_oracle(device):
 
 
 def test_launch_layouts_are_bitwise_neutral(device):
-    """The evaluation order and the pass-by-pass launch change scheduling
-    only: every output must be bit-identical. Splitting a cell over 4 lanes
+    """The evaluation order, the plain (unmasked) launch and the pass-by-pass
+    launch change scheduling only: every output must be bit-identical. Splitting a cell over 4 lanes
     regroups its float64 sums (same paths, same count; values within 1e-6)."""
     _say("case: urban canyon seed 2, 48x48 grid at 8 m, depth 3, los+refl+diff; layouts vs identity order")
     scene = workloads.urban_canyon_scene(seed=2, shape=(48, 48), spacing=8.0)
@@ -304,6 +304,13 @@ def test_launch_layouts_are_bitwise_neutral(device):
     got = outputs(out)
     for k in ref:
         assert torch.equal(got[k], ref[k]), ("z-order", k)
+    _say("layout: plain launch (chains and edges traced in the field passes, no masks)")
+    inputs, _ = solve(spatial_order=False)
+    plain = engine.radiomap_forward(dscene, inputs, tiled=False)
+    torch.cuda.synchronize()
+    got = outputs(plain)
+    for k in ref:
+        assert torch.equal(got[k], ref[k]), ("plain", k)
     _say("layout: Z-order, 4 lanes per cell")
     inputs, out = solve(spatial_order=True, lanes_per_cell=4)
     got = outputs(out)

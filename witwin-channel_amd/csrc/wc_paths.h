@@ -127,11 +127,11 @@ __device__ inline int face_containing(const SceneView& s, int g, const V3<float>
 
 // Validates one image-method chain for receiver rx: back-projects from the
 // receiver through the images, requires each hit strictly between its image
-// and the next point and inside a face of its group, then tests every leg for
-// occlusion (legs skip the groups they start and end on). On success fills
-// hits/faces and returns true. The depth D is a compile-time constant so the
-// per-bounce arrays stay in registers.
-template <int D>
+// and the next point and inside a face of its group, then (kLegs) tests every
+// leg for occlusion (legs skip the groups they start and end on). On success
+// fills hits/faces and returns true. The depth D is a compile-time constant
+// so the per-bounce arrays stay in registers.
+template <int D, bool kLegs = true>
 __device__ inline bool solve_chain(const SceneView& s, const int* groups, const double* images, const V3<float>& tx,
                                    const V3<float>& rx, V3<double>* hits, int* faces) {
     V3<double> next = to_d(rx);
@@ -152,6 +152,7 @@ __device__ inline bool solve_chain(const SceneView& s, const int* groups, const 
         faces[k] = f;
         next = h;
     }
+    if (!kLegs) return true;  // leg visibility already established (chain masks)
     // visibility of the D+1 legs
     V3<float> a = tx;
     int ga = -1;
@@ -257,8 +258,11 @@ __device__ __forceinline__ double wedge_angle(const V3<double>& u, const V3<doub
 }
 
 // Keller point by unfolding: t* = a + (b - a) rho_T / (rho_T + rho_R).
+// With observation = false the receiver's wedge angle phi (used only by the
+// field) is left unset; the existence test and every other member are the
+// same.
 __device__ inline bool diffraction_geometry(const Edge& e, const V3<float>& tx, const V3<float>& rx,
-                                            DiffGeometry& g) {
+                                            DiffGeometry& g, bool observation = true) {
     const V3<double> A{e.ax, e.ay, e.az}, d{e.dx, e.dy, e.dz};
     const V3<double> T = to_d(tx), Rr = to_d(rx);
     const double a = dot(T - A, d), b = dot(Rr - A, d);
@@ -274,6 +278,7 @@ __device__ inline bool diffraction_geometry(const Edge& e, const V3<float>& tx, 
     const double npi = (double)e.n * kPi;
     g.phi_p = wedge_angle(pt, t0, n0);
     if (g.phi_p >= npi) return false;  // source inside the wedge
+    if (!observation) return true;
     double phi = wedge_angle(pr, t0, n0);
     if (phi > npi) phi = (phi - npi < kTwoPi - phi) ? npi : 0.0;  // nearest face of the exterior domain
     g.phi = phi;

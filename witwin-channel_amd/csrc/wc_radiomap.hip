@@ -18,20 +18,39 @@ __device__ __forceinline__ int slot_of(int component) {
     }
 }
 
-// Reflection chains of one compile-time depth D for one cell.
-template <int D, class Sink>
+// Next chain of [c, c1) whose bit is set in `mask` (bit c - base), or c1.
+__device__ __forceinline__ int next_set_chain(const unsigned* mask, int base, int c, int c1) {
+    while (c < c1) {
+        const int rel = c - base;
+        const unsigned word = mask[rel >> 5] >> (rel & 31);
+        if (word) {
+            const int hit = c + __ffs(word) - 1;
+            return hit < c1 ? hit : c1;
+        }
+        c += 32 - (rel & 31);
+    }
+    return c1;
+}
+
+// Reflection chains of one compile-time depth D for one cell. With a chain
+// mask (from chain_visibility_kernel) only the chains whose legs are all
+// clear are visited and their legs are not traced again.
+template <int D, class Sink, bool kMasked>
 __device__ __forceinline__ void reflect_depth(const SceneView& s, const wc_solve& q, int t, const V3<float>& tx,
                                               const V3<float>& rx, const V3<float>& tx_pol, const V3<float>& rx_pol,
                                               float sqrt_power, float freq, double k_d, int sub, int stride,
-                                              Sink& sink) {
+                                              Sink& sink, const unsigned* chain_mask) {
+    const int base = q.chain_offset[t * kMaxDepth];
     const int c0 = q.chain_offset[t * kMaxDepth + D - 1];
     const int c1 = q.chain_offset[t * kMaxDepth + D];
-    for (int c = c0 + sub; c < c1; c += stride) {
+    for (int c = kMasked ? next_set_chain(chain_mask, base, c0, c1) : c0 + sub; c < c1;
+         c = kMasked ? next_set_chain(chain_mask, base, c + 1, c1) : c + stride) {
+        if (kMasked && (c - c0) % stride != sub) continue;
         const int* groups = q.chain_groups + c * kMaxDepth;
         const double* images = q.chain_images + c * kMaxDepth * 3;
         V3<double> hd[D];
         int faces[D];
-        if (!solve_chain<D>(s, groups, images, tx, rx, hd, faces)) continue;
+        if (!solve_chain<D, !kMasked>(s, groups, images, tx, rx, hd, faces)) continue;
         PathRecord rec;
         rec.component = kCidRefl;
         rec.depth = D;
@@ -101,9 +120,10 @@ enum EvalPass : int { kPassAll = 0, kPassMain = 1, kPassDiff = 2 };
 // canonical order; with stride L the lane `sub` of the L lanes sharing a cell
 // visits every L-th reflection chain and diffraction edge (line of sight and
 // transmission belong to sub-lane 0).
-template <class Sink, int kPass>
+template <class Sink, int kPass, bool kMasked = false>
 __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, int sub, int stride, Sink& sink,
-                          const unsigned* diff_mask = nullptr, int mask_words = 0) {
+                          const unsigned* diff_mask = nullptr, int mask_words = 0,
+                          const unsigned* chain_mask = nullptr) {
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
     const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
@@ -152,11 +172,11 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
     for (int d = 1; d <= max_depth; ++d) {
         if (kMain && (comps & kCompRefl)) {
             switch (d) {
-                case 1: reflect_depth<1>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
-                case 2: reflect_depth<2>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
-                case 3: reflect_depth<3>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
-                case 4: reflect_depth<4>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
-                default: reflect_depth<5>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink); break;
+                case 1: reflect_depth<1, Sink, kMasked>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink, chain_mask); break;
+                case 2: reflect_depth<2, Sink, kMasked>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink, chain_mask); break;
+                case 3: reflect_depth<3, Sink, kMasked>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink, chain_mask); break;
+                case 4: reflect_depth<4, Sink, kMasked>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink, chain_mask); break;
+                default: reflect_depth<5, Sink, kMasked>(s, q, t, tx, rx, tx_pol, rx_pol, sqrt_power, freq, k_d, sub, stride, sink, chain_mask); break;
             }
         }
         if (kDiff && d == 1 && (comps & kCompDiff)) {
@@ -329,12 +349,22 @@ __device__ __forceinline__ int tx_of_cell(const wc_solve& q, int cell) {
 // (32 B each) carried from the reflection pass, then the per-cell edge
 // visibility masks of the diffraction pass (mask_words(E) words per cell).
 __host__ __device__ __forceinline__ int mask_words(int edge_count) { return (edge_count + 31) >> 5; }
+__host__ __device__ __forceinline__ size_t diff_mask_bytes(int cells, int edge_count) {
+    return (size_t)cells * (size_t)mask_words(edge_count) * 4;
+}
 __host__ __device__ __forceinline__ size_t partial_bytes(int cells, int lanes) {
     return (size_t)cells * (size_t)lanes * 32;
 }
 __device__ __forceinline__ unsigned* diff_masks(const wc_radiomap_out& o, const wc_solve& q) {
     return reinterpret_cast<unsigned*>(static_cast<char*>(o.workspace) + partial_bytes(q.total_cells,
                                                                                         q.lanes_per_cell));
+}
+// after the diffraction masks: mask_words(max_tx_chains) words per cell, bit
+// c - chain_offset[t * 5] for chain c of the cell's transmitter t
+__host__ __device__ __forceinline__ unsigned* chain_masks(void* workspace, const wc_solve& q, int edge_count) {
+    return reinterpret_cast<unsigned*>(static_cast<char*>(workspace) + partial_bytes(q.total_cells,
+                                                                                     q.lanes_per_cell) +
+                                       diff_mask_bytes(q.total_cells, edge_count));
 }
 
 // cell evaluated at launch position `pos` (identity without an order table)
@@ -396,7 +426,9 @@ __device__ __forceinline__ void reduce_slot(AccumSink& a, int sl, int width) {
 // workspace when diffraction follows; the last pass reduces the group and
 // sub-lane 0 writes the cell. Descriptor segment of a lane: topology[cell]
 // [sub * cap / L ...], terminated by -1 when not full.
-template <int kPass>
+// kMasked: the tiled launch, whose reflection chains come from the chain
+// masks (chain_visibility_kernel ran first); the plain launch traces them.
+template <int kPass, bool kMasked>
 __device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o,
                                               int t, int cell, bool active, int sub, int width) {
     AccumSink a;
@@ -422,7 +454,12 @@ __device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve
             const unsigned* mask = diff_masks(o, q) + (size_t)cell * words;
             eval_cell<AccumSink, kPass>(s, q, t, cell, sub, width, a, mask, words);
         } else {
-            eval_cell<AccumSink, kPass>(s, q, t, cell, sub, width, a);
+            // without reflection chains the masked chain loops are empty
+            const unsigned* cmask =
+                kMasked && q.max_tx_chains > 0
+                    ? chain_masks(o.workspace, q, s.edge_count) + (size_t)cell * mask_words(q.max_tx_chains)
+                    : nullptr;
+            eval_cell<AccumSink, kPass, kMasked>(s, q, t, cell, sub, width, a, nullptr, 0, cmask);
         }
     }
     if (active && o.topology) {
@@ -479,7 +516,7 @@ __global__ void __launch_bounds__(256) radiomap_kernel(SceneView s, wc_solve q, 
         const int pos = base + threadIdx.x;
         const bool active = pos < q.total_cells;
         const int t = active ? tx_of_cell(q, pos) : 0;
-        radiomap_lane<kPass>(s, q, o, t, cell_at(q, active ? pos : 0), active, 0, 1);
+        radiomap_lane<kPass, false>(s, q, o, t, cell_at(q, active ? pos : 0), active, 0, 1);
     }
 }
 
@@ -503,7 +540,7 @@ __device__ __forceinline__ void radiomap_tiles(const SceneView& s, const wc_solv
         const int t = tiles[2 * tile];
         const int pos = tiles[2 * tile + 1] + local;
         const bool active = pos < q.cell_offset[t + 1];
-        radiomap_lane<kPass>(s, q, o, t, cell_at(q, active ? pos : q.cell_offset[t]), active, sub, width);
+        radiomap_lane<kPass, true>(s, q, o, t, cell_at(q, active ? pos : q.cell_offset[t]), active, sub, width);
     }
 }
 
@@ -517,6 +554,60 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_DIF
 radiomap_diff_tiles(SceneView s, wc_solve q, wc_radiomap_out o, const int* __restrict__ tiles, int tile_count,
                     int* __restrict__ counter) {
     radiomap_tiles<kPassDiff>(s, q, o, tiles, tile_count, counter);
+}
+
+// Chain validity for every (cell, reflection chain) candidate: bit c - base
+// of the cell's chain mask is set when the back-projection through the
+// chain's images lands inside a face of every group and all D + 1 legs are
+// clear. One wavefront per (tile of one transmitter, 32 consecutive chains of
+// that transmitter; the chain and its depth are wave-uniform). Only geometry
+// and traversal live here, so the kernel keeps many wavefronts per SIMD in
+// flight; the reflection field pass then visits the valid chains only.
+template <int D>
+__device__ __forceinline__ bool chain_valid(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
+                                            const V3<float>& rx) {
+    V3<double> hd[D];
+    int faces[D];
+    return solve_chain<D, true>(s, q.chain_groups + c * kMaxDepth, q.chain_images + c * kMaxDepth * 3, tx, rx, hd,
+                                faces);
+}
+
+__global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_solve q, const int* __restrict__ tiles,
+                                                               int tile_count, unsigned* __restrict__ masks,
+                                                               int words) {
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= (long long)tile_count * words) return;
+    const int tile = (int)(wave / words);
+    const int chunk = (int)(wave - (long long)tile * words);
+    const int t = tiles[2 * tile];
+    const int width = q.lanes_per_cell;
+    const int lane = threadIdx.x & 63;
+    if (lane >= 64 / width) return;
+    const int pos = tiles[2 * tile + 1] + lane;
+    if (pos >= q.cell_offset[t + 1]) return;
+    const int cell = cell_at(q, pos);
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
+    const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
+    const int base = q.chain_offset[t * kMaxDepth];
+    const int end = q.chain_offset[t * kMaxDepth + min(q.max_depth, kMaxDepth)];
+    const int c0 = base + chunk * 32;
+    const int c1 = min(c0 + 32, end);
+    unsigned bits = 0;
+    int d = 1;
+    for (int c = c0; c < c1; ++c) {
+        while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+        bool ok;
+        switch (d) {
+            case 1: ok = chain_valid<1>(s, q, c, tx, rx); break;
+            case 2: ok = chain_valid<2>(s, q, c, tx, rx); break;
+            case 3: ok = chain_valid<3>(s, q, c, tx, rx); break;
+            case 4: ok = chain_valid<4>(s, q, c, tx, rx); break;
+            default: ok = chain_valid<5>(s, q, c, tx, rx); break;
+        }
+        if (ok) bits |= 1u << (c - c0);
+    }
+    masks[(size_t)cell * words + chunk] = bits;
 }
 
 // Visibility of every (cell, edge) diffraction candidate: bit (ei & 31) of
@@ -549,7 +640,7 @@ diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, in
     for (int ei = e0; ei < e1; ++ei) {
         const Edge e = s.edges[ei];
         DiffGeometry g;
-        if (!diffraction_geometry(e, tx, rx, g)) continue;
+        if (!diffraction_geometry(e, tx, rx, g, false)) continue;
         if (edge_visible(s, e, g, tx, rx)) bits |= 1u << (ei - e0);
     }
     masks[(size_t)cell * words + chunk] = bits;
@@ -752,6 +843,16 @@ static int check_solve(const wc_solve* q) {
 
 using namespace wc;
 
+static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o, const int* tiles,
+                                    int tile_count, hipStream_t st) {
+    if (q.max_tx_chains <= 0 || !(q.components & kCompRefl) || q.max_depth <= 0) return;
+    const int words = mask_words(q.max_tx_chains);
+    unsigned* masks = chain_masks(o.workspace, q, s.edge_count);
+    const long long waves = (long long)tile_count * words;
+    hipLaunchKernelGGL(chain_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q, tiles,
+                       tile_count, masks, words);
+}
+
 static void launch_diff_visibility(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o, hipStream_t st) {
     const int words = mask_words(s.edge_count);
     if (words == 0) return;
@@ -782,8 +883,10 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
     if (solve->total_cells == 0 || passes == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     hipStream_t st = (hipStream_t)stream;
-    if (diffraction && !out->workspace)
-        return wc_fail("%s: workspace (wc_radiomap_workspace_bytes) is required with diffraction", who);
+    const bool chain_masks_used = tiles && solve->max_tx_chains > 0 && (solve->components & kCompRefl) &&
+                                  solve->max_depth > 0;
+    if ((diffraction || chain_masks_used) && !out->workspace)
+        return wc_fail("%s: workspace (wc_radiomap_workspace_bytes) is required with diffraction or reflection", who);
     if (tiles) {
         if (!tile_counter || tile_count <= 0) return wc_fail("tiled launch needs tile_counter and tile_count");
         // compute-unit count of the current device, queried once (safe during stream capture)
@@ -804,9 +907,11 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
             hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 2);
         else
             hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter + (passes == 2 ? 1 : 0), 1);
-        if (passes & 1)
+        if (passes & 1) {
+            launch_chain_visibility(s, *solve, *out, tiles, tile_count, st);
             hipLaunchKernelGGL(radiomap_main_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter);
+        }
         if (passes & 2) {
             launch_diff_visibility(s, *solve, *out, st);
             hipLaunchKernelGGL(radiomap_diff_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
@@ -840,10 +945,10 @@ extern "C" int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, co
     return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, pass, "wc_radiomap_pass", stream);
 }
 
-extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count) {
+extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count, int max_tx_chains) {
     const int c = cells > 0 ? cells : 0;
-    return partial_bytes(c, lanes_per_cell > 0 ? lanes_per_cell : 1) +
-           (size_t)c * (size_t)mask_words(edge_count > 0 ? edge_count : 0) * 4;
+    return partial_bytes(c, lanes_per_cell > 0 ? lanes_per_cell : 1) + diff_mask_bytes(c, edge_count > 0 ? edge_count : 0) +
+           (size_t)c * (size_t)mask_words(max_tx_chains > 0 ? max_tx_chains : 0) * 4;
 }
 
 extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream) {

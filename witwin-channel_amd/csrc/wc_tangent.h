@@ -129,7 +129,8 @@ __device__ inline C3<R> transmission_tangent(const SceneView& s, int count, cons
 __device__ inline void utd_term_partials(double n, double beta, int sgn, double kL, double& re, double& im,
                                          double& db_re, double& db_im, double& dk_re, double& dk_im) {
     const double arg = (kPi + sgn * beta) / (2.0 * n);
-    const double sa = sin(arg);
+    double sa, ca;
+    sincos(arg, &sa, &ca);
     if (fabs(sa) < 1.0e-9) {
         // shadow-boundary limit n sqrt(2 pi kL) sgn(eps) e^{j pi/4}: d/dkL only
         utd_term(n, beta, sgn, kL, re, im);
@@ -141,11 +142,13 @@ __device__ inline void utd_term_partials(double n, double beta, int sgn, double 
         return;
     }
     const double N = nearbyint((beta + sgn * kPi) / (2.0 * n * kPi));
-    const double c = cos((2.0 * n * kPi * N - beta) * 0.5);
+    // a = 2 cos^2(h), h = (2 pi n N - beta) / 2; da/dbeta = sin(2h) = 2 sin(h) cos(h)
+    double sh, c;
+    sincos((2.0 * n * kPi * N - beta) * 0.5, &sh, &c);
     const double a = 2.0 * c * c;
-    const double da = sin(2.0 * n * kPi * N - beta);
+    const double da = 2.0 * sh * c;
     const double X = kL * a;
-    const double cot = cos(arg) / sa;
+    const double cot = ca / sa;
     const double dcot = -(sgn / (2.0 * n)) / (sa * sa);
     double f_re, f_im;
     utd_transition(X, f_re, f_im);

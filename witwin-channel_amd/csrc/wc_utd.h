@@ -113,10 +113,12 @@ __device__ inline void fresnel_complex(double x, double& re, double& im, double*
 __device__ inline void utd_term(double n, double beta, int sgn, double kL, double& re, double& im) {
     const double arg = (kPi + sgn * beta) / (2.0 * n);
     const double N = nearbyint((beta + sgn * kPi) / (2.0 * n * kPi));
+    // a = 2 cos^2 of the half angle (keeps relative precision where a -> 0)
     const double c = cos((2.0 * n * kPi * N - beta) * 0.5);
     const double a = 2.0 * c * c;
     const double X = kL * a;
-    const double sa = sin(arg);
+    double sa, ca;
+    sincos(arg, &sa, &ca);
     if (fabs(sa) < 1.0e-9) {
         // On a shadow boundary cot -> inf and F -> 0; the product has the finite
         // limit n * sqrt(2 pi kL) * e^{j pi/4} * sign (Kouyoumjian-Pathak eq. 29).
@@ -129,7 +131,7 @@ __device__ inline void utd_term(double n, double beta, int sgn, double kL, doubl
         (void)X;
         return;
     }
-    const double cot = cos(arg) / sa;
+    const double cot = ca / sa;
     double f_re, f_im;
     utd_transition(X, f_re, f_im);
     re = cot * f_re;

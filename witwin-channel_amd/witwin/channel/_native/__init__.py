@@ -59,7 +59,7 @@ class WcSolve(ctypes.Structure):
         ("rx_shared", c_int), ("total_cells", c_int),
         ("chain_offset", c_void_p), ("chain_groups", c_void_p), ("chain_images", c_void_p),
         ("frequency_hz", c_float), ("components", c_int), ("max_depth", c_int), ("coherent", c_int),
-        ("lanes_per_cell", c_int), ("cell_order", c_void_p),
+        ("lanes_per_cell", c_int), ("cell_order", c_void_p), ("max_tx_chains", c_int),
     ]
 
 
@@ -97,7 +97,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_bvh_node_count": (c_int, [c_int]),
     "wc_bvh_build": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_size_t, c_void_p]),
-    "wc_radiomap_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "wc_radiomap_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
     "wc_radiomap_forward": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                     ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_void_p]),
     "wc_radiomap_pass": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),

@@ -185,6 +185,15 @@ def make_tiles(cell_offset: np.ndarray, tile_cells: int = TILE) -> np.ndarray:
     return np.concatenate(parts, axis=0).astype(np.int32)
 
 
+def max_tx_chains(chains: ChainPlan, tx_count: int) -> int:
+    """Largest number of reflection chains (all depths) of one transmitter."""
+    off = np.asarray(chains.offset_host, dtype=np.int64)
+    if off.size < MAX_DEPTH * tx_count + 1:
+        return 0
+    per_tx = off[MAX_DEPTH * np.arange(1, tx_count + 1)] - off[MAX_DEPTH * np.arange(tx_count)]
+    return int(per_tx.max()) if per_tx.size else 0
+
+
 def _spread_bits(v: np.ndarray) -> np.ndarray:
     """Interleave zeros into the low 21 bits of ``v`` (bit i -> bit 2i)."""
     v = v.astype(np.uint64) & np.uint64(0x1FFFFF)
@@ -286,7 +295,7 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
         rx_shared=1 if shared else 0, total_cells=int(offsets[-1]), chain_offset=p(chains.offset),
         chain_groups=p(chains.groups), chain_images=p(chains.images), frequency_hz=float(frequency_hz),
         components=int(components), max_depth=int(max_depth), coherent=1 if coherent else 0,
-        lanes_per_cell=int(lanes_per_cell), cell_order=p(order),
+        lanes_per_cell=int(lanes_per_cell), cell_order=p(order), max_tx_chains=max_tx_chains(chains, t),
     )
     return inputs
 
@@ -328,7 +337,8 @@ def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capac
             failure=torch.zeros(1, dtype=torch.int32, device=device),
             workspace=torch.empty(max(int(_native.call("wc_radiomap_workspace_bytes", cells,
                                                        int(inputs.view.lanes_per_cell),
-                                                       int(dscene.view.edge_count))), 32),
+                                                       int(dscene.view.edge_count),
+                                                       int(inputs.view.max_tx_chains))), 32),
                                   dtype=torch.uint8, device=device),
         )
     p = _native.ptr
