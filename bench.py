@@ -181,22 +181,39 @@ def main() -> int:
     ms_batched, _ = timed(step.step, args.steps, args.warmup, stream)
     value = cells / (ms_batched * 1e-3)
 
-    # each kernel of the step timed alone; the roofline is the longest one's
+    # each kernel of the step timed alone (HIP events on its stream); the
+    # roofline is the longest one's
     fwd_ms = kernel_time(lambda: engine.radiomap_forward(dscene, inputs, topology_capacity=step.capacity,
                                                          out=step.out), stream)
-    main_ms = kernel_time(lambda: engine.radiomap_pass(dscene, inputs, step.out, 1), stream)
-    diff_ms = kernel_time(lambda: engine.radiomap_pass(dscene, inputs, step.out, 2), stream,
-                          before=lambda: engine.radiomap_pass(dscene, inputs, step.out, 1))
-    vjp_ms = kernel_time(step.adjoint, stream)
+    run_pass = lambda k: (lambda: engine.radiomap_pass(dscene, inputs, step.out, k))  # noqa: E731
+
+    def passes_before(k):
+        return lambda: [engine.radiomap_pass(dscene, inputs, step.out, j) for j in range(1, k)]
+
+    chain_ms = kernel_time(run_pass(1), stream)
+    main_ms = kernel_time(run_pass(2), stream, before=passes_before(2))
+    edge_ms = kernel_time(run_pass(3), stream, before=passes_before(3))
+    diff_ms = kernel_time(run_pass(4), stream, before=passes_before(4))
+    vjp0_ms = kernel_time(lambda: step.adjoint(1), stream)
+    vjp1_ms = kernel_time(lambda: step.adjoint(2), stream, before=lambda: step.adjoint(1))
     reduce_ms = kernel_time(step.reduce, stream)
     n_main, n_diff = descriptor_split(step.out)
-    scene_bytes = (compiled.face_count * 48 + int(dscene.node_count) * 64
-                   + int(dscene.view.edge_count) * 80 + int(inputs.chains.count) * (4 * 5 + 8 * 15))
+    edges = int(dscene.view.edge_count)
+    chains = int(inputs.chains.count)
+    scene_bytes = compiled.face_count * 48 + int(dscene.node_count) * 64
+    mask_words = lambda n: (n + 31) // 32  # noqa: E731
     kernels = {
+        # reads rx + order, writes one chain-validity bit per (cell, chain); chain tables
+        "chain_visibility_kernel": (chain_ms, cells * (16 + 4 * mask_words(int(inputs.view.max_tx_chains)))
+                                    + scene_bytes + chains * (4 * 5 + 8 * 15)),
         "radiomap_main_tiles": (main_ms, cells * BYTES_CELL_MAIN + n_main * BYTES_PER_DESCRIPTOR + scene_bytes),
-        "radiomap_diff_tiles": (diff_ms, cells * BYTES_CELL_DIFF + n_diff * BYTES_PER_DESCRIPTOR + scene_bytes),
-        "vjp_kernel": (vjp_ms, cells * BYTES_CELL_VJP + (n_main + n_diff) * BYTES_PER_DESCRIPTOR + scene_bytes
-                       + inputs.tile_count * step.stride * 4),
+        # reads rx + order, writes one visibility bit per (cell, edge); edge table
+        "diff_visibility_kernel": (edge_ms, cells * (16 + 4 * mask_words(edges)) + scene_bytes + edges * 80),
+        "radiomap_diff_tiles": (diff_ms, cells * BYTES_CELL_DIFF + n_diff * BYTES_PER_DESCRIPTOR + edges * 80),
+        "vjp_kernel<0>": (vjp0_ms, cells * BYTES_CELL_VJP + n_main * BYTES_PER_DESCRIPTOR + scene_bytes
+                          + inputs.tile_count * step.stride * 4),
+        "vjp_kernel<1>": (vjp1_ms, cells * BYTES_CELL_VJP + n_diff * BYTES_PER_DESCRIPTOR + edges * 80
+                          + inputs.tile_count * step.stride * 8),
     }
     dominant = max(kernels, key=lambda k: kernels[k][0])
     dom_ms, dom_bytes = kernels[dominant]
@@ -282,8 +299,7 @@ def main() -> int:
             "plain_cells_per_s": cells / (ms_plain * 1e-3),
             "batched_over_plain": ms_plain / ms_batched,
             "forward_ms": fwd_ms,
-            "kernel_ms": {"radiomap_main_tiles": main_ms, "radiomap_diff_tiles": diff_ms, "vjp_kernel": vjp_ms,
-                          "vjp_reduce_kernel": reduce_ms},
+            "kernel_ms": {**{k: v[0] for k, v in kernels.items()}, "vjp_reduce_kernel": reduce_ms},
             "paths_reflection_pass": n_main,
             "paths_diffraction": n_diff,
             "chain_planning_s": plan_s,

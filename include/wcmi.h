@@ -181,15 +181,15 @@ size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
 
-/* One launch of wc_radiomap_forward's sequence (tiled form only), for callers
- * that time or schedule the passes apart: pass 1 zeroes tile_counter[0] and
- * runs the reflection/LoS/transmission pass, pass 2 zeroes tile_counter[1]
- * and runs the diffraction visibility and field launches, which read the
- * per-cell sums pass 1 left in `out->workspace` (so pass 2 must follow pass 1
- * of the same solve).
- * Calling pass 1 then pass 2 (when the solve has diffraction) is equivalent to
- * wc_radiomap_forward. No reference counterpart: the reference runs its
- * radiomap as one sequence of launches. */
+/* One launch of wc_radiomap_forward's tiled sequence, for callers that time
+ * or schedule its kernels apart: pass 1 = chain validity (reflection chain
+ * masks), 2 = reflection/LoS/transmission field pass (zeroes
+ * tile_counter[0]; reads the chain masks), 3 = edge visibility (diffraction
+ * masks), 4 = diffraction field pass (zeroes tile_counter[1]; reads the
+ * per-cell sums pass 2 left in `out->workspace` and the edge masks). Run in
+ * that order on one solve and output set, passes 1-4 (3 and 4 only with
+ * diffraction) equal wc_radiomap_forward. No reference counterpart: the
+ * reference runs its radiomap as one sequence of launches. */
 int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                      const int* tiles, int tile_count, int* tile_counter, int pass, void* stream);
 
@@ -224,13 +224,23 @@ typedef struct wc_adjoint {
 } wc_adjoint;
 
 int wc_vjp_partial_stride(int material_count);
-/* Deterministic reduction of the tile partials (tile order, float64 sums):
- * grad_params [2M+1] = [dL/deps_r, dL/dsigma, dL/df], grad_tx [tx_count, 3].
- * tiles are the (transmitter, first cell) pairs of the launch. */
+/* Deterministic reduction of the tile partials (float64 sums in a fixed
+ * order: strided per-thread sums, then a fixed-shape tree; one workgroup per
+ * output): grad_params [2M+1] = [dL/deps_r, dL/dsigma, dL/df],
+ * grad_tx [tx_count, 3]. tiles are the (transmitter, first cell) pairs of the
+ * launch. */
 int wc_vjp_reduce(const float* partials, int tile_count, int stride, const int* tiles, int tx_count,
                   float* grad_params, float* grad_tx, void* stream);
 int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint, const int* tiles,
                     int tile_count, void* stream);
+
+/* One launch of wc_radiomap_vjp, for callers that time or schedule the
+ * adjoint's launches apart: phase 1 (line of sight, reflection,
+ * transmission paths) writes each tile's partial vector, phase 2
+ * (diffraction paths, solves with diffraction only) adds to it, so phase 2
+ * must follow phase 1 of the same adjoint. No reference counterpart. */
+int wc_radiomap_vjp_phase(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
+                          const int* tiles, int tile_count, int phase, void* stream);
 
 /* ------------------------------------------------------------ path rows
  * Explicit path export (deterministic export_paths, path.solve,

@@ -318,13 +318,13 @@ def test_launch_layouts_are_bitwise_neutral(device):
     for k in ref:
         if k != "path_count":
             assert torch.allclose(got[k], ref[k], rtol=1e-6, atol=1e-30), ("lanes", k)
-    # pass 1 then pass 2 reproduces the fused forward, descriptors included
+    # passes 1-4 in order reproduce the fused forward, descriptors included
     inputs, out = solve(spatial_order=True)
     topo = out.topology.clone()
     for k in ref:
         getattr(out, k).zero_()
-    engine.radiomap_pass(dscene, inputs, out, 1)
-    engine.radiomap_pass(dscene, inputs, out, 2)
+    for pass_index in (1, 2, 3, 4):
+        engine.radiomap_pass(dscene, inputs, out, pass_index)
     torch.cuda.synchronize()
     got = outputs(out)
     for k in ref:
@@ -332,8 +332,8 @@ def test_launch_layouts_are_bitwise_neutral(device):
     counts = ref["path_count"].to(device)
     valid = torch.arange(out.topology_capacity, device=device).unsqueeze(0) < counts.unsqueeze(1)
     assert torch.equal(out.topology[valid], topo[valid])
-    with pytest.raises(RuntimeError, match="pass must be 1 or 2"):
-        engine.radiomap_pass(dscene, inputs, out, 3)
+    with pytest.raises(RuntimeError, match="pass must be 1, 2, 3 or 4"):
+        engine.radiomap_pass(dscene, inputs, out, 5)
 
 
 def test_urban_canyon_export_topology_bit_exact(device):

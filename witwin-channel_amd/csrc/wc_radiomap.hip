@@ -863,9 +863,13 @@ static void launch_diff_visibility(const SceneView& s, const wc_solve& q, const 
                        waves);
 }
 
-// passes: bit 0 the reflection/LoS/transmission pass, bit 1 the diffraction pass
+// Launch stages: bit 0 chain validity (tiled only), bit 1 the
+// reflection/LoS/transmission field pass, bit 2 edge visibility, bit 3 the
+// diffraction field pass.
+enum Stage : int { kStageChains = 1, kStageMain = 2, kStageEdges = 4, kStageDiff = 8, kStageAll = 15 };
+
 static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
-                           const int* tiles, int tile_count, int* tile_counter, int passes, const char* who,
+                           const int* tiles, int tile_count, int* tile_counter, int stages, const char* who,
                            void* stream) {
     if (int rc = check_scene(scene)) return rc;
     if (int rc = check_solve(solve)) return rc;
@@ -879,13 +883,13 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
     if (out->topology && out->topology_capacity % width != 0)
         return wc_fail("topology_capacity must be a multiple of lanes_per_cell");
     const bool diffraction = (solve->components & kCompDiff) != 0;
-    if (!diffraction) passes &= 1;
-    if (solve->total_cells == 0 || passes == 0) return 0;
+    const bool chains = tiles && solve->max_tx_chains > 0 && (solve->components & kCompRefl) && solve->max_depth > 0;
+    if (!diffraction) stages &= ~(kStageEdges | kStageDiff);
+    if (!chains) stages &= ~kStageChains;
+    if (solve->total_cells == 0 || stages == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     hipStream_t st = (hipStream_t)stream;
-    const bool chain_masks_used = tiles && solve->max_tx_chains > 0 && (solve->components & kCompRefl) &&
-                                  solve->max_depth > 0;
-    if ((diffraction || chain_masks_used) && !out->workspace)
+    if ((diffraction || chains) && !out->workspace)
         return wc_fail("%s: workspace (wc_radiomap_workspace_bytes) is required with diffraction or reflection", who);
     if (tiles) {
         if (!tile_counter || tile_count <= 0) return wc_fail("tiled launch needs tile_counter and tile_count");
@@ -903,29 +907,27 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
         int blocks = std::min((tile_count + 3) / 4, cached_cus * 8);
         blocks = std::max(blocks, 1);
         // a fill kernel rather than a memset node keeps the launches graph-capturable
-        if (passes == 3)
+        const bool main = (stages & kStageMain) != 0, diff = (stages & kStageDiff) != 0;
+        if (main && diff)
             hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 2);
-        else
-            hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter + (passes == 2 ? 1 : 0), 1);
-        if (passes & 1) {
-            launch_chain_visibility(s, *solve, *out, tiles, tile_count, st);
+        else if (main || diff)
+            hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter + (main ? 0 : 1), 1);
+        if (stages & kStageChains) launch_chain_visibility(s, *solve, *out, tiles, tile_count, st);
+        if (main)
             hipLaunchKernelGGL(radiomap_main_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter);
-        }
-        if (passes & 2) {
-            launch_diff_visibility(s, *solve, *out, st);
+        if (stages & kStageEdges) launch_diff_visibility(s, *solve, *out, st);
+        if (diff)
             hipLaunchKernelGGL(radiomap_diff_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter + 1);
-        }
     } else {
-        if (passes & 1)
+        if (stages & kStageMain)
             hipLaunchKernelGGL(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
                                st, s, *solve, *out);
-        if (passes & 2) {
-            launch_diff_visibility(s, *solve, *out, st);
+        if (stages & kStageEdges) launch_diff_visibility(s, *solve, *out, st);
+        if (stages & kStageDiff)
             hipLaunchKernelGGL(radiomap_kernel<kPassDiff>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
                                st, s, *solve, *out);
-        }
     }
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip(who, err);
@@ -933,16 +935,18 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
 
 extern "C" int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                                    const int* tiles, int tile_count, int* tile_counter, void* stream) {
-    return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, 3, "wc_radiomap_forward", stream);
+    return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, kStageAll, "wc_radiomap_forward",
+                           stream);
 }
 
 extern "C" int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                                 const int* tiles, int tile_count, int* tile_counter, int pass, void* stream) {
-    if (pass != 1 && pass != 2) return wc_fail("wc_radiomap_pass: pass must be 1 or 2 (got %d)", pass);
+    if (pass < 1 || pass > 4) return wc_fail("wc_radiomap_pass: pass must be 1, 2, 3 or 4 (got %d)", pass);
     if (!tiles) return wc_fail("wc_radiomap_pass: the pass-by-pass launch needs tiles");
-    if (pass == 2 && !(solve && (solve->components & kCompDiff)))
-        return wc_fail("wc_radiomap_pass: pass 2 needs a solve with diffraction");
-    return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, pass, "wc_radiomap_pass", stream);
+    if (pass >= 3 && !(solve && (solve->components & kCompDiff)))
+        return wc_fail("wc_radiomap_pass: passes 3 and 4 need a solve with diffraction");
+    return radiomap_launch(scene, solve, out, tiles, tile_count, tile_counter, 1 << (pass - 1), "wc_radiomap_pass",
+                           stream);
 }
 
 extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count, int max_tx_chains) {

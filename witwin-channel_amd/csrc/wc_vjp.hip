@@ -395,14 +395,28 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
     if (pos < q.cell_offset[t + 1]) {
         cell = q.cell_order ? q.cell_order[pos] : pos;
         seg = ga.topology + (size_t)cell * ga.topology_capacity + (size_t)(lane & (width - 1)) * cap_sub;
-        int first_diff = -1;
-        while (n_total < cap_sub) {
-            const int d = seg[n_total];
-            if (d == -1) break;
-            if (first_diff < 0 && (d >> 24) == kCidDiff) first_diff = n_total;
-            ++n_total;
+        if (width == 1) {
+            // one segment per cell: its length is the path count (capped), and
+            // the diffraction descriptors, appended last, start at the first
+            // index whose component is diffraction (binary search)
+            n_total = min(ga.path_count[cell], cap_sub);
+            int lo = 0, hi = n_total;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((seg[mid] >> 24) == kCidDiff) hi = mid;
+                else lo = mid + 1;
+            }
+            n_main = lo;
+        } else {
+            int first_diff = -1;
+            while (n_total < cap_sub) {
+                const int d = seg[n_total];
+                if (d == -1) break;
+                if (first_diff < 0 && (d >> 24) == kCidDiff) first_diff = n_total;
+                ++n_total;
+            }
+            n_main = first_diff < 0 ? n_total : first_diff;
         }
-        n_main = first_diff < 0 ? n_total : first_diff;
     }
     const int mine = kPhase == kPhaseMain ? n_main : n_total - n_main;
     const int first = kPhase == kPhaseMain ? 0 : n_main;
@@ -470,26 +484,33 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
     }
 }
 
-// Column sums of the tile partials in tile order (deterministic): thread j < 2M+1
-// sums the material and frequency columns over all tiles; the transmitter
-// columns are summed over the contiguous tile range of each transmitter.
-__global__ void vjp_reduce_kernel(const float* __restrict__ partials, int tile_count, int stride,
-                                  const int* __restrict__ tiles, int tx_count, float* __restrict__ grad_params,
-                                  float* __restrict__ grad_tx) {
+// Column sums of the tile partials: one workgroup per column. Thread j sums
+// tiles j, j + 256, ... in float64, then a fixed-shape tree in LDS combines
+// the 256 partial sums, so the result is deterministic. Material and
+// frequency columns sum every tile; transmitter columns only the tiles of
+// their transmitter.
+__global__ void __launch_bounds__(256) vjp_reduce_kernel(const float* __restrict__ partials, int tile_count,
+                                                         int stride, const int* __restrict__ tiles, int tx_count,
+                                                         float* __restrict__ grad_params,
+                                                         float* __restrict__ grad_tx) {
+    __shared__ double sums[256];
     const int m2 = stride - 3;  // 2M + 1 shared columns
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m2) {
-        double acc = 0.0;
-        for (int t = 0; t < tile_count; ++t) acc += (double)partials[(size_t)t * stride + j];
-        grad_params[j] = (float)acc;
-    } else if (j < m2 + 3 * tx_count) {
-        const int k = j - m2;
-        const int tx = k / 3, axis = k % 3;
-        double acc = 0.0;
-        for (int t = 0; t < tile_count; ++t) {
-            if (tiles[2 * t] == tx) acc += (double)partials[(size_t)t * stride + m2 + axis];
-        }
-        grad_tx[k] = (float)acc;
+    const int column = blockIdx.x;
+    const int tx = column < m2 ? -1 : (column - m2) / 3;
+    const int src = column < m2 ? column : m2 + (column - m2) % 3;
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < tile_count; t += blockDim.x) {
+        if (tx < 0 || tiles[2 * t] == tx) acc += (double)partials[(size_t)t * stride + src];
+    }
+    sums[threadIdx.x] = acc;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1) {
+        if ((int)threadIdx.x < half) sums[threadIdx.x] += sums[threadIdx.x + half];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (column < m2) grad_params[column] = (float)sums[0];
+        else grad_tx[column - m2] = (float)sums[0];
     }
 }
 
@@ -502,35 +523,47 @@ extern "C" int wc_vjp_reduce(const float* partials, int tile_count, int stride, 
                              float* grad_params, float* grad_tx, void* stream) {
     if (stride < 4 || tile_count < 0 || tx_count < 0) return wc_fail("wc_vjp_reduce: invalid sizes");
     if (!partials || !tiles || !grad_params || !grad_tx) return wc_fail("wc_vjp_reduce: null argument");
-    const int threads = (stride - 3) + 3 * tx_count;
-    hipLaunchKernelGGL(vjp_reduce_kernel, dim3((threads + 255) / 256), dim3(256), 0, (hipStream_t)stream, partials,
-                       tile_count, stride, tiles, tx_count, grad_params, grad_tx);
+    const int columns = (stride - 3) + 3 * tx_count;
+    hipLaunchKernelGGL(vjp_reduce_kernel, dim3(columns), dim3(256), 0, (hipStream_t)stream, partials, tile_count,
+                       stride, tiles, tx_count, grad_params, grad_tx);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_vjp_reduce", err);
 }
 
 extern "C" int wc_vjp_partial_stride(int material_count) { return 2 * material_count + 4; }
 
-extern "C" int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
-                               const int* tiles, int tile_count, void* stream) {
-    if (!scene || !solve || !adjoint) return wc_fail("wc_radiomap_vjp: null argument");
+static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint, const int* tiles,
+                      int tile_count, int phases, const char* who, void* stream) {
+    if (!scene || !solve || !adjoint) return wc_fail("%s: null argument", who);
     if (scene->material_count > kMaxMaterials)
-        return wc_fail("wc_radiomap_vjp: at most 16 materials are supported by the adjoint");
+        return wc_fail("%s: at most 16 materials are supported by the adjoint", who);
     if (!adjoint->topology || !adjoint->path_count || adjoint->topology_capacity <= 0)
-        return wc_fail("wc_radiomap_vjp: the forward topology (descriptors and counts) is required");
+        return wc_fail("%s: the forward topology (descriptors and counts) is required", who);
     if (!adjoint->field_total || !adjoint->component_field || !adjoint->partials)
-        return wc_fail("wc_radiomap_vjp: forward fields and the partials buffer are required");
-    if (!tiles || tile_count < 0) return wc_fail("wc_radiomap_vjp: tiles are required");
+        return wc_fail("%s: forward fields and the partials buffer are required", who);
+    if (!tiles || tile_count < 0) return wc_fail("%s: tiles are required", who);
     if (tile_count == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     if (adjoint->topology_capacity % solve->lanes_per_cell != 0)
-        return wc_fail("wc_radiomap_vjp: topology_capacity must be a multiple of lanes_per_cell");
+        return wc_fail("%s: topology_capacity must be a multiple of lanes_per_cell", who);
     const int blocks = (tile_count + 3) / 4;
-    hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve, *adjoint,
-                       tiles, tile_count);
-    if (solve->components & kCompDiff)
+    if (phases & 1)
+        hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
+                           *adjoint, tiles, tile_count);
+    if ((phases & 2) && (solve->components & kCompDiff))
         hipLaunchKernelGGL(vjp_kernel<kPhaseDiff>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
                            *adjoint, tiles, tile_count);
     const hipError_t err = hipGetLastError();
-    return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_vjp", err);
+    return err == hipSuccess ? 0 : wc_fail_hip(who, err);
+}
+
+extern "C" int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
+                               const int* tiles, int tile_count, void* stream) {
+    return vjp_launch(scene, solve, adjoint, tiles, tile_count, 3, "wc_radiomap_vjp", stream);
+}
+
+extern "C" int wc_radiomap_vjp_phase(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
+                                     const int* tiles, int tile_count, int phase, void* stream) {
+    if (phase != 1 && phase != 2) return wc_fail("wc_radiomap_vjp_phase: phase must be 1 or 2 (got %d)", phase);
+    return vjp_launch(scene, solve, adjoint, tiles, tile_count, phase, "wc_radiomap_vjp_phase", stream);
 }

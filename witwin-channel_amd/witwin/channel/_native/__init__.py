@@ -104,6 +104,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                  ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "wc_radiomap_vjp": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
                                 c_void_p, c_int, c_void_p]),
+    "wc_radiomap_vjp_phase": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
+                                      c_void_p, c_int, c_int, c_void_p]),
     "wc_vjp_partial_stride": (c_int, [c_int]),
     "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "wc_paths_count": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p]),

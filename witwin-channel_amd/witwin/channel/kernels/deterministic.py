@@ -189,11 +189,18 @@ class MeanDbStep:
         self.adjoint()
         self.reduce()
 
-    def adjoint(self) -> None:
-        """The adjoint launch alone (per-tile partial gradients)."""
+    def adjoint(self, phase: int = 0) -> None:
+        """The adjoint launches alone (per-tile partial gradients); phase 1 or 2
+        runs only the line-of-sight/reflection/transmission or the
+        diffraction launch."""
         inputs = self.inputs
-        _native.call("wc_radiomap_vjp", self.dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(self.view),
-                     _native.ptr(inputs.tiles), inputs.tile_count, _native.stream_handle(self.dscene.device))
+        args = (self.dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(self.view), _native.ptr(inputs.tiles),
+                inputs.tile_count)
+        stream = _native.stream_handle(self.dscene.device)
+        if phase:
+            _native.call("wc_radiomap_vjp_phase", *args, int(phase), stream)
+        else:
+            _native.call("wc_radiomap_vjp", *args, stream)
 
     def reduce(self) -> None:
         """Deterministic column sums of the per-tile partials."""

@@ -369,9 +369,9 @@ def _tile_counter(inputs: SolveInputs, device: torch.device) -> torch.Tensor:
 
 
 def radiomap_pass(dscene: DeviceScene, inputs: SolveInputs, out: RadiomapOutputs, pass_index: int) -> None:
-    """One launch of the tiled forward: pass 1 (LoS / reflection /
-    transmission) or pass 2 (diffraction, after pass 1 of the same solve into
-    the same ``out``)."""
+    """One kernel of the tiled forward: 1 chain validity, 2 LoS / reflection /
+    transmission fields, 3 edge visibility, 4 diffraction fields (run in
+    that order on one solve and ``out``)."""
     p = _native.ptr
     view = _out_view(out)
     counter = _tile_counter(inputs, dscene.device)
