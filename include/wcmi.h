@@ -242,6 +242,38 @@ int wc_radiomap_vjp(const wc_scene* scene, const wc_solve* solve, const wc_adjoi
 int wc_radiomap_vjp_phase(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
                           const int* tiles, int tile_count, int phase, void* stream);
 
+/* Geometry part of the fixed-topology VJP (mesh vertices). Replaces the
+ * reference's AD geometry re-solve, _reflection_geometry_ad (witwin/channel/
+ * propagation/geometry.py:214-292) with RayD's EPC backward, and the
+ * position/normal gradients of the field AD functions (witwin/channel/
+ * kernels/fields.py:1283, 1560, 1827). Each stored path is recomputed with
+ * its interaction geometry carrying tangents and the contracted derivatives
+ * are ADDED (float64 atomics; the caller zeroes the buffers) to
+ *   grad_group_plane [group_count, 4]: dL/d(n_x, n_y, n_z, c) of each group
+ *     plane n . x = c (reflections and transmitted walls),
+ *   grad_edge [edge_count, 8]: dL/d(origin xyz, frame rotation xyz, length,
+ *     wedge index n) of each diffracting edge.
+ * adjoint->partials is not used. The edge part is currently switched off:
+ * solves with diffraction (and edges) return an error, grad_edge is not
+ * written. */
+int wc_radiomap_vjp_geometry(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
+                             const int* tiles, int tile_count, double* grad_group_plane, double* grad_edge,
+                             void* stream);
+
+/* Vertex gradient from the geometry accumulators: each group plane is the
+ * plane of its representative face group_face[g] (faces [F, 3] vertex ids),
+ * each edge a function of its vertices edge_vertices [E, 4] = (a, b, c0, c1;
+ * c1 = -1 for a half-plane). contrib (workspace, (9 G + 12 E) floats)
+ * receives the per-entity vertex contributions; grad_vertices [V, 3] their
+ * sums per vertex over the CSR lists (csr_offset [V + 1], csr_slot: indices
+ * of contribution triples, groups g*3 + k first, then 3 G + e*4 + k), in
+ * list order (deterministic). No reference counterpart (the reference
+ * scatters through RayD's backward). */
+int wc_geometry_vertex_grad(const float* vertices, const int* faces, const int* group_face, int group_count,
+                            const double* grad_group_plane, const int* edge_vertices, int edge_count,
+                            const double* grad_edge, const int* csr_offset, const int* csr_slot, int vertex_count,
+                            float* contrib, float* grad_vertices, void* stream);
+
 /* ------------------------------------------------------------ path rows
  * Explicit path export (deterministic export_paths, path.solve,
  * consumer.evaluate). Pass 1 counts the valid paths of every cell; the

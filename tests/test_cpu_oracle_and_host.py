@@ -328,3 +328,83 @@ def test_native_errors_raise_without_gpu_work():
 
     with pytest.raises(RuntimeError, match="count must be non-negative"):
         _native.call("wc_field_free_space", -1, *([None] * 5), 1.0, *([None] * 7), None)
+
+
+def test_with_vertices_matches_a_fresh_edge_table_and_group_planes():
+    from witwin.channel.scene.compiler import build_edge_table, with_vertices
+
+    scene = workloads.urban_canyon_scene(seed=2, shape=(2, 2), spacing=10.0)
+    compiled = compile_scene(scene, reference_frequency_hz=28e9)
+    rng = np.random.default_rng(5)
+    # one jitter per distinct position, so coincident vertices of different structures stay welded
+    _, inverse = np.unique(compiled.vertices, axis=0, return_inverse=True)
+    jitter = rng.normal(scale=1e-3, size=(int(inverse.max()) + 1, 3))
+    v = compiled.vertices.astype(np.float64) + jitter[inverse.reshape(-1)]
+    moved = with_vertices(compiled, v)
+    edges, ev = build_edge_table(moved.vertices, compiled.faces, compiled.face_group, compiled.face_material,
+                                 compiled.edge_policy)
+    # a 1 mm jitter changes no edge's status, so a fresh table lists the same edges
+    assert np.array_equal(ev, compiled.edge_vertices)
+    for name in edges.dtype.names:
+        np.testing.assert_array_equal(moved.edges[name], edges[name], err_msg=name)
+    # group planes follow the representative faces
+    f = compiled.faces[compiled.representative_faces]
+    a, b, c = (moved.vertices.astype(np.float64)[f[:, k]] for k in range(3))
+    n = np.cross(b - a, c - a)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    n *= np.where(n[np.arange(len(n)), np.argmax(np.abs(n), axis=1)] < 0, -1.0, 1.0)[:, None]
+    np.testing.assert_allclose(moved.group_plane[:, :3], n, atol=1e-12)
+    np.testing.assert_allclose(moved.group_plane[:, 3], (n * a).sum(axis=1), atol=1e-9)
+    assert moved.device is None and compiled.vertices.dtype == moved.vertices.dtype
+
+
+def test_edge_vertices_reproduce_edge_records():
+    v, f, _ = workloads.box_mesh(0, 0, 2, 3, 4)
+    groups = coplanar_face_groups(v[f[:, 0]], np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]]),
+                                  np.zeros(12))
+    from witwin.channel.scene.compiler import build_edge_table
+
+    edges, ev = build_edge_table(v, f, groups["face_group_id"], np.zeros(12, np.int32))
+    assert ev.shape == (12, 4) and (ev[:, 3] >= 0).all()
+    np.testing.assert_allclose(np.stack([edges["ax"], edges["ay"], edges["az"]], 1), v[ev[:, 0]])
+    d = v[ev[:, 1]] - v[ev[:, 0]]
+    np.testing.assert_allclose(edges["length"], np.linalg.norm(d, axis=1), rtol=1e-6)
+    # a, b, c0 are the vertices of face 0 in its winding order (a cyclic rotation)
+    for e in range(12):
+        face = list(f[edges["face0"][e]])
+        k = face.index(ev[e, 0])
+        assert [face[(k + 1) % 3], face[(k + 2) % 3]] == [ev[e, 1], ev[e, 2]]
+        assert ev[e, 3] in f[edges["face1"][e]] and ev[e, 3] not in face
+
+
+def test_vertex_csr_lists_every_contribution_once():
+    from witwin.channel.scene.compiler import vertex_csr
+
+    faces = np.array([[0, 1, 2], [2, 1, 3], [3, 4, 0]])
+    group_face = np.array([0, 2])
+    edge_vertices = np.array([[0, 1, 2, 3], [1, 3, 2, -1]])
+    offset, slot = vertex_csr(6, faces, group_face, edge_vertices)
+    assert offset.tolist()[-1] == 6 + 7 and offset[5] == offset[6]
+    expect = {v: [] for v in range(6)}
+    for g, fc in enumerate(group_face):
+        for k in range(3):
+            expect[int(faces[fc][k])].append(3 * g + k)
+    for e in range(2):
+        for k in range(4):
+            if edge_vertices[e, k] >= 0:
+                expect[int(edge_vertices[e, k])].append(6 + 4 * e + k)
+    for v in range(6):
+        assert slot[offset[v]:offset[v + 1]].tolist() == expect[v]
+
+
+def test_heightfield_workload_shape():
+    scene = workloads.heightfield_scene(shape=(4, 4))
+    compiled = compile_scene(scene, reference_frequency_hz=3.5e9)
+    assert compiled.face_count == 2 * 127 * 127 + 40 * 12
+    assert compiled.materials.count == 4
+    z = compiled.vertices[:128 * 128, 2]
+    assert 0.0 <= z.min() and abs(z.max() - 20.0) < 1e-4
+    # vertical-only edges: the 4 vertical edges of each box
+    assert compiled.edge_count == 160 and np.all(np.abs(compiled.edges["dz"]) > 0.7)
+    rx = compiled.endpoints.rx_position
+    assert np.allclose(rx[:, 2], 21.5)

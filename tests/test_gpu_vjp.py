@@ -180,3 +180,145 @@ def test_solve_ad_mode_vjp_returns_leaf_gradients(device):
     assert tx.position.grad is not None and torch.isfinite(tx.position.grad).all()
     assert freq.grad is not None and torch.isfinite(freq.grad)
     _say(f"  grads: eps {float(eps.grad):.4e} tx {tx.position.grad.tolist()} f {float(freq.grad):.4e}")
+
+
+# ------------------------------------------------------------------ mesh vertices
+
+def _signatures(res) -> list[tuple]:
+    return [tuple((p.component, p.depth, tuple(p.prim[:p.depth]), p.edge) for p in cell) for cell in res["paths"]]
+
+
+def _vertex_loss_terms(compiled, run, vertices, cap=128):
+    """Oracle records and path signatures of the scene with moved vertices
+    (same face groups and edge list: scene.compiler.with_vertices)."""
+    from witwin.channel.scene.compiler import with_vertices
+
+    moved = with_vertices(compiled, vertices)
+    ep = compiled.endpoints
+    groups, depth = tx_chains(run, 0)
+    res = oracle.Oracle(moved).radiomap(tx=ep.tx_position[0], rx=ep.rx_position, frequency=run.frequency,
+                                        components=run.bits, max_depth=run.depth, chains=groups, chain_depth=depth,
+                                        paths_cap=cap)
+    return res["record"][:, 0], _signatures(res)
+
+
+def _vertex_case(device, scene, components, depth, frequency, picks=4, h=2e-3):
+    """dL/d(vertices) of the fixed-topology VJP against central differences of
+    the FP64 oracle with one vertex coordinate moved by +-h (Richardson
+    extrapolated from steps h and 2h; the oracle's float32 vertex and edge
+    tables make the effective step the difference of the rounded
+    coordinates). Cells whose oracle topology changes under any probe carry
+    no weight."""
+    run = run_gpu(scene, components, depth, frequency, device, topology_capacity=64)
+    compiled = run.compiled
+    cells = run.inputs.total_cells
+    ep = compiled.endpoints
+    groups, cdepth = tx_chains(run, 0)
+    base = oracle.Oracle(compiled).radiomap(tx=ep.tx_position[0], rx=ep.rx_position, frequency=frequency,
+                                            components=run.bits, max_depth=depth, chains=groups, chain_depth=cdepth,
+                                            perturb=2e-4, paths_cap=128)
+    sig0 = _signatures(base)
+    stable = ~base["ambiguous"].astype(bool)
+    rng = np.random.default_rng(1)
+    w_raw = rng.uniform(0.5, 1.5, cells)
+    p0 = base["record"][:, 0]
+    m = compiled.materials
+
+    def vjp(weights):
+        verts = torch.tensor(compiled.vertices, device=device, requires_grad=True)
+        outs, state = radiomap_vjp_outputs(run.dscene, run.inputs, eps_r=torch.tensor(m.eps_r, device=device),
+                                           sigma_e=torch.tensor(m.sigma_e, device=device),
+                                           frequency=torch.tensor(frequency, dtype=torch.float32, device=device),
+                                           tx_position=torch.tensor(ep.tx_position, device=device),
+                                           material_count=m.count, vertices=verts)
+        assert not bool((state.out.failure != 0).any()), "topology capacity exceeded"
+        loss = (outs[0] * torch.as_tensor(weights, dtype=torch.float32, device=device)).sum()
+        (grad,) = torch.autograd.grad(loss, verts)
+        torch.cuda.synchronize()
+        return grad.cpu().numpy().astype(np.float64)
+
+    w = w_raw * stable
+    w = w / max(float((w * p0).sum()), 1e-300)
+    first = vjp(w)
+    # probe the vertices with the largest gradients plus one seeded other vertex of the mesh
+    mag = np.abs(first).sum(axis=1)
+    chosen = list(np.argsort(-mag)[:picks])
+    others = [int(i) for i in rng.permutation(len(mag)) if mag[i] > 0 and int(i) not in chosen]
+    chosen += others[:1]
+    v64 = compiled.vertices.astype(np.float64)
+    probes = {}
+    for vi in chosen:
+        for a in range(3):
+            for step in (h, 2 * h):
+                for sgn in (1, -1):
+                    v = v64.copy()
+                    v[vi, a] += sgn * step
+                    p, sig = _vertex_loss_terms(compiled, run, v)
+                    probes[(vi, a, step, sgn)] = (p, np.float32(v[vi, a]))
+                    stable &= np.array([s == s0 for s, s0 in zip(sig, sig0)])
+    w = w_raw * stable
+    w = w / max(float((w * p0).sum()), 1e-300)
+    _say(f"  stable cells {int(stable.sum())} / {cells}; vertices {chosen}")
+    grad = vjp(w)
+    worst = 0.0
+    for vi in chosen:
+        for a in range(3):
+            fd = []
+            for step in (h, 2 * h):
+                (p_hi, x_hi), (p_lo, x_lo) = probes[(vi, a, step, 1)], probes[(vi, a, step, -1)]
+                fd.append(float((w * (p_hi - p_lo)).sum()) / (float(x_hi) - float(x_lo)))
+            ref = (4.0 * fd[0] - fd[1]) / 3.0
+            worst = max(worst, _check(f"v[{vi}][{'xyz'[a]}]", float(grad[vi, a]), ref))
+    return worst
+
+
+def test_vjp_vertices_two_ray_matches_fd(device):
+    _say("case: vertex VJP two-ray 16x16 at 2 m, 3.5 GHz, los + ground reflection")
+    scene = workloads.two_ray_scene(shape=(16, 16), spacing=2.0, extent=40.0)
+    worst = _vertex_case(device, scene, {"los", "reflection"}, 1, 3.5e9)
+    assert worst <= REL_TOL
+
+
+def test_vjp_vertices_urban_canyon_matches_fd(device):
+    _say("case: vertex VJP urban canyon seed 2, 16x16 at 20 m, depth 2, los+refl, 3.5 GHz")
+    scene = workloads.urban_canyon_scene(seed=2, shape=(16, 16), spacing=20.0)
+    worst = _vertex_case(device, scene, {"los", "reflection"}, 2, 3.5e9)
+    assert worst <= REL_TOL
+
+
+@pytest.mark.skip(reason="edge (diffraction) vertex gradients are switched off: their kernel stopped with a GPU "
+                         "memory aperture violation of unidentified cause")
+def test_vjp_vertices_with_diffraction_matches_fd(device):
+    _say("case: vertex VJP urban canyon seed 2, 16x16 at 20 m, depth 2, los+refl+diff, 3.5 GHz")
+    scene = workloads.urban_canyon_scene(seed=2, shape=(16, 16), spacing=20.0)
+    worst = _vertex_case(device, scene, {"los", "reflection", "diffraction"}, 2, 3.5e9)
+    assert worst <= REL_TOL
+
+
+def test_vertex_gradient_with_diffraction_is_refused(device):
+    _say("case: vertex VJP request with diffraction enabled raises (no kernel launch)")
+    scene = workloads.urban_canyon_scene(seed=2, shape=(4, 4), spacing=20.0)
+    run = run_gpu(scene, {"los", "reflection", "diffraction"}, 1, 3.5e9, device, topology_capacity=64)
+    m = run.compiled.materials
+    verts = torch.tensor(run.compiled.vertices, device=device, requires_grad=True)
+    outs, _ = radiomap_vjp_outputs(run.dscene, run.inputs, eps_r=torch.tensor(m.eps_r, device=device),
+                                   sigma_e=torch.tensor(m.sigma_e, device=device),
+                                   frequency=torch.tensor(3.5e9, dtype=torch.float32, device=device),
+                                   tx_position=torch.tensor(run.compiled.endpoints.tx_position, device=device),
+                                   material_count=m.count, vertices=verts)
+    with pytest.raises(RuntimeError, match="diffraction"):
+        outs[0].sum().backward()
+
+
+def test_vjp_vertices_transmission_walls_match_fd(device):
+    _say("case: vertex VJP thin-sheet transmission through a tilted wall + ground, 3 GHz")
+    from witwin.core import Mesh, PhysicalMaterial, Scene, Structure
+
+    v = np.array([[3.0, -6, 0], [3.4, 6, 0], [3.2, -6, 8], [3.6, 6, 8]], np.float32)
+    wall = Structure(Mesh(torch.from_numpy(v), torch.tensor([[0, 1, 2], [1, 3, 2]])),
+                     PhysicalMaterial(eps_r=4.4, sigma_e=0.02, thickness_m=0.12), surface_id=1)
+    ground = PhysicalMaterial(eps_r=5.31, sigma_e=0.0326, thickness_m=5.0)
+    scene = Scene(structures=[workloads.ground_structure(-50, -50, 50, 50, ground), wall],
+                  endpoints=[workloads.transmitter((0.0, 0.3, 2.0)), workloads.grid((8.0, -4.0, 1.5), (8, 8), 1.0)])
+    worst = _vertex_case(device, scene, {"los", "reflection", "transmission"}, 1, 3.0e9, picks=3)
+    assert worst <= REL_TOL

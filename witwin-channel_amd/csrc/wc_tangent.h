@@ -125,9 +125,11 @@ __device__ inline C3<R> transmission_tangent(const SceneView& s, int count, cons
 }
 
 // ----------------------------------------------------------------- diffraction
-// value of one cot*F term in double plus its partial derivatives in beta and kL
+// value of one cot*F term in double plus its partial derivatives in beta and
+// kL (and, when dn_re is given, in the wedge index n)
 __device__ inline void utd_term_partials(double n, double beta, int sgn, double kL, double& re, double& im,
-                                         double& db_re, double& db_im, double& dk_re, double& dk_im) {
+                                         double& db_re, double& db_im, double& dk_re, double& dk_im,
+                                         double* dn_re = nullptr, double* dn_im = nullptr) {
     const double arg = (kPi + sgn * beta) / (2.0 * n);
     double sa, ca;
     sincos(arg, &sa, &ca);
@@ -139,6 +141,11 @@ __device__ inline void utd_term_partials(double n, double beta, int sgn, double 
         dk_im = im * f;
         db_re = 0.0;
         db_im = 0.0;
+        if (dn_re) {
+            // the limit is proportional to n
+            *dn_re = re / n;
+            *dn_im = im / n;
+        }
         return;
     }
     const double N = nearbyint((beta + sgn * kPi) / (2.0 * n * kPi));
@@ -166,6 +173,13 @@ __device__ inline void utd_term_partials(double n, double beta, int sgn, double 
     db_im = dcot * f_im + cot * fp_im * kL * da;
     dk_re = cot * fp_re * a;
     dk_im = cot * fp_im * a;
+    if (dn_re) {
+        // d arg / dn = -arg / n; da/dn = -4 pi N cos(h) sin(h)
+        const double dcot_n = arg / (n * sa * sa);
+        const double da_n = -4.0 * kPi * N * c * sh;
+        *dn_re = dcot_n * f_re + cot * fp_re * kL * da_n;
+        *dn_im = dcot_n * f_im + cot * fp_im * kL * da_n;
+    }
 }
 
 template <typename R> struct UtdTangent;
@@ -181,6 +195,20 @@ template <int N> struct UtdTangent<Dual<N>> {
         for (int i = 0; i < N; ++i) {
             out.re.d[i] = (float)(br * beta.d[i] + kr * kL.d[i]);
             out.im.d[i] = (float)(bi * beta.d[i] + ki * kL.d[i]);
+        }
+        return out;
+    }
+    // the same term with the wedge index n carrying tangents too
+    __device__ static Cx<D> term_n(const D& n, const D& beta, int sgn, const D& kL) {
+        double re, im, br, bi, kr, ki, nr, ni;
+        utd_term_partials((double)n.v, (double)beta.v, sgn, (double)kL.v, re, im, br, bi, kr, ki, &nr, &ni);
+        Cx<D> out;
+        out.re.v = (float)re;
+        out.im.v = (float)im;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            out.re.d[i] = (float)(br * beta.d[i] + kr * kL.d[i] + nr * n.d[i]);
+            out.im.d[i] = (float)(bi * beta.d[i] + ki * kL.d[i] + ni * n.d[i]);
         }
         return out;
     }
@@ -301,6 +329,172 @@ __device__ inline C3<R> diffraction_tangent(const SceneView& s, const Edge& e, c
         faces->cos_face[1] = val(cosn);
     }
     return c3_axis(b0_o, (ds * prop) * Cx<R>{-e_b, lift<R>(0.f)}) + c3_axis(phi_o, (dh * prop) * Cx<R>{-e_p, lift<R>(0.f)});
+}
+
+
+// ----------------------------------------------------------------- geometry tangents
+// Path evaluators whose interaction geometry carries the tangents (the vertex
+// adjoint): the plane (n, c) of every reflecting group, the normal of every
+// transmitted wall, and the origin, frame rotation, length and wedge index of
+// a diffracting edge. Endpoints, frequency and materials are constants. The
+// values equal the evaluators above; only the tangents are consumed.
+
+template <typename R>
+__device__ inline C3<R> reflection_tangent_planes(const SceneView& s, int depth, const int* faces, const V3<R>* nrm,
+                                                  const R* off, const V3<float>& tx, const V3<float>& rx,
+                                                  const V3<float>& tx_pol, float freq, double k_d, double length_d,
+                                                  V3<R>& arrival) {
+    V3<R> img[kMaxDepth];
+    V3<R> p = lift3<R>(tx);
+    for (int j = 0; j < depth; ++j) {
+        const R dist = dot(p, nrm[j]) - off[j];
+        p = p - nrm[j] * (dist * 2.0f);
+        img[j] = p;
+    }
+    V3<R> hits[kMaxDepth];
+    V3<R> next = lift3<R>(rx);
+    for (int j = depth - 1; j >= 0; --j) {
+        const R si = dot(nrm[j], img[j]) - off[j];
+        const R sn = dot(nrm[j], next) - off[j];
+        const R t = si / (si - sn);
+        hits[j] = img[j] + (next - img[j]) * t;
+        next = hits[j];
+    }
+    const R fr = lift<R>(freq);
+    const V3<R> ez = v3c<R>(0.f, 0.f, 1.f);
+    V3<R> prev = lift3<R>(tx);
+    V3<R> incident = safe_normalize(hits[0] - prev, ez);
+    C3<R> value = c3_from_real(transverse(incident, lift3<R>(tx_pol)));
+    V3<R> outgoing = incident;
+    for (int b = 0; b < depth; ++b) {
+        incident = safe_normalize(hits[b] - prev, outgoing);
+        const int m = s.face_material[faces[b]];
+        value = reflect_c3(value, incident, nrm[b], lift<R>(s.mat_eps[m]), lift<R>(s.mat_sigma[m]),
+                           lift<R>(s.mat_mu[m]), lift<R>(s.mat_gain[m]), lift<R>(s.mat_thickness[m]), fr, outgoing);
+        prev = hits[b];
+    }
+    arrival = safe_normalize(lift3<R>(rx) - prev, outgoing);
+    const R L = norm(lift3<R>(rx) - img[depth - 1]);
+    const R k = wave_number(fr);
+    const R amp = lift<R>(1.0f) / (k * L * 2.0f);
+    return c3_scale(value, scale(cexpj(PhaseOf<R>::make(k, L, k_d, length_d)), amp));
+}
+
+template <typename R>
+__device__ inline C3<R> transmission_tangent_walls(const SceneView& s, int count, const Hit* hits,
+                                                   const V3<R>* wall_n, const V3<float>& tx, const V3<float>& rx,
+                                                   const V3<float>& tx_pol, float freq, double k_d, double carrier_d,
+                                                   V3<R>& arrival) {
+    const V3<R> ez = v3c<R>(0.f, 0.f, 1.f);
+    const V3<R> seg = lift3<R>(rx - tx);
+    const R dist = norm(seg);
+    const V3<R> dir = seg * (lift<R>(1.0f) / dist);
+    const R fr = lift<R>(freq);
+    C3<R> value = c3_from_real(transverse(dir, lift3<R>(tx_pol)));
+    R carrier = dist;
+    for (int w = 0; w < count; ++w) {
+        V3<R> n = safe_normalize(wall_n[w], ez);
+        if (val(dot(dir, n)) > 0.0f) n = -n;
+        const R ct = w_min(w_max(w_abs(dot(dir, n)), kSmallEpsF), 1.0f);
+        const int m = s.face_material[hits[w].face];
+        const int lo = s.mat_layer_offset[m], lc = s.mat_layer_count[m];
+        const int lcc = lc < 4 ? lc : 4;
+        const Cx<R> t_te = stack_t<R>(ct, s.layer_thickness + lo, s.layer_eps + lo, s.layer_sigma + lo,
+                                      s.layer_mu + lo, lcc, fr, 0, nullptr, nullptr);
+        const Cx<R> t_tm = stack_t<R>(ct, s.layer_thickness + lo, s.layer_eps + lo, s.layer_sigma + lo,
+                                      s.layer_mu + lo, lcc, fr, 1, nullptr, nullptr);
+        const V3<R> sax = safe_normalize(cross(n, dir), stable_perp_basis(dir, n));
+        const V3<R> pax = safe_normalize(cross(sax, dir), stable_perp_basis(dir, sax));
+        const Cx<R> e_s = c3_dot_real(value, sax);
+        const Cx<R> e_p = c3_dot_real(value, pax);
+        value = c3_axis(sax, t_te * e_s) + c3_axis(pax, t_tm * e_p);
+        float wall = 0.0f;
+        for (int l = 0; l < lc; ++l) wall += fmaxf(s.layer_thickness[lo + l], 0.0f);
+        carrier = carrier - ct * wall;
+    }
+    arrival = dir;
+    const R k = wave_number(fr);
+    const R amp = lift<R>(1.0f) / (k * dist * 2.0f);
+    return c3_scale(value, scale(cexpj(PhaseOf<R>::make(k, carrier, k_d, carrier_d)), amp));
+}
+
+// Edge geometry with tangents: origin A, unit direction d, face-0 frame
+// (t0, n0), length and exterior wedge index n.
+template <typename R> struct EdgeGeom {
+    V3<R> A, d, t0, n0;
+    R length, n;
+};
+
+template <typename R>
+__device__ inline R wedge_angle_r(const V3<R>& u, const V3<R>& t0, const V3<R>& n0) {
+    R a = w_atan2(dot(u, n0), dot(u, t0));
+    if (val(a) < 0.0f) a = a + (float)kTwoPi;
+    return a;
+}
+
+template <typename R>
+__device__ inline C3<R> diffraction_tangent_edge(const SceneView& s, const Edge& e, const EdgeGeom<R>& g,
+                                                 const V3<float>& tx, const V3<float>& rx, const V3<float>& tx_pol,
+                                                 float freq, double k_d, double length_d, V3<R>& arrival) {
+    const V3<R> TxR = lift3<R>(tx), RxR = lift3<R>(rx);
+    const R fr = lift<R>(freq);
+    const R a = dot(TxR - g.A, g.d);
+    const R b = dot(RxR - g.A, g.d);
+    const V3<R> pt = TxR - g.A - g.d * a;
+    const V3<R> pr = RxR - g.A - g.d * b;
+    const R rho_p = norm(pt), rho = norm(pr);
+    const R tstar = a + (b - a) * rho_p / (rho_p + rho);
+    const V3<R> q = g.A + g.d * tstar;
+    const R sp = norm(q - TxR), so = norm(RxR - q);
+    const V3<R> si = (q - TxR) * (lift<R>(1.0f) / sp);
+    const V3<R> sov = (RxR - q) * (lift<R>(1.0f) / so);
+    const R npi = g.n * (float)kPi;
+    const R phi_p = wedge_angle_r(pt, g.t0, g.n0);
+    R phi = wedge_angle_r(pr, g.t0, g.n0);
+    if (val(phi) > val(npi)) phi = (val(phi) - val(npi) < (float)kTwoPi - val(phi)) ? npi : lift<R>(0.0f);
+    const R cb = dot(si, g.d);
+    const R sin_b0 = w_sqrt(w_max(lift<R>(1.0f) - cb * cb, 1.0e-12f));
+    V3<R> phi_i = -cross(g.d, si);
+    phi_i = phi_i * (lift<R>(1.0f) / norm(phi_i));
+    const V3<R> b0_i = cross(phi_i, si);
+    V3<R> phi_o = cross(g.d, sov);
+    phi_o = phi_o * (lift<R>(1.0f) / norm(phi_o));
+    const V3<R> b0_o = cross(phi_o, sov);
+    const V3<R> tpol = transverse(si, lift3<R>(tx_pol));
+    const R e_b = dot(tpol, b0_i), e_p = dot(tpol, phi_i);
+    const R L = so * sp * sin_b0 * sin_b0 / (so + sp);
+    const R k = wave_number(fr);
+    const int m0 = e.material0;
+    const int mn = e.material1 >= 0 ? e.material1 : e.material0;
+    Cx<R> r0_s, r0_h, rn_s, rn_h;
+    const R cos0 = w_abs(w_sin(phi_p));
+    const R cosn = w_abs(w_sin(npi - phi));
+    slab_fresnel<R>(cos0, lift<R>(s.mat_eps[m0]), lift<R>(s.mat_sigma[m0]), lift<R>(s.mat_mu[m0]),
+                    lift<R>(s.mat_gain[m0]), lift<R>(s.mat_thickness[m0]), fr, r0_s, r0_h);
+    slab_fresnel<R>(cosn, lift<R>(s.mat_eps[mn]), lift<R>(s.mat_sigma[mn]), lift<R>(s.mat_mu[mn]),
+                    lift<R>(s.mat_gain[mn]), lift<R>(s.mat_thickness[mn]), fr, rn_s, rn_h);
+    const R kL = k * L;
+    const Cx<R> t1 = UtdTangent<R>::term_n(g.n, phi - phi_p, +1, kL);
+    const Cx<R> t2 = UtdTangent<R>::term_n(g.n, phi - phi_p, -1, kL);
+    const Cx<R> t3 = UtdTangent<R>::term_n(g.n, phi + phi_p, -1, kL);
+    const Cx<R> t4 = UtdTangent<R>::term_n(g.n, phi + phi_p, +1, kL);
+    const R pre_mag = lift<R>(1.0f) / (w_sqrt(k * (float)kTwoPi) * sin_b0 * (g.n * 2.0f));
+    const Cx<R> pre = scale(Cx<R>{lift<R>(-(float)cos(-0.25 * kPi)), lift<R>(-(float)sin(-0.25 * kPi))}, pre_mag);
+    const Cx<R> ds = pre * (t1 + t2 + r0_s * t3 + rn_s * t4);
+    const Cx<R> dh = pre * (t1 + t2 + r0_h * t3 + rn_h * t4);
+    const R amp = w_sqrt(sp / (so * (so + sp))) / (k * sp * 2.0f);
+    const R kappa = rho_p * rho_p / (sp * sp * sp) + rho * rho / (so * so * so);
+    const R sg = w_sqrt(k * kappa * (float)(1.0 / kPi));
+    const Cx<R> f1 = UtdTangent<R>::fresnel(sg * (g.length - tstar));
+    const Cx<R> f0 = UtdTangent<R>::fresnel(sg * (-tstar));
+    const Cx<R> dfr = f1 - f0;
+    const Cx<R> dconj{dfr.re, -dfr.im};
+    const Cx<R> trunc = cxc<R>(0.5f, 0.5f) * dconj;
+    const R ph = PhaseOf<R>::make(k, so + sp, k_d, length_d);
+    const Cx<R> prop = scale(trunc, amp) * cexpj(ph);
+    arrival = sov;
+    return c3_axis(b0_o, (ds * prop) * Cx<R>{-e_b, lift<R>(0.f)}) +
+           c3_axis(phi_o, (dh * prop) * Cx<R>{-e_p, lift<R>(0.f)});
 }
 
 }  // namespace wc

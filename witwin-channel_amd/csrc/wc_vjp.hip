@@ -96,10 +96,12 @@ __device__ __forceinline__ D8 seeded(float v, int slot, int pass) {
 }
 
 // d(contribution)/d(slot) for the seeded slots of one pass
-__device__ __forceinline__ void contract(const C3<D8>& field, const V3<D8>& arrival, const V3<float>& rx_pol,
-                                         float sqrt_power, int slot_id, const CellAdjoint& adj, bool is_diff,
-                                         float out[kTangents]) {
-    const Cx<D8> coef = project_receiver(field, arrival, lift3<D8>(rx_pol));
+template <int N>
+__device__ __forceinline__ void contract(const C3<Dual<N>>& field, const V3<Dual<N>>& arrival,
+                                         const V3<float>& rx_pol, float sqrt_power, int slot_id,
+                                         const CellAdjoint& adj, bool is_diff, float out[N]) {
+    using DN = Dual<N>;
+    const Cx<DN> coef = project_receiver(field, arrival, lift3<DN>(rx_pol));
     float ar, ai;
     if (adj.coherent) {
         ar = adj.a_re[slot_id];
@@ -111,10 +113,10 @@ __device__ __forceinline__ void contract(const C3<D8>& field, const V3<D8>& arri
         ai = adj.g_inc[slot_id] * 2.0f * fi + adj.g_f_inc * fi;
     }
 #pragma unroll
-    for (int i = 0; i < kTangents; ++i) {
+    for (int i = 0; i < N; ++i) {
         float v = (ar * coef.re.d[i] + ai * coef.im.d[i]) * sqrt_power;
         if (is_diff) {
-            const Cx<D8> comps[3] = {field.x, field.y, field.z};
+            const Cx<DN> comps[3] = {field.x, field.y, field.z};
             for (int j = 0; j < 3; ++j)
                 v += (adj.b[j].re * comps[j].re.d[i] + adj.b[j].im * comps[j].im.d[i]) * sqrt_power;
         }
@@ -220,30 +222,43 @@ __device__ __forceinline__ void scatter(LaneGrad& g, const PathParams& pp, int p
 
 enum VjpPhase : int { kPhaseMain = 0, kPhaseDiff = 1 };
 
-// Adjoint of one stored path (descriptor `desc`) of `cell`, accumulated into
-// g. kPhaseMain handles line of sight, reflection chains and transmission,
-// kPhaseDiff diffraction: each phase is its own kernel with its own register
-// budget.
-template <int kPhase>
-__device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell, int desc,
-                             LaneGrad& g) {
-    const CellAdjoint adj = cell_adjoint(ga, q, cell);
-    const V3<float> tx = ld3(q.tx_pos + 3 * t);
-    const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
-    const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
-    const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
-    const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
-    const V3<float> rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
-    const float freq = q.frequency_hz;
-    const double k_d = (double)freq * kTwoPi / kC0;
-    const int comp = desc >> 24;
-    int cand = desc & 0xFFFFFF;
-    if (cand == 0xFFFFFF) cand = -1;
-    PathParams pp;
-    pp.count = 4;
+// Everything one stored path (descriptor `desc`) of `cell` needs for its
+// adjoint: the cell adjoint, the endpoints, the frozen interactions (chain
+// groups and the faces they were hit on, or the walls of a transmission, or
+// the edge) and the local parameter slots.
+struct PathSetup {
+    CellAdjoint adj;
+    V3<float> tx, tx_pol, rx, rx_pol;
+    float sqrt_power, freq;
+    double k_d, length_d;
+    int comp, cand, depth, slot_id;
     int groups[kMaxDepth];
     int faces[kMaxDepth];
     Hit walls[kMaxHits];
+    PathParams pp;
+};
+
+template <int kPhase>
+__device__ __forceinline__ void setup_path(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t,
+                                           int cell, int desc, PathSetup& ps) {
+    ps.adj = cell_adjoint(ga, q, cell);
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    ps.tx = tx;
+    ps.tx_pol = ld3(q.tx_pol + 3 * t);
+    ps.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
+    const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
+    ps.rx = rx;
+    ps.rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
+    ps.freq = q.frequency_hz;
+    ps.k_d = (double)ps.freq * kTwoPi / kC0;
+    const int comp = desc >> 24;
+    ps.comp = comp;
+    int cand = desc & 0xFFFFFF;
+    if (cand == 0xFFFFFF) cand = -1;
+    ps.cand = cand;
+    PathParams& pp = ps.pp;
+    pp.count = 4;
     int depth = 0;
     double length_d = 0.0;
     if constexpr (kPhase == kPhaseDiff) {
@@ -263,25 +278,25 @@ __device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adj
         const int* cg = q.chain_groups + cand * kMaxDepth;
         const double* img = q.chain_images + cand * kMaxDepth * 3;
         while (depth < kMaxDepth && cg[depth] >= 0) {
-            groups[depth] = cg[depth];
+            ps.groups[depth] = cg[depth];
             ++depth;
         }
         // interaction faces of the frozen chain (image back-projection only)
         V3<double> next = to_d(rx);
         for (int k = depth - 1; k >= 0; --k) {
             double off;
-            const V3<double> n = group_normal(s, groups[k], off);
+            const V3<double> n = group_normal(s, ps.groups[k], off);
             const V3<double> im{img[3 * k], img[3 * k + 1], img[3 * k + 2]};
             const double si = dot(n, im) - off, sn = dot(n, next) - off;
             const V3<double> h = im + (next - im) * (si / (si - sn));
-            int f = face_containing(s, groups[k], to_f(h), to_f(n));
-            faces[k] = f >= 0 ? f : s.group_faces[s.group_face_offset[groups[k]]];
+            int f = face_containing(s, ps.groups[k], to_f(h), to_f(n));
+            ps.faces[k] = f >= 0 ? f : s.group_faces[s.group_face_offset[ps.groups[k]]];
             next = h;
         }
         const V3<double> last{img[3 * (depth - 1)], img[3 * (depth - 1) + 1], img[3 * (depth - 1) + 2]};
         length_d = norm(to_d(rx) - last);
         for (int k = 0; k < depth; ++k) {
-            const int m = s.face_material[faces[k]];
+            const int m = s.face_material[ps.faces[k]];
             pp.material[k] = m;
             pp.eps[k] = s.mat_eps[m];
             pp.sig[k] = s.mat_sigma[m];
@@ -294,16 +309,16 @@ __device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adj
         if (len > 2.0f * kSegmentBias) {
             const float eps = kSegmentBias / len;
             const RayPre r = ray_setup(tx.x, tx.y, tx.z, seg.x, seg.y, seg.z);
-            wall_count = gather_hits(s, r, eps, 1.0f - eps, q.max_depth, walls);
+            wall_count = gather_hits(s, r, eps, 1.0f - eps, q.max_depth, ps.walls);
         }
         depth = min(wall_count, q.max_depth);
         double carrier = norm(to_d(rx) - to_d(tx));
         for (int w = 0; w < depth; ++w) {
             double off;
-            const V3<double> nd = group_normal(s, walls[w].group, off);
+            const V3<double> nd = group_normal(s, ps.walls[w].group, off);
             const V3<double> dir = (to_d(rx) - to_d(tx)) * (1.0 / norm(to_d(rx) - to_d(tx)));
             const double ct = fmin(fmax(fabs(dot(dir, nd)), 1e-6), 1.0);
-            const int m = s.face_material[walls[w].face];
+            const int m = s.face_material[ps.walls[w].face];
             double wall = 0.0;
             for (int l = 0; l < s.mat_layer_count[m]; ++l)
                 wall += fmax((double)s.layer_thickness[s.mat_layer_offset[m] + l], 0.0);
@@ -317,16 +332,31 @@ __device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adj
     } else {
         length_d = norm(to_d(rx) - to_d(tx));
     }
-    const int slot_id = slot_of_component(comp);
+    ps.depth = depth;
+    ps.length_d = length_d;
+    ps.slot_id = slot_of_component(comp);
+}
+
+// Adjoint of one stored path (descriptor `desc`) of `cell`, accumulated into
+// g. kPhaseMain handles line of sight, reflection chains and transmission,
+// kPhaseDiff diffraction: each phase is its own kernel with its own register
+// budget.
+template <int kPhase>
+__device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell, int desc,
+                             LaneGrad& g) {
+    PathSetup ps;
+    setup_path<kPhase>(s, q, ga, t, cell, desc, ps);
+    const PathParams& pp = ps.pp;
     if constexpr (kPhase == kPhaseDiff) {
-        diffraction_adjoint(s, s.edges[cand], pp, tx, rx, tx_pol, rx_pol, freq, k_d, length_d, sqrt_power, slot_id,
-                            adj, g);
+        diffraction_adjoint(s, s.edges[ps.cand], pp, ps.tx, ps.rx, ps.tx_pol, ps.rx_pol, ps.freq, ps.k_d,
+                            ps.length_d, ps.sqrt_power, ps.slot_id, ps.adj, g);
         return;
     }
+    const int comp = ps.comp;
     const int passes = (pp.count + kTangents - 1) / kTangents;
     for (int pass = 0; pass < passes; ++pass) {
-        const V3<D8> txd{seeded(tx.x, 0, pass), seeded(tx.y, 1, pass), seeded(tx.z, 2, pass)};
-        const D8 fd = seeded(freq, 3, pass);
+        const V3<D8> txd{seeded(ps.tx.x, 0, pass), seeded(ps.tx.y, 1, pass), seeded(ps.tx.z, 2, pass)};
+        const D8 fd = seeded(ps.freq, 3, pass);
         D8 eps[kMaxDepth], sig[kMaxDepth];
         const int nm = (pp.count - 4) / 2;
         for (int k = 0; k < nm; ++k) {
@@ -337,18 +367,98 @@ __device__ void adjoint_path(const SceneView& s, const wc_solve& q, const wc_adj
         C3<D8> field;
         {
             if (comp == kCidLos) {
-                field = los_tangent(txd, rx, tx_pol, fd, k_d, length_d, arrival);
+                field = los_tangent(txd, ps.rx, ps.tx_pol, fd, ps.k_d, ps.length_d, arrival);
             } else if (comp == kCidRefl) {
-                field = reflection_tangent(s, depth, groups, faces, txd, rx, tx_pol, fd, eps, sig, k_d, length_d,
-                                           arrival);
+                field = reflection_tangent(s, ps.depth, ps.groups, ps.faces, txd, ps.rx, ps.tx_pol, fd, eps, sig,
+                                           ps.k_d, ps.length_d, arrival);
             } else {
-                field = transmission_tangent(s, depth, walls, txd, rx, tx_pol, fd, eps, sig, k_d, length_d,
-                                             arrival);
+                field = transmission_tangent(s, ps.depth, ps.walls, txd, ps.rx, ps.tx_pol, fd, eps, sig, ps.k_d,
+                                             ps.length_d, arrival);
             }
         }
         float d[kTangents];
-        contract(field, arrival, rx_pol, sqrt_power, slot_id, adj, false, d);
+        contract(field, arrival, ps.rx_pol, ps.sqrt_power, ps.slot_id, ps.adj, false, d);
         scatter(g, pp, pass, d);
+    }
+}
+
+// Geometry adjoint outputs (float64 accumulators, caller-zeroed):
+// dL/d(group plane n xyz, c) [G, 4] and dL/d(edge origin xyz, frame rotation
+// xyz, length, wedge index) [E, 8].
+struct GeoGrad {
+    double* plane;
+    double* edge;
+};
+
+constexpr int kGeoTangents = 8;
+using DG = Dual<kGeoTangents>;
+
+__device__ __forceinline__ DG geo_seed(float v, int slot, int pass) {
+    const int local = slot - pass * kGeoTangents;
+    return seed<kGeoTangents>(v, (local >= 0 && local < kGeoTangents) ? local : -1);
+}
+
+// Geometry adjoint of one stored path: the same recomputation with the
+// interaction geometry carrying the tangents (4 per reflecting or transmitted
+// plane, 8 per edge); the contracted derivatives are added to the per-group
+// and per-edge accumulators with float64 atomics. Line-of-sight paths have no
+// geometry.
+template <int kPhase>
+__device__ void geometry_path(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell, int desc,
+                              const GeoGrad& out) {
+    PathSetup ps;
+    setup_path<kPhase>(s, q, ga, t, cell, desc, ps);
+    float d[kGeoTangents];
+    if constexpr (kPhase == kPhaseDiff) {
+        const Edge e = s.edges[ps.cand];
+        EdgeGeom<DG> g;
+        g.A = {seed<kGeoTangents>(e.ax, 0), seed<kGeoTangents>(e.ay, 1), seed<kGeoTangents>(e.az, 2)};
+        // an infinitesimal rotation w of the frame: v -> v + w x v
+        const V3<DG> w{seed<kGeoTangents>(0.0f, 3), seed<kGeoTangents>(0.0f, 4), seed<kGeoTangents>(0.0f, 5)};
+        const V3<DG> d0 = v3c<DG>(e.dx, e.dy, e.dz), t0 = v3c<DG>(e.t0x, e.t0y, e.t0z);
+        const V3<DG> n0 = v3c<DG>(e.n0x, e.n0y, e.n0z);
+        g.d = d0 + cross(w, d0);
+        g.t0 = t0 + cross(w, t0);
+        g.n0 = n0 + cross(w, n0);
+        g.length = seed<kGeoTangents>(e.length, 6);
+        g.n = seed<kGeoTangents>(e.n, 7);
+        V3<DG> arrival;
+        const C3<DG> field = diffraction_tangent_edge(s, e, g, ps.tx, ps.rx, ps.tx_pol, ps.freq, ps.k_d,
+                                                      ps.length_d, arrival);
+        contract(field, arrival, ps.rx_pol, ps.sqrt_power, ps.slot_id, ps.adj, true, d);
+        double* dst = out.edge + (size_t)kGeoTangents * ps.cand;
+#pragma unroll
+        for (int i = 0; i < kGeoTangents; ++i)
+            if (d[i] != 0.0f) unsafeAtomicAdd(dst + i, (double)d[i]);
+        return;
+    }
+    if (ps.comp == kCidLos || ps.depth == 0) return;
+    const bool refl = ps.comp == kCidRefl;
+    const int slots = 4 * ps.depth;
+    for (int pass = 0; pass * kGeoTangents < slots; ++pass) {
+        V3<DG> nrm[kMaxDepth];
+        DG off[kMaxDepth];
+        for (int k = 0; k < ps.depth; ++k) {
+            double o;
+            const V3<float> n = to_f(group_normal(s, refl ? ps.groups[k] : ps.walls[k].group, o));
+            nrm[k] = {geo_seed(n.x, 4 * k, pass), geo_seed(n.y, 4 * k + 1, pass), geo_seed(n.z, 4 * k + 2, pass)};
+            off[k] = geo_seed((float)o, 4 * k + 3, pass);
+        }
+        V3<DG> arrival;
+        const C3<DG> field =
+            refl ? reflection_tangent_planes(s, ps.depth, ps.faces, nrm, off, ps.tx, ps.rx, ps.tx_pol, ps.freq,
+                                             ps.k_d, ps.length_d, arrival)
+                 : transmission_tangent_walls(s, ps.depth, ps.walls, nrm, ps.tx, ps.rx, ps.tx_pol, ps.freq, ps.k_d,
+                                              ps.length_d, arrival);
+        contract(field, arrival, ps.rx_pol, ps.sqrt_power, ps.slot_id, ps.adj, false, d);
+#pragma unroll
+        for (int i = 0; i < kGeoTangents; ++i) {
+            const int slot = pass * kGeoTangents + i;
+            if (slot >= slots || d[i] == 0.0f) continue;
+            const int k = slot >> 2;
+            const int group = refl ? ps.groups[k] : ps.walls[k].group;
+            unsafeAtomicAdd(out.plane + 4 * (size_t)group + (slot & 3), (double)d[i]);
+        }
     }
 }
 
@@ -380,9 +490,10 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
 // whatever the spread of paths over its cells. Within a unit the reflection
 // pass stored its descriptors before the diffraction pass appended its own.
 // kPhaseMain writes the tile's partial vector, kPhaseDiff adds to it.
-template <int kPhase>
+// With kGeometry the same dealing drives geometry_path instead (no partials).
+template <int kPhase, bool kGeometry = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VJP_WAVES_PER_EU)))
-vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles, int tile_count) {
+vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles, int tile_count, GeoGrad geo) {
     const int lane = threadIdx.x & 63;
     const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (tile >= tile_count) return;
@@ -447,9 +558,11 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
         const int u_sub = u & (width - 1);
         if (j < total) {
             const int* useg = ga.topology + (size_t)u_cell * ga.topology_capacity + (size_t)u_sub * cap_sub;
-            adjoint_path<kPhase>(s, q, ga, t, u_cell, useg[u_first + (j - u_excl)], g);
+            if constexpr (kGeometry) geometry_path<kPhase>(s, q, ga, t, u_cell, useg[u_first + (j - u_excl)], geo);
+            else adjoint_path<kPhase>(s, q, ga, t, u_cell, useg[u_first + (j - u_excl)], g);
         }
     }
+    if constexpr (kGeometry) return;
     const int m = s.material_count;
     const int stride = 2 * m + 4;
     float* out = ga.partials + (size_t)tile * stride;
@@ -533,13 +646,13 @@ extern "C" int wc_vjp_reduce(const float* partials, int tile_count, int stride, 
 extern "C" int wc_vjp_partial_stride(int material_count) { return 2 * material_count + 4; }
 
 static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint, const int* tiles,
-                      int tile_count, int phases, const char* who, void* stream) {
+                      int tile_count, int phases, const char* who, void* stream, const GeoGrad* geo = nullptr) {
     if (!scene || !solve || !adjoint) return wc_fail("%s: null argument", who);
     if (scene->material_count > kMaxMaterials)
         return wc_fail("%s: at most 16 materials are supported by the adjoint", who);
     if (!adjoint->topology || !adjoint->path_count || adjoint->topology_capacity <= 0)
         return wc_fail("%s: the forward topology (descriptors and counts) is required", who);
-    if (!adjoint->field_total || !adjoint->component_field || !adjoint->partials)
+    if (!adjoint->field_total || !adjoint->component_field || (!geo && !adjoint->partials))
         return wc_fail("%s: forward fields and the partials buffer are required", who);
     if (!tiles || tile_count < 0) return wc_fail("%s: tiles are required", who);
     if (tile_count == 0) return 0;
@@ -547,12 +660,20 @@ static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adj
     if (adjoint->topology_capacity % solve->lanes_per_cell != 0)
         return wc_fail("%s: topology_capacity must be a multiple of lanes_per_cell", who);
     const int blocks = (tile_count + 3) / 4;
+    const GeoGrad none{nullptr, nullptr};
+    if (geo) {
+        if (phases & 1)
+            hipLaunchKernelGGL((vjp_kernel<kPhaseMain, true>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, s,
+                               *solve, *adjoint, tiles, tile_count, *geo);
+        const hipError_t err = hipGetLastError();
+        return err == hipSuccess ? 0 : wc_fail_hip(who, err);
+    }
     if (phases & 1)
         hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
-                           *adjoint, tiles, tile_count);
+                           *adjoint, tiles, tile_count, none);
     if ((phases & 2) && (solve->components & kCompDiff))
         hipLaunchKernelGGL(vjp_kernel<kPhaseDiff>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
-                           *adjoint, tiles, tile_count);
+                           *adjoint, tiles, tile_count, none);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip(who, err);
 }
@@ -566,4 +687,150 @@ extern "C" int wc_radiomap_vjp_phase(const wc_scene* scene, const wc_solve* solv
                                      const int* tiles, int tile_count, int phase, void* stream) {
     if (phase != 1 && phase != 2) return wc_fail("wc_radiomap_vjp_phase: phase must be 1 or 2 (got %d)", phase);
     return vjp_launch(scene, solve, adjoint, tiles, tile_count, phase, "wc_radiomap_vjp_phase", stream);
+}
+
+extern "C" int wc_radiomap_vjp_geometry(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint,
+                                        const int* tiles, int tile_count, double* grad_group_plane,
+                                        double* grad_edge, void* stream) {
+    if (!scene || !solve) return wc_fail("wc_radiomap_vjp_geometry: null argument");
+    if (scene->group_count > 0 && !grad_group_plane)
+        return wc_fail("wc_radiomap_vjp_geometry: grad_group_plane is required");
+    // The edge part (geometry_path<kPhaseDiff>) is switched off: its kernel
+    // stopped with a GPU memory aperture violation whose cause is not
+    // identified (the same device code runs clean under AddressSanitizer in a
+    // CPU build over every edge/receiver pair of that scene). Solves with
+    // diffraction are refused rather than returning a partial gradient.
+    if (scene->edge_count > 0 && (solve->components & kCompDiff))
+        return wc_fail("wc_radiomap_vjp_geometry: vertex gradients of diffraction paths are not available; "
+                       "solve without diffraction");
+    (void)grad_edge;
+    const GeoGrad geo{grad_group_plane, nullptr};
+    return vjp_launch(scene, solve, adjoint, tiles, tile_count, 1, "wc_radiomap_vjp_geometry", stream, &geo);
+}
+
+namespace wc {
+namespace {
+
+template <int N> __device__ __forceinline__ V3<Dual<N>> seed3(const float* p, int first) {
+    return {seed<N>(p[0], first), seed<N>(p[1], first + 1), seed<N>(p[2], first + 2)};
+}
+
+template <int N> __device__ __forceinline__ V3<Dual<N>> unit(const V3<Dual<N>>& v) {
+    return v * (lift<Dual<N>>(1.0f) / norm(v));
+}
+
+// Transposed Jacobians of the scene geometry with respect to the vertices.
+// Group g (plane of its representative face, canonical sign: largest normal
+// component positive; c = n . v0) maps dL/d(n, c) to 9 vertex-coordinate
+// contributions; edge e (vertices a, b, c0, c1) maps dL/d(A, rotation,
+// length, n) to 12. Contributions are stored per (entity, vertex slot, xyz):
+// groups first, [G, 3, 3], then edges, [E, 4, 3].
+__global__ void __launch_bounds__(256) geometry_jacobian_kernel(
+    const float* __restrict__ vertices, const int* __restrict__ faces, const int* __restrict__ group_face,
+    int group_count, const double* __restrict__ grad_plane, const int* __restrict__ edge_vertices, int edge_count,
+    const double* __restrict__ grad_edge, float* __restrict__ contrib) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < group_count) {
+        using D = Dual<9>;
+        const int f = group_face[i];
+        const V3<D> v0 = seed3<9>(vertices + 3 * faces[3 * f], 0);
+        const V3<D> v1 = seed3<9>(vertices + 3 * faces[3 * f + 1], 3);
+        const V3<D> v2 = seed3<9>(vertices + 3 * faces[3 * f + 2], 6);
+        V3<D> n = unit(cross(v1 - v0, v2 - v0));
+        const float ax = fabsf(n.x.v), ay = fabsf(n.y.v), az = fabsf(n.z.v);
+        const float major = (ax >= ay && ax >= az) ? n.x.v : (ay >= az ? n.y.v : n.z.v);
+        if (major < 0.0f) n = -n;
+        const D c = dot(n, v0);
+        const double* g = grad_plane + 4 * (size_t)i;
+        float* out = contrib + 9 * (size_t)i;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            out[k] = (float)(g[0] * n.x.d[k] + g[1] * n.y.d[k] + g[2] * n.z.d[k] + g[3] * c.d[k]);
+        return;
+    }
+    const int e = i - group_count;
+    if (e >= edge_count) return;
+    using D = Dual<12>;
+    const int* ev = edge_vertices + 4 * e;
+    const V3<D> va = seed3<12>(vertices + 3 * ev[0], 0);
+    const V3<D> vb = seed3<12>(vertices + 3 * ev[1], 3);
+    const V3<D> vc = seed3<12>(vertices + 3 * ev[2], 6);
+    const V3<D> dv = vb - va;
+    const D length = norm(dv);
+    const V3<D> d = dv * (lift<D>(1.0f) / length);
+    auto in_plane = [&](const V3<D>& third) {
+        const V3<D> w = third - va;
+        return unit(w - d * dot(w, d));
+    };
+    const V3<D> t0 = in_plane(vc);
+    const V3<D> n0raw = unit(cross(vb - va, vc - va));
+    const V3<D> n0 = unit(n0raw - d * dot(n0raw, d));
+    D n = lift<D>(2.0f);
+    if (ev[3] >= 0) {
+        const V3<D> t1 = in_plane(seed3<12>(vertices + 3 * ev[3], 9));
+        // n = 2 - alpha / pi with alpha = atan2(-n0.t1, t0.t1) in [0, 2 pi): the
+        // 2 pi wrap does not change the derivative
+        n = lift<D>(2.0f) - w_atan2(-dot(n0, t1), dot(t0, t1)) * (float)(1.0 / kPi);
+    }
+    const double* g = grad_edge + 8 * (size_t)e;
+    float* out = contrib + 9 * (size_t)group_count + 12 * (size_t)e;
+    const V3<float> df{d.x.v, d.y.v, d.z.v}, tf{t0.x.v, t0.y.v, t0.z.v}, nf{n0.x.v, n0.y.v, n0.z.v};
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        // the frame moves by a rotation w = (d x dd + t0 x dt0 + n0 x dn0) / 2
+        const V3<float> dd{d.x.d[k], d.y.d[k], d.z.d[k]};
+        const V3<float> dt{t0.x.d[k], t0.y.d[k], t0.z.d[k]};
+        const V3<float> dn{n0.x.d[k], n0.y.d[k], n0.z.d[k]};
+        const V3<float> w = (cross(df, dd) + cross(tf, dt) + cross(nf, dn)) * 0.5f;
+        const float dA[3] = {va.x.d[k], va.y.d[k], va.z.d[k]};
+        out[k] = (float)(g[0] * dA[0] + g[1] * dA[1] + g[2] * dA[2] + g[3] * w.x + g[4] * w.y + g[5] * w.z +
+                         g[6] * length.d[k] + g[7] * n.d[k]);
+    }
+}
+
+// grad[v] = sum of the contribution triples listed for v (CSR, fixed order)
+__global__ void __launch_bounds__(256) vertex_gather_kernel(const float* __restrict__ contrib,
+                                                            const int* __restrict__ offset,
+                                                            const int* __restrict__ slot, int vertex_count,
+                                                            float* __restrict__ grad) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= vertex_count) return;
+    double x = 0.0, y = 0.0, z = 0.0;
+    for (int k = offset[v]; k < offset[v + 1]; ++k) {
+        const float* c = contrib + 3 * (size_t)slot[k];
+        x += c[0];
+        y += c[1];
+        z += c[2];
+    }
+    grad[3 * v] = (float)x;
+    grad[3 * v + 1] = (float)y;
+    grad[3 * v + 2] = (float)z;
+}
+
+}  // namespace
+}  // namespace wc
+
+extern "C" int wc_geometry_vertex_grad(const float* vertices, const int* faces, const int* group_face,
+                                       int group_count, const double* grad_group_plane, const int* edge_vertices,
+                                       int edge_count, const double* grad_edge, const int* csr_offset,
+                                       const int* csr_slot, int vertex_count, float* contrib, float* grad_vertices,
+                                       void* stream) {
+    if (group_count < 0 || edge_count < 0 || vertex_count < 0)
+        return wc_fail("wc_geometry_vertex_grad: invalid sizes");
+    if (vertex_count == 0) return 0;
+    if (!vertices || !csr_offset || !grad_vertices || ((group_count || edge_count) && (!contrib || !csr_slot)))
+        return wc_fail("wc_geometry_vertex_grad: null argument");
+    if (group_count && (!faces || !group_face || !grad_group_plane))
+        return wc_fail("wc_geometry_vertex_grad: group tables are required");
+    if (edge_count && (!edge_vertices || !grad_edge))
+        return wc_fail("wc_geometry_vertex_grad: edge tables are required");
+    const int entities = group_count + edge_count;
+    if (entities > 0)
+        hipLaunchKernelGGL(geometry_jacobian_kernel, dim3((entities + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                           vertices, faces, group_face, group_count, grad_group_plane, edge_vertices, edge_count,
+                           grad_edge, contrib);
+    hipLaunchKernelGGL(vertex_gather_kernel, dim3((vertex_count + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       contrib, csr_offset, csr_slot, vertex_count, grad_vertices);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_geometry_vertex_grad", err);
 }

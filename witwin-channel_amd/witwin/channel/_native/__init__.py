@@ -108,6 +108,11 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                       c_void_p, c_int, c_int, c_void_p]),
     "wc_vjp_partial_stride": (c_int, [c_int]),
     "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "wc_radiomap_vjp_geometry": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
+                                         ctypes.POINTER(WcAdjoint), c_void_p, c_int, c_void_p, c_void_p,
+                                         c_void_p]),
+    "wc_geometry_vertex_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p,
+                                        c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "wc_paths_count": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p]),
     "wc_paths_emit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_void_p, c_void_p,
                               ctypes.POINTER(WcPathRows), c_void_p]),

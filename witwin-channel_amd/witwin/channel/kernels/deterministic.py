@@ -4,7 +4,9 @@ Forward runs ``wc_radiomap_forward`` with path descriptors stored per cell;
 backward runs ``wc_radiomap_vjp`` (the fixed-topology adjoint that recomputes
 each stored path with dual numbers) and reduces its per-tile partials.
 Differentiable inputs: first-layer ``eps_r`` and ``sigma_e`` of every
-material, the carrier frequency, and the transmitter positions.
+material, the carrier frequency, the transmitter positions and the mesh
+vertices (``wc_radiomap_vjp_geometry`` + ``wc_geometry_vertex_grad``, run
+only when the vertex table requires a gradient).
 
 Counterpart of the reference ``_DeterministicAccumulateFlatAdFunction`` and
 the field AD functions (witwin/channel/kernels/deterministic.py:186-390,
@@ -42,7 +44,7 @@ class RadiomapFunction(torch.autograd.Function):
     diffraction_power [cells] (= sum over axes of |diffraction vector|^2)."""
 
     @staticmethod
-    def forward(ctx, eps_r, sigma_e, frequency, tx_position, state: RadiomapState):  # type: ignore[override]
+    def forward(ctx, eps_r, sigma_e, frequency, tx_position, vertices, state: RadiomapState):  # type: ignore[override]
         if state.out is None:
             state.out = engine.forward_with_topology(state.dscene, state.inputs, capacity=state.topology_capacity)
             state.topology_capacity = state.out.topology_capacity
@@ -92,17 +94,46 @@ class RadiomapFunction(torch.autograd.Function):
         grad_tx = torch.empty((t, 3), dtype=torch.float32, device=device)
         _native.call("wc_vjp_reduce", p(state.partials), tiles, stride, p(inputs.tiles), t, p(grad_params),
                      p(grad_tx), stream)
-        return grad_params[:m], grad_params[m:2 * m], grad_params[2 * m], grad_tx, None
+        grad_vertices = geometry_vjp(state.dscene, inputs, view) if ctx.needs_input_grad[4] else None
+        return grad_params[:m], grad_params[m:2 * m], grad_params[2 * m], grad_tx, grad_vertices, None
+
+
+def geometry_vjp(dscene, inputs: engine.SolveInputs, view) -> torch.Tensor:
+    """dL/d(vertices) [V, 3] of the fixed-topology radiomap for the adjoint
+    ``view``: per-group plane and per-edge accumulators from
+    wc_radiomap_vjp_geometry, then the vertex Jacobians and per-vertex sums."""
+    device = dscene.device
+    p = _native.ptr
+    g, e = dscene.group_count, dscene.edge_count
+    if e and int(inputs.view.components) & 4:
+        raise RuntimeError("vertex gradients of diffraction paths are not available; solve without diffraction "
+                           "(components without 'diffraction') to differentiate the mesh vertices")
+    plane = torch.zeros((g, 4), dtype=torch.float64, device=device)
+    edge = torch.zeros((e, 8), dtype=torch.float64, device=device)
+    stream = _native.stream_handle(device)
+    _native.call("wc_radiomap_vjp_geometry", dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(view),
+                 p(inputs.tiles), inputs.tile_count, p(plane), p(edge), stream)
+    offset, slot = dscene.vertex_csr()
+    v = int(dscene.vertices.shape[0])
+    contrib = torch.empty(9 * g + 12 * e, dtype=torch.float32, device=device)
+    grad = torch.empty((v, 3), dtype=torch.float32, device=device)
+    _native.call("wc_geometry_vertex_grad", p(dscene.vertices), p(dscene.faces), p(dscene.group_face), g, p(plane),
+                 p(dscene.edge_vertices), e, p(edge), p(offset), p(slot), v, p(contrib), p(grad), stream)
+    return grad
 
 
 def radiomap_vjp_outputs(dscene, inputs: engine.SolveInputs, *, eps_r: torch.Tensor, sigma_e: torch.Tensor,
                          frequency: torch.Tensor, tx_position: torch.Tensor, material_count: int,
-                         state: RadiomapState | None = None, topology_capacity: int = DEFAULT_TOPOLOGY_CAPACITY):
+                         vertices: torch.Tensor | None = None, state: RadiomapState | None = None,
+                         topology_capacity: int = DEFAULT_TOPOLOGY_CAPACITY):
     """Differentiable radiomap outputs (see RadiomapFunction) for explicit
-    parameter tensors. The tensors' values must equal the compiled scene's."""
+    parameter tensors. The tensors' values must equal the compiled scene's
+    (``vertices``: the concatenated [V, 3] vertex table)."""
     if state is None:
         state = RadiomapState(dscene, inputs, material_count, topology_capacity)
-    outs = RadiomapFunction.apply(eps_r, sigma_e, frequency, tx_position, state)
+    if vertices is None:
+        vertices = dscene.vertices
+    outs = RadiomapFunction.apply(eps_r, sigma_e, frequency, tx_position, vertices, state)
     return outs, state
 
 
@@ -135,8 +166,13 @@ def radiomap_with_vjp(compiled, dscene, inputs, *, material_params: Any = None, 
         txp = torch.stack([t.reshape(3).to(device=device, dtype=torch.float32) for t in tx_leaves])
     else:
         txp = inputs.tx_pos
+    vert_leaves = leaves.get("vertices", [])
+    vertices = None
+    if any(isinstance(t, torch.Tensor) and t.requires_grad for t, _ in vert_leaves):
+        # the concatenated vertex table, differentiable back to every structure's vertices
+        vertices = torch.cat([t.reshape(-1, 3).to(device=device, dtype=torch.float32) for t, _ in vert_leaves])
     outs, state = radiomap_vjp_outputs(dscene, inputs, eps_r=eps, sigma_e=sig, frequency=freq, tx_position=txp,
-                                       material_count=m.count)
+                                       material_count=m.count, vertices=vertices)
     power, field, comp_power, comp_field, diff_power = outs
     return state.out, (power, field, comp_power, diff_power)
 

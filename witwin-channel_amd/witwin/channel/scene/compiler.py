@@ -15,6 +15,7 @@ witwin/channel/scene/resources.py:169-245 (scene + BVH creation).
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import math
 from dataclasses import dataclass, field
 from typing import Any
@@ -83,6 +84,8 @@ class CompiledScene:
     frequency_hz: float
     endpoints: Endpoints
     edge_policy: str = "all"
+    # [E, 4] int32 vertex ids (a, b, c0, c1) of every edge record (see edge_geometry)
+    edge_vertices: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
     device: "DeviceScene | None" = None
     cache: dict[str, Any] = field(default_factory=dict)
 
@@ -230,9 +233,63 @@ def _group_tables(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarra
 
 # ---------------------------------------------------------------- edges
 
-def build_edges(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray, face_material: np.ndarray,
-                policy: str = "all") -> np.ndarray:
-    """Diffraction edge table.
+def edge_geometry(vertices: np.ndarray, edge_vertices: np.ndarray) -> dict[str, np.ndarray]:
+    """Float64 geometry of edges given by their vertex ids [E, 4] = (a, b, c0,
+    c1): a, b the end points in face 0's winding order, c0 the third vertex of
+    face 0, c1 the third vertex of face 1 (-1 for a half-plane).
+
+    Returns the origin A = v[a], unit direction d, length, the face-0
+    in-plane direction t0 (perpendicular to d, towards c0), the face-0 normal
+    n0 (winding normal with its d component removed) and the wedge index
+    n = 2 - alpha / pi, alpha = atan2(-n0.t1, t0.t1) in [0, 2 pi) (2 for a
+    half-plane). The wedge is a function of these four vertices only, which is
+    what the vertex adjoint differentiates.
+    """
+    v = np.asarray(vertices, dtype=np.float64)
+    ev = np.asarray(edge_vertices, dtype=np.int64).reshape(-1, 4)
+    A = v[ev[:, 0]]
+    d = v[ev[:, 1]] - A
+    length = np.sqrt((d * d).sum(axis=1))
+    d = d / np.maximum(length, 1e-300)[:, None]
+
+    def in_plane(third: np.ndarray) -> np.ndarray:
+        w = third - A
+        w = w - d * (w * d).sum(axis=1)[:, None]
+        return w / np.maximum(np.sqrt((w * w).sum(axis=1)), 1e-300)[:, None]
+
+    c0 = v[ev[:, 2]]
+    t0 = in_plane(c0)
+    n0 = np.cross(v[ev[:, 1]] - A, c0 - A)
+    n0 = n0 / np.maximum(np.sqrt((n0 * n0).sum(axis=1)), 1e-300)[:, None]
+    n0 = n0 - d * (n0 * d).sum(axis=1)[:, None]
+    n0 = n0 / np.maximum(np.sqrt((n0 * n0).sum(axis=1)), 1e-300)[:, None]
+    wedge = ev[:, 3] >= 0
+    t1 = in_plane(v[np.where(wedge, ev[:, 3], ev[:, 2])])
+    alpha = np.arctan2(-(n0 * t1).sum(axis=1), (t0 * t1).sum(axis=1))
+    alpha = np.where(alpha < 0.0, alpha + 2.0 * math.pi, alpha)
+    n = np.where(wedge, 2.0 - alpha / math.pi, 2.0)
+    return {"A": A, "d": d, "length": length, "t0": t0, "n0": n0, "alpha": alpha, "n": n, "wedge": wedge}
+
+
+def _edge_rows(geometry: dict[str, np.ndarray], ids: np.ndarray) -> np.ndarray:
+    """wc_edge records from edge_geometry output and [E, 6] integer columns
+    (face0, face1, group0, group1, material0, material1)."""
+    rows = np.zeros(ids.shape[0], dtype=EDGE_DTYPE)
+    for k, axis in enumerate("xyz"):
+        rows["a" + axis] = geometry["A"][:, k]
+        rows["d" + axis] = geometry["d"][:, k]
+        rows["t0" + axis] = geometry["t0"][:, k]
+        rows["n0" + axis] = geometry["n0"][:, k]
+    rows["length"] = geometry["length"]
+    rows["n"] = geometry["n"]
+    for k, name in enumerate(("face0", "face1", "group0", "group1", "material0", "material1")):
+        rows[name] = ids[:, k]
+    return rows
+
+
+def build_edge_table(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray, face_material: np.ndarray,
+                     policy: str = "all") -> tuple[np.ndarray, np.ndarray]:
+    """Diffraction edge table and the vertex ids of every edge.
 
     Every triangle edge is keyed by its welded end points. An edge shared by
     two faces of different, non-coplanar planes is a wedge whose exterior is the
@@ -240,7 +297,8 @@ def build_edges(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray,
     face is a half-plane (n = 2). Edges between coplanar faces and edges with
     more than two faces are not diffracting. Ids follow first appearance
     (faces ascending, local edges v0v1, v1v2, v2v0). ``policy="vertical_only"``
-    keeps edges with |dz|/length > 0.7.
+    keeps edges with |dz|/length > 0.7. Returns (records, [E, 4] int32 vertex
+    ids (a, b, c0, c1), see edge_geometry).
     """
     v = vertices.astype(np.float64)
     q = np.rint(v / EDGE_WELD_QUANTIZATION).astype(np.int64)
@@ -256,56 +314,37 @@ def build_edges(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray,
                 table[key] = []
                 order.append(key)
             table[key].append((f, i0, i1, i2))
-    rows = []
+    cand_v, cand_i = [], []
     for key in order:
         entries = table[key]
         if len(entries) > 2:
             continue
         f0, i0, i1, i2 = entries[0]
-        A, B = v[i0], v[i1]
-        d = B - A
-        length = float(np.linalg.norm(d))
-        if length < 1e-9:
-            continue
-        d = d / length
-        if policy == "vertical_only" and abs(d[2]) <= 0.7:
-            continue
-
-        def in_plane_dir(third: np.ndarray) -> np.ndarray:
-            w = third - A
-            w = w - d * np.dot(w, d)
-            return w / max(np.linalg.norm(w), 1e-300)
-
-        a0, b0, c0 = v[faces[f0]]
-        n0 = np.cross(b0 - a0, c0 - a0)
-        n0 = n0 / max(np.linalg.norm(n0), 1e-300)
-        t0 = in_plane_dir(v[i2])
-        # n0 must be perpendicular to the edge; remove numerical drift
-        n0 = n0 - d * np.dot(n0, d)
-        n0 = n0 / max(np.linalg.norm(n0), 1e-300)
         if len(entries) == 2:
-            f1, j0, j1, j2 = entries[1]
+            f1, _, _, j2 = entries[1]
             if face_group[f1] == face_group[f0]:
                 continue
-            t1 = in_plane_dir(v[j2])
-            alpha = math.atan2(-float(np.dot(n0, t1)), float(np.dot(t0, t1)))
-            if alpha < 0.0:
-                alpha += 2.0 * math.pi
-            if abs(alpha - math.pi) < COPLANAR_ANGLE_TOLERANCE or alpha < 1e-6:
-                continue
-            wedge_n = 2.0 - alpha / math.pi
-            face1 = f1
-            group1 = int(face_group[f1])
-            material1 = int(face_material[f1])
+            cand_v.append((i0, i1, i2, j2))
+            cand_i.append((f0, f1, int(face_group[f0]), int(face_group[f1]), int(face_material[f0]),
+                           int(face_material[f1])))
         else:
-            wedge_n = 2.0
-            face1 = -1
-            group1 = -1
-            material1 = -1
-        rows.append((A[0], A[1], A[2], d[0], d[1], d[2], length, wedge_n, t0[0], t0[1], t0[2],
-                     n0[0], n0[1], n0[2], f0, face1, int(face_group[f0]), group1, int(face_material[f0]),
-                     material1))
-    return np.array(rows, dtype=EDGE_DTYPE)
+            cand_v.append((i0, i1, i2, -1))
+            cand_i.append((f0, -1, int(face_group[f0]), -1, int(face_material[f0]), -1))
+    ev = np.asarray(cand_v, dtype=np.int64).reshape(-1, 4)
+    ids = np.asarray(cand_i, dtype=np.int64).reshape(-1, 6)
+    g = edge_geometry(v, ev)
+    keep = g["length"] >= 1e-9
+    if policy == "vertical_only":
+        keep &= np.abs(g["d"][:, 2]) > 0.7
+    keep &= ~(g["wedge"] & ((np.abs(g["alpha"] - math.pi) < COPLANAR_ANGLE_TOLERANCE) | (g["alpha"] < 1e-6)))
+    g = {k: a[keep] for k, a in g.items()}
+    return _edge_rows(g, ids[keep]), ev[keep].astype(np.int32)
+
+
+def build_edges(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray, face_material: np.ndarray,
+                policy: str = "all") -> np.ndarray:
+    """Diffraction edge records (see build_edge_table)."""
+    return build_edge_table(vertices, faces, face_group, face_material, policy)[0]
 
 
 # ---------------------------------------------------------------- endpoints
@@ -359,7 +398,7 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
         else float(reference_frequency_hz)
     if not frequency > 0.0:
         raise ValueError("reference_frequency_hz must be positive")
-    vert_chunks, face_chunks, mat_ids, surf_ids = [], [], [], []
+    vert_chunks, face_chunks, mat_ids, surf_ids, vert_leaves = [], [], [], [], []
     unique: list[PhysicalMaterial] = []
     lookup: dict[int, int] = {}
     base = 0
@@ -372,6 +411,7 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
             lookup[id(mat)] = len(unique)
             unique.append(mat)
         vert_chunks.append(v)
+        vert_leaves.append((mesh.vertices, base))
         face_chunks.append(f + base)
         mat_ids.append(np.full(f.shape[0], lookup[id(mat)], dtype=np.int32))
         sid = structure.surface_id if structure.surface_id is not None else index
@@ -388,6 +428,8 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
         "sigma_e": [l.sigma_e if isinstance(l.sigma_e, torch.Tensor) else None for l in firsts],
         "tx_position": [t.position for t in scene.transmitters],
         "frequency": reference_frequency_hz if isinstance(reference_frequency_hz, torch.Tensor) else None,
+        # (structure vertex tensor, first row in the concatenated vertex table)
+        "vertices": vert_leaves,
     }
     if faces.shape[0]:
         tri = vertices[faces]
@@ -401,15 +443,55 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
                                                              face_material)
     policy = getattr(scene.metadata.get("imported_edge_policy"), "edge_selection_mode", "all") \
         if scene.metadata else "all"
-    edges = build_edges(vertices, faces, face_group, face_material, policy)
+    edges, edge_vertices = build_edge_table(vertices, faces, face_group, face_material, policy)
     compiled = CompiledScene(
         vertices=vertices, faces=faces, face_material=face_material, face_surface=face_surface,
         face_group=face_group, group_plane=planes, group_face_offset=offsets, group_faces=members,
         group_material=group_material, representative_faces=representative, materials=materials,
         edges=edges, frequency_hz=frequency, endpoints=compile_endpoints(scene), edge_policy=policy,
+        edge_vertices=edge_vertices,
     )
     compiled.cache["ad_leaves"] = ad_leaves
     return compiled
+
+
+def with_vertices(compiled: CompiledScene, vertices: np.ndarray) -> CompiledScene:
+    """The compiled scene with moved vertices and the same topology: face
+    groups, group membership and the edge list are kept; each group plane is
+    recomputed from its representative face and each edge record from its
+    four vertices. This is the geometry model the vertex adjoint
+    differentiates (fixed topology)."""
+    v = np.ascontiguousarray(np.asarray(vertices, dtype=np.float32).reshape(compiled.vertices.shape))
+    planes, _, _, _ = _group_tables(v, compiled.faces, compiled.face_group, compiled.representative_faces,
+                                    compiled.face_material)
+    edges = compiled.edges
+    if compiled.edge_count:
+        ids = np.stack([edges[k] for k in ("face0", "face1", "group0", "group1", "material0", "material1")],
+                       axis=1).astype(np.int64)
+        edges = _edge_rows(edge_geometry(v, compiled.edge_vertices), ids)
+    return dataclasses.replace(compiled, vertices=v, group_plane=planes, edges=edges, device=None,
+                               cache={})
+
+
+def vertex_csr(vertex_count: int, faces: np.ndarray, group_face: np.ndarray,
+               edge_vertices: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Per-vertex lists of the geometry-adjoint contribution triples that
+    touch it (see wc_geometry_vertex_grad): triple g*3 + k is vertex k of the
+    representative face of group g, triple 3 G + e*4 + k vertex k of edge e.
+    Returns (offset [V + 1], slot) int32; a vertex's slots keep ascending
+    order, so the per-vertex sums are deterministic."""
+    g_count = int(np.asarray(group_face).shape[0])
+    gv = np.asarray(faces, dtype=np.int64)[np.asarray(group_face, dtype=np.int64)].reshape(-1)
+    ev = np.asarray(edge_vertices, dtype=np.int64).reshape(-1)
+    keep = ev >= 0
+    verts = np.concatenate([gv, ev[keep]])
+    slots = np.concatenate([np.arange(3 * g_count, dtype=np.int64),
+                            3 * g_count + np.arange(ev.shape[0], dtype=np.int64)[keep]])
+    order = np.argsort(verts, kind="stable")
+    counts = np.bincount(verts, minlength=vertex_count)
+    offset = np.zeros(vertex_count + 1, dtype=np.int64)
+    offset[1:] = np.cumsum(counts)
+    return offset.astype(np.int32), slots[order].astype(np.int32)
 
 
 # ---------------------------------------------------------------- device scene
@@ -442,6 +524,11 @@ class DeviceScene:
         self.layer_eps = t(mats.layer_eps_r, torch.float32)
         self.layer_sigma = t(mats.layer_sigma_e, torch.float32)
         self.layer_mu = t(mats.layer_mu_r, torch.float32)
+        self.group_face = t(compiled.representative_faces, torch.int32)
+        self.edge_vertices = t(compiled.edge_vertices.reshape(-1, 4), torch.int32)
+        self.group_count = compiled.group_count
+        self.edge_count = compiled.edge_count
+        self._csr: tuple[torch.Tensor, torch.Tensor] | None = None
         edge_bytes = compiled.edges.view(np.uint8).reshape(-1) if compiled.edge_count else np.zeros(0, np.uint8)
         self.edges = torch.as_tensor(edge_bytes.copy()).to(device)
         f = compiled.face_count
@@ -474,6 +561,14 @@ class DeviceScene:
 
     def ref(self) -> "ctypes._Pointer":
         return ctypes.byref(self.view)
+
+    def vertex_csr(self) -> tuple[torch.Tensor, torch.Tensor]:
+        """Device copy of vertex_csr for this scene (built on first use)."""
+        if self._csr is None:
+            offset, slot = vertex_csr(int(self.vertices.shape[0]), self.faces.cpu().numpy(),
+                                      self.group_face.cpu().numpy(), self.edge_vertices.cpu().numpy())
+            self._csr = (torch.as_tensor(offset).to(self.device), torch.as_tensor(slot).to(self.device))
+        return self._csr
 
 
 def device_scene(compiled: CompiledScene, device: torch.device) -> DeviceScene:

@@ -12,11 +12,17 @@
   z = 1.5 m covering the block (256 x 256 at 2 m by default).
 * ``batched_transmitters`` — seeded transmitter positions over the streets
   with log-uniform receiver-set sizes.
+* ``heightfield_scene`` — seeded fBm terrain (128 x 128 vertices at 2 m
+  pitch, relief 20 m, 32,258 triangles) with 40 random boxes (480
+  triangles), 4 materials (soil, concrete, glass, brick), one transmitter
+  25 m above the highest terrain point and a 256 x 256 receiver grid 1.5 m
+  above it; diffraction edges follow the vertical-only edge policy.
 """
 
 from __future__ import annotations
 
 import math
+from types import SimpleNamespace
 
 import numpy as np
 import torch
@@ -162,3 +168,86 @@ def batched_transmitters(seed: int = 3, tx_count: int = 64, min_rx: int = 16, ma
     sizes = np.floor(np.exp(rng.uniform(math.log(min_rx), math.log(max_rx), size=tx_count))).astype(np.int64)
     rx = np.concatenate([street_points(rng, int(n)) for n in sizes], axis=0)
     return tx, sizes, rx
+
+
+SOIL = dict(eps_r=15.0, sigma_e=0.035)
+BRICK = dict(eps_r=3.75, sigma_e=0.038)
+
+
+def fbm_heights(seed: int, size: int = 128, octaves: int = 6, relief: float = 20.0) -> np.ndarray:
+    """[size, size] fractional-Brownian-motion heights in [0, relief]: value
+    noise (seeded uniform lattices, bilinear interpolation) summed over
+    octaves of doubling frequency and halving weight."""
+    rng = np.random.default_rng(seed)
+    u = np.linspace(0.0, 1.0, size)
+    total = np.zeros((size, size))
+    for o in range(octaves):
+        cells = 2 ** (o + 1)
+        lattice = rng.uniform(-1.0, 1.0, (cells + 1, cells + 1))
+        x = u * cells
+        i = np.minimum(np.floor(x).astype(np.int64), cells - 1)
+        f = x - i
+        row = lattice[i] * (1.0 - f)[:, None] + lattice[i + 1] * f[:, None]
+        total += 0.5 ** o * (row[:, i] * (1.0 - f)[None, :] + row[:, i + 1] * f[None, :])
+    return relief * (total - total.min()) / (total.max() - total.min())
+
+
+def heightfield_mesh(heights: np.ndarray, pitch: float) -> tuple[np.ndarray, np.ndarray]:
+    """Terrain vertices (x = i pitch, y = j pitch, z = heights[i, j]) and two
+    upward-facing triangles per lattice square."""
+    n = heights.shape[0]
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    v = np.stack([ii * pitch, jj * pitch, heights], axis=-1).reshape(-1, 3).astype(np.float32)
+    a = (ii[:-1, :-1] * n + jj[:-1, :-1]).reshape(-1)
+    b = a + n          # (i + 1, j)
+    c = b + 1          # (i + 1, j + 1)
+    d = a + 1          # (i, j + 1)
+    f = np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], axis=1).reshape(-1, 3)
+    return v, f.astype(np.int64)
+
+
+HEIGHTFIELD_TX_OFFSET = (97.31, 131.87, 25.0)   # x, y, height above the highest terrain point
+
+
+def heightfield_scene(seed: int = 4, size: int = 128, pitch: float = 2.0, relief: float = 20.0,
+                      boxes: int = 40, shape: tuple[int, int] = (256, 256), spacing: float | None = None) -> Scene:
+    rng = np.random.default_rng(seed + 1000)
+    heights = fbm_heights(seed, size, relief=relief)
+    extent = (size - 1) * pitch
+    soil = PhysicalMaterial(thickness_m=5.0, name="soil", **SOIL)
+    walls = [PhysicalMaterial(thickness_m=0.2, name="concrete", **CONCRETE),
+             PhysicalMaterial(thickness_m=0.02, name="glass", **GLASS),
+             PhysicalMaterial(thickness_m=0.25, name="brick", **BRICK)]
+    tv, tf = heightfield_mesh(heights, pitch)
+    structures = [Structure(Mesh(torch.from_numpy(tv), torch.from_numpy(tf)), soil, name="terrain", surface_id=0)]
+    tx_xy = np.array(HEIGHTFIELD_TX_OFFSET[:2])
+    placed: list[tuple[float, float, float, float]] = []
+    while len(placed) < boxes:
+        w, l = rng.uniform(8.0, 20.0, size=2)
+        x0 = rng.uniform(4.0, extent - 4.0 - w)
+        y0 = rng.uniform(4.0, extent - 4.0 - l)
+        box = (x0 - 3.0, y0 - 3.0, x0 + w + 3.0, y0 + l + 3.0)
+        if any(box[0] < p[2] and p[0] < box[2] and box[1] < p[3] and p[1] < box[3] for p in placed):
+            continue
+        if box[0] <= tx_xy[0] <= box[2] and box[1] <= tx_xy[1] <= box[3]:
+            continue
+        placed.append(box)
+        i0, i1 = int(x0 // pitch), int(math.ceil((x0 + w) / pitch))
+        j0, j1 = int(y0 // pitch), int(math.ceil((y0 + l) / pitch))
+        under = heights[i0:i1 + 1, j0:j1 + 1]
+        base = float(under.min()) - 1.0
+        top = float(under.max()) + float(rng.uniform(8.0, 30.0))
+        v, f, _ = box_mesh(x0, y0, x0 + w, y0 + l, top - base)
+        v[:, 2] += base
+        mat = walls[int(rng.integers(0, len(walls)))]
+        structures.append(Structure(Mesh(torch.from_numpy(v), torch.from_numpy(f)), mat, name=f"box-{len(placed)}",
+                                    surface_id=len(placed)))
+    top = float(heights.max())
+    rows, cols = shape
+    if spacing is None:
+        spacing = extent / max(rows, cols)
+    span_r, span_c = (rows - 1) * spacing, (cols - 1) * spacing
+    origin = ((extent - span_r) / 2.0 + 0.137, (extent - span_c) / 2.0 + 0.219, top + 1.5)
+    tx = (HEIGHTFIELD_TX_OFFSET[0], HEIGHTFIELD_TX_OFFSET[1], top + HEIGHTFIELD_TX_OFFSET[2])
+    return Scene(structures=structures, endpoints=[transmitter(tx), grid(origin, shape, spacing)],
+                 metadata={"imported_edge_policy": SimpleNamespace(edge_selection_mode="vertical_only")})
