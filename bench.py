@@ -137,6 +137,7 @@ def main() -> int:
     parser.add_argument("--tx-batch", type=int, default=64)
     parser.add_argument("--no-cpu", action="store_true")
     parser.add_argument("--no-batch", action="store_true")
+    parser.add_argument("--no-heightfield", action="store_true")
     parser.add_argument("--copy-calibration", action="store_true",
                         help="only run a plain device copy of known size (HBM counter calibration)")
     args = parser.parse_args()
@@ -252,6 +253,33 @@ def main() -> int:
                  "batched_cells_per_s": bcells / (ms_b * 1e-3), "plain_cells_per_s": bcells / (ms_p * 1e-3),
                  "batched_over_plain": ms_p / ms_b}
 
+    terrain = None
+    if not args.no_heightfield:
+        # differentiable terrain radiomap: fBm heightfield + boxes, gradient w.r.t. materials, frequency,
+        # transmitter and every mesh vertex
+        h_freq = 3.5e9
+        h_bits = component_bits({"los", "reflection"})
+        h_compiled = compile_scene(workloads.heightfield_scene(seed=4), reference_frequency_hz=h_freq)
+        h_dscene = device_scene(h_compiled, device)
+        h_inputs = engine.solve_inputs(h_compiled, h_dscene, components=h_bits, max_depth=2, coherent=True,
+                                       frequency_hz=h_freq)
+        h_step = MeanDbStep(h_dscene, h_inputs, h_compiled.materials.count, vertices=True)
+        if bool((h_step.out.failure != 0).any()):
+            raise RuntimeError("topology capacity exceeded (heightfield)")
+        h_step.capture()
+        h_ms, _ = timed(h_step.step, max(5, args.steps // 2), 2, stream)
+        h_geo_ms = kernel_time(h_step.geometry, stream)
+        h_fwd_ms = kernel_time(lambda: engine.radiomap_forward(h_dscene, h_inputs, topology_capacity=h_step.capacity,
+                                                               out=h_step.out), stream)
+        h_cells = h_inputs.total_cells
+        terrain = {"workload": f"fBm heightfield 128x128 at 2 m (relief 20 m) + 40 boxes, {h_compiled.face_count} "
+                               "tris, 4 materials, 256x256 grid 1.5 m above the highest point, 3.5 GHz, LoS + "
+                               "depth-2 reflection, loss mean dB; VJP w.r.t. eps_r, sigma, f, Tx and all "
+                               f"{h_compiled.vertices.shape[0]} vertices",
+                   "cells": h_cells, "chains": h_inputs.chains.count,
+                   "paths": int(h_step.out.path_count.sum().item()), "ms_per_step": h_ms,
+                   "cells_per_s": h_cells / (h_ms * 1e-3), "forward_ms": h_fwd_ms, "vertex_vjp_ms": h_geo_ms}
+
     cpu = None
     if not args.no_cpu:
         from tests.gpu_helpers import Run, run_oracle
@@ -304,6 +332,7 @@ def main() -> int:
             "paths_diffraction": n_diff,
             "chain_planning_s": plan_s,
             "multi_tx": batch,
+            "heightfield": terrain,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(dominant, workload_name),

@@ -322,3 +322,38 @@ def test_vjp_vertices_transmission_walls_match_fd(device):
                   endpoints=[workloads.transmitter((0.0, 0.3, 2.0)), workloads.grid((8.0, -4.0, 1.5), (8, 8), 1.0)])
     worst = _vertex_case(device, scene, {"los", "reflection", "transmission"}, 1, 3.0e9, picks=3)
     assert worst <= REL_TOL
+
+
+def test_mean_db_step_vertex_gradient_matches_autograd_and_replays(device):
+    _say("case: MeanDbStep(vertices=True) on the two-ray 16x16 map: eager, graph replays, autograd")
+    import math
+
+    from witwin.channel.kernels.deterministic import MeanDbStep
+
+    scene = workloads.two_ray_scene(shape=(16, 16), spacing=2.0, extent=40.0)
+    run = run_gpu(scene, {"los", "reflection"}, 1, 3.5e9, device, topology_capacity=64)
+    m = run.compiled.materials
+    step = MeanDbStep(run.dscene, run.inputs, m.count, vertices=True)
+    step.launch()
+    torch.cuda.synchronize()
+    eager = step.grad_vertices.clone()
+    step.capture()
+    for _ in range(3):
+        step.step()
+        torch.cuda.synchronize()
+        assert torch.allclose(step.grad_vertices, eager, rtol=1e-6, atol=1e-12)
+    verts = torch.tensor(run.compiled.vertices, device=device, requires_grad=True)
+    outs, _ = radiomap_vjp_outputs(run.dscene, run.inputs, eps_r=torch.tensor(m.eps_r, device=device),
+                                   sigma_e=torch.tensor(m.sigma_e, device=device),
+                                   frequency=torch.tensor(3.5e9, dtype=torch.float32, device=device),
+                                   tx_position=torch.tensor(run.compiled.endpoints.tx_position, device=device),
+                                   material_count=m.count, vertices=verts)
+    power = outs[0]
+    lit = power > 0
+    loss = (10.0 * torch.log10(power[lit])).sum() / max(int(lit.sum()), 1)
+    loss.backward()
+    torch.cuda.synchronize()
+    scale = float(eager.abs().max())
+    diff = float((verts.grad - eager).abs().max())
+    _say(f"  max |autograd - step| {diff:.3e} (max |grad| {scale:.3e})")
+    assert math.isfinite(scale) and scale > 0 and diff <= 1e-4 * scale

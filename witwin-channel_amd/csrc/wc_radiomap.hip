@@ -560,9 +560,12 @@ radiomap_diff_tiles(SceneView s, wc_solve q, wc_radiomap_out o, const int* __res
 // of the cell's chain mask is set when the back-projection through the
 // chain's images lands inside a face of every group and all D + 1 legs are
 // clear. One wavefront per (tile of one transmitter, 32 consecutive chains of
-// that transmitter; the chain and its depth are wave-uniform). Only geometry
-// and traversal live here, so the kernel keeps many wavefronts per SIMD in
-// flight; the reflection field pass then visits the valid chains only.
+// that transmitter; the chain and its depth are wave-uniform). The wavefront
+// first culls its 32 chains against the bounding box of the tile's receivers
+// (chain_reaches_box, one chain per lane), then every receiver tests the
+// surviving chains exactly. Only geometry and traversal live here, so the
+// kernel keeps many wavefronts per SIMD in flight; the reflection field pass
+// then visits the valid chains only.
 template <int D>
 __device__ __forceinline__ bool chain_valid(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
                                             const V3<float>& rx) {
@@ -570,6 +573,83 @@ __device__ __forceinline__ bool chain_valid(const SceneView& s, const wc_solve& 
     int faces[D];
     return solve_chain<D, true>(s, q.chain_groups + c * kMaxDepth, q.chain_images + c * kMaxDepth * 3, tx, rx, hd,
                                 faces);
+}
+
+// Conservative tile test of one chain: can the last bounce of chain c (depth
+// D) land inside its group for some receiver in the box [lo, hi]? The hit of
+// the last leg is the central projection of the receiver from the last image
+// onto the group plane, so over the box it stays inside the convex hull of the
+// projected box corners when every corner lies across the plane from the
+// image; with no corner across, no receiver of the box can reach the plane.
+// false only when the chain fails solve_chain's last-bounce tests for every
+// receiver of the box (margin 1e-3 m plus 1e-5 of the coordinates).
+__device__ inline bool chain_reaches_box(const SceneView& s, const wc_solve& q, int c, int D, const V3<float>& lo,
+                                         const V3<float>& hi) {
+    const int g = q.chain_groups[c * kMaxDepth + D - 1];
+    const double* im = q.chain_images + ((size_t)c * kMaxDepth + D - 1) * 3;
+    double off;
+    const V3<double> n = group_normal(s, g, off);
+    const V3<double> I{im[0], im[1], im[2]};
+    const double si = dot(n, I) - off;
+    V3<double> x[8];
+    int across = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const V3<double> corner{(k & 1) ? hi.x : lo.x, (k & 2) ? hi.y : lo.y, (k & 4) ? hi.z : lo.z};
+        const double sn = dot(n, corner) - off;
+        if (si * sn < 0.0) ++across;
+        x[k] = I + (corner - I) * (si / (si - sn));
+    }
+    if (across == 0) return false;
+    if (across < 8) return true;
+    V3<double> plo = x[0], phi = x[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+        plo = {fmin(plo.x, x[k].x), fmin(plo.y, x[k].y), fmin(plo.z, x[k].z)};
+        phi = {fmax(phi.x, x[k].x), fmax(phi.y, x[k].y), fmax(phi.z, x[k].z)};
+    }
+    // the hull of the projected corners misses a face when their boxes are
+    // apart or all corners lie outside one of its edges; the chain misses the
+    // box when that holds for every face of the group
+    for (int i = s.group_face_offset[g]; i < s.group_face_offset[g + 1]; ++i) {
+        const Tri tri = s.faces[s.group_faces[i]];
+        const V3<double> v[3] = {{tri.v0.x, tri.v0.y, tri.v0.z}, {tri.v1.x, tri.v1.y, tri.v1.z},
+                                 {tri.v2.x, tri.v2.y, tri.v2.z}};
+        const V3<double> flo{fmin(v[0].x, fmin(v[1].x, v[2].x)), fmin(v[0].y, fmin(v[1].y, v[2].y)),
+                             fmin(v[0].z, fmin(v[1].z, v[2].z))};
+        const V3<double> fhi{fmax(v[0].x, fmax(v[1].x, v[2].x)), fmax(v[0].y, fmax(v[1].y, v[2].y)),
+                             fmax(v[0].z, fmax(v[1].z, v[2].z))};
+        const double mb = 1.0e-3 + 1.0e-5 * (fabs(flo.x) + fabs(flo.y) + fabs(flo.z) + fabs(fhi.x) + fabs(fhi.y) +
+                                             fabs(fhi.z));
+        if (plo.x > fhi.x + mb || phi.x < flo.x - mb || plo.y > fhi.y + mb || phi.y < flo.y - mb ||
+            plo.z > fhi.z + mb || phi.z < flo.z - mb)
+            continue;
+        const double orient = dot(cross(v[1] - v[0], v[2] - v[0]), n) >= 0.0 ? 1.0 : -1.0;
+        bool separated = false;
+        for (int e = 0; e < 3 && !separated; ++e) {
+            const V3<double> a = v[e], b = v[(e + 1) % 3];
+            const V3<double> ab = b - a;
+            const double m = (1.0e-3 + 1.0e-5 * (fabs(a.x) + fabs(a.y) + fabs(a.z))) * norm(ab);
+            bool all_out = true;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) all_out = all_out && orient * dot(cross(ab, x[k] - a), n) < -m;
+            separated = all_out;
+        }
+        if (!separated) return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
 }
 
 __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_solve q, const int* __restrict__ tiles,
@@ -582,20 +662,37 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
     const int t = tiles[2 * tile];
     const int width = q.lanes_per_cell;
     const int lane = threadIdx.x & 63;
-    if (lane >= 64 / width) return;
     const int pos = tiles[2 * tile + 1] + lane;
-    if (pos >= q.cell_offset[t + 1]) return;
-    const int cell = cell_at(q, pos);
+    const bool live = lane < 64 / width && pos < q.cell_offset[t + 1];
+    const int cell = live ? cell_at(q, pos) : 0;
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
-    const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
-    const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
+    V3<float> rx{0.0f, 0.0f, 0.0f};
+    if (live) rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[t] : cell));
+    // receiver box of the tile (the whole wave takes part)
+    const V3<float> lo{wave_min(live ? rx.x : INFINITY), wave_min(live ? rx.y : INFINITY),
+                       wave_min(live ? rx.z : INFINITY)};
+    const V3<float> hi{wave_max(live ? rx.x : -INFINITY), wave_max(live ? rx.y : -INFINITY),
+                       wave_max(live ? rx.z : -INFINITY)};
     const int base = q.chain_offset[t * kMaxDepth];
     const int end = q.chain_offset[t * kMaxDepth + min(q.max_depth, kMaxDepth)];
     const int c0 = base + chunk * 32;
     const int c1 = min(c0 + 32, end);
+    // lane l < 32 culls chain c0 + l against the box
+    bool reach = false;
+    if (lane < c1 - c0) {
+        const int c = c0 + lane;
+        int d = 1;
+        while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+        reach = chain_reaches_box(s, q, c, d, lo, hi);
+    }
+    unsigned todo = (unsigned)__ballot(reach);
+    if (!live) return;
     unsigned bits = 0;
     int d = 1;
-    for (int c = c0; c < c1; ++c) {
+    while (todo) {
+        const int b = __ffs(todo) - 1;
+        todo &= todo - 1;
+        const int c = c0 + b;
         while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
         bool ok;
         switch (d) {
@@ -605,7 +702,7 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
             case 4: ok = chain_valid<4>(s, q, c, tx, rx); break;
             default: ok = chain_valid<5>(s, q, c, tx, rx); break;
         }
-        if (ok) bits |= 1u << (c - c0);
+        if (ok) bits |= 1u << b;
     }
     masks[(size_t)cell * words + chunk] = bits;
 }

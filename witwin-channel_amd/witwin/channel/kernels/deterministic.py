@@ -184,11 +184,12 @@ class MeanDbStep:
 
     dL/dP_c = 10 / (ln 10 * N * P_c) on the N cells lit in the first forward
     (the fixed topology keeps that set). Produces ``grad_params`` =
-    [dL/deps_r (M), dL/dsigma (M), dL/df] and ``grad_tx`` [T, 3].
+    [dL/deps_r (M), dL/dsigma (M), dL/df] and ``grad_tx`` [T, 3], and with
+    ``vertices=True`` also ``grad_vertices`` [V, 3].
     """
 
     def __init__(self, dscene, inputs: engine.SolveInputs, material_count: int,
-                 topology_capacity: int = DEFAULT_TOPOLOGY_CAPACITY) -> None:
+                 topology_capacity: int = DEFAULT_TOPOLOGY_CAPACITY, vertices: bool = False) -> None:
         self.dscene = dscene
         self.inputs = inputs
         self.m = material_count
@@ -217,6 +218,17 @@ class MeanDbStep:
             partials=p(self.partials),
         )
         self.graph: torch.cuda.CUDAGraph | None = None
+        self.vertices = vertices
+        if vertices:
+            if dscene.edge_count and int(inputs.view.components) & 4:
+                raise RuntimeError("vertex gradients of diffraction paths are not available; solve without "
+                                   "diffraction to differentiate the mesh vertices")
+            g, e = dscene.group_count, dscene.edge_count
+            self.grad_plane = torch.zeros((g, 4), dtype=torch.float64, device=device)
+            self.grad_edge = torch.zeros((e, 8), dtype=torch.float64, device=device)
+            self.contrib = torch.empty(9 * g + 12 * e, dtype=torch.float32, device=device)
+            self.grad_vertices = torch.empty((int(dscene.vertices.shape[0]), 3), dtype=torch.float32, device=device)
+            self.csr = dscene.vertex_csr()
 
     def launch(self) -> None:
         engine.radiomap_forward(self.dscene, self.inputs, topology_capacity=self.capacity, out=self.out)
@@ -224,6 +236,22 @@ class MeanDbStep:
         torch.div(self.scale, torch.clamp_min(power, 1.0e-38), out=self.grad_power)
         self.adjoint()
         self.reduce()
+        if self.vertices:
+            self.geometry()
+
+    def geometry(self) -> None:
+        """Vertex gradient of the same adjoint (plane accumulators cleared by a
+        fill kernel, then wc_radiomap_vjp_geometry and the per-vertex sums)."""
+        d = self.dscene
+        p = _native.ptr
+        stream = _native.stream_handle(d.device)
+        self.grad_plane.zero_()
+        _native.call("wc_radiomap_vjp_geometry", d.ref(), ctypes.byref(self.inputs.view), ctypes.byref(self.view),
+                     p(self.inputs.tiles), self.inputs.tile_count, p(self.grad_plane), p(self.grad_edge), stream)
+        offset, slot = self.csr
+        _native.call("wc_geometry_vertex_grad", p(d.vertices), p(d.faces), p(d.group_face), d.group_count,
+                     p(self.grad_plane), p(d.edge_vertices), d.edge_count, p(self.grad_edge), p(offset), p(slot),
+                     int(d.vertices.shape[0]), p(self.contrib), p(self.grad_vertices), stream)
 
     def adjoint(self, phase: int = 0) -> None:
         """The adjoint launches alone (per-tile partial gradients); phase 1 or 2
