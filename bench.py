@@ -212,9 +212,9 @@ def main() -> int:
         "diff_visibility_kernel": (edge_ms, cells * (16 + 4 * mask_words(edges)) + scene_bytes + edges * 80),
         "radiomap_diff_tiles": (diff_ms, cells * BYTES_CELL_DIFF + n_diff * BYTES_PER_DESCRIPTOR + edges * 80),
         "vjp_kernel<0>": (vjp0_ms, cells * BYTES_CELL_VJP + n_main * BYTES_PER_DESCRIPTOR + scene_bytes
-                          + inputs.tile_count * step.stride * 4),
+                          + step.partials.numel() * 4),
         "vjp_kernel<1>": (vjp1_ms, cells * BYTES_CELL_VJP + n_diff * BYTES_PER_DESCRIPTOR + edges * 80
-                          + inputs.tile_count * step.stride * 8),
+                          + step.partials.numel() * 8),
     }
     dominant = max(kernels, key=lambda k: kernels[k][0])
     dom_ms, dom_bytes = kernels[dominant]

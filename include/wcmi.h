@@ -202,11 +202,13 @@ int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, const wc_radi
  * Instead of tapes it recomputes every path stored in `topology` (from the
  * forward call) with forward-mode dual numbers over the path's transmitter
  * position, frequency and per-interaction material parameters, contracts
- * with the cell adjoint, and writes one partial gradient vector per tile:
- * partials [tile_count, wc_vjp_partial_stride(M)] =
+ * with the cell adjoint, and writes one partial gradient vector per work
+ * item (a slice of one tile's paths):
+ * partials [wc_vjp_partial_rows(tile_count), wc_vjp_partial_stride(M)] =
  *   [dL/deps_r[0..M), dL/dsigma[0..M), dL/df, dL/dtx_x, dL/dtx_y, dL/dtx_z]
- * where the transmitter terms belong to the tile's transmitter. The upstream
- * gradients may be NULL (zero). Materials are the first-layer view (M <= 16). */
+ * where the transmitter terms belong to the item's tile's transmitter (row r
+ * belongs to tile r / (rows / tile_count)). The upstream gradients may be
+ * NULL (zero). Materials are the first-layer view (M <= 16). */
 typedef struct wc_adjoint {
     const float* grad_power_total;       /* [cells] */
     const float* grad_field_total;       /* [cells] c64 */
@@ -224,7 +226,8 @@ typedef struct wc_adjoint {
 } wc_adjoint;
 
 int wc_vjp_partial_stride(int material_count);
-/* Deterministic reduction of the tile partials (float64 sums in a fixed
+int wc_vjp_partial_rows(int tile_count);
+/* Deterministic reduction of the partial rows (float64 sums in a fixed
  * order: strided per-thread sums, then a fixed-shape tree; one workgroup per
  * output): grad_params [2M+1] = [dL/deps_r, dL/dsigma, dL/df],
  * grad_tx [tx_count, 3]. tiles are the (transmitter, first cell) pairs of the

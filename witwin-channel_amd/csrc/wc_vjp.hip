@@ -483,20 +483,29 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
 #define WC_VJP_WAVES_PER_EU 1
 #endif
 
-// One wavefront per tile of 64 units (unit = one lane-segment of a cell's
-// descriptors; 64 / L cells). The tile's paths of this phase are flattened
-// over its units (wave prefix sum of their counts) and dealt to the 64 lanes
-// round-robin, so a tile costs ceil(paths / 64) path evaluations per lane
-// whatever the spread of paths over its cells. Within a unit the reflection
-// pass stored its descriptors before the diffraction pass appended its own.
-// kPhaseMain writes the tile's partial vector, kPhaseDiff adds to it.
-// With kGeometry the same dealing drives geometry_path instead (no partials).
+// Work items of the adjoint per tile: the tile's flattened path list is cut
+// into this many slices of (nearly) equal iteration counts, one wavefront
+// each, so a tile holding many paths (cells near the transmitter see hundreds
+// of edges) spreads over many SIMDs while the slices of light tiles end at
+// once. Each (tile, slice) owns one partial row.
+constexpr int kVjpSlices = 16;
+
+// One wavefront per (tile of 64 units, slice) (unit = one lane-segment of a
+// cell's descriptors; 64 / L cells). The tile's paths of this phase are
+// flattened over its units (wave prefix sum of their counts) and dealt to the
+// 64 lanes round-robin; slice k takes iterations [k I / S, (k + 1) I / S) of
+// the I = ceil(paths / 64) the tile needs. Within a unit the reflection pass
+// stored its descriptors before the diffraction pass appended its own.
+// kPhaseMain writes the item's partial vector, kPhaseDiff adds to it. With
+// kGeometry the same dealing drives geometry_path instead (no partials).
 template <int kPhase, bool kGeometry = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VJP_WAVES_PER_EU)))
 vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles, int tile_count, GeoGrad geo) {
     const int lane = threadIdx.x & 63;
-    const int tile = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (tile >= tile_count) return;
+    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (item >= tile_count * kVjpSlices) return;
+    const int tile = item / kVjpSlices;
+    const int slice = item - tile * kVjpSlices;
     const int width = q.lanes_per_cell;
     const int t = tiles[2 * tile];
     const int pos = tiles[2 * tile + 1] + lane / width;
@@ -542,7 +551,10 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
     }
     g.freq = 0.0f;
     g.tx[0] = g.tx[1] = g.tx[2] = 0.0f;
-    for (int base = 0; base < total; base += 64) {
+    const int iterations = (total + 63) >> 6;
+    const int it0 = (int)((long long)iterations * slice / kVjpSlices);
+    const int it1 = (int)((long long)iterations * (slice + 1) / kVjpSlices);
+    for (int base = it0 * 64; base < it1 * 64; base += 64) {
         // owner unit u of path j: the last unit whose exclusive prefix is <= j
         // (all lanes take part in the cross-lane reads)
         const int j = base + lane;
@@ -565,7 +577,7 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
     if constexpr (kGeometry) return;
     const int m = s.material_count;
     const int stride = 2 * m + 4;
-    float* out = ga.partials + (size_t)tile * stride;
+    float* out = ga.partials + (size_t)item * stride;
 #pragma unroll
     for (int i = 0; i < kMaxMaterials; ++i) {
         const float se = wave_sum(g.eps[i]);
@@ -597,11 +609,11 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
     }
 }
 
-// Column sums of the tile partials: one workgroup per column. Thread j sums
-// tiles j, j + 256, ... in float64, then a fixed-shape tree in LDS combines
-// the 256 partial sums, so the result is deterministic. Material and
-// frequency columns sum every tile; transmitter columns only the tiles of
-// their transmitter.
+// Column sums of the partial rows (kVjpSlices per tile): one workgroup per
+// column. Thread j sums rows j, j + 256, ... in float64, then a fixed-shape
+// tree in LDS combines the 256 partial sums, so the result is deterministic.
+// Material and frequency columns sum every row; transmitter columns only the
+// rows of their transmitter's tiles.
 __global__ void __launch_bounds__(256) vjp_reduce_kernel(const float* __restrict__ partials, int tile_count,
                                                          int stride, const int* __restrict__ tiles, int tx_count,
                                                          float* __restrict__ grad_params,
@@ -612,8 +624,9 @@ __global__ void __launch_bounds__(256) vjp_reduce_kernel(const float* __restrict
     const int tx = column < m2 ? -1 : (column - m2) / 3;
     const int src = column < m2 ? column : m2 + (column - m2) % 3;
     double acc = 0.0;
-    for (int t = threadIdx.x; t < tile_count; t += blockDim.x) {
-        if (tx < 0 || tiles[2 * t] == tx) acc += (double)partials[(size_t)t * stride + src];
+    const int rows = tile_count * kVjpSlices;
+    for (int r = threadIdx.x; r < rows; r += blockDim.x) {
+        if (tx < 0 || tiles[2 * (r / kVjpSlices)] == tx) acc += (double)partials[(size_t)r * stride + src];
     }
     sums[threadIdx.x] = acc;
     __syncthreads();
@@ -645,6 +658,8 @@ extern "C" int wc_vjp_reduce(const float* partials, int tile_count, int stride, 
 
 extern "C" int wc_vjp_partial_stride(int material_count) { return 2 * material_count + 4; }
 
+extern "C" int wc_vjp_partial_rows(int tile_count) { return tile_count * kVjpSlices; }
+
 static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* adjoint, const int* tiles,
                       int tile_count, int phases, const char* who, void* stream, const GeoGrad* geo = nullptr) {
     if (!scene || !solve || !adjoint) return wc_fail("%s: null argument", who);
@@ -659,7 +674,8 @@ static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adj
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     if (adjoint->topology_capacity % solve->lanes_per_cell != 0)
         return wc_fail("%s: topology_capacity must be a multiple of lanes_per_cell", who);
-    const int blocks = (tile_count + 3) / 4;
+    if (tile_count > 0x7fffffff / kVjpSlices) return wc_fail("%s: too many tiles (%d)", who, tile_count);
+    const int blocks = (tile_count * kVjpSlices + 3) / 4;
     const GeoGrad none{nullptr, nullptr};
     if (geo) {
         if (phases & 1)

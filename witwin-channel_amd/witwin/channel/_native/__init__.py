@@ -107,6 +107,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_radiomap_vjp_phase": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
                                       c_void_p, c_int, c_int, c_void_p]),
     "wc_vjp_partial_stride": (c_int, [c_int]),
+    "wc_vjp_partial_rows": (c_int, [c_int]),
     "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "wc_radiomap_vjp_geometry": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                          ctypes.POINTER(WcAdjoint), c_void_p, c_int, c_void_p, c_void_p,
@@ -138,7 +139,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
 
 
 # int-returning entry points whose result is a value, not a status
-_VALUE_RETURNING = frozenset({"wc_abi_version", "wc_bvh_node_count", "wc_vjp_partial_stride"})
+_VALUE_RETURNING = frozenset({"wc_abi_version", "wc_bvh_node_count", "wc_vjp_partial_stride",
+                              "wc_vjp_partial_rows"})
 
 
 def library_path() -> Path:

@@ -65,8 +65,9 @@ class RadiomapFunction(torch.autograd.Function):
         m = state.material_count
         stride = int(_native.call("wc_vjp_partial_stride", m))
         tiles = inputs.tile_count
-        if state.partials is None or state.partials.shape != (tiles, stride):
-            state.partials = torch.empty((tiles, stride), dtype=torch.float32, device=device)
+        rows = int(_native.call("wc_vjp_partial_rows", tiles))
+        if state.partials is None or state.partials.shape != (rows, stride):
+            state.partials = torch.empty((rows, stride), dtype=torch.float32, device=device)
 
         def prep(g, dtype):
             if g is None:
@@ -204,7 +205,8 @@ class MeanDbStep:
                                  torch.zeros_like(power))
         self.grad_power = torch.empty_like(power)
         self.stride = int(_native.call("wc_vjp_partial_stride", material_count))
-        self.partials = torch.empty((inputs.tile_count, self.stride), dtype=torch.float32, device=device)
+        rows = int(_native.call("wc_vjp_partial_rows", inputs.tile_count))
+        self.partials = torch.empty((rows, self.stride), dtype=torch.float32, device=device)
         self.grad_params = torch.empty(2 * material_count + 1, dtype=torch.float32, device=device)
         self.grad_tx = torch.empty((int(inputs.view.tx_count), 3), dtype=torch.float32, device=device)
         self.loss = torch.empty((), dtype=torch.float32, device=device)
