@@ -171,13 +171,15 @@ typedef struct wc_radiomap_out {
     int* topology;
     int topology_capacity;
     int* failure;
-    void* workspace; /* wc_radiomap_workspace_bytes(cells, L, edge_count, max_tx_chains):
-                        per-lane sums carried from the reflection pass to the
-                        diffraction pass, one edge-visibility bit per (cell, edge)
-                        and one chain-validity bit per (cell, chain) */
+    void* workspace; /* wc_radiomap_workspace_bytes(cells, L, edge_count, max_tx_chains,
+                        tx_count): per-lane sums carried from the reflection pass to
+                        the diffraction pass, one edge-visibility bit per (cell, edge),
+                        one chain-validity bit per (cell, chain) and 64 transmitter-leg
+                        shadow bins per (transmitter, edge) */
 } wc_radiomap_out;
 
-size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count, int max_tx_chains);
+size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count, int max_tx_chains,
+                                   int tx_count);
 int wc_radiomap_forward(const wc_scene* scene, const wc_solve* solve, const wc_radiomap_out* out,
                         const int* tiles, int tile_count, int* tile_counter, void* stream);
 

@@ -367,6 +367,17 @@ __host__ __device__ __forceinline__ unsigned* chain_masks(void* workspace, const
                                        diff_mask_bytes(q.total_cells, edge_count));
 }
 
+// after the chain masks (8-byte aligned): the transmitter-leg shadow bins of
+// every (transmitter, edge), one 64-bit word each (edge_shadow_kernel)
+__host__ __device__ __forceinline__ size_t chain_mask_bytes(int cells, int max_tx_chains) {
+    return (size_t)cells * (size_t)mask_words(max_tx_chains > 0 ? max_tx_chains : 0) * 4;
+}
+__host__ __device__ __forceinline__ size_t shadow_offset(int cells, int lanes, int edge_count, int max_tx_chains) {
+    const size_t o = partial_bytes(cells, lanes) + diff_mask_bytes(cells, edge_count) +
+                     chain_mask_bytes(cells, max_tx_chains);
+    return (o + 7) & ~(size_t)7;
+}
+
 // cell evaluated at launch position `pos` (identity without an order table)
 __device__ __forceinline__ int cell_at(const wc_solve& q, int pos) {
     return q.cell_order ? q.cell_order[pos] : pos;
@@ -718,8 +729,146 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
 #ifndef WC_VIS_WAVES_PER_EU
 #define WC_VIS_WAVES_PER_EU 1
 #endif
+
+// Transmitter-leg shadow of one edge cast by one triangle: the edge
+// parameter range [0, length] is cut into 64 bins; bit b is set when the
+// triangle blocks every segment from the transmitter T to a point of bin b.
+// All those segments lie in the plane through T and the edge line; the
+// triangle meets that plane in a segment whose central projection from T onto
+// the edge line is the interval of blocked edge parameters. The decision
+// keeps margins that put every counted crossing well inside the triangle
+// (shrunk by 1 mm + 1e-5 of the coordinates), at least 0.1 % of the leg away
+// from both of its ends, and at a crossing angle of at least ~6 degrees in the
+// plane and between the planes, so the exact segment test (watertight, end
+// bias 1e-4 m) blocks those legs too: a set bit only ever removes candidates
+// the full test rejects.
+struct ShadowFrame {
+    V3<double> A, d, w, m;  // edge origin and direction, unit towards T in the fan plane, plane normal
+    double a, rho, L, bw, shrink, dt;
+    bool valid;
+};
+
+__host__ __device__ inline ShadowFrame shadow_frame(const Edge& e, const V3<float>& tx) {
+    ShadowFrame f;
+    const V3<double> T{tx.x, tx.y, tx.z};
+    f.A = {e.ax, e.ay, e.az};
+    f.d = {e.dx, e.dy, e.dz};
+    f.L = e.length;
+    f.a = dot(T - f.A, f.d);
+    const V3<double> w = T - f.A - f.d * f.a;
+    f.rho = sqrt(dot(w, w));
+    f.valid = f.rho > 1.0e-2 && f.L > 0.0;
+    f.w = f.valid ? w * (1.0 / f.rho) : V3<double>{0.0, 0.0, 0.0};
+    f.m = cross(f.d, f.w);
+    const double scale = fabs(f.A.x) + fabs(f.A.y) + fabs(f.A.z) + fabs(T.x) + fabs(T.y) + fabs(T.z) + f.L;
+    f.shrink = 1.0e-3 + 1.0e-5 * scale;
+    f.bw = f.L / 64.0;
+    f.dt = 1.0e-4 + 1.0e-6 * scale;
+    return f;
+}
+
+__host__ __device__ inline unsigned long long triangle_shadow_bins(const ShadowFrame& f, const Tri& tri) {
+    const V3<double> v[3] = {{tri.v0.x, tri.v0.y, tri.v0.z}, {tri.v1.x, tri.v1.y, tri.v1.z},
+                             {tri.v2.x, tri.v2.y, tri.v2.z}};
+    V3<double> nt = cross(v[1] - v[0], v[2] - v[0]);
+    const double nn = sqrt(dot(nt, nt));
+    if (!(nn > 0.0)) return 0;
+    nt = nt * (1.0 / nn);
+    // dihedral angle between the triangle and the fan plane
+    const V3<double> c = cross(nt, f.m);
+    if (dot(c, c) < 0.01) return 0;
+    double sd[3];
+    for (int k = 0; k < 3; ++k) sd[k] = dot(f.m, v[k] - f.A);
+    V3<double> P[2];
+    int np = 0;
+    for (int k = 0; k < 3; ++k) {
+        const int j = (k + 1) % 3;
+        if ((sd[k] < 0.0 && sd[j] > 0.0) || (sd[k] > 0.0 && sd[j] < 0.0)) {
+            if (np < 2) P[np] = v[k] + (v[j] - v[k]) * (sd[k] / (sd[k] - sd[j]));
+            ++np;
+        }
+    }
+    if (np != 2) return 0;
+    // 2-D coordinates in the fan plane: x along the edge, y towards the transmitter
+    double x0 = dot(P[0] - f.A, f.d), y0 = dot(P[0] - f.A, f.w);
+    double x1 = dot(P[1] - f.A, f.d), y1 = dot(P[1] - f.A, f.w);
+    const double len = sqrt((x1 - x0) * (x1 - x0) + (y1 - y0) * (y1 - y0));
+    if (len <= 2.0 * f.shrink) return 0;
+    {
+        const double ux = (x1 - x0) / len, uy = (y1 - y0) / len;
+        x0 += ux * f.shrink;
+        y0 += uy * f.shrink;
+        x1 -= ux * f.shrink;
+        y1 -= uy * f.shrink;
+    }
+    // clip to the band of crossings at least 0.1 % of the leg from both ends:
+    // lambda = 1 - y / rho in [1e-3, 1 - 1e-3]
+    const double ylo = 1.0e-3 * f.rho, yhi = (1.0 - 1.0e-3) * f.rho;
+    if ((y0 < ylo && y1 < ylo) || (y0 > yhi && y1 > yhi)) return 0;
+    if (y0 != y1) {
+        const double dy = y1 - y0;
+        const double sa = (ylo - y0) / dy, sb = (yhi - y0) / dy;
+        const double s0 = fmax(0.0, fmin(sa, sb)), s1 = fmin(1.0, fmax(sa, sb));
+        if (!(s0 < s1)) return 0;
+        const double cx0 = x0 + (x1 - x0) * s0, cy0 = y0 + dy * s0;
+        const double cx1 = x0 + (x1 - x0) * s1, cy1 = y0 + dy * s1;
+        x0 = cx0; y0 = cy0; x1 = cx1; y1 = cy1;
+    }
+    // in-plane crossing angle with the legs at both ends
+    const double ex = x1 - x0, ey = y1 - y0, el = sqrt(ex * ex + ey * ey);
+    if (!(el > 0.0)) return 0;
+    const double ends[2][2] = {{x0, y0}, {x1, y1}};
+    for (int k = 0; k < 2; ++k) {
+        const double rx = ends[k][0] - f.a, ry = ends[k][1] - f.rho;
+        const double rl = sqrt(rx * rx + ry * ry);
+        if (!(rl > 0.0) || fabs(ex * ry - ey * rx) < 0.1 * el * rl) return 0;
+    }
+    // central projection from the transmitter onto the edge line
+    const double t0 = f.a + (x0 - f.a) * f.rho / (f.rho - y0);
+    const double t1 = f.a + (x1 - f.a) * f.rho / (f.rho - y1);
+    const double lo = fmin(t0, t1) + f.dt, hi = fmax(t0, t1) - f.dt;
+    if (!(hi > lo)) return 0;
+    const double b0 = ceil(lo / f.bw), b1 = floor(hi / f.bw) - 1.0;
+    const int first = (int)fmax(b0, 0.0);
+    const int last = (int)fmin(b1, 63.0);
+    if (first > last) return 0;
+    const int count = last - first + 1;
+    return (count >= 64 ? ~0ull : ((1ull << count) - 1ull)) << first;
+}
+
+// Shadow bins of every (transmitter, edge) over all triangles outside the
+// edge's own face groups (which the leg test ignores). One wavefront per
+// (transmitter, edge); lanes stride over the triangles.
+__global__ void __launch_bounds__(256) edge_shadow_kernel(SceneView s, wc_solve q,
+                                                          unsigned long long* __restrict__ shadow) {
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int edges = s.edge_count;
+    if (wave >= (long long)q.tx_count * edges) return;
+    const int lane = threadIdx.x & 63;
+    const int t = (int)(wave / edges);
+    const int ei = (int)(wave - (long long)t * edges);
+    const Edge e = s.edges[ei];
+    const ShadowFrame f = shadow_frame(e, ld3(q.tx_pos + 3 * t));
+    unsigned long long bits = 0;
+    if (f.valid) {
+        for (int k = lane; k < s.face_count; k += 64) {
+            const Tri tri = s.faces[k];
+            const int group = __float_as_int(tri.v0.w);
+            if (group == e.group0 || group == e.group1) continue;
+            bits |= triangle_shadow_bins(f, tri);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned lo32 = __shfl_xor((unsigned)bits, off), hi32 = __shfl_xor((unsigned)(bits >> 32), off);
+        bits |= ((unsigned long long)hi32 << 32) | lo32;
+    }
+    if (lane == 0) shadow[wave] = bits;
+}
+
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VIS_WAVES_PER_EU)))
-diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, int words, long long waves) {
+diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, int words, long long waves,
+                       const unsigned long long* __restrict__ shadow) {
     const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= waves) return;
     const int chunk = (int)(wave % words);
@@ -736,8 +885,16 @@ diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, in
     unsigned bits = 0;
     for (int ei = e0; ei < e1; ++ei) {
         const Edge e = s.edges[ei];
+        const unsigned long long sh = shadow ? shadow[(size_t)t * s.edge_count + ei] : 0ull;
+        if (sh == ~0ull) continue;  // no point of the edge sees the transmitter
         DiffGeometry g;
         if (!diffraction_geometry(e, tx, rx, g, false)) continue;
+        if (sh) {
+            // the transmitter leg of a stationary point in a shadowed bin is blocked
+            const double tc = fmin(fmax(g.tstar, 0.0), (double)e.length);
+            const int bin = min((int)(tc * (64.0 / (double)e.length)), 63);
+            if ((sh >> bin) & 1ull) continue;
+        }
         if (edge_visible(s, e, g, tx, rx)) bits |= 1u << (ei - e0);
     }
     masks[(size_t)cell * words + chunk] = bits;
@@ -950,14 +1107,26 @@ static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const
                        tile_count, masks, words);
 }
 
+// The shadow table costs transmitters x edges x triangles segment-plane
+// tests; above this many it is skipped (the visibility test alone is exact).
+constexpr long long kShadowBudget = 4000000000LL;
+
 static void launch_diff_visibility(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o, hipStream_t st) {
     const int words = mask_words(s.edge_count);
     if (words == 0) return;
     unsigned* masks = reinterpret_cast<unsigned*>(static_cast<char*>(o.workspace) +
                                                   partial_bytes(q.total_cells, q.lanes_per_cell));
+    unsigned long long* shadow = nullptr;
+    const long long pairs = (long long)q.tx_count * s.edge_count;
+    if (pairs > 0 && pairs * (long long)s.face_count <= kShadowBudget) {
+        shadow = reinterpret_cast<unsigned long long*>(
+            static_cast<char*>(o.workspace) + shadow_offset(q.total_cells, q.lanes_per_cell, s.edge_count,
+                                                            q.max_tx_chains));
+        hipLaunchKernelGGL(edge_shadow_kernel, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, st, s, q, shadow);
+    }
     const long long waves = ((long long)q.total_cells + 63) / 64 * words;
     hipLaunchKernelGGL(diff_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q, masks, words,
-                       waves);
+                       waves, (const unsigned long long*)shadow);
 }
 
 // Launch stages: bit 0 chain validity (tiled only), bit 1 the
@@ -1046,10 +1215,12 @@ extern "C" int wc_radiomap_pass(const wc_scene* scene, const wc_solve* solve, co
                            stream);
 }
 
-extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count, int max_tx_chains) {
+extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int edge_count, int max_tx_chains,
+                                              int tx_count) {
     const int c = cells > 0 ? cells : 0;
-    return partial_bytes(c, lanes_per_cell > 0 ? lanes_per_cell : 1) + diff_mask_bytes(c, edge_count > 0 ? edge_count : 0) +
-           (size_t)c * (size_t)mask_words(max_tx_chains > 0 ? max_tx_chains : 0) * 4;
+    const int e = edge_count > 0 ? edge_count : 0;
+    return shadow_offset(c, lanes_per_cell > 0 ? lanes_per_cell : 1, e, max_tx_chains) +
+           (size_t)(tx_count > 0 ? tx_count : 0) * (size_t)e * 8;
 }
 
 extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream) {

@@ -97,7 +97,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_bvh_node_count": (c_int, [c_int]),
     "wc_bvh_build": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_size_t, c_void_p]),
-    "wc_radiomap_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "wc_radiomap_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int]),
     "wc_radiomap_forward": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                     ctypes.POINTER(WcRadiomapOut), c_void_p, c_int, c_void_p, c_void_p]),
     "wc_radiomap_pass": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),

@@ -338,7 +338,8 @@ def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capac
             workspace=torch.empty(max(int(_native.call("wc_radiomap_workspace_bytes", cells,
                                                        int(inputs.view.lanes_per_cell),
                                                        int(dscene.view.edge_count),
-                                                       int(inputs.view.max_tx_chains))), 32),
+                                                       int(inputs.view.max_tx_chains),
+                                                       int(inputs.view.tx_count))), 32),
                                   dtype=torch.uint8, device=device),
         )
     p = _native.ptr
