@@ -380,6 +380,39 @@ int wc_accumulate_flat(int count, const unsigned char* valid, const int* tx_id, 
 int wc_visibility(const wc_scene* scene, int count, const float* source, const float* target,
                   unsigned char* visible, void* stream);
 
+/* ------------------------------------------------------------ fixed-topology replay
+ * Replays frozen path rows at new endpoints (consumer reevaluate). Replaces
+ * the per-bucket replay of witwin/channel/propagation/consumer.py:3029-3096
+ * (_los_inputs :2770 with the RayD visibility gate, _reflection_inputs :2734
+ * -> reflection_epc_paths, propagation/geometry.py:214, then
+ * field_free_space / field_reflection_sequence). component 0 (line of sight,
+ * depth 0) re-tests its segment; component 1 (reflection, depth 1..width)
+ * re-solves the image method over the planes of its frozen faces' groups and
+ * re-checks faces and legs. Rows that no longer exist get valid = 0 and zero
+ * fields; other components are invalid. One launch covers every depth. */
+typedef struct wc_fixed_rows {
+    int count;
+    int width;                    /* interaction slots per row */
+    const int* component;         /* [N] */
+    const int* depth;             /* [N] */
+    const int* faces;             /* [N, width] frozen face ids */
+    const float* source;          /* [N, 3] */
+    const float* target;          /* [N, 3] */
+    const float* tx_power;        /* [N] W */
+    const float* tx_polarization; /* [N, 3] */
+    const float* rx_polarization; /* [N, 3] */
+    unsigned char* valid;         /* [N] out */
+    float* path_field;            /* [N] c64 out: coefficient * sqrt(power) */
+    float* field_xyz;             /* [N, 3] c64 out: unit-excitation field at the target */
+    float* path_length;           /* [N] out, m */
+    float* delay;                 /* [N] out, s */
+    float* direction;             /* [N, 3] out: arrival direction */
+    float* positions;             /* [N, width, 3] out (zeros past the depth or when invalid) */
+    float* normals;               /* [N, width, 3] out */
+} wc_fixed_rows;
+
+int wc_fixed_rows_evaluate(const wc_scene* scene, const wc_fixed_rows* rows, float frequency_hz, void* stream);
+
 /* ------------------------------------------------------------ discovery
  * Transmitter-launched specular discovery rays (reference
  * native/channel/rayd/reflection.cpp:9-21, witwin/channel/interactions/

@@ -126,3 +126,135 @@ def test_consumer_los_jones_closed_form(device):
         # bases transverse to the LoS direction
         dn = d / dist
         assert abs(tb[0] @ dn) < 1e-5 and abs(rb[1] @ dn) < 1e-5
+
+
+# ------------------------------------------------------------------ fixed-topology replay
+
+def _replay_request(sources, sinks, topology, response="scalar_transport", **kw):
+    return consumer.FixedTopologyRequest(sources=sources, sinks=sinks, reference_frequency_hz=28e9,
+                                         topology=topology, response=response, ad_mode="none", **kw)
+
+
+def _street_sinks(rng, n):
+    """Sinks along the two streets that cross at the canyon transmitter, so
+    most of them see it and its wall reflections."""
+    along = rng.uniform(0.0, 370.0, n)
+    across = rng.uniform(46.0, 59.0, n)
+    ns = rng.uniform(size=n) < 0.5
+    pts = np.stack([np.where(ns, across, along), np.where(ns, along, across), np.full(n, 1.5)], axis=1)
+    return pts.astype(np.float32)
+
+
+def _discover(device, tx, rx, compiled, comps=frozenset({"los", "reflection"}), depth=2):
+    sources, sinks = _batches(device, tx, [1.0] * len(tx), rx)
+    req = consumer.PropagationRequest(sources=sources, sinks=sinks, reference_frequency_hz=28e9, components=comps,
+                                      max_depth=depth, response="scalar_transport", topology_mode="discover",
+                                      ad_mode="none")
+    return consumer.evaluate(compiled, req), sources, sinks
+
+
+def _keys(paths, face_group):
+    """(pair, component, coplanar groups of the sequence): a replayed row keeps
+    its frozen face labels when a hit slides onto a coplanar twin triangle."""
+    seq = paths.topology.primitive_sequence.cpu().numpy()
+    groups = np.where(seq >= 0, face_group[np.maximum(seq, 0)], -1)
+    comp = paths.topology.component_id.cpu().tolist()
+    pair = paths.pair_index.cpu().tolist()
+    return [(p, c, tuple(g)) for p, c, g in zip(pair, comp, groups.tolist())]
+
+
+def test_replay_at_discovery_endpoints_reproduces_evaluate(device):
+    _say("case: reevaluate at the discovery endpoints, canyon 2 sources x 9 sinks, depth 2, los+refl")
+    rng = np.random.default_rng(21)
+    compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    ev, sources, sinks = _discover(device, [workloads.URBAN_CANYON_TX, (52.7, 140.3, 12.0)],
+                                   _street_sinks(rng, 9), compiled)
+    prepared = consumer.prepare_fixed_topology(ev.paths.topology)
+    out = consumer.reevaluate(compiled, _replay_request(sources, sinks, prepared))
+    p, q = ev.paths, out.paths
+    assert q.path_count == p.path_count > 0
+    assert bool(out.row_valid.all())
+    assert torch.equal(q.pair_index, p.pair_index) and torch.equal(q.pair_offsets, p.pair_offsets)
+    a, b = q.transport.coefficient.cpu(), p.transport.coefficient.cpu()
+    comp = p.topology.component_id.cpu()
+    for c in (0, 1):
+        sel = comp == c
+        if bool(sel.any()):
+            rel = ((a[sel] - b[sel]).abs() / b[sel].abs().clamp_min(1e-30)).max()
+            _say(f"  component {c}: {int(sel.sum())} rows, identical {int((a[sel] == b[sel]).sum())}, "
+                 f"max relative difference {float(rel):.2e}")
+    assert torch.allclose(a, b, rtol=1e-6, atol=0.0)
+    assert torch.equal(q.geometry.delay_s.cpu(), p.geometry.delay_s.cpu())
+    assert torch.allclose(q.geometry.interaction_positions_m.cpu(), p.geometry.interaction_positions_m.cpu(),
+                          rtol=0.0, atol=1e-5)
+    assert torch.equal(q.geometry.interaction_normals.cpu(), p.geometry.interaction_normals.cpu())
+    assert torch.allclose(q.geometry.field_direction.cpu(), p.geometry.field_direction.cpu(), rtol=0.0, atol=1e-6)
+    assert out.diagnostics.validation_sync_count == 1 and out.diagnostics.discovery_launch_count == 0
+
+
+def test_replay_with_moved_sinks_matches_fresh_discovery(device):
+    _say("case: reevaluate with every sink moved 0.7 m vs a fresh evaluate at the moved sinks")
+    rng = np.random.default_rng(22)
+    compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    tx = [workloads.URBAN_CANYON_TX, (52.7, 140.3, 12.0)]
+    rx = _street_sinks(rng, 12)
+    ev, sources, _ = _discover(device, tx, rx, compiled)
+    moved = rx + np.array([0.7, -0.4, 0.3], np.float32)
+    _, sinks_moved = _batches(device, tx, [1.0] * 2, moved)
+    out = consumer.reevaluate(compiled, _replay_request(sources, sinks_moved,
+                                                        consumer.prepare_fixed_topology(ev.paths.topology)))
+    fresh, _, _ = _discover(device, tx, moved, compiled)
+    fresh_rows = {k: i for i, k in enumerate(_keys(fresh.paths, compiled.face_group))}
+    valid = out.row_valid.cpu().tolist()
+    coeff = out.paths.transport.coefficient.cpu()
+    fcoeff = fresh.paths.transport.coefficient.cpu()
+    kept = dropped = 0
+    for r, key in enumerate(_keys(out.paths, compiled.face_group)):
+        if valid[r]:
+            assert key in fresh_rows, f"replayed row {key} is valid but absent from the fresh discovery"
+            assert torch.allclose(coeff[r], fcoeff[fresh_rows[key]], rtol=1e-5, atol=1e-12)
+            kept += 1
+        else:
+            assert key not in fresh_rows
+            assert coeff[r] == 0
+            dropped += 1
+    _say(f"  rows kept {kept}, dropped {dropped}, new in the fresh discovery "
+         f"{len(fresh_rows) - kept}")
+    assert kept > 0
+
+
+def test_replay_slots_and_wideband_columns(device):
+    _say("case: reevaluate over 3 block-diagonal slots and 3 frequency offsets")
+    rng = np.random.default_rng(23)
+    compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    tx = [workloads.URBAN_CANYON_TX]
+    rx = _street_sinks(rng, 5)
+    ev, sources, sinks = _discover(device, tx, rx, compiled)
+    prepared = consumer.prepare_fixed_topology(ev.paths.topology)
+    shifts = [np.zeros(3, np.float32), np.array([0.3, 0.2, 0.0], np.float32), np.array([-0.5, 0.1, 0.2], np.float32)]
+    singles = []
+    for sh in shifts:
+        _, s1 = _batches(device, tx, [1.0], rx + sh)
+        singles.append(consumer.reevaluate(compiled, _replay_request(sources, s1, prepared)))
+    src3, snk3 = _batches(device, tx * 3, [1.0] * 3, np.concatenate([rx + sh for sh in shifts]))
+    rep = consumer.replicate_over_slots(prepared, 3, source_count=1, sink_count=5)
+    out = consumer.reevaluate(compiled, _replay_request(src3, snk3, rep, slot_count=3))
+    k = prepared.row_count
+    assert out.paths.pair_count == 3 * 5 and out.paths.path_count == 3 * k
+    for t, single in enumerate(singles):
+        sl = slice(t * k, (t + 1) * k)
+        assert torch.equal(out.paths.transport.coefficient[sl].cpu(), single.paths.transport.coefficient.cpu())
+        assert torch.equal(out.row_valid[sl].cpu(), single.row_valid.cpu())
+        assert torch.equal(out.paths.pair_index[sl].cpu() - t * 5, single.paths.pair_index.cpu())
+    # wideband: column j equals a replay on the scene compiled at 28 GHz + df_j
+    offsets = (-2.0e6, 0.0, 5.0e6)
+    wb = consumer.reevaluate(compiled, _replay_request(sources, sinks, prepared, frequency_offsets_hz=offsets))
+    cols = wb.paths.transport.coefficient_offsets
+    assert cols.shape == (k, 3)
+    assert torch.equal(cols[:, 1].cpu(), wb.paths.transport.coefficient.cpu())
+    for j, df in enumerate(offsets):
+        other = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9 + df)
+        req = consumer.FixedTopologyRequest(sources=sources, sinks=sinks, reference_frequency_hz=28e9 + df,
+                                            topology=prepared, response="scalar_transport", ad_mode="none")
+        ref = consumer.reevaluate(other, req)
+        assert torch.equal(cols[:, j].cpu(), ref.paths.transport.coefficient.cpu())

@@ -79,6 +79,16 @@ class WcPathRows(ctypes.Structure):
         "departure_direction")]
 
 
+class WcFixedRows(ctypes.Structure):
+    _fields_ = [
+        ("count", c_int), ("width", c_int), ("component", c_void_p), ("depth", c_void_p), ("faces", c_void_p),
+        ("source", c_void_p), ("target", c_void_p), ("tx_power", c_void_p), ("tx_polarization", c_void_p),
+        ("rx_polarization", c_void_p), ("valid", c_void_p), ("path_field", c_void_p), ("field_xyz", c_void_p),
+        ("path_length", c_void_p), ("delay", c_void_p), ("direction", c_void_p), ("positions", c_void_p),
+        ("normals", c_void_p),
+    ]
+
+
 class WcAdjoint(ctypes.Structure):
     _fields_ = [
         ("grad_power_total", c_void_p), ("grad_field_total", c_void_p), ("grad_component_power", c_void_p),
@@ -109,6 +119,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_vjp_partial_stride": (c_int, [c_int]),
     "wc_vjp_partial_rows": (c_int, [c_int]),
     "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "wc_fixed_rows_evaluate": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcFixedRows), ctypes.c_float,
+                                       c_void_p]),
     "wc_radiomap_vjp_geometry": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                          ctypes.POINTER(WcAdjoint), c_void_p, c_int, c_void_p, c_void_p,
                                          c_void_p]),

@@ -7,7 +7,11 @@ consumer.py): the vocabulary (:405-435), ``EndpointBatch`` (:515-588),
 ``PropagationRequest`` (:623-653), ``PropagationTopology`` /
 ``PropagationGeometry`` (:656-793), the transports (:796-916),
 ``PropagationPathBatch`` (:919-960), ``PropagationConvention`` (:963-1005),
-``capabilities`` (:1181) and ``evaluate`` (:3478-3520).
+``capabilities`` (:1181), ``evaluate`` (:3478-3520) and the fixed-topology
+replay: ``prepare_fixed_topology`` (:1518), ``replicate_over_slots`` (:1573),
+``FixedTopologyRequest`` (:1307), ``FixedTopologyEvaluation`` (:1421) and
+``reevaluate`` (:3965-4110), with slot batching and wideband frequency
+offsets.
 
 The rows come from the same HIP engine as ``deterministic.solve`` with
 ``export_paths`` (count -> one count read -> emit -> canonical order), run on
@@ -17,7 +21,9 @@ response is line-of-sight only, as in the reference; it excites the native
 free-space row operator with the two source basis vectors and projects onto
 the sink basis. Forward evaluation only: ``ad_mode`` other than ``"none"`` is
 refused (the differentiable route of this build is ``deterministic.solve``
-with ``ad_mode="vjp"``).
+with ``ad_mode="vjp"``). ``reevaluate`` replays frozen line-of-sight and
+reflection rows at new endpoints in one native launch per frequency column
+(``wc_fixed_rows_evaluate``).
 """
 
 from __future__ import annotations
@@ -39,6 +45,9 @@ COMPONENTS: frozenset[str] = frozenset({"los", "reflection", "transmission", "di
 RESPONSES: frozenset[str] = frozenset({"scalar_transport", "complex3_transport", "polarimetric_transport"})
 TOPOLOGY_MODES: frozenset[str] = frozenset({"discover"})
 AD_MODES: frozenset[str] = frozenset({"none", "jvp", "vjp"})
+WORLD_MOTIONS: frozenset[str] = frozenset({"frozen_world", "fixed_winner_replay"})
+# component ids a frozen topology may carry into a replay (reference consumer.py:211-214)
+FIXED_TOPOLOGY_COMPONENT_IDS: tuple[tuple[str, int], ...] = (("los", 0), ("reflection", 1))
 MAX_DEPTH = 5
 
 
@@ -270,6 +279,13 @@ class PropagationPathBatch:
 class PropagationConvention:
     contract_version: int = CONTRACT_VERSION
     pair_layout: str = "sink_major_source_minor:pair_index=sink_index*source_count+source_index"
+    # block-diagonal law of a slot-batched replay (reference consumer.py:975-983)
+    slot_pair_layout: str = ("block_diagonal_slots:row=slot*frozen_row_count+frozen_row;"
+                             "source_index=slot*slot_source_count+slot_source_index;"
+                             "sink_index=slot*slot_sink_count+slot_sink_index;"
+                             "pair_index=slot*slot_source_count*slot_sink_count"
+                             "+slot_sink_index*slot_source_count+slot_source_index;"
+                             "pair_count=slot_count*slot_source_count*slot_sink_count")
     distance_unit: str = "m"
     delay_unit: str = "s"
     phasor: str = PHASOR
@@ -290,7 +306,13 @@ class PropagationCapabilities:
     response_components: tuple[tuple[str, frozenset[str]], ...]
     response_ad_modes: tuple[tuple[str, frozenset[str]], ...]
     supports_los_jones: bool = True
-    supports_fixed_topology: bool = False
+    supports_fixed_topology: bool = True
+    fixed_topology_components: frozenset[str] = frozenset({"los", "reflection"})
+    fixed_topology_responses: frozenset[str] = frozenset(
+        {"scalar_transport", "complex3_transport", "polarimetric_transport"})
+    fixed_topology_row_validity_components: frozenset[str] = frozenset({"los", "reflection"})
+    wideband_responses: frozenset[str] = frozenset({"scalar_transport", "complex3_transport"})
+    world_motions: frozenset[str] = WORLD_MOTIONS
 
     def components_for(self, response: str) -> frozenset[str]:
         return dict(self.response_components)[response]
@@ -520,10 +542,383 @@ def evaluate(compiled_scene: CompiledScene, request: PropagationRequest) -> Prop
     return PropagationEvaluation(paths=paths, diagnostics=diagnostics)
 
 
+# ---------------------------------------------------------------- fixed-topology replay
+
+@dataclass(frozen=True, slots=True, eq=False)
+class FixedTopologyBucket:
+    """One ``(component, depth)`` partition of a frozen topology: ascending
+    row indices, so a bucket keeps frozen row order (reference
+    consumer.py:1228-1249)."""
+
+    component: str
+    depth: int
+    rows: torch.Tensor
+
+    def __post_init__(self) -> None:
+        _require_vocabulary("component", self.component, COMPONENTS)
+        if type(self.depth) is not int or not 0 <= self.depth <= MAX_DEPTH:
+            raise ValueError(f"depth must be an int in [0, {MAX_DEPTH}]")
+        _require_tensor("rows", self.rows, dtype=torch.int64)
+
+    @property
+    def row_count(self) -> int:
+        return int(self.rows.shape[0])
+
+
+@dataclass(frozen=True, slots=True, eq=False)
+class PreparedFixedTopology:
+    """A frozen topology partitioned once for repeated replay (reference
+    consumer.py:1252-1287). The counters describe the preparation itself."""
+
+    topology: PropagationTopology
+    buckets: tuple[FixedTopologyBucket, ...]
+    prepare_d2h_copies: int
+    prepare_d2h_bytes: int
+    prepare_synchronizations: int
+    slot_count: int = 1
+    slot_source_count: int = 0
+    slot_sink_count: int = 0
+
+    @property
+    def row_count(self) -> int:
+        return self.topology.row_count
+
+    @property
+    def device(self) -> torch.device:
+        return self.topology.primitive_sequence.device
+
+    @property
+    def provenance(self) -> Any:
+        return self.topology.provenance
+
+
+def _component_name(component_id: int) -> str:
+    for name, cid in FIXED_TOPOLOGY_COMPONENT_IDS:
+        if cid == component_id:
+            return name
+    raise NotImplementedError(
+        f"fixed-topology reevaluation supports components "
+        f"{sorted(_CAPABILITIES.fixed_topology_components)}; the frozen topology carries component id "
+        f"{component_id}")
+
+
+def _malformed_rows(topology: PropagationTopology, width: int) -> torch.Tensor:
+    depth = topology.depth.to(torch.int64)
+    bad = (depth < 0) | (depth > width)
+    if width:
+        slots = torch.arange(width, device=depth.device).unsqueeze(0)
+        used = slots < depth.unsqueeze(1)
+        seq = topology.primitive_sequence
+        bad = bad | ((seq >= 0) != used).any(dim=1)
+    return bad
+
+
+def prepare_fixed_topology(topology: PropagationTopology) -> PreparedFixedTopology:
+    """Partition a frozen topology by (component, depth) once (reference
+    consumer.py:1518-1570). Synchronizes; call it once per frozen topology."""
+    if not isinstance(topology, PropagationTopology):
+        raise TypeError("topology must be a PropagationTopology")
+    if topology.row_count == 0:
+        return PreparedFixedTopology(topology=topology, buckets=(), prepare_d2h_copies=0, prepare_d2h_bytes=0,
+                                     prepare_synchronizations=0)
+    width = int(topology.primitive_sequence.shape[1])
+    if bool(_malformed_rows(topology, width).any().item()):
+        raise ValueError("frozen topology rows disagree with their interaction sequence padding; depth must be "
+                         "in [0, sequence width] and unused slots must hold -1")
+    key = topology.component_id.to(torch.int64) * (width + 1) + topology.depth.to(torch.int64)
+    distinct = torch.unique(key).tolist()
+    buckets = []
+    for value in distinct:
+        component_id, depth = divmod(int(value), width + 1)
+        component = _component_name(component_id)
+        if (component == "los") != (depth == 0):
+            raise ValueError(f"a {component} row cannot have depth {depth}")
+        buckets.append(FixedTopologyBucket(component=component, depth=depth,
+                                           rows=torch.nonzero(key == value, as_tuple=False).reshape(-1)))
+    return PreparedFixedTopology(topology=topology, buckets=tuple(buckets), prepare_d2h_copies=2 + len(buckets),
+                                 prepare_d2h_bytes=1 + 8 * (len(distinct) + len(buckets)),
+                                 prepare_synchronizations=2 + len(buckets))
+
+
+def _tile(t: torch.Tensor, slot_count: int) -> torch.Tensor:
+    return t.repeat((slot_count,) + (1,) * (t.ndim - 1))
+
+
+def replicate_over_slots(prepared: PreparedFixedTopology, slot_count: int, *, source_count: int,
+                         sink_count: int) -> PreparedFixedTopology:
+    """Tile a prepared topology over ``slot_count`` block-diagonal slots
+    (reference consumer.py:1573-1660): row ``t*K + r`` is frozen row ``r`` in
+    slot ``t``, its source and sink indices shifted by ``t`` times the
+    per-slot counts. Index arithmetic only; ``slot_count == 1`` returns the
+    handle unchanged."""
+    if not isinstance(prepared, PreparedFixedTopology):
+        raise TypeError("prepared must be a PreparedFixedTopology")
+    if type(slot_count) is not int or slot_count <= 0:
+        raise ValueError("slot_count must be a positive int")
+    for name, value in (("source_count", source_count), ("sink_count", sink_count)):
+        if type(value) is not int or value <= 0:
+            raise ValueError(f"{name} must be a positive int")
+    if slot_count == 1:
+        return prepared
+    if prepared.slot_count != 1:
+        raise ValueError("the topology is already replicated over slots")
+    topo = prepared.topology
+    k = topo.row_count
+    device = topo.primitive_sequence.device
+    slot = torch.arange(slot_count, device=device, dtype=torch.int32).repeat_interleave(k)
+    source_index = _tile(topo.source_index, slot_count) + slot * source_count
+    sink_index = _tile(topo.sink_index, slot_count) + slot * sink_count
+    replicated = PropagationTopology(
+        source_index=source_index.to(torch.int32), sink_index=sink_index.to(torch.int32),
+        source_id=_tile(topo.source_id, slot_count), sink_id=_tile(topo.sink_id, slot_count),
+        depth=_tile(topo.depth, slot_count), component_id=_tile(topo.component_id, slot_count),
+        primitive_id=_tile(topo.primitive_id, slot_count), edge_id=_tile(topo.edge_id, slot_count),
+        material_id=_tile(topo.material_id, slot_count),
+        primitive_sequence=_tile(topo.primitive_sequence, slot_count),
+        material_sequence=_tile(topo.material_sequence, slot_count),
+        interaction_type=_tile(topo.interaction_type, slot_count), provenance=topo.provenance)
+    offsets = torch.arange(slot_count, device=device, dtype=torch.int64) * k
+    buckets = tuple(FixedTopologyBucket(component=b.component, depth=b.depth,
+                                        rows=(b.rows.unsqueeze(0) + offsets.unsqueeze(1)).reshape(-1))
+                    for b in prepared.buckets)
+    return PreparedFixedTopology(topology=replicated, buckets=buckets, prepare_d2h_copies=0, prepare_d2h_bytes=0,
+                                 prepare_synchronizations=0, slot_count=slot_count, slot_source_count=source_count,
+                                 slot_sink_count=sink_count)
+
+
+def _require_frequency_offsets(offsets: Any) -> tuple[float, ...] | None:
+    if offsets is None:
+        return None
+    if not isinstance(offsets, tuple) or not offsets:
+        raise ValueError("frequency_offsets_hz must be a non-empty tuple of floats or None")
+    values = []
+    for o in offsets:
+        if isinstance(o, bool) or not isinstance(o, (int, float)) or not (abs(float(o)) < float("inf")):
+            raise ValueError("frequency_offsets_hz entries must be finite floats")
+        values.append(float(o))
+    return tuple(values)
+
+
+@dataclass(frozen=True, slots=True, eq=False)
+class FixedTopologyRequest:
+    """A replay request against an already-discovered topology (reference
+    consumer.py:1307-1418). ``topology`` is a raw :class:`PropagationTopology`
+    for the zero-interaction line-of-sight route, or a
+    :class:`PreparedFixedTopology` for any route with interactions."""
+
+    sources: EndpointBatch
+    sinks: EndpointBatch
+    reference_frequency_hz: float | torch.Tensor
+    topology: PropagationTopology | PreparedFixedTopology
+    response: str
+    ad_mode: str
+    world_motion: str = "frozen_world"
+    slot_count: int = 1
+    frequency_offsets_hz: tuple[float, ...] | None = None
+
+    def __post_init__(self) -> None:
+        if not isinstance(self.sources, EndpointBatch) or not isinstance(self.sinks, EndpointBatch):
+            raise TypeError("sources and sinks must be EndpointBatch instances")
+        if type(self.slot_count) is not int or self.slot_count <= 0:
+            raise ValueError("slot_count must be a positive int")
+        if self.slot_count > 1:
+            if isinstance(self.topology, PropagationTopology):
+                raise NotImplementedError("slot_count > 1 requires a PreparedFixedTopology; call "
+                                          "prepare_fixed_topology first")
+            for name, count in (("sources", self.sources.count), ("sinks", self.sinks.count),
+                                ("topology rows", self.frozen_topology.row_count)):
+                if count % self.slot_count:
+                    raise ValueError(f"{name} carries {count} entries, which is not a multiple of "
+                                     f"slot_count={self.slot_count}; every slot must hold the same number")
+        object.__setattr__(self, "frequency_offsets_hz", _require_frequency_offsets(self.frequency_offsets_hz))
+        if not isinstance(self.topology, (PropagationTopology, PreparedFixedTopology)):
+            raise TypeError("topology must be a PropagationTopology or a PreparedFixedTopology")
+        response = _require_vocabulary("response", self.response, RESPONSES)
+        _require_vocabulary("ad_mode", self.ad_mode, AD_MODES)
+        _require_vocabulary("world_motion", self.world_motion, WORLD_MOTIONS)
+        if response not in _CAPABILITIES.fixed_topology_responses:
+            raise NotImplementedError(f"fixed-topology reevaluation does not support {response!r}")
+        if self.frequency_offsets_hz is not None and response not in _CAPABILITIES.wideband_responses:
+            raise NotImplementedError(f"frequency_offsets_hz is not supported for {response!r}; "
+                                      f"capabilities().wideband_responses is "
+                                      f"{sorted(_CAPABILITIES.wideband_responses)}")
+
+    @property
+    def frozen_topology(self) -> PropagationTopology:
+        if isinstance(self.topology, PreparedFixedTopology):
+            return self.topology.topology
+        return self.topology
+
+
+@dataclass(frozen=True, slots=True, eq=False)
+class FixedTopologyEvaluation:
+    """A replayed batch of frozen rows (reference consumer.py:1421-1480).
+    ``row_valid`` (when present) is the sole authority on whether a row still
+    exists; replay is subtractive: paths that appear at the new endpoints are
+    not discovered here."""
+
+    paths: PropagationPathBatch
+    convention: PropagationConvention
+    capabilities: PropagationCapabilities
+    diagnostics: PropagationDiagnostics
+    row_valid: torch.Tensor | None = None
+
+    def __post_init__(self) -> None:
+        if self.row_valid is not None:
+            _require_tensor("row_valid", self.row_valid, dtype=torch.bool, shape=(self.paths.path_count,))
+
+
+def _replay_rows(dscene: Any, topology: PropagationTopology, source: torch.Tensor, target: torch.Tensor,
+                 tx_power: torch.Tensor, tx_pol: torch.Tensor, rx_pol: torch.Tensor,
+                 frequency_hz: float) -> dict[str, torch.Tensor]:
+    """One native launch of wc_fixed_rows_evaluate over every frozen row."""
+    k = topology.row_count
+    width = int(topology.primitive_sequence.shape[1])
+    device = source.device
+    out = {
+        "valid": torch.empty(k, dtype=torch.uint8, device=device),
+        "path_field": torch.empty(k, dtype=torch.complex64, device=device),
+        "field_xyz": torch.empty((k, 3), dtype=torch.complex64, device=device),
+        "path_length": torch.empty(k, dtype=torch.float32, device=device),
+        "delay": torch.empty(k, dtype=torch.float32, device=device),
+        "direction": torch.empty((k, 3), dtype=torch.float32, device=device),
+        "positions": torch.zeros((k, width, 3), dtype=torch.float32, device=device),
+        "normals": torch.zeros((k, width, 3), dtype=torch.float32, device=device),
+    }
+    if k == 0:
+        return out
+    p = _native.ptr
+    component = topology.component_id.to(torch.int32).contiguous()
+    depth = topology.depth.to(torch.int32).contiguous()
+    faces = topology.primitive_sequence.to(torch.int32).contiguous()
+    view = _native.WcFixedRows(
+        count=k, width=width, component=p(component), depth=p(depth), faces=p(faces) if width else None,
+        source=p(source), target=p(target), tx_power=p(tx_power), tx_polarization=p(tx_pol),
+        rx_polarization=p(rx_pol), valid=p(out["valid"]), path_field=p(torch.view_as_real(out["path_field"])),
+        field_xyz=p(torch.view_as_real(out["field_xyz"])), path_length=p(out["path_length"]), delay=p(out["delay"]),
+        direction=p(out["direction"]), positions=p(out["positions"]) if width else None,
+        normals=p(out["normals"]) if width else None)
+    _native.call("wc_fixed_rows_evaluate", dscene.ref(), ctypes.byref(view), ctypes.c_float(frequency_hz),
+                 _native.stream_handle(device))
+    return out
+
+
+def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> FixedTopologyEvaluation:
+    """Replay frozen rows at the request's endpoints without discovery or
+    compaction (reference consumer.py:3965-4110): line-of-sight rows re-test
+    their segment, reflection rows re-solve the image method over their frozen
+    faces' planes and re-check faces and legs; rows that no longer exist are
+    published with ``row_valid = False`` and zero transport."""
+    if not isinstance(compiled_scene, CompiledScene):
+        raise TypeError("reevaluate requires a CompiledScene")
+    if not isinstance(request, FixedTopologyRequest):
+        raise TypeError("request must be a FixedTopologyRequest")
+    compiled = compiled_scene
+    freq = float(request.reference_frequency_hz.detach()) if isinstance(request.reference_frequency_hz,
+                                                                         torch.Tensor) \
+        else float(request.reference_frequency_hz)
+    if abs(freq - compiled.frequency_hz) > 1e-6 * compiled.frequency_hz:
+        raise ValueError(f"reference_frequency_hz {freq} does not match the compiled scene's "
+                         f"{compiled.frequency_hz}")
+    if request.ad_mode not in _CAPABILITIES.ad_modes:
+        raise NotImplementedError(f"ad_mode {request.ad_mode!r} is not supported by this build; supported: "
+                                  f"{sorted(_CAPABILITIES.ad_modes)}")
+    topo = request.frozen_topology
+    width = int(topo.primitive_sequence.shape[1])
+    prepared = request.topology if isinstance(request.topology, PreparedFixedTopology) else None
+    if prepared is None and width != 0:
+        raise NotImplementedError("fixed LoS reevaluation requires zero-width interaction sequences; call "
+                                  "prepare_fixed_topology first to reevaluate a topology that carries interactions")
+    if prepared is not None and request.slot_count != prepared.slot_count:
+        raise ValueError(f"slot_count={request.slot_count} does not match the topology's "
+                         f"{prepared.slot_count} slots (replicate_over_slots)")
+    components = {b.component for b in prepared.buckets} if prepared is not None else {"los"}
+    if request.response == "polarimetric_transport":
+        if request.sources.polarization_basis is None or request.sinks.polarization_basis is None:
+            raise ValueError("polarimetric_transport requires source and sink polarization_basis tensors")
+        if components - {"los"}:
+            raise NotImplementedError("polarimetric_transport replays line-of-sight rows only")
+    if topo.primitive_sequence.device != request.sources.positions_m.device:
+        raise ValueError("fixed topology and endpoint batches must share a device")
+    device = _native.require_gpu()
+    dscene = device_scene(compiled, device)
+    sources, sinks = request.sources, request.sinks
+    k = topo.row_count
+    slots = request.slot_count
+    s_count = sources.count // slots
+    r_count = sinks.count // slots
+    src = topo.source_index.to(torch.int64)
+    snk = topo.sink_index.to(torch.int64)
+    # the one validation read: every index in range and, with slots, inside its row's slot
+    if k:
+        slot_of_row = torch.arange(k, device=src.device) // max(k // slots, 1)
+        bad = (src < 0) | (src >= sources.count) | (snk < 0) | (snk >= sinks.count)
+        if slots > 1:
+            bad = bad | (src // s_count != slot_of_row) | (snk // r_count != slot_of_row)
+        if bool(bad.any().item()):
+            raise ValueError("frozen rows name endpoints outside the request's batches (or outside their slot)")
+    source = sources.positions_m[src].contiguous()
+    target = sinks.positions_m[snk].contiguous()
+    tx_power = sources.powers_w[src].to(torch.float32).contiguous()
+    tx_pol = sources.polarizations[src].to(torch.float32).contiguous()
+    rx_pol = sinks.polarizations[snk].to(torch.float32).contiguous()
+    out = _replay_rows(dscene, topo, source, target, tx_power, tx_pol, rx_pol, compiled.frequency_hz)
+    valid = out["valid"].bool()
+    amplitude = torch.sqrt(torch.clamp(tx_power, min=0.0)).unsqueeze(-1)
+    offsets = request.frequency_offsets_hz
+    if slots > 1:
+        slot = src // s_count
+        pair_index = slot * (s_count * r_count) + (snk - slot * r_count) * s_count + (src - slot * s_count)
+        pair_count = slots * s_count * r_count
+    else:
+        pair_index = snk * sources.count + src
+        pair_count = sources.count * sinks.count
+    counts = torch.bincount(pair_index, minlength=pair_count) if pair_count else \
+        torch.zeros(0, dtype=torch.int64, device=device)
+    pair_offsets = torch.zeros(pair_count + 1, dtype=torch.int64, device=device)
+    if pair_count:
+        pair_offsets[1:] = torch.cumsum(counts, 0)
+    geometry = PropagationGeometry(path_length_m=out["path_length"], delay_s=out["delay"],
+                                   field_direction=out["direction"], interaction_positions_m=out["positions"],
+                                   interaction_normals=out["normals"])
+    columns = []
+    if offsets is not None:
+        for df in offsets:
+            f = compiled.frequency_hz + df
+            if not f > 0.0:
+                raise ValueError(f"frequency offset {df} gives a non-positive frequency")
+            columns.append(_replay_rows(dscene, topo, source, target, tx_power, tx_pol, rx_pol, f))
+    if request.response == "scalar_transport":
+        payload = torch.stack([c["path_field"] for c in columns], dim=1) if columns else None
+        transport: Any = ScalarTransport(coefficient=out["path_field"], coefficient_offsets=payload,
+                                         frequency_offsets_hz=offsets)
+    elif request.response == "complex3_transport":
+        payload = torch.stack([c["field_xyz"] * amplitude for c in columns], dim=1) if columns else None
+        transport = Complex3Transport(field=(out["field_xyz"] * amplitude).contiguous(), direction=out["direction"],
+                                      field_offsets=payload, frequency_offsets_hz=offsets)
+    else:
+        transport = _los_jones(source, target, sources.polarization_basis[src], sinks.polarization_basis[snk],
+                               compiled.frequency_hz)
+        zero = torch.zeros((), dtype=transport.matrix.dtype, device=device)
+        transport = JonesTransport(matrix=torch.where(valid.reshape(-1, 1, 1), transport.matrix, zero),
+                                   source_basis=transport.source_basis, sink_basis=transport.sink_basis)
+    diagnostics = PropagationDiagnostics(
+        discovery_launch_count=0, candidate_count=0, visibility_rejection_count=0, compact_count_d2h_copies=0,
+        compact_count_d2h_bytes=0, compact_sync_count=0, validation_d2h_copies=1 if k else 0,
+        validation_d2h_bytes=1 if k else 0, validation_sync_count=1 if k else 0,
+        frequency_column_count=1 if offsets is None else len(offsets))
+    paths = PropagationPathBatch(pair_count=pair_count, path_count=k, pair_index=pair_index.contiguous(),
+                                 pair_offsets=pair_offsets, topology=topo, geometry=geometry, transport=transport)
+    publish = bool(components & _CAPABILITIES.fixed_topology_row_validity_components)
+    return FixedTopologyEvaluation(paths=paths, convention=PropagationConvention(), capabilities=_CAPABILITIES,
+                                   diagnostics=diagnostics, row_valid=valid if publish else None)
+
+
 __all__ = [
     "AD_MODES", "COMPONENTS", "CONTRACT_VERSION", "Complex3Transport", "EndpointBatch", "JonesTransport",
     "MAX_DEPTH", "PropagationCapabilities", "PropagationConvention", "PropagationDiagnostics",
     "PropagationEvaluation", "PropagationGeometry", "PropagationPathBatch", "PropagationRequest",
     "PropagationTopology", "RESPONSES", "ScalarTransport", "TOPOLOGY_MODES", "capabilities", "evaluate",
-    "transverse_basis",
+    "transverse_basis", "WORLD_MOTIONS", "FixedTopologyBucket", "PreparedFixedTopology", "prepare_fixed_topology",
+    "replicate_over_slots", "FixedTopologyRequest", "FixedTopologyEvaluation", "reevaluate",
 ]
