@@ -378,6 +378,36 @@ __host__ __device__ __forceinline__ size_t shadow_offset(int cells, int lanes, i
     return (o + 7) & ~(size_t)7;
 }
 
+// after the shadow table: the staged diffraction pass (diff_stage_layout):
+// per-cell pair offsets, control words and the pair records
+struct DiffRecord {
+    float pf_re, pf_im, gain;  // path field (x sqrt power) and gain; gain < 0: below the gain floor
+    float f[6];                // unit-excitation field xyz (re, im)
+    int edge;
+};
+static_assert(sizeof(DiffRecord) == 40, "DiffRecord layout");
+constexpr long long kDiffRecordCap = 1LL << 22;  // staged pairs (4 M x 40 B)
+__host__ __device__ __forceinline__ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+__host__ __device__ __forceinline__ long long diff_record_capacity(int cells, int edge_count) {
+    const long long all = (long long)(cells > 0 ? cells : 0) * (long long)(edge_count > 0 ? edge_count : 0);
+    return all < kDiffRecordCap ? all : kDiffRecordCap;
+}
+struct DiffStageLayout {
+    size_t offsets, ctrl, records, end;
+};
+__host__ __device__ inline DiffStageLayout diff_stage_layout(size_t base, int cells, int edge_count) {
+    DiffStageLayout l;
+    l.offsets = align256(base);
+    l.ctrl = align256(l.offsets + ((size_t)cells + 1) * 4);
+    l.records = align256(l.ctrl + 64);
+    l.end = l.records + (size_t)diff_record_capacity(cells, edge_count) * sizeof(DiffRecord);
+    return l;
+}
+__host__ __device__ inline size_t shadow_end(int cells, int lanes, int edge_count, int max_tx_chains, int tx_count) {
+    return shadow_offset(cells, lanes, edge_count, max_tx_chains) +
+           (size_t)(tx_count > 0 ? tx_count : 0) * (size_t)(edge_count > 0 ? edge_count : 0) * 8;
+}
+
 // cell evaluated at launch position `pos` (identity without an order table)
 __device__ __forceinline__ int cell_at(const wc_solve& q, int pos) {
     return q.cell_order ? q.cell_order[pos] : pos;
@@ -563,8 +593,182 @@ radiomap_main_tiles(SceneView s, wc_solve q, wc_radiomap_out o, const int* __res
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_DIFF_WAVES_PER_EU)))
 radiomap_diff_tiles(SceneView s, wc_solve q, wc_radiomap_out o, const int* __restrict__ tiles, int tile_count,
-                    int* __restrict__ counter) {
+                    int* __restrict__ counter, const int* __restrict__ staged) {
+    if (staged && staged[2]) return;  // the staged pass below covered every pair
     radiomap_tiles<kPassDiff>(s, q, o, tiles, tile_count, counter);
+}
+
+// ---- staged diffraction field pass ------------------------------------------
+// With one lane per cell, the field pass lasts as long as the busiest cell
+// (a few hundred visible edges near the transmitter against a mean of ~15),
+// and a tile of such cells cannot be shared between wavefronts without
+// changing the order of its float64 sums. The staged pass separates the two:
+//   1. diff_count_kernel + diff_scan_kernel: visible pairs per cell and their
+//      exclusive prefix over cells (one workgroup);
+//   2. diff_record_kernel: every (cell, visible edge) pair evaluated once,
+//      the pairs dealt in chunks of kDiffChunk over all wavefronts (the same
+//      geometry, field and gain-floor test as the per-cell pass), one
+//      40-byte record per pair;
+//   3. diff_sum_kernel: one lane per cell sums its records in ascending edge
+//      order into the carried float64 totals and appends the descriptors,
+//      exactly as the per-cell pass does, so the outputs are bit-identical.
+// When the pairs exceed the record capacity the control word says so, these
+// kernels return at once and the per-cell pass (radiomap_diff_tiles) runs.
+constexpr int kDiffChunk = 256;
+
+__global__ void __launch_bounds__(256) diff_count_kernel(wc_solve q, const unsigned* __restrict__ masks, int words,
+                                                         int* __restrict__ offsets) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= q.total_cells) return;
+    int n = 0;
+    for (int w = 0; w < words; ++w) n += __popc(masks[(size_t)cell * words + w]);
+    offsets[cell] = n;
+}
+
+// In-place exclusive scan of offsets[0..n) (one workgroup of 1024 threads,
+// contiguous chunks per thread), offsets[n] = total; ctrl[0] = total,
+// ctrl[1] = 0 (chunk counter), ctrl[2] = 1 when the records fit.
+__global__ void __launch_bounds__(1024) diff_scan_kernel(int* __restrict__ offsets, int n, long long capacity,
+                                                         int* __restrict__ ctrl) {
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const long long per = ((long long)n + 1023) / 1024;
+    const long long lo = (long long)t * per, hi = lo + per < n ? lo + per : n;
+    long long sum = 0;
+    for (long long i = lo; i < hi; ++i) sum += offsets[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const long long v = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = part[t] - sum;  // exclusive prefix of this thread's chunk
+    for (long long i = lo; i < hi; ++i) {
+        const int c = offsets[i];
+        offsets[i] = (int)run;
+        run += c;
+    }
+    if (t == 1023) {
+        const long long total = part[1023];
+        const bool fits = total <= capacity && total < 0x7fffffffLL;
+        offsets[n] = fits ? (int)total : 0;
+        ctrl[0] = fits ? (int)total : 0;
+        ctrl[1] = 0;
+        ctrl[2] = fits ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_DIFF_WAVES_PER_EU)))
+diff_record_kernel(SceneView s, wc_solve q, const unsigned* __restrict__ masks, int words,
+                   const int* __restrict__ offsets, int* __restrict__ ctrl, DiffRecord* __restrict__ records) {
+    const int lane = threadIdx.x & 63;
+    if (!ctrl[2]) return;
+    const int total = ctrl[0];
+    const int cells = q.total_cells;
+    const float freq = q.frequency_hz;
+    const double k_d = (double)freq * kTwoPi / kC0;
+    while (true) {
+        int chunk = 0;
+        if (lane == 0) chunk = atomicAdd(ctrl + 1, 1);
+        chunk = __shfl(chunk, 0);
+        const long long base = (long long)chunk * kDiffChunk;
+        if (base >= total) break;
+#pragma unroll 1
+        for (int j = 0; j < kDiffChunk / 64; ++j) {
+            const long long p64 = base + j * 64 + lane;
+            if (p64 >= total) break;
+            const int p = (int)p64;
+            // owner cell: the last cell whose offset is <= p
+            int lo = 0, hi = cells - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (offsets[mid] <= p) lo = mid;
+                else hi = mid - 1;
+            }
+            const int cell = lo;
+            int k = p - offsets[cell];
+            const unsigned* m = masks + (size_t)cell * words;
+            int w = 0;
+            unsigned bits = m[0];
+            while (k >= __popc(bits)) {
+                k -= __popc(bits);
+                bits = m[++w];
+            }
+            for (int i = 0; i < k; ++i) bits &= bits - 1;
+            const int ei = w * 32 + __ffs(bits) - 1;
+            const int t = tx_of_cell(q, cell);
+            const V3<float> tx = ld3(q.tx_pos + 3 * t);
+            const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
+            const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+            const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
+            const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
+            const V3<float> rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
+            const Edge e = s.edges[ei];
+            DiffRecord r;
+            r.edge = ei;
+            r.pf_re = r.pf_im = 0.0f;
+            r.gain = -1.0f;
+            for (int i = 0; i < 6; ++i) r.f[i] = 0.0f;
+            DiffGeometry g;
+            if (diffraction_geometry(e, tx, rx, g)) {
+                V3<float> arrival, departure;
+                const C3<float> v = diffraction_field(s, e, g, tx, rx, tx_pol, freq, k_d, arrival, departure);
+                PathRecord rec;
+                finish_record(rec, v, arrival, rx_pol, sqrt_power);
+                if (rec.gain >= kGainFloor) {
+                    r.pf_re = rec.path_field.re;
+                    r.pf_im = rec.path_field.im;
+                    r.gain = rec.gain;
+                    r.f[0] = v.x.re; r.f[1] = v.x.im;
+                    r.f[2] = v.y.re; r.f[3] = v.y.im;
+                    r.f[4] = v.z.re; r.f[5] = v.z.im;
+                }
+            }
+            records[p] = r;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) diff_sum_kernel(wc_solve q, wc_radiomap_out o, const int* __restrict__ offsets,
+                                                       const int* __restrict__ ctrl,
+                                                       const DiffRecord* __restrict__ records) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= q.total_cells || !ctrl[2]) return;
+    const int t = tx_of_cell(q, cell);
+    AccumSink a;
+    a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    const CellPartial p = reinterpret_cast<const CellPartial*>(o.workspace)[cell];
+    a.s_re = p.s_re;
+    a.s_im = p.s_im;
+    a.p_inc = p.p_inc;
+    a.count = p.count;
+    int cap = 0;
+    if (o.topology) {
+        cap = o.topology_capacity;
+        a.topo = o.topology + (size_t)cell * cap;
+        a.cap = cap;
+    }
+    for (int i = offsets[cell]; i < offsets[cell + 1]; ++i) {
+        const DiffRecord r = records[i];
+        if (!(r.gain >= 0.0f)) continue;
+        PathRecord rec;
+        rec.component = kCidDiff;
+        rec.candidate = r.edge;
+        rec.path_field = {r.pf_re, r.pf_im};
+        rec.gain = r.gain;
+        rec.field.x = {r.f[0], r.f[1]};
+        rec.field.y = {r.f[2], r.f[3]};
+        rec.field.z = {r.f[4], r.f[5]};
+        a.path(rec);
+    }
+    if (o.topology) {
+        if (a.count < cap) a.topo[a.count] = -1;
+        if (a.count > cap && o.failure) atomicOr(o.failure, 2);
+    }
+    write_slot(q, o, cell, 2, a);
+    write_totals(q, o, cell, a);
 }
 
 // Chain validity for every (cell, reflection chain) candidate: bit c - base
@@ -1183,9 +1387,31 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
             hipLaunchKernelGGL(radiomap_main_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter);
         if (stages & kStageEdges) launch_diff_visibility(s, *solve, *out, st);
-        if (diff)
+        if (diff) {
+            const int* staged = nullptr;
+            const int words = mask_words(s.edge_count);
+            const int n = solve->total_cells;
+            if (width == 1 && words > 0) {
+                const DiffStageLayout l = diff_stage_layout(
+                    shadow_end(n, width, s.edge_count, solve->max_tx_chains, solve->tx_count), n, s.edge_count);
+                char* ws = static_cast<char*>(out->workspace);
+                int* offsets = reinterpret_cast<int*>(ws + l.offsets);
+                int* ctrl = reinterpret_cast<int*>(ws + l.ctrl);
+                DiffRecord* records = reinterpret_cast<DiffRecord*>(ws + l.records);
+                const unsigned* masks = reinterpret_cast<const unsigned*>(ws + partial_bytes(n, width));
+                hipLaunchKernelGGL(diff_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, *solve, masks,
+                                   words, offsets);
+                hipLaunchKernelGGL(diff_scan_kernel, dim3(1), dim3(1024), 0, st, offsets, n,
+                                   diff_record_capacity(n, s.edge_count), ctrl);
+                hipLaunchKernelGGL(diff_record_kernel, dim3(cached_cus * 4), dim3(256), 0, st, s, *solve, masks,
+                                   words, offsets, ctrl, records);
+                hipLaunchKernelGGL(diff_sum_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, *solve, *out,
+                                   offsets, ctrl, records);
+                staged = ctrl;
+            }
             hipLaunchKernelGGL(radiomap_diff_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
-                               tile_count, tile_counter + 1);
+                               tile_count, tile_counter + 1, staged);
+        }
     } else {
         if (stages & kStageMain)
             hipLaunchKernelGGL(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
@@ -1219,8 +1445,10 @@ extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int
                                               int tx_count) {
     const int c = cells > 0 ? cells : 0;
     const int e = edge_count > 0 ? edge_count : 0;
-    return shadow_offset(c, lanes_per_cell > 0 ? lanes_per_cell : 1, e, max_tx_chains) +
-           (size_t)(tx_count > 0 ? tx_count : 0) * (size_t)e * 8;
+    const int lanes = lanes_per_cell > 0 ? lanes_per_cell : 1;
+    const size_t base = shadow_end(c, lanes, e, max_tx_chains, tx_count);
+    // the staged diffraction pass runs with one lane per cell only
+    return lanes == 1 && e > 0 ? diff_stage_layout(base, c, e).end : base;
 }
 
 extern "C" int wc_paths_count(const wc_scene* scene, const wc_solve* solve, int* path_count, void* stream) {

@@ -488,7 +488,10 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
 // each, so a tile holding many paths (cells near the transmitter see hundreds
 // of edges) spreads over many SIMDs while the slices of light tiles end at
 // once. Each (tile, slice) owns one partial row.
-constexpr int kVjpSlices = 16;
+#ifndef WC_VJP_SLICES
+#define WC_VJP_SLICES 16
+#endif
+constexpr int kVjpSlices = WC_VJP_SLICES;
 
 // One wavefront per (tile of 64 units, slice) (unit = one lane-segment of a
 // cell's descriptors; 64 / L cells). The tile's paths of this phase are
