@@ -323,6 +323,78 @@ def test_value_returning_entry_points():
     assert _native.call("wc_accumulate_workspace_bytes", 10, 2, 3) > 0
 
 
+def _sah_tables(compiled):
+    from witwin.channel import _native
+
+    f = compiled.face_count
+    nodes = np.zeros(_native.call("wc_bvh_node_count", f) * 64, np.uint8)
+    sorted_tris = np.zeros(f * 48, np.uint8)
+    face_tris = np.zeros(f * 48, np.uint8)
+    arrays = [np.ascontiguousarray(compiled.vertices, np.float32), np.ascontiguousarray(compiled.faces, np.int32),
+              np.ascontiguousarray(compiled.face_group, np.int32), np.ascontiguousarray(compiled.face_material, np.int32)]
+    _native.call("wc_bvh_build_sah", *(a.ctypes.data for a in arrays), f, nodes.ctypes.data, sorted_tris.ctypes.data,
+                 face_tris.ctypes.data)
+    return nodes, sorted_tris.view(np.float32).reshape(f, 12), face_tris.view(np.float32).reshape(f, 12)
+
+
+@pytest.mark.parametrize("which", ["canyon", "heightfield", "one", "two"])
+def test_sah_bvh_tables_are_a_valid_tree(which):
+    """Host SAH tree: every triangle is one leaf, every internal node is
+    reached once from the root, each child box holds its whole subtree, the
+    depth stays below the traversal stack (64), and the triangle rows carry
+    group / face / material exactly as the device LBVH writes them."""
+    if which == "canyon":
+        compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    elif which == "heightfield":
+        compiled = compile_scene(workloads.heightfield_scene(seed=4, size=48, boxes=12, shape=(2, 2)),
+                                 reference_frequency_hz=3.5e9)
+    else:
+        n = 1 if which == "one" else 2
+        verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [3, 0, 1], [4, 0, 1], [3, 1, 1]], np.float32)
+        faces = np.array([[0, 1, 2], [3, 4, 5]], np.int64)[:n]
+        mesh = Mesh(vertices=torch.as_tensor(verts), faces=torch.as_tensor(faces))
+        scene = Scene(structures=[Structure(name="tri", geometry=mesh, material=PhysicalMaterial(eps_r=3.0))])
+        compiled = compile_scene(scene, reference_frequency_hz=28e9)
+    f = compiled.face_count
+    nodes, sorted_tris, face_tris = _sah_tables(compiled)
+    node_f = nodes.view(np.float32).reshape(-1, 16)
+    node_i = nodes.view(np.int32).reshape(-1, 16)
+    tri = compiled.vertices[compiled.faces].astype(np.float32)  # [f, 3, 3]
+    np.testing.assert_array_equal(face_tris.reshape(f, 3, 4)[:, :, :3], tri)
+    np.testing.assert_array_equal(face_tris[:, 3].view(np.int32), compiled.face_group)
+    np.testing.assert_array_equal(face_tris[:, 7].view(np.int32), np.arange(f))
+    np.testing.assert_array_equal(face_tris[:, 11].view(np.int32), compiled.face_material)
+    order = sorted_tris[:, 7].view(np.int32)
+    assert sorted(order.tolist()) == list(range(f))
+    np.testing.assert_array_equal(sorted_tris, face_tris[order])
+    seen_leaf = np.zeros(f, np.int64)
+    seen_node = np.zeros(len(node_i), np.int64)
+    max_depth = 0
+
+    def visit(node, depth):
+        nonlocal max_depth
+        seen_node[node] += 1
+        max_depth = max(max_depth, depth)
+        boxes = [node_f[node, 0:6], node_f[node, 6:12]]
+        leaves = []
+        for side, child in enumerate(node_i[node, 12:14]):
+            if f == 1 and side == 1:
+                assert boxes[1][0] > boxes[1][3]  # the empty second child
+                continue
+            sub = [~child] if child < 0 else visit(child, depth + 1)
+            pts = sorted_tris[sub].reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
+            assert (pts >= boxes[side][:3]).all() and (pts <= boxes[side][3:]).all()
+            leaves += sub
+        return leaves
+
+    leaves = visit(0, 1)
+    for k in leaves:
+        seen_leaf[k] += 1
+    assert (seen_leaf == 1).all()
+    assert (seen_node == 1).all() and len(seen_node) == max(f - 1, 1)
+    assert max_depth < 64
+
+
 def test_native_errors_raise_without_gpu_work():
     from witwin.channel import _native
 

@@ -8,6 +8,8 @@
 #include "wc_scene.h"
 #include "wc_math.h"
 
+#include <cfloat>
+
 namespace wc {
 
 __device__ __forceinline__ int bits_i(float f) { return __float_as_int(f); }
@@ -72,13 +74,24 @@ __device__ __forceinline__ float tri_hit(const RayPre& r, const Tri& tri, float 
     return t;
 }
 
+// Relative widening of the far slab bound. The slab distances and the
+// triangle distance are each rounded, so an unwidened box test can reject a
+// box whose triangle the watertight test hits (a ray grazing the rim of a
+// flat box at a shared edge or corner). Widening the far bound by a few
+// ulps (Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013, uses 1 + 2 gamma_3;
+// 32 ulps also covers the rounding of the triangle distance short of
+// near-parallel hits) lets every walk order reach the same triangles, so
+// any-hit, closest-hit and gathered answers do not depend on the tree.
+constexpr float kSlabWiden = 1.0f + 32.0f * FLT_EPSILON;
+
 __device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                      float tmin, float tmax, float& tnear) {
     const float t0x = (lx - r.ox) * r.ix, t1x = (hx - r.ox) * r.ix;
     const float t0y = (ly - r.oy) * r.iy, t1y = (hy - r.oy) * r.iy;
     const float t0z = (lz - r.oz) * r.iz, t1z = (hz - r.oz) * r.iz;
     const float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
-    const float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    // only the box's own bound is widened: the segment's end bound stays exact
+    const float tf = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kSlabWiden, tmax);
     tnear = tn;
     return tn <= tf;
 }
@@ -137,7 +150,25 @@ struct Hit {
     int group;
 };
 
-// Closest hit in (tmin, tmax), skipping group `ignore`; t < 0 when nothing is hit.
+// Closest hit in (tmin, tmax), skipping group `ignore`; t < 0 when nothing is
+// hit. Equal hit distances (a ray through an edge or corner shared by two
+// faces) resolve to the lower face id, so the answer does not depend on the
+// order in which the tree is walked (the widened box test, with the limit
+// itself widened once a hit exists, reaches every tied triangle).
+__device__ inline void closest_candidate(const RayPre& r, const Tri& tri, float tmin, float tmax, int ignore,
+                                         float& tbest, Hit& best) {
+    const int g = bits_i(tri.v0.w);
+    if (g == ignore) return;
+    const float lim = best.face >= 0 ? nextafterf(tbest, INFINITY) : tbest;
+    const float t = tri_hit(r, tri, tmin, lim);
+    if (t < 0.0f) return;
+    const int f = bits_i(tri.v1.w);
+    if (t < tbest || f < best.face) {
+        tbest = t;
+        best = {t, f, g};
+    }
+}
+
 __device__ inline Hit closest_hit(const SceneView& s, const RayPre& r, float tmin, float tmax, int ignore) {
     Hit best{-1.0f, -1, -1};
     if (s.tri_count == 0) return best;
@@ -148,30 +179,22 @@ __device__ inline Hit closest_hit(const SceneView& s, const RayPre& r, float tmi
     while (true) {
         const Node nd = s.nodes[node];
         float tn0, tn1;
-        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, tbest, tn0);
-        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, tbest, tn1);
+        // once a hit exists, boxes up to a few ulps past it still hold ties
+        const float limit = best.face >= 0 ? tbest * kSlabWiden : tbest;
+        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, limit, tn0);
+        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, limit, tn1);
         int c0 = nd.d.x, c1 = nd.d.y;
         int near_child = -1, far_child = -1;
         if (h0) {
             if (c0 < 0) {
-                const Tri tri = s.tris[~c0];
-                const int g = bits_i(tri.v0.w);
-                if (g != ignore) {
-                    const float t = tri_hit(r, tri, tmin, tbest);
-                    if (t >= 0.0f) { tbest = t; best = {t, bits_i(tri.v1.w), g}; }
-                }
+                closest_candidate(r, s.tris[~c0], tmin, tmax, ignore, tbest, best);
             } else {
                 near_child = c0;
             }
         }
         if (h1) {
             if (c1 < 0) {
-                const Tri tri = s.tris[~c1];
-                const int g = bits_i(tri.v0.w);
-                if (g != ignore) {
-                    const float t = tri_hit(r, tri, tmin, tbest);
-                    if (t >= 0.0f) { tbest = t; best = {t, bits_i(tri.v1.w), g}; }
-                }
+                closest_candidate(r, s.tris[~c1], tmin, tmax, ignore, tbest, best);
             } else {
                 if (near_child < 0) {
                     near_child = c1;
@@ -189,6 +212,12 @@ __device__ inline Hit closest_hit(const SceneView& s, const RayPre& r, float tmi
         node = near_child;
     }
     return best;
+}
+
+// Order of gathered hits: distance, then group id for equal distances (a
+// segment through an edge shared by two walls), independent of walk order.
+__device__ __forceinline__ bool hit_before(const Hit& a, const Hit& b) {
+    return a.t < b.t || (a.t == b.t && a.group < b.group);
 }
 
 // All hits on the segment in (tmin, tmax), ordered by t, at most one per
@@ -224,15 +253,27 @@ __device__ inline int gather_hits(const SceneView& s, const RayPre& r, float tmi
                     for (int i = 0; i < count && i < cap; ++i) {
                         if (hits[i].group == g) {
                             dup = true;
-                            if (f < hits[i].face) { hits[i].face = f; hits[i].t = t; }
+                            if (f < hits[i].face) {
+                                // the group's lowest face and its distance; restore the order
+                                hits[i].face = f;
+                                hits[i].t = t;
+                                int j = i;
+                                while (j > 0 && hit_before(hits[j], hits[j - 1])) {
+                                    const Hit tmp = hits[j]; hits[j] = hits[j - 1]; hits[j - 1] = tmp; --j;
+                                }
+                                while (j + 1 < count && j + 1 < cap && hit_before(hits[j + 1], hits[j])) {
+                                    const Hit tmp = hits[j]; hits[j] = hits[j + 1]; hits[j + 1] = tmp; ++j;
+                                }
+                            }
                         }
                     }
                     if (!dup) {
                         if (count < cap) {
-                            // insertion by t
+                            // insertion by (t, group)
+                            const Hit h{t, f, g};
                             int j = count;
-                            while (j > 0 && hits[j - 1].t > t) { hits[j] = hits[j - 1]; --j; }
-                            hits[j] = {t, f, g};
+                            while (j > 0 && hit_before(h, hits[j - 1])) { hits[j] = hits[j - 1]; --j; }
+                            hits[j] = h;
                         }
                         ++count;
                         if (count > cap) return cap + 1;

@@ -496,13 +496,26 @@ def vertex_csr(vertex_count: int, faces: np.ndarray, group_face: np.ndarray,
 
 # ---------------------------------------------------------------- device scene
 
-class DeviceScene:
-    """HBM copy of a compiled scene plus its LBVH, and the wc_scene view."""
+BVH_BUILDERS = ("sah", "lbvh")
 
-    def __init__(self, compiled: CompiledScene, device: torch.device) -> None:
+
+class DeviceScene:
+    """HBM copy of a compiled scene plus its BVH, and the wc_scene view.
+
+    ``bvh="sah"`` (default) builds a binned surface-area-heuristic tree on the
+    host (``wc_bvh_build_sah``): fewer nodes per segment walk for every
+    visibility test. ``bvh="lbvh"`` builds the Morton-order tree on the
+    device (``wc_bvh_build``), for geometry that is rebuilt every step. Both
+    trees answer every visibility and hit query identically; they differ only
+    in how many nodes a walk visits."""
+
+    def __init__(self, compiled: CompiledScene, device: torch.device, bvh: str = "sah") -> None:
         from witwin.channel import _native
 
+        if bvh not in BVH_BUILDERS:
+            raise ValueError(f"bvh must be one of {BVH_BUILDERS}, got {bvh!r}")
         self.device = device
+        self.bvh = bvh
         t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a)).to(device=device, dtype=dt).contiguous()  # noqa: E731
         mats = compiled.materials
         self.vertices = t(compiled.vertices, torch.float32)
@@ -536,7 +549,19 @@ class DeviceScene:
         self.nodes = torch.zeros(self.node_count * 64, dtype=torch.uint8, device=device)
         self.sorted_tris = torch.zeros(max(f, 1) * 48, dtype=torch.uint8, device=device)
         self.face_tris = torch.zeros(max(f, 1) * 48, dtype=torch.uint8, device=device)
-        if f:
+        if f and bvh == "sah":
+            host = lambda a, dt: np.ascontiguousarray(a, dtype=dt)  # noqa: E731
+            verts, tri_faces = host(compiled.vertices, np.float32), host(compiled.faces, np.int32)
+            groups, materials = host(compiled.face_group, np.int32), host(compiled.face_material, np.int32)
+            nodes = np.zeros(self.node_count * 64, np.uint8)
+            sorted_tris = np.zeros(f * 48, np.uint8)
+            face_tris = np.zeros(f * 48, np.uint8)
+            _native.call("wc_bvh_build_sah", verts.ctypes.data, tri_faces.ctypes.data, groups.ctypes.data,
+                         materials.ctypes.data, f, nodes.ctypes.data, sorted_tris.ctypes.data, face_tris.ctypes.data)
+            self.nodes.copy_(torch.from_numpy(nodes))
+            self.sorted_tris.copy_(torch.from_numpy(sorted_tris))
+            self.face_tris.copy_(torch.from_numpy(face_tris))
+        elif f:
             ws_bytes = int(_native.call("wc_bvh_workspace_bytes", f))
             workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
             _native.call("wc_bvh_build", _native.ptr(self.vertices), _native.ptr(self.faces),
@@ -571,7 +596,9 @@ class DeviceScene:
         return self._csr
 
 
-def device_scene(compiled: CompiledScene, device: torch.device) -> DeviceScene:
-    if compiled.device is None or compiled.device.device != device:
-        compiled.device = DeviceScene(compiled, device)
+def device_scene(compiled: CompiledScene, device: torch.device, bvh: str = "sah") -> DeviceScene:
+    """The compiled scene's device copy (cached on the scene per device and
+    BVH builder)."""
+    if compiled.device is None or compiled.device.device != device or compiled.device.bvh != bvh:
+        compiled.device = DeviceScene(compiled, device, bvh=bvh)
     return compiled.device
