@@ -38,7 +38,8 @@ const char* wc_build_target(void);
  * Replaces the RayD scene creation behind _channel.rayd_scene_create
  * (native/channel/binding/rayd.cpp:367, native/channel/rayd/scene.cpp:19-57):
  * builds a linear BVH (Morton codes, LDS radix sort, Karras hierarchy,
- * bottom-up boxes) over the triangles.
+ * bottom-up boxes) over the triangles. The output is the BINARY tree
+ * (64 B nodes); the scene's nodes are its 4-wide collapse (wc_bvh_widen).
  *   nodes_out:        wc_bvh_node_count(face_count) x 64 B
  *   sorted_tris_out:  face_count x 48 B (BVH leaf order)
  *   face_tris_out:    face_count x 48 B (original face order)
@@ -51,12 +52,24 @@ int wc_bvh_build(const float* vertices, const int* faces, const int* face_group,
                  size_t workspace_bytes, void* stream);
 /* The same scene creation (rayd.cpp:367, scene.cpp:19-57) with a binned
  * surface-area-heuristic tree instead: 16 centroid bins per axis, one
- * triangle per leaf, depth below the traversal stack. HOST pointers in and
- * out, same layouts as wc_bvh_build (upload the three tables afterwards).
+ * triangle per leaf, collapsed to the 4-wide nodes the tracers walk. Host
+ * pointers in and out (upload the three tables afterwards).
+ *   nodes_out_host:   wc_bvh_node_count(face_count) x 128 B (4-wide nodes;
+ *                     slots past the tree's node count stay untouched)
+ *   sorted_tris_out_host / face_tris_out_host: as for wc_bvh_build
  * Every visibility and hit query answers as with the LBVH; a walk visits
  * about a third fewer nodes on building and terrain meshes. */
-int wc_bvh_build_sah(const float* vertices, const int* faces, const int* face_group, const int* face_material,
-                     int face_count, void* nodes_out, void* sorted_tris_out, void* face_tris_out);
+int wc_bvh_build_sah(const float* vertices_host, const int* faces_host, const int* face_group_host,
+                     const int* face_material_host, int face_count, void* nodes_out_host,
+                     void* sorted_tris_out_host, void* face_tris_out_host);
+/* Collapse of a binary tree (wc_bvh_build's nodes, copied to the host) into
+ * the 4-wide nodes of the scene view: each wide node takes a binary node's
+ * children and opens the internal child of largest surface area until it
+ * holds four (half the dependent node fetches per walk). Fails, rather than
+ * let a walk overflow, when no collapse fits the 96-entry walk stack. Host
+ * pointers; wide_nodes_out_host holds wc_bvh_node_count(face_count) x 128 B;
+ * wide_count_host (optional) receives the wide node count. */
+int wc_bvh_widen(const void* binary_nodes_host, int face_count, void* wide_nodes_out_host, int* wide_count_host);
 
 /* Scene view consumed by the path kernels. Group planes are float64 (unit
  * normal xyz, offset). Material arrays hold the first-layer scalar view;

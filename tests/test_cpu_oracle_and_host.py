@@ -327,7 +327,7 @@ def _sah_tables(compiled):
     from witwin.channel import _native
 
     f = compiled.face_count
-    nodes = np.zeros(_native.call("wc_bvh_node_count", f) * 64, np.uint8)
+    nodes = np.zeros(_native.call("wc_bvh_node_count", f) * 128, np.uint8)
     sorted_tris = np.zeros(f * 48, np.uint8)
     face_tris = np.zeros(f * 48, np.uint8)
     arrays = [np.ascontiguousarray(compiled.vertices, np.float32), np.ascontiguousarray(compiled.faces, np.int32),
@@ -337,28 +337,41 @@ def _sah_tables(compiled):
     return nodes, sorted_tris.view(np.float32).reshape(f, 12), face_tris.view(np.float32).reshape(f, 12)
 
 
+def _wide_children(nodes, node):
+    """(lo [k,3], hi [k,3], child [k]) of a 128-byte wide node."""
+    fl = nodes.view(np.float32).reshape(-1, 32)[node]
+    it = nodes.view(np.int32).reshape(-1, 32)[node]
+    k = int(it[28])
+    lo = np.stack([fl[0:4], fl[4:8], fl[8:12]], 1)[:k]
+    hi = np.stack([fl[12:16], fl[16:20], fl[20:24]], 1)[:k]
+    return lo, hi, it[24:24 + k]
+
+
+def _build_case(which):
+    if which == "canyon":
+        return compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    if which == "heightfield":
+        return compile_scene(workloads.heightfield_scene(seed=4, size=48, boxes=12, shape=(2, 2)),
+                             reference_frequency_hz=3.5e9)
+    n = 1 if which == "one" else 2
+    verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [3, 0, 1], [4, 0, 1], [3, 1, 1]], np.float32)
+    faces = np.array([[0, 1, 2], [3, 4, 5]], np.int64)[:n]
+    mesh = Mesh(vertices=torch.as_tensor(verts), faces=torch.as_tensor(faces))
+    scene = Scene(structures=[Structure(name="tri", geometry=mesh, material=PhysicalMaterial(eps_r=3.0))])
+    return compile_scene(scene, reference_frequency_hz=28e9)
+
+
 @pytest.mark.parametrize("which", ["canyon", "heightfield", "one", "two"])
 def test_sah_bvh_tables_are_a_valid_tree(which):
-    """Host SAH tree: every triangle is one leaf, every internal node is
-    reached once from the root, each child box holds its whole subtree, the
-    depth stays below the traversal stack (64), and the triangle rows carry
-    group / face / material exactly as the device LBVH writes them."""
-    if which == "canyon":
-        compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
-    elif which == "heightfield":
-        compiled = compile_scene(workloads.heightfield_scene(seed=4, size=48, boxes=12, shape=(2, 2)),
-                                 reference_frequency_hz=3.5e9)
-    else:
-        n = 1 if which == "one" else 2
-        verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [3, 0, 1], [4, 0, 1], [3, 1, 1]], np.float32)
-        faces = np.array([[0, 1, 2], [3, 4, 5]], np.int64)[:n]
-        mesh = Mesh(vertices=torch.as_tensor(verts), faces=torch.as_tensor(faces))
-        scene = Scene(structures=[Structure(name="tri", geometry=mesh, material=PhysicalMaterial(eps_r=3.0))])
-        compiled = compile_scene(scene, reference_frequency_hz=28e9)
+    """Host SAH tree, collapsed to 4-wide nodes: every triangle is one leaf,
+    every wide node is reached once from the root and holds 1..4 children
+    (2..4 unless the scene has one triangle), each child box holds its whole
+    subtree, the walk's worst-case stack (the unvisited siblings of a path)
+    fits the 96-entry stack, and the triangle rows carry group / face /
+    material exactly as the device LBVH writes them."""
+    compiled = _build_case(which)
     f = compiled.face_count
     nodes, sorted_tris, face_tris = _sah_tables(compiled)
-    node_f = nodes.view(np.float32).reshape(-1, 16)
-    node_i = nodes.view(np.int32).reshape(-1, 16)
     tri = compiled.vertices[compiled.faces].astype(np.float32)  # [f, 3, 3]
     np.testing.assert_array_equal(face_tris.reshape(f, 3, 4)[:, :, :3], tri)
     np.testing.assert_array_equal(face_tris[:, 3].view(np.int32), compiled.face_group)
@@ -368,31 +381,76 @@ def test_sah_bvh_tables_are_a_valid_tree(which):
     assert sorted(order.tolist()) == list(range(f))
     np.testing.assert_array_equal(sorted_tris, face_tris[order])
     seen_leaf = np.zeros(f, np.int64)
-    seen_node = np.zeros(len(node_i), np.int64)
-    max_depth = 0
+    seen_node = {}
+    worst_stack = 0
 
-    def visit(node, depth):
-        nonlocal max_depth
-        seen_node[node] += 1
-        max_depth = max(max_depth, depth)
-        boxes = [node_f[node, 0:6], node_f[node, 6:12]]
+    def visit(node, held):
+        nonlocal worst_stack
+        seen_node[node] = seen_node.get(node, 0) + 1
+        lo, hi, child = _wide_children(nodes, node)
+        assert 1 <= len(child) <= 4 and (len(child) >= 2 or f == 1)
+        internal = int((child >= 0).sum())
+        held = held + max(internal - 1, 0)
+        worst_stack = max(worst_stack, held)
         leaves = []
-        for side, child in enumerate(node_i[node, 12:14]):
-            if f == 1 and side == 1:
-                assert boxes[1][0] > boxes[1][3]  # the empty second child
-                continue
-            sub = [~child] if child < 0 else visit(child, depth + 1)
+        for k, c in enumerate(child):
+            sub = [~c] if c < 0 else visit(int(c), held)
             pts = sorted_tris[sub].reshape(-1, 3, 4)[:, :, :3].reshape(-1, 3)
-            assert (pts >= boxes[side][:3]).all() and (pts <= boxes[side][3:]).all()
+            assert (pts >= lo[k]).all() and (pts <= hi[k]).all()
             leaves += sub
         return leaves
 
-    leaves = visit(0, 1)
-    for k in leaves:
+    for k in visit(0, 0):
         seen_leaf[k] += 1
     assert (seen_leaf == 1).all()
-    assert (seen_node == 1).all() and len(seen_node) == max(f - 1, 1)
-    assert max_depth < 64
+    assert all(v == 1 for v in seen_node.values())
+    assert len(seen_node) <= max(f - 1, 1)
+    assert worst_stack <= 96
+
+
+def _slab_np(o, inv, lo, hi, tmin, tmax):
+    t0, t1 = (lo - o) * inv, (hi - o) * inv
+    tn = max(np.minimum(t0, t1).max(), tmin)
+    tf = min(np.maximum(t0, t1).min() * (1.0 + 32.0 * np.finfo(np.float32).eps), tmax)
+    return tn <= tf
+
+
+def test_wide_walk_reaches_every_crossed_triangle():
+    """The any-hit walk over the wide tree (the traversal order of the
+    tracers, in numpy) reaches every triangle a segment crosses: for random
+    segments through the canyon, the set of triangles whose Moller-Trumbore
+    test hits equals the set reached through boxes the segment crosses."""
+    compiled = _build_case("canyon")
+    nodes, sorted_tris, _ = _sah_tables(compiled)
+    tri = sorted_tris.reshape(-1, 3, 4)[:, :, :3].astype(np.float64)
+    v0, e1, e2 = tri[:, 0], tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        a = np.array([rng.uniform(-10, 200), rng.uniform(-10, 200), rng.uniform(0.5, 45)])
+        b = np.array([rng.uniform(-10, 200), rng.uniform(-10, 200), rng.uniform(0.5, 45)])
+        d = b - a
+        with np.errstate(divide="ignore"):
+            inv = 1.0 / d
+        p = np.cross(d, e2)
+        det = (e1 * p).sum(1)
+        ok = np.abs(det) > 1e-12
+        inv_det = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)
+        tv = a - v0
+        u = (tv * p).sum(1) * inv_det
+        q = np.cross(tv, e1)
+        v = (d * q).sum(1) * inv_det
+        t = (e2 * q).sum(1) * inv_det
+        crossed = set(np.nonzero(ok & (u >= 1e-6) & (v >= 1e-6) & (u + v <= 1 - 1e-6) & (t > 1e-4) & (t < 1 - 1e-4))[0])
+        reached, stack = set(), [0]
+        while stack:
+            lo, hi, child = _wide_children(nodes, stack.pop())
+            for k, c in enumerate(child):
+                if _slab_np(a, inv, lo[k], hi[k], 0.0, 1.0):
+                    if c < 0:
+                        reached.add(int(~c))
+                    else:
+                        stack.append(int(c))
+        assert crossed <= reached
 
 
 def test_native_errors_raise_without_gpu_work():

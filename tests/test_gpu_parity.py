@@ -124,10 +124,16 @@ def test_reflection_sequence_known_answer_and_oracle(device):
 
 # ----------------------------------------------------------------- BVH
 
-def test_bvh_visibility_matches_brute_force(device):
-    _say("case: LBVH visibility, urban canyon seed 2, 4096 random segments")
+@pytest.mark.parametrize("bvh", ["sah", "lbvh"])
+def test_bvh_visibility_matches_brute_force(device, bvh):
+    """Segment visibility through the 4-wide walk, over the host SAH tree and
+    over the collapsed device LBVH, against Moller-Trumbore on every triangle."""
+    _say(f"case: {bvh} visibility, urban canyon seed 2, 4096 random segments")
+    from witwin.channel.scene.compiler import device_scene
+
     scene = workloads.urban_canyon_scene(seed=2, shape=(2, 2))
     run = run_gpu(scene, {"los"}, 0, 28e9, device)
+    run.dscene = device_scene(run.compiled, device, bvh=bvh)
     rng = np.random.default_rng(7)
     n = 4096
     a = rng.uniform(-10, 390, (n, 3)).astype(np.float32)

@@ -1,14 +1,22 @@
-// Surface-area-heuristic BVH, built on the host for static scenes.
+// Host-side BVH work: the surface-area-heuristic build for static scenes and
+// the collapse of any binary tree into the 4-wide tree the tracers walk.
 //
-// Top-down binned SAH (16 centroid bins per axis, one triangle per leaf) with
-// the same output layout as the device LBVH: node 0 is the root, each node
-// holds both child boxes inline, a negative child ~k names triangle k of the
-// leaf-ordered table. Against the LBVH it cuts the nodes an occlusion walk
-// visits by about a third on building and terrain meshes (long wall and
-// roof triangles spoil Morton order), which every segment test of the
-// radiomap pays. Equal split costs keep the first candidate of a fixed scan
-// order and partitions are stable, so the tree is a pure function of the
-// input arrays.
+// SAH build: top-down binned SAH (16 centroid bins per axis, one triangle per
+// leaf) into the binary layout of the device LBVH (node 0 the root, both
+// child boxes inline, a negative child ~k names triangle k of the
+// leaf-ordered table). Against the LBVH it cuts the nodes an occlusion walk
+// visits by about a third on building and terrain meshes (long wall and roof
+// triangles spoil Morton order). Equal split costs keep the first candidate of
+// a fixed scan order and partitions are stable, so the tree is a pure
+// function of the input arrays.
+//
+// Collapse: every wide node takes a binary node's two children and keeps
+// opening the internal child with the largest surface area until it holds
+// four, which halves the dependent node fetches of a walk. The walk keeps the
+// unvisited siblings of its path on a stack; the collapse computes the
+// tree's exact worst case and, should it exceed the tracers' stack, redoes
+// the collapse opening every internal child once (depth halves, so a binary
+// tree of depth <= 60 needs at most 90 entries).
 #include "wc_api_internal.h"
 #include "wc_scene.h"
 
@@ -178,14 +186,161 @@ void write_child(Node& nd, int which, const HostBox& b, int child) {
     }
 }
 
+struct ChildRef {
+    int child;
+    HostBox box;
+};
+
+inline HostBox node_child_box(const Node& nd, int which) {
+    HostBox b;
+    if (which == 0) {
+        b.lo[0] = nd.a.x; b.lo[1] = nd.a.y; b.lo[2] = nd.a.z;
+        b.hi[0] = nd.a.w; b.hi[1] = nd.b.x; b.hi[2] = nd.b.y;
+    } else {
+        b.lo[0] = nd.b.z; b.lo[1] = nd.b.w; b.lo[2] = nd.c.x;
+        b.hi[0] = nd.c.y; b.hi[1] = nd.c.z; b.hi[2] = nd.c.w;
+    }
+    return b;
+}
+
+struct Widener {
+    const Node* bin;
+    int bin_count;
+    int leaf_count;
+    bool greedy;
+    std::vector<WideNode> out;
+    int stack_need = 0;
+    bool bad = false;
+
+    void open(std::vector<ChildRef>& kids, size_t i) {
+        const int c = kids[i].child;
+        if (c < 0 || c >= bin_count) {
+            bad = true;
+            return;
+        }
+        kids[i] = {bin[c].d.x, node_child_box(bin[c], 0)};
+        kids.push_back({bin[c].d.y, node_child_box(bin[c], 1)});
+    }
+
+    // Emits the wide node for binary node `node`; `above` counts the stack
+    // entries the walk may hold for the ancestors.
+    int emit(int node, int above, int depth) {
+        if (node < 0 || node >= bin_count || depth > bin_count) {
+            bad = true;
+            return 0;
+        }
+        std::vector<ChildRef> kids = {{bin[node].d.x, node_child_box(bin[node], 0)},
+                                      {bin[node].d.y, node_child_box(bin[node], 1)}};
+        if (greedy) {
+            while (kids.size() < 4 && !bad) {
+                int best = -1;
+                double best_area = -1.0;
+                for (size_t i = 0; i < kids.size(); ++i) {
+                    if (kids[i].child < 0) continue;
+                    const double a = half_area(kids[i].box);
+                    if (a > best_area) {
+                        best_area = a;
+                        best = (int)i;
+                    }
+                }
+                if (best < 0) break;
+                open(kids, (size_t)best);
+            }
+        } else {
+            const size_t n0 = kids.size();
+            for (size_t i = 0; i < n0 && !bad; ++i)
+                if (kids[i].child >= 0) open(kids, i);
+        }
+        if (bad) return 0;
+        const int me = (int)out.size();
+        out.push_back(WideNode{});
+        int internal = 0;
+        for (const ChildRef& k : kids) internal += k.child >= 0;
+        const int held = above + (internal > 1 ? internal - 1 : 0);
+        if (held > stack_need) stack_need = held;
+        float lo[3][4] = {}, hi[3][4] = {};
+        int child[4] = {0, 0, 0, 0};
+        for (size_t k = 0; k < kids.size(); ++k) {
+            for (int a = 0; a < 3; ++a) {
+                lo[a][k] = kids[k].box.lo[a];
+                hi[a][k] = kids[k].box.hi[a];
+            }
+            const int c = kids[k].child;
+            if (c < 0 && ~c >= leaf_count) bad = true;
+            child[k] = c < 0 ? c : emit(c, held, depth + 1);
+            if (bad) return 0;
+        }
+        WideNode& w = out[me];  // out may have grown during the recursion
+        w.lo_x = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+        w.lo_y = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+        w.lo_z = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+        w.hi_x = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+        w.hi_y = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+        w.hi_z = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        w.child = make_int4(child[0], child[1], child[2], child[3]);
+        w.meta = make_int4((int)kids.size(), 0, 0, 0);
+        return me;
+    }
+};
+
+// Collapses a binary tree of `face_count` leaves into wide nodes; returns the
+// wide node count or -1 (malformed tree, or no collapse fits the stack).
+int widen(const Node* bin, int face_count, std::vector<WideNode>& out) {
+    const int bin_count = face_count <= 1 ? 1 : face_count - 1;
+    if (face_count == 1) {
+        // one triangle: the binary root's second child is an empty placeholder
+        WideNode w{};
+        const HostBox b = node_child_box(bin[0], 0);
+        w.lo_x.x = b.lo[0]; w.lo_y.x = b.lo[1]; w.lo_z.x = b.lo[2];
+        w.hi_x.x = b.hi[0]; w.hi_y.x = b.hi[1]; w.hi_z.x = b.hi[2];
+        if (bin[0].d.x != ~0) return -1;
+        w.child = make_int4(~0, 0, 0, 0);
+        w.meta = make_int4(1, 0, 0, 0);
+        out.assign(1, w);
+        return 1;
+    }
+    for (int pass = 0; pass < 2; ++pass) {
+        Widener w{bin, bin_count, face_count, pass == 0};
+        w.out.reserve(bin_count);
+        w.emit(0, 0, 0);
+        if (w.bad) return -1;
+        if (w.stack_need <= kWideStackSize) {
+            out.swap(w.out);
+            return (int)out.size();
+        }
+    }
+    return -1;
+}
+
 }  // namespace
 }  // namespace wc
 
 using namespace wc;
 
-extern "C" int wc_bvh_build_sah(const float* vertices, const int* faces, const int* face_group,
-                                const int* face_material, int face_count, void* nodes_out, void* sorted_tris_out,
-                                void* face_tris_out) {
+extern "C" int wc_bvh_widen(const void* binary_nodes_host, int face_count, void* wide_nodes_out_host,
+                            int* wide_count_host) {
+    if (face_count <= 0) return wc_fail("wc_bvh_widen: face_count must be positive");
+    if (!binary_nodes_host || !wide_nodes_out_host) return wc_fail("wc_bvh_widen: null pointer argument");
+    std::vector<WideNode> wide;
+    const int count = widen(static_cast<const Node*>(binary_nodes_host), face_count, wide);
+    if (count < 0)
+        return wc_fail("wc_bvh_widen: the binary tree is malformed or too deep for a %d-entry walk stack",
+                       kWideStackSize);
+    std::memcpy(wide_nodes_out_host, wide.data(), sizeof(WideNode) * (size_t)count);
+    if (wide_count_host) *wide_count_host = count;
+    return 0;
+}
+
+extern "C" int wc_bvh_build_sah(const float* vertices_host, const int* faces_host, const int* face_group_host,
+                                const int* face_material_host, int face_count, void* nodes_out_host,
+                                void* sorted_tris_out_host, void* face_tris_out_host) {
+    const float* vertices = vertices_host;
+    const int* faces = faces_host;
+    const int* face_group = face_group_host;
+    const int* face_material = face_material_host;
+    void* nodes_out = nodes_out_host;
+    void* sorted_tris_out = sorted_tris_out_host;
+    void* face_tris_out = face_tris_out_host;
     if (face_count < 0) return wc_fail("wc_bvh_build_sah: face_count must be non-negative");
     if (face_count == 0) return 0;
     if (!vertices || !faces || !face_group || !face_material || !nodes_out || !sorted_tris_out || !face_tris_out)
@@ -193,7 +348,8 @@ extern "C" int wc_bvh_build_sah(const float* vertices, const int* faces, const i
     const int n = face_count;
     Tri* sorted = static_cast<Tri*>(sorted_tris_out);
     Tri* by_face = static_cast<Tri*>(face_tris_out);
-    Node* nodes = static_cast<Node*>(nodes_out);
+    std::vector<Node> bin_nodes(n <= 1 ? 1 : n - 1);
+    Node* nodes = bin_nodes.data();
     std::vector<HostBox> box(n);
     std::vector<float> cen(3 * (size_t)n);
     for (int f = 0; f < n; ++f) {
@@ -224,7 +380,7 @@ extern "C" int wc_bvh_build_sah(const float* vertices, const int* faces, const i
         nd.c = make_float4(FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX);
         nd.d = make_int4(~0, ~0, 0, 0);
         nodes[0] = nd;
-        return 0;
+        return wc_bvh_widen(nodes, 1, nodes_out, nullptr);
     }
     std::vector<int> idx(n);
     for (int i = 0; i < n; ++i) idx[i] = i;
@@ -257,5 +413,5 @@ extern "C" int wc_bvh_build_sah(const float* vertices, const int* faces, const i
     }
     if (next_node != n - 1) return wc_fail("wc_bvh_build_sah: internal node count %d != %d", next_node, n - 1);
     for (int i = 0; i < n; ++i) sorted[i] = by_face[idx[i]];
-    return 0;
+    return wc_bvh_widen(nodes, n, nodes_out, nullptr);
 }

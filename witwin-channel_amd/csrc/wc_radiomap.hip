@@ -931,7 +931,7 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
 // hide the traversal's memory latency; the field pass then evaluates the
 // visible edges only.
 #ifndef WC_VIS_WAVES_PER_EU
-#define WC_VIS_WAVES_PER_EU 1
+#define WC_VIS_WAVES_PER_EU 4
 #endif
 
 // Transmitter-leg shadow of one edge cast by one triangle: the edge

@@ -10,7 +10,8 @@ namespace wc {
 
 constexpr int kMaxDepth = 5;          // public reflection/transmission depth cap
 constexpr int kMaxHits = 8;           // ordered wall hits gathered on a straight segment
-constexpr int kStackSize = 64;        // BVH traversal stack (LBVH depth for <= 2^32 leaves with margin)
+constexpr int kStackSize = 64;        // depth bound of the binary trees the builders produce
+constexpr int kWideStackSize = 96;    // traversal stack of the 4-wide tree (checked at build time)
 
 // Triangle in BVH (Morton) order for intersection: three float4 rows,
 // v0.w = group id, v1.w = original face id, v2.w = material id (bit casts).
@@ -42,9 +43,20 @@ struct Edge {
     int material0, material1;
 };
 
+// 4-wide node the tracers walk (128 B, two cache lines): child k's box in
+// component k of the six bound vectors. child >= 0: wide node index;
+// child < 0: leaf holding sorted triangle ~child. Slots k >= meta.x are unused.
+// Built on the host from a binary tree (wc_bvh_widen).
+struct WideNode {
+    float4 lo_x, lo_y, lo_z;
+    float4 hi_x, hi_y, hi_z;
+    int4 child;
+    int4 meta;  // x = child count (1..4)
+};
+
 struct SceneView {
-    // BVH over triangles in sorted order
-    const Node* nodes;
+    // 4-wide BVH over triangles in sorted order
+    const WideNode* nodes;
     const Tri* tris;
     int tri_count;
     int node_count;

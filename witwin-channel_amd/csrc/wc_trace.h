@@ -96,52 +96,79 @@ __device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float 
     return tn <= tf;
 }
 
-// Any hit on the segment o + t*d, t in (tmin, tmax), skipping triangles of
-// groups ig0/ig1 (the surfaces the segment starts or ends on).
-__device__ inline bool occluded(const SceneView& s, const RayPre& r, float tmin, float tmax, int ig0, int ig1) {
-    if (s.tri_count == 0) return false;
-    int stack[kStackSize];
+__device__ __forceinline__ float lane4(const float4& v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+__device__ __forceinline__ int lane4(const int4& v, int k) {
+    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+}
+
+// Walks the 4-wide tree over the segment (tmin, limit), handing every leaf
+// whose box it crosses to `leaf` (which returns true to end the walk). Of the
+// internal children a node's segment crosses, the nearest is entered next and
+// the others wait on the stack, so a blocker close to the segment's start ends
+// an any-hit walk early. `limit` is read at every box test: a closest-hit
+// leaf may shrink it. The stack never overflows: the host build checks the
+// tree's worst-case stack need against kWideStackSize.
+template <class Leaf>
+__device__ __forceinline__ void walk_wide(const SceneView& s, const RayPre& r, float tmin, const float& limit,
+                                          Leaf&& leaf) {
+    int stack[kWideStackSize];
     int sp = 0;
     int node = 0;
     while (true) {
-        const Node nd = s.nodes[node];
-        float tn0, tn1;
-        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, tmax, tn0);
-        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, tmax, tn1);
+        const WideNode nd = s.nodes[node];
         int next = -1;
-        int c0 = nd.d.x, c1 = nd.d.y;
-        if (h0) {
-            if (c0 < 0) {
-                const Tri tri = s.tris[~c0];
-                const int g = bits_i(tri.v0.w);
-                if (g != ig0 && g != ig1 && tri_hit(r, tri, tmin, tmax) >= 0.0f) return true;
-            } else {
-                next = c0;
-            }
-        }
-        if (h1) {
-            if (c1 < 0) {
-                const Tri tri = s.tris[~c1];
-                const int g = bits_i(tri.v0.w);
-                if (g != ig0 && g != ig1 && tri_hit(r, tri, tmin, tmax) >= 0.0f) return true;
-            } else {
-                if (next >= 0) {
-                    // nearer box first: a blocker close to the segment's start ends the walk sooner
-                    int far_child = c1;
-                    if (tn1 < tn0) { far_child = next; next = c1; }
-                    if (sp < kStackSize) stack[sp++] = far_child;
+        float tnext = 0.0f;
+        unsigned leaves = 0;  // slots of crossed leaves, tested after the box tests
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float tn;
+            if (k < nd.meta.x && slab(r, lane4(nd.lo_x, k), lane4(nd.lo_y, k), lane4(nd.lo_z, k), lane4(nd.hi_x, k),
+                                      lane4(nd.hi_y, k), lane4(nd.hi_z, k), tmin, limit, tn)) {
+                const int c = lane4(nd.child, k);
+                if (c < 0) {
+                    leaves |= 1u << k;
+                } else if (next < 0) {
+                    next = c;
+                    tnext = tn;
                 } else {
-                    next = c1;
+                    int wait = c;
+                    if (tn < tnext) {
+                        wait = next;
+                        next = c;
+                        tnext = tn;
+                    }
+                    if (sp < kWideStackSize) stack[sp++] = wait;
                 }
             }
         }
+        // one triangle per lane per round: lanes whose leaves sit in different
+        // slots share the rounds instead of taking turns slot by slot
+        while (leaves) {
+            const int k = __ffs(leaves) - 1;
+            leaves &= leaves - 1;
+            if (leaf(s.tris[~lane4(nd.child, k)])) return;
+        }
         if (next < 0) {
-            if (sp == 0) break;
+            if (sp == 0) return;
             next = stack[--sp];
         }
         node = next;
     }
-    return false;
+}
+
+// Any hit on the segment o + t*d, t in (tmin, tmax), skipping triangles of
+// groups ig0/ig1 (the surfaces the segment starts or ends on).
+__device__ inline bool occluded(const SceneView& s, const RayPre& r, float tmin, float tmax, int ig0, int ig1) {
+    if (s.tri_count == 0) return false;
+    bool blocked = false;
+    walk_wide(s, r, tmin, tmax, [&](const Tri& tri) {
+        const int g = bits_i(tri.v0.w);
+        blocked = g != ig0 && g != ig1 && tri_hit(r, tri, tmin, tmax) >= 0.0f;
+        return blocked;
+    });
+    return blocked;
 }
 
 struct Hit {
@@ -155,8 +182,8 @@ struct Hit {
 // faces) resolve to the lower face id, so the answer does not depend on the
 // order in which the tree is walked (the widened box test, with the limit
 // itself widened once a hit exists, reaches every tied triangle).
-__device__ inline void closest_candidate(const RayPre& r, const Tri& tri, float tmin, float tmax, int ignore,
-                                         float& tbest, Hit& best) {
+__device__ inline void closest_candidate(const RayPre& r, const Tri& tri, float tmin, int ignore, float& tbest,
+                                         Hit& best) {
     const int g = bits_i(tri.v0.w);
     if (g == ignore) return;
     const float lim = best.face >= 0 ? nextafterf(tbest, INFINITY) : tbest;
@@ -172,45 +199,14 @@ __device__ inline void closest_candidate(const RayPre& r, const Tri& tri, float 
 __device__ inline Hit closest_hit(const SceneView& s, const RayPre& r, float tmin, float tmax, int ignore) {
     Hit best{-1.0f, -1, -1};
     if (s.tri_count == 0) return best;
-    int stack[kStackSize];
-    int sp = 0;
-    int node = 0;
     float tbest = tmax;
-    while (true) {
-        const Node nd = s.nodes[node];
-        float tn0, tn1;
+    float limit = tmax;
+    walk_wide(s, r, tmin, limit, [&](const Tri& tri) {
+        closest_candidate(r, tri, tmin, ignore, tbest, best);
         // once a hit exists, boxes up to a few ulps past it still hold ties
-        const float limit = best.face >= 0 ? tbest * kSlabWiden : tbest;
-        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, limit, tn0);
-        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, limit, tn1);
-        int c0 = nd.d.x, c1 = nd.d.y;
-        int near_child = -1, far_child = -1;
-        if (h0) {
-            if (c0 < 0) {
-                closest_candidate(r, s.tris[~c0], tmin, tmax, ignore, tbest, best);
-            } else {
-                near_child = c0;
-            }
-        }
-        if (h1) {
-            if (c1 < 0) {
-                closest_candidate(r, s.tris[~c1], tmin, tmax, ignore, tbest, best);
-            } else {
-                if (near_child < 0) {
-                    near_child = c1;
-                } else {
-                    // visit the nearer box first
-                    if (tn1 < tn0) { far_child = near_child; near_child = c1; } else { far_child = c1; }
-                }
-            }
-        }
-        if (far_child >= 0 && sp < kStackSize) stack[sp++] = far_child;
-        if (near_child < 0) {
-            if (sp == 0) break;
-            near_child = stack[--sp];
-        }
-        node = near_child;
-    }
+        limit = best.face >= 0 ? tbest * kSlabWiden : tbest;
+        return false;
+    });
     return best;
 }
 
@@ -222,82 +218,46 @@ __device__ __forceinline__ bool hit_before(const Hit& a, const Hit& b) {
 
 // All hits on the segment in (tmin, tmax), ordered by t, at most one per
 // coplanar group (a segment crossing a wall through the shared diagonal of
-// its two triangles counts once). Returns the number of distinct hits, or
-// cap + 1 when there are more than `cap`.
+// its two triangles counts once, with the group's lowest face id). Returns
+// the number of distinct hits, or cap + 1 when there are more than `cap`.
 __device__ inline int gather_hits(const SceneView& s, const RayPre& r, float tmin, float tmax, int cap,
                                   Hit* hits) {
     int count = 0;
     if (s.tri_count == 0) return 0;
-    int stack[kStackSize];
-    int sp = 0;
-    int node = 0;
-    while (true) {
-        const Node nd = s.nodes[node];
-        float tn0, tn1;
-        const bool h0 = slab(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.b.x, nd.b.y, tmin, tmax, tn0);
-        const bool h1 = slab(r, nd.b.z, nd.b.w, nd.c.x, nd.c.y, nd.c.z, nd.c.w, tmin, tmax, tn1);
-        int next = -1;
-        const int cs[2] = {nd.d.x, nd.d.y};
-        const bool hs[2] = {h0, h1};
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if (!hs[k]) continue;
-            const int c = cs[k];
-            if (c < 0) {
-                const Tri tri = s.tris[~c];
-                const float t = tri_hit(r, tri, tmin, tmax);
-                if (t >= 0.0f) {
-                    const int g = bits_i(tri.v0.w);
-                    const int f = bits_i(tri.v1.w);
-                    bool dup = false;
-                    for (int i = 0; i < count && i < cap; ++i) {
-                        if (hits[i].group == g) {
-                            dup = true;
-                            if (f < hits[i].face) {
-                                // the group's lowest face and its distance; restore the order
-                                hits[i].face = f;
-                                hits[i].t = t;
-                                int j = i;
-                                while (j > 0 && hit_before(hits[j], hits[j - 1])) {
-                                    const Hit tmp = hits[j]; hits[j] = hits[j - 1]; hits[j - 1] = tmp; --j;
-                                }
-                                while (j + 1 < count && j + 1 < cap && hit_before(hits[j + 1], hits[j])) {
-                                    const Hit tmp = hits[j]; hits[j] = hits[j + 1]; hits[j + 1] = tmp; ++j;
-                                }
-                            }
-                        }
-                    }
-                    if (!dup) {
-                        if (count < cap) {
-                            // insertion by (t, group)
-                            const Hit h{t, f, g};
-                            int j = count;
-                            while (j > 0 && hit_before(h, hits[j - 1])) { hits[j] = hits[j - 1]; --j; }
-                            hits[j] = h;
-                        }
-                        ++count;
-                        if (count > cap) return cap + 1;
-                    }
+    walk_wide(s, r, tmin, tmax, [&](const Tri& tri) {
+        const float t = tri_hit(r, tri, tmin, tmax);
+        if (t < 0.0f) return false;
+        const int g = bits_i(tri.v0.w);
+        const int f = bits_i(tri.v1.w);
+        bool dup = false;
+        for (int i = 0; i < count && i < cap; ++i) {
+            if (hits[i].group != g) continue;
+            dup = true;
+            if (f < hits[i].face) {
+                // the group's lowest face and its distance; restore the order
+                hits[i].face = f;
+                hits[i].t = t;
+                int j = i;
+                while (j > 0 && hit_before(hits[j], hits[j - 1])) {
+                    const Hit tmp = hits[j]; hits[j] = hits[j - 1]; hits[j - 1] = tmp; --j;
                 }
-            } else {
-                if (next < 0) next = c;
-                else if (sp < kStackSize) stack[sp++] = c;
+                while (j + 1 < count && j + 1 < cap && hit_before(hits[j + 1], hits[j])) {
+                    const Hit tmp = hits[j]; hits[j] = hits[j + 1]; hits[j + 1] = tmp; ++j;
+                }
             }
         }
-        if (next < 0) {
-            if (sp == 0) break;
-            next = stack[--sp];
+        if (dup) return false;
+        if (count < cap) {
+            // insertion by (t, group)
+            const Hit h{t, f, g};
+            int j = count;
+            while (j > 0 && hit_before(h, hits[j - 1])) { hits[j] = hits[j - 1]; --j; }
+            hits[j] = h;
         }
-        node = next;
-    }
-    // keep ascending order after face-id replacements
-    for (int i = 1; i < count; ++i) {
-        Hit h = hits[i];
-        int j = i;
-        while (j > 0 && hits[j - 1].t > h.t) { hits[j] = hits[j - 1]; --j; }
-        hits[j] = h;
-    }
-    return count;
+        ++count;
+        return count > cap;
+    });
+    return count > cap ? cap + 1 : count;
 }
 
 }  // namespace wc

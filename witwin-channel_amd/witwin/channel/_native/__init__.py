@@ -105,6 +105,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_build_target": (c_char_p, []),
     "wc_bvh_workspace_bytes": (c_size_t, [c_int]),
     "wc_bvh_node_count": (c_int, [c_int]),
+    "wc_bvh_widen": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "wc_bvh_build_sah": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "wc_bvh_build": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p,
                              c_void_p, c_size_t, c_void_p]),

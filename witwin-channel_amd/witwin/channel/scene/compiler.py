@@ -497,6 +497,8 @@ def vertex_csr(vertex_count: int, faces: np.ndarray, group_face: np.ndarray,
 # ---------------------------------------------------------------- device scene
 
 BVH_BUILDERS = ("sah", "lbvh")
+BINARY_NODE_BYTES = 64   # wc_bvh_build / SAH build layout
+WIDE_NODE_BYTES = 128    # the 4-wide nodes the tracers walk (wc_bvh_widen)
 
 
 class DeviceScene:
@@ -505,9 +507,10 @@ class DeviceScene:
     ``bvh="sah"`` (default) builds a binned surface-area-heuristic tree on the
     host (``wc_bvh_build_sah``): fewer nodes per segment walk for every
     visibility test. ``bvh="lbvh"`` builds the Morton-order tree on the
-    device (``wc_bvh_build``), for geometry that is rebuilt every step. Both
-    trees answer every visibility and hit query identically; they differ only
-    in how many nodes a walk visits."""
+    device (``wc_bvh_build``), for geometry that is rebuilt every step. Either
+    binary tree is collapsed on the host into the 4-wide nodes the tracers
+    walk. Both trees answer every visibility and hit query identically; they
+    differ only in how many nodes a walk visits."""
 
     def __init__(self, compiled: CompiledScene, device: torch.device, bvh: str = "sah") -> None:
         from witwin.channel import _native
@@ -546,14 +549,14 @@ class DeviceScene:
         self.edges = torch.as_tensor(edge_bytes.copy()).to(device)
         f = compiled.face_count
         self.node_count = int(_native.call("wc_bvh_node_count", f))
-        self.nodes = torch.zeros(self.node_count * 64, dtype=torch.uint8, device=device)
+        self.nodes = torch.zeros(self.node_count * WIDE_NODE_BYTES, dtype=torch.uint8, device=device)
         self.sorted_tris = torch.zeros(max(f, 1) * 48, dtype=torch.uint8, device=device)
         self.face_tris = torch.zeros(max(f, 1) * 48, dtype=torch.uint8, device=device)
         if f and bvh == "sah":
             host = lambda a, dt: np.ascontiguousarray(a, dtype=dt)  # noqa: E731
             verts, tri_faces = host(compiled.vertices, np.float32), host(compiled.faces, np.int32)
             groups, materials = host(compiled.face_group, np.int32), host(compiled.face_material, np.int32)
-            nodes = np.zeros(self.node_count * 64, np.uint8)
+            nodes = np.zeros(self.node_count * WIDE_NODE_BYTES, np.uint8)
             sorted_tris = np.zeros(f * 48, np.uint8)
             face_tris = np.zeros(f * 48, np.uint8)
             _native.call("wc_bvh_build_sah", verts.ctypes.data, tri_faces.ctypes.data, groups.ctypes.data,
@@ -564,12 +567,16 @@ class DeviceScene:
         elif f:
             ws_bytes = int(_native.call("wc_bvh_workspace_bytes", f))
             workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+            binary = torch.zeros(self.node_count * BINARY_NODE_BYTES, dtype=torch.uint8, device=device)
             _native.call("wc_bvh_build", _native.ptr(self.vertices), _native.ptr(self.faces),
                          _native.ptr(self.face_group), _native.ptr(self.face_material), f,
-                         _native.ptr(self.nodes), _native.ptr(self.sorted_tris), _native.ptr(self.face_tris),
+                         _native.ptr(binary), _native.ptr(self.sorted_tris), _native.ptr(self.face_tris),
                          _native.ptr(workspace), ws_bytes, _native.stream_handle(device))
-            torch.cuda.current_stream(device).synchronize()
-            del workspace
+            binary_host = binary.cpu().numpy()  # synchronizes with the build
+            del workspace, binary
+            wide = np.zeros(self.node_count * WIDE_NODE_BYTES, np.uint8)
+            _native.call("wc_bvh_widen", binary_host.ctypes.data, f, wide.ctypes.data, None)
+            self.nodes.copy_(torch.from_numpy(wide))
         p = _native.ptr
         self.view = _native.WcScene(
             nodes=p(self.nodes), tris=p(self.sorted_tris), tri_count=f, node_count=self.node_count,
