@@ -1040,34 +1040,95 @@ __host__ __device__ inline unsigned long long triangle_shadow_bins(const ShadowF
     return (count >= 64 ? ~0ull : ((1ull << count) - 1ull)) << first;
 }
 
+// Float pre-test of triangle_shadow_bins. Every leg from T to a point of the
+// edge lies in the fan plane, inside the triangle (T, A, A + L d); a blocking
+// triangle must cut the plane inside it. The section of a triangle with the
+// plane lies within the convex hull of its vertices' in-plane coordinates
+// (x along the edge, y towards T), so the test is true when the three
+// vertices lie strictly on one side of the plane, or their (x, y) lie
+// strictly outside one side of (T, A, B), by more than the frame's shrink
+// margin (>= 1 mm). Float distances of scene coordinates below 10 km differ
+// from the double ones by well under 1 mm, so the double test would find no
+// crossing inside (T, A, B) and return 0 as well.
+struct FanPlane {
+    float ax, ay, az;  // edge origin
+    float mx, my, mz;  // plane normal
+    float dx, dy, dz;  // along the edge
+    float wx, wy, wz;  // in-plane, towards T
+    float a, rho, len, tol;
+    float at_x, at_y;  // unit outward normal of side A-T in (x, y)
+    float bt_x, bt_y;  // unit outward normal of side B-T
+};
+
+__host__ __device__ inline FanPlane fan_plane(const ShadowFrame& f) {
+    FanPlane p;
+    p.ax = (float)f.A.x; p.ay = (float)f.A.y; p.az = (float)f.A.z;
+    p.mx = (float)f.m.x; p.my = (float)f.m.y; p.mz = (float)f.m.z;
+    p.dx = (float)f.d.x; p.dy = (float)f.d.y; p.dz = (float)f.d.z;
+    p.wx = (float)f.w.x; p.wy = (float)f.w.y; p.wz = (float)f.w.z;
+    p.a = (float)f.a; p.rho = (float)f.rho; p.len = (float)f.L; p.tol = (float)f.shrink;
+    // side A(0,0)-T(a,rho): outward (away from B(L,0)) normal; side B-T likewise
+    const double la = sqrt(f.a * f.a + f.rho * f.rho), lb = sqrt((f.a - f.L) * (f.a - f.L) + f.rho * f.rho);
+    p.at_x = (float)(-f.rho / la); p.at_y = (float)(f.a / la);
+    p.bt_x = (float)(f.rho / lb); p.bt_y = (float)((f.L - f.a) / lb);
+    return p;
+}
+
+__host__ __device__ inline bool fan_plane_misses(const FanPlane& p, const Tri& tri) {
+    const float vx[3] = {tri.v0.x - p.ax, tri.v1.x - p.ax, tri.v2.x - p.ax};
+    const float vy[3] = {tri.v0.y - p.ay, tri.v1.y - p.ay, tri.v2.y - p.ay};
+    const float vz[3] = {tri.v0.z - p.az, tri.v1.z - p.az, tri.v2.z - p.az};
+    bool above = true, below = true, behind = true, out_a = true, out_b = true;
+    for (int k = 0; k < 3; ++k) {
+        const float sd = p.mx * vx[k] + p.my * vy[k] + p.mz * vz[k];
+        const float x = p.dx * vx[k] + p.dy * vy[k] + p.dz * vz[k];
+        const float y = p.wx * vx[k] + p.wy * vy[k] + p.wz * vz[k];
+        above = above && sd > p.tol;
+        below = below && sd < -p.tol;
+        behind = behind && y < -p.tol;
+        out_a = out_a && p.at_x * x + p.at_y * y > p.tol;
+        out_b = out_b && p.bt_x * (x - p.len) + p.bt_y * y > p.tol;
+    }
+    return above || below || behind || out_a || out_b;
+}
+
 // Shadow bins of every (transmitter, edge) over all triangles outside the
-// edge's own face groups (which the leg test ignores). One wavefront per
-// (transmitter, edge); lanes stride over the triangles.
-__global__ void __launch_bounds__(256) edge_shadow_kernel(SceneView s, wc_solve q,
+// edge's own face groups (which the leg test ignores). A wavefront serves
+// 64 / P edges of one transmitter with P lanes each: lane P j + p takes edge
+// j of the group and triangles p, p + P, ..., so a triangle fetch serves
+// 64 / P edges (P = 8 when there are many pairs, 64 when few, to fill the
+// chip). Only the triangles that reach an edge's fan plane take the
+// double-precision section test; the P lanes of an edge then OR their bins.
+__global__ void __launch_bounds__(256) edge_shadow_kernel(SceneView s, wc_solve q, int lanes_per_edge,
                                                           unsigned long long* __restrict__ shadow) {
     const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int edges = s.edge_count;
-    if (wave >= (long long)q.tx_count * edges) return;
+    const int per_wave = 64 / lanes_per_edge;
+    const int groups = (edges + per_wave - 1) / per_wave;
+    if (wave >= (long long)q.tx_count * groups) return;
     const int lane = threadIdx.x & 63;
-    const int t = (int)(wave / edges);
-    const int ei = (int)(wave - (long long)t * edges);
-    const Edge e = s.edges[ei];
-    const ShadowFrame f = shadow_frame(e, ld3(q.tx_pos + 3 * t));
+    const int t = (int)(wave / groups);
+    const int ei = (int)(wave - (long long)t * groups) * per_wave + lane / lanes_per_edge;
+    const int phase = lane & (lanes_per_edge - 1);
     unsigned long long bits = 0;
-    if (f.valid) {
-        for (int k = lane; k < s.face_count; k += 64) {
-            const Tri tri = s.faces[k];
-            const int group = __float_as_int(tri.v0.w);
-            if (group == e.group0 || group == e.group1) continue;
-            bits |= triangle_shadow_bins(f, tri);
+    if (ei < edges) {
+        const Edge e = s.edges[ei];
+        const ShadowFrame f = shadow_frame(e, ld3(q.tx_pos + 3 * t));
+        if (f.valid) {
+            const FanPlane plane = fan_plane(f);
+            for (int k = phase; k < s.face_count; k += lanes_per_edge) {
+                const Tri tri = s.faces[k];
+                const int group = __float_as_int(tri.v0.w);
+                if (group == e.group0 || group == e.group1 || fan_plane_misses(plane, tri)) continue;
+                bits |= triangle_shadow_bins(f, tri);
+            }
         }
     }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
+    for (int off = 1; off < lanes_per_edge; off <<= 1) {
         const unsigned lo32 = __shfl_xor((unsigned)bits, off), hi32 = __shfl_xor((unsigned)(bits >> 32), off);
         bits |= ((unsigned long long)hi32 << 32) | lo32;
     }
-    if (lane == 0) shadow[wave] = bits;
+    if (ei < edges && phase == 0) shadow[(size_t)t * edges + ei] = bits;
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VIS_WAVES_PER_EU)))
@@ -1311,6 +1372,22 @@ static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const
                        tile_count, masks, words);
 }
 
+// Compute-unit count of the current device, queried once per device and
+// thread (safe during stream capture).
+static int device_cus() {
+    static thread_local int cached_device = -1;
+    static thread_local int cached_cus = 256;
+    int device = 0;
+    (void)hipGetDevice(&device);
+    if (device != cached_device) {
+        int cus_query = 256;
+        if (hipDeviceGetAttribute(&cus_query, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
+            cached_cus = cus_query;
+        cached_device = device;
+    }
+    return cached_cus;
+}
+
 // The shadow table costs transmitters x edges x triangles segment-plane
 // tests; above this many it is skipped (the visibility test alone is exact).
 constexpr long long kShadowBudget = 4000000000LL;
@@ -1326,7 +1403,13 @@ static void launch_diff_visibility(const SceneView& s, const wc_solve& q, const 
         shadow = reinterpret_cast<unsigned long long*>(
             static_cast<char*>(o.workspace) + shadow_offset(q.total_cells, q.lanes_per_cell, s.edge_count,
                                                             q.max_tx_chains));
-        hipLaunchKernelGGL(edge_shadow_kernel, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, st, s, q, shadow);
+        // 64 lanes per edge: on the urban canyon with 64 transmitters, 8 lanes
+        // per edge (fewer triangle fetches) measured slower
+        const int lanes_per_edge = 64;
+        const int per_wave = 64 / lanes_per_edge;
+        const long long shadow_waves = (long long)q.tx_count * ((s.edge_count + per_wave - 1) / per_wave);
+        hipLaunchKernelGGL(edge_shadow_kernel, dim3((unsigned)((shadow_waves + 3) / 4)), dim3(256), 0, st, s, q,
+                           lanes_per_edge, shadow);
     }
     const long long waves = ((long long)q.total_cells + 63) / 64 * words;
     hipLaunchKernelGGL(diff_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q, masks, words,
@@ -1363,17 +1446,7 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
         return wc_fail("%s: workspace (wc_radiomap_workspace_bytes) is required with diffraction or reflection", who);
     if (tiles) {
         if (!tile_counter || tile_count <= 0) return wc_fail("tiled launch needs tile_counter and tile_count");
-        // compute-unit count of the current device, queried once (safe during stream capture)
-        static thread_local int cached_device = -1;
-        static thread_local int cached_cus = 256;
-        int device = 0;
-        (void)hipGetDevice(&device);
-        if (device != cached_device) {
-            int cus_query = 256;
-            if (hipDeviceGetAttribute(&cus_query, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
-                cached_cus = cus_query;
-            cached_device = device;
-        }
+        const int cached_cus = device_cus();
         int blocks = std::min((tile_count + 3) / 4, cached_cus * 8);
         blocks = std::max(blocks, 1);
         // a fill kernel rather than a memset node keeps the launches graph-capturable
