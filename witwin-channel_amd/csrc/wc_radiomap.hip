@@ -625,36 +625,53 @@ __global__ void __launch_bounds__(256) diff_count_kernel(wc_solve q, const unsig
     offsets[cell] = n;
 }
 
-// In-place exclusive scan of offsets[0..n) (one workgroup of 1024 threads,
-// contiguous chunks per thread), offsets[n] = total; ctrl[0] = total,
+// In-place exclusive scan of offsets[0..n) by one workgroup of 1024 threads,
+// 4096 consecutive entries per pass (4 per thread, coalesced): each thread's
+// 4, a scan inside each wavefront, a scan of the 16 wavefront totals in LDS,
+// and a running carry across passes. offsets[n] = total; ctrl[0] = total,
 // ctrl[1] = 0 (chunk counter), ctrl[2] = 1 when the records fit.
 __global__ void __launch_bounds__(1024) diff_scan_kernel(int* __restrict__ offsets, int n, long long capacity,
                                                          int* __restrict__ ctrl) {
-    __shared__ long long part[1024];
-    const int t = threadIdx.x;
-    const long long per = ((long long)n + 1023) / 1024;
-    const long long lo = (long long)t * per, hi = lo + per < n ? lo + per : n;
-    long long sum = 0;
-    for (long long i = lo; i < hi; ++i) sum += offsets[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const long long v = t >= off ? part[t - off] : 0;
+    __shared__ int wave_total[16];
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    long long carry = 0;  // the same in every thread
+    for (int base = 0; base < n; base += 4096) {
+        const int i0 = base + 4 * t;
+        int v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? offsets[i0 + k] : 0;
+        const int mine = v[0] + v[1] + v[2] + v[3];
+        int x = mine;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wave_total[wid] = x;
         __syncthreads();
-        part[t] += v;
+        if (wid == 0) {
+            int w = lane < 16 ? wave_total[lane] : 0;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const int y = __shfl_up(w, off);
+                if (lane >= off) w += y;
+            }
+            if (lane < 16) wave_total[lane] = w;
+        }
         __syncthreads();
+        long long run = carry + (wid > 0 ? wave_total[wid - 1] : 0) + x - mine;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < n) offsets[i0 + k] = (int)run;
+            run += v[k];
+        }
+        carry += wave_total[15];
+        __syncthreads();  // wave_total is rewritten by the next pass
     }
-    long long run = part[t] - sum;  // exclusive prefix of this thread's chunk
-    for (long long i = lo; i < hi; ++i) {
-        const int c = offsets[i];
-        offsets[i] = (int)run;
-        run += c;
-    }
-    if (t == 1023) {
-        const long long total = part[1023];
-        const bool fits = total <= capacity && total < 0x7fffffffLL;
-        offsets[n] = fits ? (int)total : 0;
-        ctrl[0] = fits ? (int)total : 0;
+    if (t == 0) {
+        const bool fits = carry <= capacity && carry < 0x7fffffffLL;
+        offsets[n] = fits ? (int)carry : 0;
+        ctrl[0] = fits ? (int)carry : 0;
         ctrl[1] = 0;
         ctrl[2] = fits ? 1 : 0;
     }
