@@ -54,7 +54,7 @@ int wc_bvh_build(const float* vertices, const int* faces, const int* face_group,
  * surface-area-heuristic tree instead: 16 centroid bins per axis, one
  * triangle per leaf, collapsed to the 4-wide nodes the tracers walk. Host
  * pointers in and out (upload the three tables afterwards).
- *   nodes_out_host:   wc_bvh_node_count(face_count) x 128 B (4-wide nodes;
+ *   nodes_out_host:   wc_bvh_node_count(face_count) x 64 B (4-wide nodes;
  *                     slots past the tree's node count stay untouched)
  *   sorted_tris_out_host / face_tris_out_host: as for wc_bvh_build
  * Every visibility and hit query answers as with the LBVH; a walk visits
@@ -65,9 +65,11 @@ int wc_bvh_build_sah(const float* vertices_host, const int* faces_host, const in
 /* Collapse of a binary tree (wc_bvh_build's nodes, copied to the host) into
  * the 4-wide nodes of the scene view: each wide node takes a binary node's
  * children and opens the internal child of largest surface area until it
- * holds four (half the dependent node fetches per walk). Fails, rather than
+ * holds four (half the dependent node fetches per walk), its child boxes as
+ * 8-bit steps of a per-axis grid chosen so every decoded bound is an exact
+ * float (64 B per node, one cache line). Fails, rather than
  * let a walk overflow, when no collapse fits the 96-entry walk stack. Host
- * pointers; wide_nodes_out_host holds wc_bvh_node_count(face_count) x 128 B;
+ * pointers; wide_nodes_out_host holds wc_bvh_node_count(face_count) x 64 B;
  * wide_count_host (optional) receives the wide node count. */
 int wc_bvh_widen(const void* binary_nodes_host, int face_count, void* wide_nodes_out_host, int* wide_count_host);
 

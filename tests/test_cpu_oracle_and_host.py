@@ -327,7 +327,7 @@ def _sah_tables(compiled):
     from witwin.channel import _native
 
     f = compiled.face_count
-    nodes = np.zeros(_native.call("wc_bvh_node_count", f) * 128, np.uint8)
+    nodes = np.zeros(_native.call("wc_bvh_node_count", f) * 64, np.uint8)
     sorted_tris = np.zeros(f * 48, np.uint8)
     face_tris = np.zeros(f * 48, np.uint8)
     arrays = [np.ascontiguousarray(compiled.vertices, np.float32), np.ascontiguousarray(compiled.faces, np.int32),
@@ -338,13 +338,24 @@ def _sah_tables(compiled):
 
 
 def _wide_children(nodes, node):
-    """(lo [k,3], hi [k,3], child [k]) of a 128-byte wide node."""
-    fl = nodes.view(np.float32).reshape(-1, 32)[node]
-    it = nodes.view(np.int32).reshape(-1, 32)[node]
-    k = int(it[28])
-    lo = np.stack([fl[0:4], fl[4:8], fl[8:12]], 1)[:k]
-    hi = np.stack([fl[12:16], fl[16:20], fl[20:24]], 1)[:k]
-    return lo, hi, it[24:24 + k]
+    """(lo [k,3], hi [k,3], child [k]) of a 64-byte wide node, decoded from
+    its 8-bit grid steps exactly as the tracers decode them."""
+    rec = nodes[node * 64:(node + 1) * 64]
+    fl = rec.view(np.float32)
+    u = rec.view(np.uint32)
+    it = rec.view(np.int32)
+    exps = int(u[3])
+    k = exps >> 24
+    step = np.array([np.ldexp(1.0, ((exps >> (8 * a)) & 255) - 127) for a in range(3)], np.float32)
+    origin = fl[0:3]
+    q_lo = np.array([[(int(u[4 + a]) >> (8 * j)) & 255 for a in range(3)] for j in range(k)], np.float32)
+    q_hi = np.array([[(int(u[7 + a]) >> (8 * j)) & 255 for a in range(3)] for j in range(k)], np.float32)
+    lo = (q_lo * step + origin).astype(np.float32)
+    hi = (q_hi * step + origin).astype(np.float32)
+    # the decode is exact: the float64 value equals the float32 one
+    np.testing.assert_array_equal(lo.astype(np.float64), q_lo.astype(np.float64) * step + origin.astype(np.float64))
+    np.testing.assert_array_equal(hi.astype(np.float64), q_hi.astype(np.float64) * step + origin.astype(np.float64))
+    return lo, hi, it[12:12 + k]
 
 
 def _build_case(which):
@@ -363,7 +374,8 @@ def _build_case(which):
 
 @pytest.mark.parametrize("which", ["canyon", "heightfield", "one", "two"])
 def test_sah_bvh_tables_are_a_valid_tree(which):
-    """Host SAH tree, collapsed to 4-wide nodes: every triangle is one leaf,
+    """Host SAH tree, collapsed to 64-byte 4-wide nodes with quantized child
+    boxes (decoded exactly): every triangle is one leaf,
     every wide node is reached once from the root and holds 1..4 children
     (2..4 unless the scene has one triangle), each child box holds its whole
     subtree, the walk's worst-case stack (the unvisited siblings of a path)

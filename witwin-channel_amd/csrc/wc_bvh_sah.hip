@@ -12,7 +12,8 @@
 //
 // Collapse: every wide node takes a binary node's two children and keeps
 // opening the internal child with the largest surface area until it holds
-// four, which halves the dependent node fetches of a walk. The walk keeps the
+// four, which halves the dependent node fetches of a walk; its child boxes are
+// stored as 8-bit steps of an exact per-axis grid (64 B per node). The walk keeps the
 // unvisited siblings of its path on a stack; the collapse computes the
 // tree's exact worst case and, should it exceed the tracers' stack, redoes
 // the collapse opening every internal child once (depth halves, so a binary
@@ -21,6 +22,7 @@
 #include "wc_scene.h"
 
 #include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -203,6 +205,64 @@ inline HostBox node_child_box(const Node& nd, int which) {
     return b;
 }
 
+// Grid of one axis: the spacing 2^p and an origin that is a multiple of it,
+// with p large enough that the node's extent fits 255 steps and that every
+// multiple of 2^p up to the origin plus 255 steps is a float exactly (at most
+// 24 significant bits). Returns false when no exponent in range works.
+bool axis_grid(double lo, double hi, int& p_out, double& origin_out) {
+    const double extent = hi - lo;
+    int p = extent > 0.0 ? (int)std::ceil(std::log2(extent / 255.0)) : -126;
+    for (; p <= 127; ++p) {
+        if (p < -126) continue;
+        const double step = std::ldexp(1.0, p);
+        const double origin = std::floor(lo / step) * step;
+        const double top = origin + 255.0 * step;
+        if (top < hi) continue;
+        const double mag = std::fmax(std::fabs(origin), std::fabs(top));
+        if (mag >= std::ldexp(1.0, p + 24)) continue;  // not exact in float
+        p_out = p;
+        origin_out = origin;
+        return true;
+    }
+    return false;
+}
+
+// Quantized boxes of a wide node's children (always containing the true
+// boxes, decoded exactly); false if an axis has no usable grid.
+bool quantize(const std::vector<ChildRef>& kids, WideNode& w) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (const ChildRef& k : kids)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::fmin(lo[a], (double)k.box.lo[a]);
+            hi[a] = std::fmax(hi[a], (double)k.box.hi[a]);
+        }
+    double origin[3], step[3];
+    unsigned exps = (unsigned)kids.size() << 24;
+    for (int a = 0; a < 3; ++a) {
+        int p;
+        if (!axis_grid(lo[a], hi[a], p, origin[a])) return false;
+        step[a] = std::ldexp(1.0, p);
+        exps |= (unsigned)(p + 127) << (8 * a);
+    }
+    unsigned qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+    for (size_t k = 0; k < kids.size(); ++k)
+        for (int a = 0; a < 3; ++a) {
+            const double l = std::floor(((double)kids[k].box.lo[a] - origin[a]) / step[a]);
+            const double h = std::ceil(((double)kids[k].box.hi[a] - origin[a]) / step[a]);
+            if (l < 0.0 || h > 255.0 || l > h) return false;
+            qlo[a] |= (unsigned)l << (8 * k);
+            qhi[a] |= (unsigned)h << (8 * k);
+        }
+    w.ox = (float)origin[0];
+    w.oy = (float)origin[1];
+    w.oz = (float)origin[2];
+    w.exps = exps;
+    w.lo_x = qlo[0]; w.lo_y = qlo[1]; w.lo_z = qlo[2];
+    w.hi_x = qhi[0]; w.hi_y = qhi[1]; w.hi_z = qhi[2];
+    w.pad0 = w.pad1 = 0;
+    return true;
+}
+
 struct Widener {
     const Node* bin;
     int bin_count;
@@ -258,27 +318,19 @@ struct Widener {
         for (const ChildRef& k : kids) internal += k.child >= 0;
         const int held = above + (internal > 1 ? internal - 1 : 0);
         if (held > stack_need) stack_need = held;
-        float lo[3][4] = {}, hi[3][4] = {};
         int child[4] = {0, 0, 0, 0};
         for (size_t k = 0; k < kids.size(); ++k) {
-            for (int a = 0; a < 3; ++a) {
-                lo[a][k] = kids[k].box.lo[a];
-                hi[a][k] = kids[k].box.hi[a];
-            }
             const int c = kids[k].child;
             if (c < 0 && ~c >= leaf_count) bad = true;
             child[k] = c < 0 ? c : emit(c, held, depth + 1);
             if (bad) return 0;
         }
         WideNode& w = out[me];  // out may have grown during the recursion
-        w.lo_x = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
-        w.lo_y = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
-        w.lo_z = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
-        w.hi_x = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
-        w.hi_y = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
-        w.hi_z = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+        if (!quantize(kids, w)) {
+            bad = true;
+            return 0;
+        }
         w.child = make_int4(child[0], child[1], child[2], child[3]);
-        w.meta = make_int4((int)kids.size(), 0, 0, 0);
         return me;
     }
 };
@@ -289,13 +341,11 @@ int widen(const Node* bin, int face_count, std::vector<WideNode>& out) {
     const int bin_count = face_count <= 1 ? 1 : face_count - 1;
     if (face_count == 1) {
         // one triangle: the binary root's second child is an empty placeholder
-        WideNode w{};
-        const HostBox b = node_child_box(bin[0], 0);
-        w.lo_x.x = b.lo[0]; w.lo_y.x = b.lo[1]; w.lo_z.x = b.lo[2];
-        w.hi_x.x = b.hi[0]; w.hi_y.x = b.hi[1]; w.hi_z.x = b.hi[2];
         if (bin[0].d.x != ~0) return -1;
+        WideNode w{};
+        const std::vector<ChildRef> only = {{~0, node_child_box(bin[0], 0)}};
+        if (!quantize(only, w)) return -1;
         w.child = make_int4(~0, 0, 0, 0);
-        w.meta = make_int4(1, 0, 0, 0);
         out.assign(1, w);
         return 1;
     }

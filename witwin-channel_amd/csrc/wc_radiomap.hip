@@ -751,9 +751,11 @@ diff_record_kernel(SceneView s, wc_solve q, const unsigned* __restrict__ masks, 
 __global__ void __launch_bounds__(256) diff_sum_kernel(wc_solve q, wc_radiomap_out o, const int* __restrict__ offsets,
                                                        const int* __restrict__ ctrl,
                                                        const DiffRecord* __restrict__ records) {
-    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
-    if (cell >= q.total_cells || !ctrl[2]) return;
-    const int t = tx_of_cell(q, cell);
+    // evaluation order: neighbouring receivers (similar record counts) share a wavefront
+    const int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= q.total_cells || !ctrl[2]) return;
+    const int cell = cell_at(q, pos);
+    const int t = tx_of_cell(q, pos);
     AccumSink a;
     a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
     const CellPartial p = reinterpret_cast<const CellPartial*>(o.workspace)[cell];

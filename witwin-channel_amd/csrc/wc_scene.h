@@ -43,16 +43,22 @@ struct Edge {
     int material0, material1;
 };
 
-// 4-wide node the tracers walk (128 B, two cache lines): child k's box in
-// component k of the six bound vectors. child >= 0: wide node index;
-// child < 0: leaf holding sorted triangle ~child. Slots k >= meta.x are unused.
-// Built on the host from a binary tree (wc_bvh_widen).
+// 4-wide node the tracers walk: 64 B, one cache line. Child boxes are 8-bit
+// offsets on a per-axis grid of spacing 2^(e - 127) from `origin`, a multiple
+// of that spacing; the host picks e so that every decoded bound
+// origin + q * 2^(e - 127) (q = 0..255) is a float exactly, hence a decoded
+// box always contains the child's true box. child >= 0: wide node index;
+// child < 0: leaf holding sorted triangle ~child. Slots k >= the child count
+// are unused. Built on the host from a binary tree (wc_bvh_widen).
 struct WideNode {
-    float4 lo_x, lo_y, lo_z;
-    float4 hi_x, hi_y, hi_z;
+    float ox, oy, oz;
+    unsigned exps;              // bytes 0..2: biased exponents e of x, y, z; byte 3: child count (1..4)
+    unsigned lo_x, lo_y, lo_z;  // byte k: child k's lower bound in grid steps
+    unsigned hi_x, hi_y, hi_z;  // byte k: child k's upper bound in grid steps
+    unsigned pad0, pad1;
     int4 child;
-    int4 meta;  // x = child count (1..4)
 };
+static_assert(sizeof(WideNode) == 64, "a wide node fills one 64-byte cache line");
 
 struct SceneView {
     // 4-wide BVH over triangles in sorted order

@@ -96,20 +96,23 @@ __device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float 
     return tn <= tf;
 }
 
-__device__ __forceinline__ float lane4(const float4& v, int k) {
-    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+__device__ __forceinline__ float grid_step(unsigned exps, int axis) {
+    return __int_as_float((int)((exps >> (8 * axis)) & 0xFFu) << 23);
 }
-__device__ __forceinline__ int lane4(const int4& v, int k) {
-    return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w));
+
+// bound of child k on one axis: origin + q * step, exact by construction
+__device__ __forceinline__ float grid_bound(unsigned packed, int k, float step, float origin) {
+    return fmaf((float)((packed >> (8 * k)) & 0xFFu), step, origin);
 }
 
 // Walks the 4-wide tree over the segment (tmin, limit), handing every leaf
 // whose box it crosses to `leaf` (which returns true to end the walk). Of the
 // internal children a node's segment crosses, the nearest is entered next and
 // the others wait on the stack, so a blocker close to the segment's start ends
-// an any-hit walk early. `limit` is read at every box test: a closest-hit
-// leaf may shrink it. The stack never overflows: the host build checks the
-// tree's worst-case stack need against kWideStackSize.
+// an any-hit walk early; a node's leaves are tested after its box tests, one
+// triangle per lane per round. `limit` is read at every box test: a
+// closest-hit leaf may shrink it. The stack never overflows: the host build
+// checks the tree's worst-case stack need against kWideStackSize.
 template <class Leaf>
 __device__ __forceinline__ void walk_wide(const SceneView& s, const RayPre& r, float tmin, const float& limit,
                                           Leaf&& leaf) {
@@ -118,15 +121,19 @@ __device__ __forceinline__ void walk_wide(const SceneView& s, const RayPre& r, f
     int node = 0;
     while (true) {
         const WideNode nd = s.nodes[node];
+        const float sx = grid_step(nd.exps, 0), sy = grid_step(nd.exps, 1), sz = grid_step(nd.exps, 2);
+        const int count = (int)(nd.exps >> 24);
         int next = -1;
         float tnext = 0.0f;
         unsigned leaves = 0;  // slots of crossed leaves, tested after the box tests
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             float tn;
-            if (k < nd.meta.x && slab(r, lane4(nd.lo_x, k), lane4(nd.lo_y, k), lane4(nd.lo_z, k), lane4(nd.hi_x, k),
-                                      lane4(nd.hi_y, k), lane4(nd.hi_z, k), tmin, limit, tn)) {
-                const int c = lane4(nd.child, k);
+            if (k < count &&
+                slab(r, grid_bound(nd.lo_x, k, sx, nd.ox), grid_bound(nd.lo_y, k, sy, nd.oy),
+                     grid_bound(nd.lo_z, k, sz, nd.oz), grid_bound(nd.hi_x, k, sx, nd.ox),
+                     grid_bound(nd.hi_y, k, sy, nd.oy), grid_bound(nd.hi_z, k, sz, nd.oz), tmin, limit, tn)) {
+                const int c = k == 0 ? nd.child.x : (k == 1 ? nd.child.y : (k == 2 ? nd.child.z : nd.child.w));
                 if (c < 0) {
                     leaves |= 1u << k;
                 } else if (next < 0) {
@@ -148,7 +155,8 @@ __device__ __forceinline__ void walk_wide(const SceneView& s, const RayPre& r, f
         while (leaves) {
             const int k = __ffs(leaves) - 1;
             leaves &= leaves - 1;
-            if (leaf(s.tris[~lane4(nd.child, k)])) return;
+            const int c = k == 0 ? nd.child.x : (k == 1 ? nd.child.y : (k == 2 ? nd.child.z : nd.child.w));
+            if (leaf(s.tris[~c])) return;
         }
         if (next < 0) {
             if (sp == 0) return;

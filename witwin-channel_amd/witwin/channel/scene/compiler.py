@@ -498,7 +498,7 @@ def vertex_csr(vertex_count: int, faces: np.ndarray, group_face: np.ndarray,
 
 BVH_BUILDERS = ("sah", "lbvh")
 BINARY_NODE_BYTES = 64   # wc_bvh_build / SAH build layout
-WIDE_NODE_BYTES = 128    # the 4-wide nodes the tracers walk (wc_bvh_widen)
+WIDE_NODE_BYTES = 64     # the 4-wide nodes the tracers walk (wc_bvh_widen)
 
 
 class DeviceScene:
