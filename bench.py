@@ -155,7 +155,7 @@ def main() -> int:
     from witwin.channel.components import component_bits
     from witwin.channel.kernels.deterministic import MeanDbStep
     from witwin.channel.propagation import engine
-    from witwin.channel.scene.compiler import compile as compile_scene, device_scene
+    from witwin.channel.scene.compiler import DeviceScene, compile as compile_scene, device_scene, with_vertices
 
     device = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(device)
@@ -272,13 +272,35 @@ def main() -> int:
         h_fwd_ms = kernel_time(lambda: engine.radiomap_forward(h_dscene, h_inputs, topology_capacity=h_step.capacity,
                                                                out=h_step.out), stream)
         h_cells = h_inputs.total_cells
+        # geometry step of a vertex loop on the same mesh (every vertex lifted by up to 5 cm): device refit
+        # (triangle tables, tree boxes, planes, edges; wall time incl. the failure-word read) against a host
+        # recompile of the moved scene plus a fresh SAH tree and upload
+        lift = np.random.default_rng(1).uniform(0.0, 0.05, h_compiled.vertices.shape[0]).astype(np.float32)
+        moved = h_compiled.vertices.copy()
+        moved[:, 2] += lift
+        refit_scene = DeviceScene(h_compiled, device)
+        moved_t = torch.as_tensor(moved).to(device)
+        refit_scene.refit(moved_t)
+        torch.cuda.synchronize()
+        r0 = time.perf_counter()
+        for _ in range(5):
+            refit_scene.refit(moved_t)
+        torch.cuda.synchronize()
+        refit_ms = (time.perf_counter() - r0) * 1e3 / 5
+        r0 = time.perf_counter()
+        for _ in range(2):
+            DeviceScene(with_vertices(h_compiled, moved), device)
+        torch.cuda.synchronize()
+        rebuild_ms = (time.perf_counter() - r0) * 1e3 / 2
+        del refit_scene
         terrain = {"workload": f"fBm heightfield 128x128 at 2 m (relief 20 m) + 40 boxes, {h_compiled.face_count} "
                                "tris, 4 materials, 256x256 grid 1.5 m above the highest point, 3.5 GHz, LoS + "
                                "depth-2 reflection, loss mean dB; VJP w.r.t. eps_r, sigma, f, Tx and all "
                                f"{h_compiled.vertices.shape[0]} vertices",
                    "cells": h_cells, "chains": h_inputs.chains.count,
                    "paths": int(h_step.out.path_count.sum().item()), "ms_per_step": h_ms,
-                   "cells_per_s": h_cells / (h_ms * 1e-3), "forward_ms": h_fwd_ms, "vertex_vjp_ms": h_geo_ms}
+                   "cells_per_s": h_cells / (h_ms * 1e-3), "forward_ms": h_fwd_ms, "vertex_vjp_ms": h_geo_ms,
+                   "geometry_update": {"device_refit_ms": refit_ms, "host_recompile_and_build_ms": rebuild_ms}}
 
     cpu = None
     if not args.no_cpu:

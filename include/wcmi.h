@@ -156,6 +156,32 @@ typedef struct wc_solve {
                               radiomap (0: no chain masks) */
 } wc_solve;
 
+/* ------------------------------------------------------------ scene refit
+ * Moved vertices, same topology: replaces the per-world-version recompile and
+ * RayD scene rebuild (scene/compiler.py:1306-1450, scene/resources.py:169-245)
+ * for geometry steps such as a vertex-gradient loop. In place, on `stream`:
+ * the triangle tables (tris, faces) get the new positions, every 4-wide node
+ * is refit after its children (levels deepest first) and its child boxes are
+ * re-quantized on the exact grid of the host build, group planes are
+ * recomputed from their representative faces and edge records from their four
+ * vertices (float64, the host compile's formulas). The tree's shape, groups
+ * and edge list stay; walks stay exact, they may visit more nodes as the
+ * boxes drift from a fresh build. A box that cannot be quantized sets
+ * *failure (the tree is then unusable: rebuild).
+ *   node_order:        wide node ids grouped by depth, deepest level first
+ *   level_offset_host: [levels + 1] host offsets into node_order */
+typedef struct wc_refit {
+    const float* vertices;        /* [V,3] new positions */
+    const int* faces;             /* [F,3] */
+    const int* group_face;        /* [G] representative face of each group */
+    const int* edge_vertices;     /* [E,4] (a, b, c0, c1), c1 = -1 for a half-plane */
+    const int* node_order;
+    const int* level_offset_host;
+    int levels;
+    int* failure;                 /* device int, caller-zeroed */
+} wc_refit;
+int wc_scene_refit(const wc_scene* scene, const wc_refit* refit, void* stream);
+
 /* ------------------------------------------------------------ radiomap
  * Fused deterministic radiomap: per (transmitter, cell) discovery of every
  * line-of-sight, reflection chain, diffraction edge and transmission path,

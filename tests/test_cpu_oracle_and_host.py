@@ -420,6 +420,25 @@ def test_sah_bvh_tables_are_a_valid_tree(which):
     assert worst_stack <= 96
 
 
+def test_wide_node_levels_put_children_before_parents():
+    """Refit order of the wide tree: every reachable node once, each internal
+    child in a deeper level (refit earlier) than its parent, the root last."""
+    from witwin.channel.scene.compiler import wide_node_levels
+
+    for which in ("canyon", "heightfield", "one"):
+        compiled = _build_case(which)
+        nodes, _, _ = _sah_tables(compiled)
+        order, offsets = wide_node_levels(nodes)
+        assert offsets[0] == 0 and offsets[-1] == len(order) and (np.diff(offsets) > 0).all()
+        assert order[-1] == 0 and len(set(order.tolist())) == len(order)
+        level_of = {int(n): i for i in range(len(offsets) - 1) for n in order[offsets[i]:offsets[i + 1]]}
+        for n in order:
+            _, _, child = _wide_children(nodes, int(n))
+            for c in child:
+                if c >= 0:
+                    assert level_of[int(c)] < level_of[int(n)]
+
+
 def _slab_np(o, inv, lo, hi, tmin, tmax):
     t0, t1 = (lo - o) * inv, (hi - o) * inv
     tn = max(np.minimum(t0, t1).max(), tmin)

@@ -89,6 +89,13 @@ class WcFixedRows(ctypes.Structure):
     ]
 
 
+class WcRefit(ctypes.Structure):
+    _fields_ = [
+        ("vertices", c_void_p), ("faces", c_void_p), ("group_face", c_void_p), ("edge_vertices", c_void_p),
+        ("node_order", c_void_p), ("level_offset_host", c_void_p), ("levels", c_int), ("failure", c_void_p),
+    ]
+
+
 class WcAdjoint(ctypes.Structure):
     _fields_ = [
         ("grad_power_total", c_void_p), ("grad_field_total", c_void_p), ("grad_component_power", c_void_p),
@@ -121,6 +128,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_vjp_partial_stride": (c_int, [c_int]),
     "wc_vjp_partial_rows": (c_int, [c_int]),
     "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "wc_scene_refit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcRefit), c_void_p]),
     "wc_fixed_rows_evaluate": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcFixedRows), ctypes.c_float,
                                        c_void_p]),
     "wc_radiomap_vjp_geometry": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),

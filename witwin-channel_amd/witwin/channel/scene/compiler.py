@@ -501,6 +501,23 @@ BINARY_NODE_BYTES = 64   # wc_bvh_build / SAH build layout
 WIDE_NODE_BYTES = 64     # the 4-wide nodes the tracers walk (wc_bvh_widen)
 
 
+def wide_node_levels(nodes: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Refit order of a 4-wide tree (raw 64-byte nodes): the node ids reachable
+    from the root grouped by depth, deepest level first, and the [levels + 1]
+    offsets of the groups (the order wc_scene_refit refits them in)."""
+    words = np.ascontiguousarray(nodes).view(np.int32).reshape(-1, WIDE_NODE_BYTES // 4)
+    counts = words[:, 3].view(np.uint32) >> 24
+    levels: list[list[int]] = [[0]]
+    while True:
+        nxt = [int(c) for n in levels[-1] for c in words[n, 12:12 + int(counts[n])] if c >= 0]
+        if not nxt:
+            break
+        levels.append(nxt)
+    order = np.array([n for level in reversed(levels) for n in level], dtype=np.int32)
+    offsets = np.cumsum([0] + [len(level) for level in reversed(levels)]).astype(np.int32)
+    return order, offsets
+
+
 class DeviceScene:
     """HBM copy of a compiled scene plus its BVH, and the wc_scene view.
 
@@ -564,6 +581,7 @@ class DeviceScene:
             self.nodes.copy_(torch.from_numpy(nodes))
             self.sorted_tris.copy_(torch.from_numpy(sorted_tris))
             self.face_tris.copy_(torch.from_numpy(face_tris))
+            wide_host = nodes
         elif f:
             ws_bytes = int(_native.call("wc_bvh_workspace_bytes", f))
             workspace = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
@@ -577,6 +595,12 @@ class DeviceScene:
             wide = np.zeros(self.node_count * WIDE_NODE_BYTES, np.uint8)
             _native.call("wc_bvh_widen", binary_host.ctypes.data, f, wide.ctypes.data, None)
             self.nodes.copy_(torch.from_numpy(wide))
+            wide_host = wide
+        self._refit_order: torch.Tensor | None = None
+        self._refit_offsets = np.zeros(1, np.int32)
+        if f:
+            order, self._refit_offsets = wide_node_levels(wide_host)
+            self._refit_order = torch.as_tensor(order).to(device)
         p = _native.ptr
         self.view = _native.WcScene(
             nodes=p(self.nodes), tris=p(self.sorted_tris), tri_count=f, node_count=self.node_count,
@@ -593,6 +617,27 @@ class DeviceScene:
 
     def ref(self) -> "ctypes._Pointer":
         return ctypes.byref(self.view)
+
+    def refit(self, vertices: torch.Tensor) -> None:
+        """Move the vertices (same topology) on the device: triangle tables,
+        the tree's boxes, group planes and edge records are recomputed in place
+        by ``wc_scene_refit``; faces, groups, the edge list and the tree's
+        shape stay. The host CompiledScene is not updated
+        (``with_vertices`` gives its host counterpart)."""
+        from witwin.channel import _native
+
+        v = vertices.detach().to(device=self.device, dtype=torch.float32).reshape(self.vertices.shape)
+        self.vertices.copy_(v)
+        failure = torch.zeros(1, dtype=torch.int32, device=self.device)
+        p = _native.ptr
+        levels = np.ascontiguousarray(self._refit_offsets, dtype=np.int32)
+        refit = _native.WcRefit(
+            vertices=p(self.vertices), faces=p(self.faces), group_face=p(self.group_face),
+            edge_vertices=p(self.edge_vertices), node_order=p(self._refit_order) if self._refit_order is not None
+            else None, level_offset_host=levels.ctypes.data, levels=len(levels) - 1, failure=p(failure))
+        _native.call("wc_scene_refit", self.ref(), ctypes.byref(refit), _native.stream_handle(self.device))
+        if int(failure.item()):
+            raise RuntimeError("wc_scene_refit: a node box could not be quantized; rebuild the scene")
 
     def vertex_csr(self) -> tuple[torch.Tensor, torch.Tensor]:
         """Device copy of vertex_csr for this scene (built on first use)."""
