@@ -1,6 +1,10 @@
 """The conservative culls checked on the CPU against the exact float tests,
 on the device sources themselves (compiled as host C++ through a shim).
 
+Reflection chains per tile: a chain that chain_reaches_box drops for an 8 x 8
+tile of receivers must not back-project into faces of all its groups for any
+receiver of that tile.
+
 Transmitter-leg shadow bins: every bin a triangle sets must be blocked by
 that triangle for legs sampled across the bin (float watertight test, the
 leg test's end bias); the float pre-test that skips triangles whose in-plane
@@ -34,15 +38,65 @@ def _dump(scene, txs: np.ndarray, out: Path, frequency: float) -> None:
     np.asarray(txs, np.float32).reshape(-1, 3).tofile(out / "txs.bin")
 
 
-@pytest.fixture(scope="module")
-def checker(tmp_path_factory) -> Path:
+def _build(tmp_path_factory, name: str) -> Path:
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ is not available")
-    exe = tmp_path_factory.mktemp("cull") / "shadow_check"
-    subprocess.run([gxx, "-O2", "-std=c++17", f"-I{HERE / 'cpu_geo'}", str(HERE / "cpu_geo" / "shadow_check.cpp"),
+    exe = tmp_path_factory.mktemp("cull") / name
+    subprocess.run([gxx, "-O2", "-std=c++17", f"-I{HERE / 'cpu_geo'}", str(HERE / "cpu_geo" / f"{name}.cpp"),
                     "-o", str(exe)], check=True, capture_output=True, timeout=300)
     return exe
+
+
+@pytest.fixture(scope="module")
+def checker(tmp_path_factory) -> Path:
+    return _build(tmp_path_factory, "shadow_check")
+
+
+@pytest.fixture(scope="module")
+def chain_checker(tmp_path_factory) -> Path:
+    return _build(tmp_path_factory, "chain_cull_check")
+
+
+def _dump_chains(scene, out: Path, seed: int, depth1: int, depth2: int) -> None:
+    """Scene tables, a random sample of depth-1 and depth-2 group chains with
+    their transmitter images, and the scene's square receiver grid."""
+    compiled = compile_scene(scene, reference_frequency_hz=3.5e9)
+    out.mkdir(parents=True, exist_ok=True)
+    g = compiled.group_count
+    rng = np.random.default_rng(seed)
+    one = rng.choice(g, size=min(g, depth1), replace=False)
+    two = rng.integers(0, g, size=(depth2, 2))
+    two = two[two[:, 0] != two[:, 1]]
+    chains = np.full((len(one) + len(two), 5), -1, np.int32)
+    chains[:len(one), 0] = one
+    chains[len(one):, :2] = two
+    tx = compiled.endpoints.tx_position[0].astype(np.float64)
+    n, off = compiled.group_plane[:, :3], compiled.group_plane[:, 3]
+    images = np.zeros((len(chains), 5, 3))
+    p = np.repeat(tx[None], len(chains), 0)
+    for j in range(2):
+        grp = chains[:, j]
+        ok = grp >= 0
+        gg = np.where(ok, grp, 0)
+        d = (p * n[gg]).sum(1) - off[gg]
+        p = np.where(ok[:, None], p - 2.0 * d[:, None] * n[gg], p)
+        images[:, j] = np.where(ok[:, None], p, 0.0)
+    tri = np.zeros((compiled.face_count, 12), np.float32)
+    for k in range(3):
+        tri[:, 4 * k:4 * k + 3] = compiled.vertices[compiled.faces[:, k]]
+    tri[:, 3] = compiled.face_group.astype(np.int32).view(np.float32)
+    rx = np.asarray(compiled.endpoints.rx_position, np.float32)
+    side = int(round(np.sqrt(len(rx))))
+    compiled.group_plane.astype(np.float64).tofile(out / "plane.bin")
+    compiled.group_face_offset.astype(np.int32).tofile(out / "goff.bin")
+    compiled.group_faces.astype(np.int32).tofile(out / "gfaces.bin")
+    tri.tofile(out / "tris.bin")
+    chains.tofile(out / "chains.bin")
+    images.tofile(out / "images.bin")
+    rx.tofile(out / "rx.bin")
+    tx.astype(np.float32).tofile(out / "tx.bin")
+    np.array([g, compiled.face_count, len(chains), side], np.int32).tofile(out / "counts.bin")
 
 
 def _run(exe: Path, data: Path) -> dict[str, int]:
@@ -68,3 +122,15 @@ def test_shadow_bins_and_pretest_on_heightfield(checker, tmp_path):
     c = _run(checker, tmp_path / "terrain")
     assert c["violations"] == 0 and c["skipped_with_bins"] == 0
     assert c["set_bins"] > 0 and c["skipped"] > c["tests"] // 2
+
+
+@pytest.mark.parametrize("which", ["canyon", "heightfield"])
+def test_chain_tile_cull_drops_no_valid_chain(chain_checker, tmp_path, which):
+    if which == "canyon":
+        scene = workloads.urban_canyon_scene(seed=2, shape=(24, 24), spacing=14.0)
+    else:
+        scene = workloads.heightfield_scene(seed=4, size=40, boxes=10, shape=(24, 24))
+    _dump_chains(scene, tmp_path / which, seed=0, depth1=1500, depth2=1500)
+    c = _run(chain_checker, tmp_path / which)
+    assert c["wrong"] == 0
+    assert c["culled"] > c["kept"] // 4 and c["kept_valid"] > 0
