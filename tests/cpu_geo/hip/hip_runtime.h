@@ -58,3 +58,5 @@ inline int atomicOr(int* p, int v) { int o = *p; *p |= v; return o; }
 inline int atomicAdd(int* p, int v) { int o = *p; *p += v; return o; }
 inline unsigned atomicAdd(unsigned* p, unsigned v) { unsigned o = *p; *p += v; return o; }
 inline double unsafeAtomicAdd(double* p, double v) { double o = *p; *p += v; return o; }
+#define __builtin_amdgcn_wave_barrier() ((void)0)
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
