@@ -131,6 +131,30 @@ __device__ inline int face_containing(const SceneView& s, int g, const V3<float>
 // leg for occlusion (legs skip the groups they start and end on). On success
 // fills hits/faces and returns true. The depth D is a compile-time constant
 // so the per-bounce arrays stay in registers.
+// Visibility of the D+1 legs of a reflection chain through its hit points
+// (each leg ignores the groups it starts and ends on).
+template <int D>
+__device__ inline bool chain_legs_clear(const SceneView& s, const int* groups, const V3<double>* hits,
+                                        const V3<float>& tx, const V3<float>& rx) {
+    V3<float> a = tx;
+    int ga = -1;
+#pragma unroll
+    for (int k = 0; k <= D; ++k) {
+        const V3<float> b = k < D ? to_f(hits[k]) : rx;
+        const int gb = k < D ? groups[k] : -1;
+        const V3<float> seg = b - a;
+        const float len = norm(seg);
+        if (len > 2.0f * kSegmentBias) {
+            const float eps = kSegmentBias / len;
+            const RayPre r = ray_setup(a.x, a.y, a.z, seg.x, seg.y, seg.z);
+            if (occluded(s, r, eps, 1.0f - eps, ga, gb)) return false;
+        }
+        a = b;
+        ga = gb;
+    }
+    return true;
+}
+
 template <int D, bool kLegs = true>
 __device__ inline bool solve_chain(const SceneView& s, const int* groups, const double* images, const V3<float>& tx,
                                    const V3<float>& rx, V3<double>* hits, int* faces) {
@@ -153,24 +177,7 @@ __device__ inline bool solve_chain(const SceneView& s, const int* groups, const 
         next = h;
     }
     if (!kLegs) return true;  // leg visibility already established (chain masks)
-    // visibility of the D+1 legs
-    V3<float> a = tx;
-    int ga = -1;
-#pragma unroll
-    for (int k = 0; k <= D; ++k) {
-        const V3<float> b = k < D ? to_f(hits[k]) : rx;
-        const int gb = k < D ? groups[k] : -1;
-        const V3<float> seg = b - a;
-        const float len = norm(seg);
-        if (len > 2.0f * kSegmentBias) {
-            const float eps = kSegmentBias / len;
-            const RayPre r = ray_setup(a.x, a.y, a.z, seg.x, seg.y, seg.z);
-            if (occluded(s, r, eps, 1.0f - eps, ga, gb)) return false;
-        }
-        a = b;
-        ga = gb;
-    }
-    return true;
+    return chain_legs_clear<D>(s, groups, hits, tx, rx);
 }
 
 // Field of a reflection chain with the interaction points fixed: the Jones

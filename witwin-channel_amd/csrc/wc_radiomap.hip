@@ -941,6 +941,137 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
     masks[(size_t)cell * words + chunk] = bits;
 }
 
+// Chain validity for per-transmitter point sets (scattered receivers: a
+// wavefront's cells gain little from testing the same chain together). After
+// the tile cull, each lane back-projects its cell through the surviving
+// chains (no traversal); the (cell, chain) pairs that land in faces queue up
+// chain by chain (lanes in order within a chain) and their D+1 legs are traced
+// 64 pairs per round with every lane busy. The bits are those of
+// chain_visibility_kernel (same back-projection and leg code).
+template <int D>
+__device__ __forceinline__ bool chain_lands(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
+                                            const V3<float>& rx) {
+    V3<double> hd[D];
+    int faces[D];
+    return solve_chain<D, false>(s, q.chain_groups + c * kMaxDepth, q.chain_images + c * kMaxDepth * 3, tx, rx, hd,
+                                 faces);
+}
+
+template <int D>
+__device__ __forceinline__ bool chain_legs(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
+                                           const V3<float>& rx) {
+    V3<double> hd[D];
+    int faces[D];
+    const int* groups = q.chain_groups + c * kMaxDepth;
+    solve_chain<D, false>(s, groups, q.chain_images + c * kMaxDepth * 3, tx, rx, hd, faces);
+    return chain_legs_clear<D>(s, groups, hd, tx, rx);
+}
+
+__global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s, wc_solve q,
+                                                                     const int* __restrict__ tiles, int tile_count,
+                                                                     unsigned* __restrict__ masks, int words) {
+    __shared__ unsigned short queue[4][128];
+    __shared__ float4 lane_rx[4][64];
+    __shared__ unsigned found[4][64];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * 4 + wid;
+    const bool wave_live = wave < (long long)tile_count * words;
+    const int tile = wave_live ? (int)(wave / words) : 0;
+    const int chunk = wave_live ? (int)(wave - (long long)tile * words) : 0;
+    const int t = tiles[2 * tile];
+    const int width = q.lanes_per_cell;
+    const int pos = tiles[2 * tile + 1] + lane;
+    const bool live = wave_live && lane < 64 / width && pos < q.cell_offset[t + 1];
+    const int cell = live ? cell_at(q, pos) : 0;
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    V3<float> rx{0.0f, 0.0f, 0.0f};
+    if (live) rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[t] : cell));
+    const V3<float> lo{wave_min(live ? rx.x : INFINITY), wave_min(live ? rx.y : INFINITY),
+                       wave_min(live ? rx.z : INFINITY)};
+    const V3<float> hi{wave_max(live ? rx.x : -INFINITY), wave_max(live ? rx.y : -INFINITY),
+                       wave_max(live ? rx.z : -INFINITY)};
+    const int base = q.chain_offset[t * kMaxDepth];
+    const int end = q.chain_offset[t * kMaxDepth + min(q.max_depth, kMaxDepth)];
+    const int c0 = base + chunk * 32;
+    const int c1 = min(c0 + 32, end);
+    // 1. tile cull, one chain per lane
+    bool reach = false;
+    if (wave_live && lane < c1 - c0) {
+        const int c = c0 + lane;
+        int d = 1;
+        while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+        reach = chain_reaches_box(s, q, c, d, lo, hi);
+    }
+    unsigned todo = (unsigned)__ballot(reach);
+    // 2. back-projection of the lane's cell through the surviving chains
+    unsigned cand = 0;
+    if (live) {
+        unsigned rest = todo;
+        int d = 1;
+        while (rest) {
+            const int b = __ffs(rest) - 1;
+            rest &= rest - 1;
+            const int c = c0 + b;
+            while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+            bool ok;
+            switch (d) {
+                case 1: ok = chain_lands<1>(s, q, c, tx, rx); break;
+                case 2: ok = chain_lands<2>(s, q, c, tx, rx); break;
+                case 3: ok = chain_lands<3>(s, q, c, tx, rx); break;
+                case 4: ok = chain_lands<4>(s, q, c, tx, rx); break;
+                default: ok = chain_lands<5>(s, q, c, tx, rx); break;
+            }
+            if (ok) cand |= 1u << b;
+        }
+    }
+    // 3. the landing pairs queue up chain by chain; legs traced 64 per round
+    lane_rx[wid][lane] = make_float4(rx.x, rx.y, rx.z, 0.0f);
+    found[wid][lane] = 0;
+    __syncthreads();
+    int queued = 0;  // wave-uniform
+    const int nb = wave_live ? c1 - c0 : 0;
+    for (int b = 0; b <= nb; ++b) {
+        if (b < nb) {
+            const bool pred = (cand >> b) & 1u;
+            const unsigned long long ball = __ballot(pred);
+            if (pred) queue[wid][queued + (int)__popcll(ball & ((1ull << lane) - 1ull))] = (unsigned short)(lane | (b << 6));
+            queued += (int)__popcll(ball);
+            if (queued < 64) continue;
+        } else if (queued == 0) {
+            break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int item = lane < queued ? (int)queue[wid][lane] : -1;
+        const int carry = lane + 64 < queued ? (int)queue[wid][lane + 64] : -1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (carry >= 0) queue[wid][lane] = (unsigned short)carry;
+        queued = queued > 64 ? queued - 64 : 0;
+        if (item >= 0) {
+            const int o = item & 63, bit = item >> 6;
+            const float4 r4 = lane_rx[wid][o];
+            const V3<float> orx{r4.x, r4.y, r4.z};
+            const int c = c0 + bit;
+            int d = 1;
+            while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+            bool ok;
+            switch (d) {
+                case 1: ok = chain_legs<1>(s, q, c, tx, orx); break;
+                case 2: ok = chain_legs<2>(s, q, c, tx, orx); break;
+                case 3: ok = chain_legs<3>(s, q, c, tx, orx); break;
+                case 4: ok = chain_legs<4>(s, q, c, tx, orx); break;
+                default: ok = chain_legs<5>(s, q, c, tx, orx); break;
+            }
+            if (ok) atomicOr(&found[wid][o], 1u << bit);
+        }
+    }
+    __syncthreads();
+    if (live) masks[(size_t)cell * words + chunk] = found[wid][lane];
+}
+
 // Visibility of every (cell, edge) diffraction candidate: bit (ei & 31) of
 // word ei >> 5 of the cell's mask is set when the stationary point, clamped
 // to the finite edge, sees both the receiver and the transmitter. One
@@ -1479,8 +1610,14 @@ static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const
     const int words = mask_words(q.max_tx_chains);
     unsigned* masks = chain_masks(o.workspace, q, s.edge_count);
     const long long waves = (long long)tile_count * words;
-    hipLaunchKernelGGL(chain_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q, tiles,
-                       tile_count, masks, words);
+    // a shared receiver grid tests each chain coherently across the tile;
+    // per-transmitter point sets take the queued form
+    if (q.rx_shared || q.lanes_per_cell != 1)
+        hipLaunchKernelGGL(chain_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q,
+                           tiles, tile_count, masks, words);
+    else
+        hipLaunchKernelGGL(chain_visibility_queue_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s,
+                           q, tiles, tile_count, masks, words);
 }
 
 // Compute-unit count of the current device, queried once per device and
