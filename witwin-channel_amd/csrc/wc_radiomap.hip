@@ -408,6 +408,28 @@ __host__ __device__ inline size_t shadow_end(int cells, int lanes, int edge_coun
            (size_t)(tx_count > 0 ? tx_count : 0) * (size_t)(edge_count > 0 ? edge_count : 0) * 8;
 }
 
+// after the shadow table (256-aligned): the supertile cull of the chain
+// validity pass for one transmitter with many chains (supertile = 16 tiles =
+// 1,024 consecutive positions): one receiver box (6 floats) per supertile,
+// then mask_words(max_tx_chains) words of surviving chains per supertile
+constexpr int kSuperTiles = 16;
+constexpr int kSuperCells = 64 * kSuperTiles;
+struct SuperLayout {
+    size_t boxes, masks, end;
+};
+__host__ __device__ inline SuperLayout super_layout(size_t base, int cells, int max_tx_chains) {
+    SuperLayout l;
+    const size_t supers = (size_t)((cells > 0 ? cells : 0) + kSuperCells - 1) / kSuperCells;
+    l.boxes = align256(base);
+    l.masks = align256(l.boxes + supers * 24);
+    l.end = l.masks + supers * (size_t)mask_words(max_tx_chains > 0 ? max_tx_chains : 0) * 4;
+    return l;
+}
+// start of the staged diffraction pass
+__host__ __device__ inline size_t stage_base(int cells, int lanes, int edge_count, int max_tx_chains, int tx_count) {
+    return super_layout(shadow_end(cells, lanes, edge_count, max_tx_chains, tx_count), cells, max_tx_chains).end;
+}
+
 // cell evaluated at launch position `pos` (identity without an order table)
 __device__ __forceinline__ int cell_at(const wc_solve& q, int pos) {
     return q.cell_order ? q.cell_order[pos] : pos;
@@ -886,9 +908,62 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// Supertile cull (one transmitter, positions [1024 k, 1024 k + 1024)): the
+// receiver box of each supertile, then every chain against each box, so the
+// tile pass culls only the chains its supertile kept (a chain that misses the
+// supertile box misses every tile box inside it: chain_reaches_box only
+// grows with the box).
+__global__ void __launch_bounds__(256) super_box_kernel(wc_solve q, int supers, float* __restrict__ boxes) {
+    const int sup = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (sup >= supers) return;
+    const int lane = threadIdx.x & 63;
+    const int first = sup * kSuperCells, last = min(first + kSuperCells, q.total_cells);
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int pos = first + lane; pos < last; pos += 64) {
+        const int cell = cell_at(q, pos);
+        const V3<float> rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[0] : cell));
+        lo[0] = fminf(lo[0], rx.x); lo[1] = fminf(lo[1], rx.y); lo[2] = fminf(lo[2], rx.z);
+        hi[0] = fmaxf(hi[0], rx.x); hi[1] = fmaxf(hi[1], rx.y); hi[2] = fmaxf(hi[2], rx.z);
+    }
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min(lo[a]);
+        hi[a] = wave_max(hi[a]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 3; ++a) {
+            boxes[6 * sup + a] = lo[a];
+            boxes[6 * sup + 3 + a] = hi[a];
+        }
+}
+
+__global__ void __launch_bounds__(256) super_cull_kernel(SceneView s, wc_solve q, int supers, int words,
+                                                         const float* __restrict__ boxes,
+                                                         unsigned* __restrict__ super_masks) {
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= (long long)supers * words) return;
+    const int sup = (int)(wave / words);
+    const int chunk = (int)(wave - (long long)sup * words);
+    const int lane = threadIdx.x & 63;
+    const V3<float> lo{boxes[6 * sup], boxes[6 * sup + 1], boxes[6 * sup + 2]};
+    const V3<float> hi{boxes[6 * sup + 3], boxes[6 * sup + 4], boxes[6 * sup + 5]};
+    const int base = q.chain_offset[0];
+    const int end = q.chain_offset[min(q.max_depth, kMaxDepth)];
+    const int c0 = base + chunk * 32;
+    const int c1 = min(c0 + 32, end);
+    bool reach = false;
+    if (lane < c1 - c0) {
+        const int c = c0 + lane;
+        int d = 1;
+        while (c >= q.chain_offset[d]) ++d;
+        reach = chain_reaches_box(s, q, c, d, lo, hi);
+    }
+    const unsigned bits = (unsigned)__ballot(reach);
+    if (lane == 0) super_masks[(size_t)sup * words + chunk] = bits;
+}
+
 __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_solve q, const int* __restrict__ tiles,
                                                                int tile_count, unsigned* __restrict__ masks,
-                                                               int words) {
+                                                               int words, const unsigned* __restrict__ super_masks) {
     const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)tile_count * words) return;
     const int tile = (int)(wave / words);
@@ -899,6 +974,11 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
     const int pos = tiles[2 * tile + 1] + lane;
     const bool live = lane < 64 / width && pos < q.cell_offset[t + 1];
     const int cell = live ? cell_at(q, pos) : 0;
+    // chains the tile's supertile kept (all when there is no supertile cull;
+    // with it the masks were zero-filled first, so a tile whose supertile kept
+    // none of the 32 chains has nothing to write)
+    const unsigned kept = super_masks ? super_masks[(size_t)(tile / kSuperTiles) * words + chunk] : ~0u;
+    if (kept == 0u) return;
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     V3<float> rx{0.0f, 0.0f, 0.0f};
     if (live) rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[t] : cell));
@@ -913,7 +993,7 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
     const int c1 = min(c0 + 32, end);
     // lane l < 32 culls chain c0 + l against the box
     bool reach = false;
-    if (lane < c1 - c0) {
+    if (lane < c1 - c0 && ((kept >> lane) & 1u)) {
         const int c = c0 + lane;
         int d = 1;
         while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
@@ -1604,6 +1684,17 @@ static int check_solve(const wc_solve* q) {
 
 using namespace wc;
 
+// Zero fill (coalesced 4-byte stores; a fill kernel, not a memset node, keeps
+// the launch graph-capturable).
+__global__ void __launch_bounds__(256) fill_zero_kernel(unsigned* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+// chain words (32 chains each) from which the supertile cull pays for its
+// two extra launches
+constexpr int kSuperMinWords = 64;
+
 static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o, const int* tiles,
                                     int tile_count, hipStream_t st) {
     if (q.max_tx_chains <= 0 || !(q.components & kCompRefl) || q.max_depth <= 0) return;
@@ -1612,9 +1703,31 @@ static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const
     const long long waves = (long long)tile_count * words;
     // a shared receiver grid tests each chain coherently across the tile;
     // per-transmitter point sets take the queued form
-    if (q.rx_shared || q.lanes_per_cell != 1)
+    if (q.rx_shared || q.lanes_per_cell != 1) {
+        // one transmitter with many chains: cull chains per supertile first
+        // (tiles are then the 64-position blocks in order, 16 per supertile)
+        const unsigned* super_masks = nullptr;
+        if (q.tx_count == 1 && q.lanes_per_cell == 1 && words >= kSuperMinWords) {
+            const SuperLayout l = super_layout(
+                shadow_end(q.total_cells, q.lanes_per_cell, s.edge_count, q.max_tx_chains, q.tx_count),
+                q.total_cells, q.max_tx_chains);
+            char* ws = static_cast<char*>(o.workspace);
+            float* boxes = reinterpret_cast<float*>(ws + l.boxes);
+            unsigned* sm = reinterpret_cast<unsigned*>(ws + l.masks);
+            const int supers = (q.total_cells + kSuperCells - 1) / kSuperCells;
+            hipLaunchKernelGGL(super_box_kernel, dim3((unsigned)((supers + 3) / 4)), dim3(256), 0, st, q, supers,
+                               boxes);
+            const long long sw = (long long)supers * words;
+            hipLaunchKernelGGL(super_cull_kernel, dim3((unsigned)((sw + 3) / 4)), dim3(256), 0, st, s, q, supers,
+                               words, boxes, sm);
+            const size_t mask_count = (size_t)q.total_cells * (size_t)words;
+            hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((mask_count + 255) / 256, 65536)),
+                               dim3(256), 0, st, masks, mask_count);
+            super_masks = sm;
+        }
         hipLaunchKernelGGL(chain_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q,
-                           tiles, tile_count, masks, words);
+                           tiles, tile_count, masks, words, super_masks);
+    }
     else
         hipLaunchKernelGGL(chain_visibility_queue_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s,
                            q, tiles, tile_count, masks, words);
@@ -1722,7 +1835,7 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
             const int n = solve->total_cells;
             if (width == 1 && words > 0) {
                 const DiffStageLayout l = diff_stage_layout(
-                    shadow_end(n, width, s.edge_count, solve->max_tx_chains, solve->tx_count), n, s.edge_count);
+                    stage_base(n, width, s.edge_count, solve->max_tx_chains, solve->tx_count), n, s.edge_count);
                 char* ws = static_cast<char*>(out->workspace);
                 int* offsets = reinterpret_cast<int*>(ws + l.offsets);
                 int* ctrl = reinterpret_cast<int*>(ws + l.ctrl);
@@ -1775,7 +1888,7 @@ extern "C" size_t wc_radiomap_workspace_bytes(int cells, int lanes_per_cell, int
     const int c = cells > 0 ? cells : 0;
     const int e = edge_count > 0 ? edge_count : 0;
     const int lanes = lanes_per_cell > 0 ? lanes_per_cell : 1;
-    const size_t base = shadow_end(c, lanes, e, max_tx_chains, tx_count);
+    const size_t base = stage_base(c, lanes, e, max_tx_chains, tx_count);
     // the staged diffraction pass runs with one lane per cell only
     return lanes == 1 && e > 0 ? diff_stage_layout(base, c, e).end : base;
 }
