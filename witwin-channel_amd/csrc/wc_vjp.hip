@@ -492,6 +492,17 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
 #define WC_VJP_SLICES 16
 #endif
 constexpr int kVjpSlices = WC_VJP_SLICES;
+// The line-of-sight / reflection / transmission phase deals each tile's paths
+// over fewer work items: its tiles hold tens to hundreds of paths, not the
+// thousands of diffraction paths near the transmitter, and a work item's fixed
+// cost (descriptor scan, 2M + 4 wavefront sums, at one wavefront per SIMD)
+// then dominates. Its work items fill the first of the tile's partial rows
+// and zero the rest, which the diffraction phase adds to.
+#ifndef WC_VJP_MAIN_SLICES
+#define WC_VJP_MAIN_SLICES 4
+#endif
+constexpr int kVjpMainSlices = WC_VJP_MAIN_SLICES;
+static_assert(kVjpMainSlices >= 1 && kVjpMainSlices <= kVjpSlices, "main-phase slices");
 
 // One wavefront per (tile of 64 units, slice) (unit = one lane-segment of a
 // cell's descriptors; 64 / L cells). The tile's paths of this phase are
@@ -503,12 +514,14 @@ constexpr int kVjpSlices = WC_VJP_SLICES;
 // kGeometry the same dealing drives geometry_path instead (no partials).
 template <int kPhase, bool kGeometry = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VJP_WAVES_PER_EU)))
-vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles, int tile_count, GeoGrad geo) {
+vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles, int tile_count, GeoGrad geo,
+           int slices) {
     const int lane = threadIdx.x & 63;
-    const int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (item >= tile_count * kVjpSlices) return;
-    const int tile = item / kVjpSlices;
-    const int slice = item - tile * kVjpSlices;
+    const int work = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (work >= tile_count * slices) return;
+    const int tile = work / slices;
+    const int slice = work - tile * slices;
+    const int item = tile * kVjpSlices + slice;  // the work item's partial row
     const int width = q.lanes_per_cell;
     const int t = tiles[2 * tile];
     const int pos = tiles[2 * tile + 1] + lane / width;
@@ -555,8 +568,8 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
     g.freq = 0.0f;
     g.tx[0] = g.tx[1] = g.tx[2] = 0.0f;
     const int iterations = (total + 63) >> 6;
-    const int it0 = (int)((long long)iterations * slice / kVjpSlices);
-    const int it1 = (int)((long long)iterations * (slice + 1) / kVjpSlices);
+    const int it0 = (int)((long long)iterations * slice / slices);
+    const int it1 = (int)((long long)iterations * (slice + 1) / slices);
     for (int base = it0 * 64; base < it1 * 64; base += 64) {
         // owner unit u of path j: the last unit whose exclusive prefix is <= j
         // (all lanes take part in the cross-lane reads)
@@ -581,6 +594,11 @@ vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles
     const int m = s.material_count;
     const int stride = 2 * m + 4;
     float* out = ga.partials + (size_t)item * stride;
+    if (kPhase == kPhaseMain && slice == 0) {
+        // the tile's rows this phase's work items do not cover
+        for (int r = slices; r < kVjpSlices; ++r)
+            for (int j = lane; j < stride; j += 64) ga.partials[((size_t)tile * kVjpSlices + r) * stride + j] = 0.0f;
+    }
 #pragma unroll
     for (int i = 0; i < kMaxMaterials; ++i) {
         const float se = wave_sum(g.eps[i]);
@@ -679,20 +697,21 @@ static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adj
         return wc_fail("%s: topology_capacity must be a multiple of lanes_per_cell", who);
     if (tile_count > 0x7fffffff / kVjpSlices) return wc_fail("%s: too many tiles (%d)", who, tile_count);
     const int blocks = (tile_count * kVjpSlices + 3) / 4;
+    const int main_blocks = (tile_count * kVjpMainSlices + 3) / 4;
     const GeoGrad none{nullptr, nullptr};
     if (geo) {
         if (phases & 1)
             hipLaunchKernelGGL((vjp_kernel<kPhaseMain, true>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, s,
-                               *solve, *adjoint, tiles, tile_count, *geo);
+                               *solve, *adjoint, tiles, tile_count, *geo, kVjpSlices);
         const hipError_t err = hipGetLastError();
         return err == hipSuccess ? 0 : wc_fail_hip(who, err);
     }
     if (phases & 1)
-        hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
-                           *adjoint, tiles, tile_count, none);
+        hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3(main_blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
+                           *adjoint, tiles, tile_count, none, kVjpMainSlices);
     if ((phases & 2) && (solve->components & kCompDiff))
         hipLaunchKernelGGL(vjp_kernel<kPhaseDiff>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
-                           *adjoint, tiles, tile_count, none);
+                           *adjoint, tiles, tile_count, none, kVjpSlices);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip(who, err);
 }
