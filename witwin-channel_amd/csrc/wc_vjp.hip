@@ -497,7 +497,8 @@ constexpr int kVjpSlices = WC_VJP_SLICES;
 // thousands of diffraction paths near the transmitter, and a work item's fixed
 // cost (descriptor scan, 2M + 4 wavefront sums, at one wavefront per SIMD)
 // then dominates. Its work items fill the first of the tile's partial rows
-// and zero the rest, which the diffraction phase adds to.
+// and zero the rest, which the diffraction phase adds to. The geometry adjoint
+// of the same paths (no partial rows) is dealt the same way.
 #ifndef WC_VJP_MAIN_SLICES
 #define WC_VJP_MAIN_SLICES 4
 #endif
@@ -701,8 +702,8 @@ static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adj
     const GeoGrad none{nullptr, nullptr};
     if (geo) {
         if (phases & 1)
-            hipLaunchKernelGGL((vjp_kernel<kPhaseMain, true>), dim3(blocks), dim3(256), 0, (hipStream_t)stream, s,
-                               *solve, *adjoint, tiles, tile_count, *geo, kVjpSlices);
+            hipLaunchKernelGGL((vjp_kernel<kPhaseMain, true>), dim3(main_blocks), dim3(256), 0,
+                               (hipStream_t)stream, s, *solve, *adjoint, tiles, tile_count, *geo, kVjpMainSlices);
         const hipError_t err = hipGetLastError();
         return err == hipSuccess ? 0 : wc_fail_hip(who, err);
     }
