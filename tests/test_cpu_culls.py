@@ -5,6 +5,12 @@ Reflection chains per tile: a chain that chain_reaches_box drops for an 8 x 8
 tile of receivers must not back-project into faces of all its groups for any
 receiver of that tile.
 
+The 4-wide tree walk: any-hit and closest-hit walks over the host-built tree
+must give the answers of a test against every triangle, for random segments
+and for segments aimed through shared vertices and edges (where the widened
+box test and the face-id tie-break matter); closest hits may differ only
+where two surfaces meet within the triangle test's rounding of the distance.
+
 Transmitter-leg shadow bins: every bin a triangle sets must be blocked by
 that triangle for legs sampled across the bin (float watertight test, the
 leg test's end bias); the float pre-test that skips triangles whose in-plane
@@ -134,3 +140,36 @@ def test_chain_tile_cull_drops_no_valid_chain(chain_checker, tmp_path, which):
     c = _run(chain_checker, tmp_path / which)
     assert c["wrong"] == 0
     assert c["culled"] > c["kept"] // 4 and c["kept_valid"] > 0
+
+
+@pytest.fixture(scope="module")
+def walk_checker(tmp_path_factory) -> Path:
+    return _build(tmp_path_factory, "walk_check")
+
+
+def test_wide_walk_matches_every_triangle(walk_checker, tmp_path):
+    from test_cpu_oracle_and_host import _sah_tables
+
+    compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    nodes, sorted_tris, _ = _sah_tables(compiled)
+    rng = np.random.default_rng(21)
+    lo, hi = compiled.vertices.min(axis=0), compiled.vertices.max(axis=0)
+    n = 3000
+    a = rng.uniform(lo - 5.0, hi + 5.0, (n, 3))
+    b = rng.uniform(lo - 5.0, hi + 5.0, (n, 3))
+    # segments through vertices and edge midpoints, in random directions
+    tri = compiled.vertices[compiled.faces].astype(np.float64)
+    through = np.concatenate([tri.reshape(-1, 3), 0.5 * (tri[:, 0] + tri[:, 1]), 0.5 * (tri[:, 1] + tri[:, 2])])
+    pick = through[rng.integers(0, len(through), 3000)]
+    dirs = rng.normal(size=(3000, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    reach = rng.uniform(2.0, 60.0, (3000, 1))
+    segs = np.concatenate([np.concatenate([a, b], 1), np.concatenate([pick - dirs * reach, pick + dirs * reach], 1)])
+    out = tmp_path / "walk"
+    out.mkdir()
+    nodes.tofile(out / "nodes.bin")
+    sorted_tris.astype(np.float32).tofile(out / "tris.bin")
+    segs.astype(np.float32).tofile(out / "segs.bin")
+    c = _run(walk_checker, out)
+    assert c["any_wrong"] == 0 and c["closest_wrong"] == 0
+    assert c["hit"] > 1000 and c["near_ties"] <= c["segments"] // 1000

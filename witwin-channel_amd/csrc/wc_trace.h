@@ -81,7 +81,11 @@ __device__ __forceinline__ float tri_hit(const RayPre& r, const Tri& tri, float 
 // ulps (Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013, uses 1 + 2 gamma_3;
 // 32 ulps also covers the rounding of the triangle distance short of
 // near-parallel hits) lets every walk order reach the same triangles, so
-// any-hit, closest-hit and gathered answers do not depend on the tree.
+// any-hit and gathered answers do not depend on the tree. The triangle test's
+// distance carries an absolute error of a few ulps of the vertex coordinates
+// (a large triangle far from the segment's origin): closest hits on two
+// surfaces that meet within that error along the segment may resolve either
+// way, as they would under a different rounding.
 constexpr float kSlabWiden = 1.0f + 32.0f * FLT_EPSILON;
 
 __device__ __forceinline__ bool slab(const RayPre& r, float lx, float ly, float lz, float hx, float hy, float hz,
@@ -189,7 +193,8 @@ struct Hit {
 // hit. Equal hit distances (a ray through an edge or corner shared by two
 // faces) resolve to the lower face id, so the answer does not depend on the
 // order in which the tree is walked (the widened box test, with the limit
-// itself widened once a hit exists, reaches every tied triangle).
+// itself widened once a hit exists, reaches every tied triangle), up to the
+// rounding of the triangle distance noted at kSlabWiden.
 __device__ inline void closest_candidate(const RayPre& r, const Tri& tri, float tmin, int ignore, float& tbest,
                                          Hit& best) {
     const int g = bits_i(tri.v0.w);
