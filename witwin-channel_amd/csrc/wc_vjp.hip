@@ -707,9 +707,15 @@ static int vjp_launch(const wc_scene* scene, const wc_solve* solve, const wc_adj
         const hipError_t err = hipGetLastError();
         return err == hipSuccess ? 0 : wc_fail_hip(who, err);
     }
-    if (phases & 1)
-        hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3(main_blocks), dim3(256), 0, (hipStream_t)stream, s, *solve,
-                           *adjoint, tiles, tile_count, none, kVjpMainSlices);
+    if (phases & 1) {
+        // a shared receiver grid concentrates its reflection paths next to the
+        // transmitter (8 items per tile: canyon step 3.16 -> 3.08 ms); point
+        // sets spread them (4: 64-Tx batch 3.42 vs 3.62 ms with 8)
+        const int main_slices = solve->rx_shared ? 2 * kVjpMainSlices : kVjpMainSlices;
+        hipLaunchKernelGGL(vjp_kernel<kPhaseMain>, dim3((tile_count * main_slices + 3) / 4), dim3(256), 0,
+                           (hipStream_t)stream, s, *solve, *adjoint, tiles, tile_count, none,
+                           main_slices <= kVjpSlices ? main_slices : kVjpSlices);
+    }
     if ((phases & 2) && (solve->components & kCompDiff)) {
         // a shared receiver grid has the heavy tiles next to the transmitter
         // (16 work items each); per-transmitter point sets spread their
