@@ -139,7 +139,7 @@ typedef struct wc_solve {
     const int* chain_offset;
     const int* chain_groups;
     const double* chain_images;
-    float frequency_hz;
+    double frequency_hz; /* carrier (Hz); the phase -k L uses it in double */
     int components;  /* bit 0 los, 1 reflection, 2 diffraction, 3 transmission */
     int max_depth;
     int coherent;

@@ -102,6 +102,7 @@ def load() -> ctypes.CDLL:
         lib.oracle_transition.argtypes = [d, vp]
         lib.oracle_fresnel.argtypes = [d, vp]
         lib.oracle_utd.argtypes = [d, d, d, d, d, d, vp, vp]
+        lib.oracle_utd_finite.argtypes = [d, d, d, d, d, d, vp, vp, d, vp]
         lib.oracle_free_space.argtypes = [vp, vp, d, vp, vp, d, vp]
         lib.oracle_reflection_sequence.argtypes = [i, vp, vp, vp, vp, d, vp, vp, vp, vp, vp, vp, vp, d, vp]
         lib.oracle_pair_paths.argtypes = [ctypes.POINTER(OracleScene), ctypes.POINTER(OracleSolve),
@@ -158,6 +159,17 @@ def utd(n, phi, phi_p, sin_beta0, k, L, r0s=-1, rns=-1, r0h=1, rnh=1) -> tuple[c
             complex(r0h).real, complex(r0h).imag, complex(rnh).real, complex(rnh).imag], np.float64)
     out = np.zeros(4)
     load().oracle_utd(n, phi, phi_p, sin_beta0, k, L, _p(r), _p(out))
+    return complex(out[0], out[1]), complex(out[2], out[3])
+
+
+def utd_finite(n, phi, phi_p, sin_beta0, k, L, trunc, gamma, r0s=-1, rns=-1, r0h=1, rnh=1) -> tuple[complex, complex]:
+    """Wedge coefficients with the finite-edge truncation factor ``trunc``
+    and corner blend ``gamma`` folded into the terms (E1d assembly)."""
+    r = _c([complex(r0s).real, complex(r0s).imag, complex(rns).real, complex(rns).imag,
+            complex(r0h).real, complex(r0h).imag, complex(rnh).real, complex(rnh).imag], np.float64)
+    t = _c([complex(trunc).real, complex(trunc).imag], np.float64)
+    out = np.zeros(4)
+    load().oracle_utd_finite(n, phi, phi_p, sin_beta0, k, L, _p(r), _p(t), float(gamma), _p(out))
     return complex(out[0], out[1]), complex(out[2], out[3])
 
 

@@ -18,8 +18,8 @@
 //                           tests/reference/em_oracle.py:204
 //   transmission sequence . tests/ad/_reference_fields.py:269-336
 //   UTD ................... Kouyoumjian & Pathak 1974; Luebbers 1984;
-//                           finite-edge factor of
-//                           docs/dev/audit/utd-continuity-fix-design.md:91-140
+//                           finite-edge factor and E1d assembly of
+//                           docs/dev/audit/utd-continuity-fix-design.md:91-205
 //   accumulation .......... native/channel/kernels/deterministic.cu:829-970
 //   canonical order ....... native/channel/kernels/evaluated_paths.cu:126-310,654-700
 //
@@ -229,13 +229,35 @@ cd utd_term(double n, double beta, int sgn, double kL) {
     return (std::cos(arg) / sa) * transition(kL * a);
 }
 
+// Wedge coefficients with the E1d finite-edge assembly
+// (reference docs/dev/audit/utd-continuity-fix-design.md:160-205). Each cot*F
+// group (incident: beta = phi - phi', boundaries -pi / +pi; reflection:
+// beta = phi + phi', boundaries pi / (2n-1) pi) splits the term that is
+// singular at the boundary b* nearest to beta into an odd part
+//   odd_w = (t(beta) - t(2 b* - beta)) / 2 * exp(-(delta / delta_w)^2),
+//   delta = beta - b*, delta_w = 4 sqrt(2 pi / kL),
+// and enters as (t - odd_w) T + odd_w gamma; every other term enters as t T.
+// T is the finite-edge truncation factor, gamma = 1 / (1 + exp(u / 0.1)) the
+// corner blend of the stationary point's exit coordinate u (u < 0 inside the
+// edge). With T = gamma = 1 this is the plain Kouyoumjian-Pathak wedge.
 void utd_wedge(double n, double phi, double phip, double sinb, double k, double L, cd r0s, cd rns, cd r0h, cd rnh,
-               cd& ds, cd& dh) {
+               cd trunc, double gamma, cd& ds, cd& dh) {
     const double kL = k * L;
-    const cd t1 = utd_term(n, phi - phip, +1, kL);
-    const cd t2 = utd_term(n, phi - phip, -1, kL);
-    const cd t3 = utd_term(n, phi + phip, -1, kL);
-    const cd t4 = utd_term(n, phi + phip, +1, kL);
+    const double dw = 4.0 * std::sqrt(2.0 * PI / kL);
+    auto term = [&](double beta, int sgn, bool active, double bstar) {
+        const cd t = utd_term(n, beta, sgn, kL);
+        if (!active) return t * trunc;
+        const double delta = (beta - bstar) / dw;
+        const cd odd = 0.5 * (t - utd_term(n, 2.0 * bstar - beta, sgn, kL)) * std::exp(-delta * delta);
+        return (t - odd) * trunc + odd * gamma;
+    };
+    const double bi = phi - phip, br = phi + phip;
+    const bool up = bi >= 0.0;                                             // nearest incident boundary +pi
+    const bool face0 = std::fabs(br - PI) <= std::fabs(br - (2.0 * n - 1.0) * PI);  // nearest reflection boundary pi
+    const cd t1 = term(bi, +1, !up, -PI);
+    const cd t2 = term(bi, -1, up, PI);
+    const cd t3 = term(br, -1, face0, PI);
+    const cd t4 = term(br, +1, !face0, (2.0 * n - 1.0) * PI);
     const cd pre = -std::exp(cd(0.0, -PI / 4.0)) / (2.0 * n * std::sqrt(2.0 * PI * k) * std::max(sinb, 1.0e-6));
     ds = pre * (t1 + t2 + r0s * t3 + rns * t4);
     dh = pre * (t1 + t2 + r0h * t3 + rnh * t4);
@@ -378,7 +400,17 @@ void oracle_fresnel(double x, double* out2) { fresnel(x, out2[0], out2[1]); }
 
 void oracle_utd(double n, double phi, double phip, double sinb, double k, double L, const double* r8, double* out4) {
     cd ds, dh;
-    utd_wedge(n, phi, phip, sinb, k, L, cd(r8[0], r8[1]), cd(r8[2], r8[3]), cd(r8[4], r8[5]), cd(r8[6], r8[7]), ds, dh);
+    utd_wedge(n, phi, phip, sinb, k, L, cd(r8[0], r8[1]), cd(r8[2], r8[3]), cd(r8[4], r8[5]), cd(r8[6], r8[7]),
+              cd(1.0, 0.0), 1.0, ds, dh);
+    out4[0] = ds.real(); out4[1] = ds.imag(); out4[2] = dh.real(); out4[3] = dh.imag();
+}
+
+// Wedge coefficients with finite-edge factors T = t2[0] + j t2[1] and gamma.
+void oracle_utd_finite(double n, double phi, double phip, double sinb, double k, double L, const double* r8,
+                       const double* t2, double gamma, double* out4) {
+    cd ds, dh;
+    utd_wedge(n, phi, phip, sinb, k, L, cd(r8[0], r8[1]), cd(r8[2], r8[3]), cd(r8[4], r8[5]), cd(r8[6], r8[7]),
+              cd(t2[0], t2[1]), gamma, ds, dh);
     out4[0] = ds.real(); out4[1] = ds.imag(); out4[2] = dh.real(); out4[3] = dh.imag();
 }
 
@@ -606,8 +638,6 @@ int oracle_pair_paths(const Scene* sp, const Solve* qp, OraclePath* out, int cap
                              s.mat_thick[m0], f, r0s, r0h);
                 slab_fresnel(std::fabs(std::sin(npi - phi)), s.mat_eps[mn], s.mat_sigma[mn], s.mat_mu[mn],
                              s.mat_gain[mn], s.mat_thick[mn], f, rns, rnh);
-                cd ds, dh;
-                utd_wedge(n, phi, phip, sinb, k, L, r0s, rns, r0h, rnh, ds, dh);
                 const double amp = std::sqrt(sp_ / (so_ * (so_ + sp_))) / (2.0 * k * sp_);
                 const double kappa = rp * rp / (sp_ * sp_ * sp_) + rr * rr / (so_ * so_ * so_);
                 const double sg = std::sqrt(k * kappa / PI);
@@ -615,7 +645,11 @@ int oracle_pair_paths(const Scene* sp, const Solve* qp, OraclePath* out, int cap
                 fresnel(sg * (elen - ts), c1, s1);
                 fresnel(sg * (0.0 - ts), c0, s0);
                 const cd trunc = cd(0.5, 0.5) * std::conj(cd(c1 - c0, s1 - s0));
-                const cd prop = amp * trunc * std::exp(cd(0.0, -k * (so_ + sp_)));
+                const double u = sg * std::max(-ts, ts - elen);
+                const double gamma = 1.0 / (1.0 + std::exp(u / 0.1));
+                cd ds, dh;
+                utd_wedge(n, phi, phip, sinb, k, L, r0s, rns, r0h, rnh, trunc, gamma, ds, dh);
+                const cd prop = amp * std::exp(cd(0.0, -k * (so_ + sp_)));
                 const CV v = cv_axis(b0_o, -ds * eb * prop) + cv_axis(phi_o, -dh * epp * prop);
                 OraclePath p = blank(2, 1);
                 p.edge = e;

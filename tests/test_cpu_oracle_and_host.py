@@ -569,3 +569,141 @@ def test_heightfield_workload_shape():
     assert compiled.edge_count == 160 and np.all(np.abs(compiled.edges["dz"]) > 0.7)
     rx = compiled.endpoints.rx_position
     assert np.allclose(rx[:, 2], 21.5)
+
+
+def _sommerfeld_half_plane(k, rho, phi, phi_p, soft=True):
+    """Exact total field of a perfectly conducting half-plane (faces at phi = 0
+    and 2 pi) lit by the unit plane wave e^{jk rho cos(phi - phi')} (time
+    factor e^{jwt}): u = v(phi - phi') -+ v(phi + phi'),
+    v(psi) = e^{jk rho cos psi} e^{j pi/4} / sqrt(pi) int_{-inf}^{a} e^{-j t^2} dt,
+    a = sqrt(2 k rho) cos(psi / 2) (Sommerfeld 1896; Balanis, Advanced
+    Engineering Electromagnetics, ch. 13)."""
+    from scipy.special import fresnel
+
+    def v(psi):
+        a = math.sqrt(2.0 * k * rho) * math.cos(psi / 2.0)
+        s, c = fresnel(a * math.sqrt(2.0 / math.pi))
+        integral = math.sqrt(math.pi / 2.0) * (complex(c, -s) + complex(0.5, -0.5))
+        return cmath.exp(1j * k * rho * math.cos(psi)) * cmath.exp(1j * math.pi / 4) / math.sqrt(math.pi) * integral
+
+    return v(phi - phi_p) - v(phi + phi_p) if soft else v(phi - phi_p) + v(phi + phi_p)
+
+
+@pytest.mark.parametrize("soft", [True, False])
+def test_utd_half_plane_matches_sommerfeld(soft):
+    """Geometrical optics plus the oracle's UTD diffracted field of a PEC
+    half-plane (n = 2, plane-wave incidence, L = rho) against the exact
+    Sommerfeld solution, at every observation angle including both shadow
+    boundaries. A wrong sign, cot argument or transition-function convention
+    in any of the four terms gives an O(1) error."""
+    k = 2 * math.pi / 0.1
+    rho = 3.0
+    phi_p = 1.1
+    r = -1.0 if soft else 1.0
+    worst = 0.0
+    for phi in np.linspace(0.05, 2 * math.pi - 0.05, 181):
+        go = 0.0
+        if abs(phi - phi_p) < math.pi:
+            go += cmath.exp(1j * k * rho * math.cos(phi - phi_p))
+        if phi + phi_p < math.pi:
+            go += r * cmath.exp(1j * k * rho * math.cos(phi + phi_p))
+        ds, dh = oracle.utd(2.0, float(phi), phi_p, 1.0, k, rho)
+        d = ds if soft else dh
+        total = go + d * cmath.exp(-1j * k * rho) / math.sqrt(rho)
+        exact = _sommerfeld_half_plane(k, rho, float(phi), phi_p, soft)
+        worst = max(worst, abs(total - exact))
+    assert worst < 1e-8, worst
+
+
+def _finite(n, phi, phi_p, k, L, trunc, gamma):
+    ds, _ = oracle.utd_finite(n, phi, phi_p, 1.0, k, L, trunc, gamma)
+    return ds
+
+
+def test_utd_finite_edge_assembly_keeps_go_compensation():
+    """E1d assembly (reference docs/dev/audit/utd-continuity-fix-design.md:
+    160-205). With the corner blend gamma = 1 the coefficient's jump across
+    the incident and both reflection shadow boundaries is the infinite
+    wedge's, whatever the truncation factor T (the step-carrying odd part is
+    not multiplied by T); with gamma = 0 the jump vanishes; outside the
+    locality window the coefficient is T times the infinite wedge's. The
+    E1c form (T times the whole coefficient) fails the first check."""
+    k, L, n, phi_p = 2 * math.pi / 0.1, 4.0, 1.5, 0.7
+    trunc = 1.27 * cmath.exp(1j * math.radians(15.0))
+    d = 1e-9
+    for boundary in (math.pi + phi_p, math.pi - phi_p, (2 * n - 1) * math.pi - phi_p):
+        plain = _finite(n, boundary + d, phi_p, k, L, 1.0, 1.0) - _finite(n, boundary - d, phi_p, k, L, 1.0, 1.0)
+        kept = _finite(n, boundary + d, phi_p, k, L, trunc, 1.0) - _finite(n, boundary - d, phi_p, k, L, trunc, 1.0)
+        gone = _finite(n, boundary + d, phi_p, k, L, trunc, 0.0) - _finite(n, boundary - d, phi_p, k, L, trunc, 0.0)
+        assert abs(plain) > 1e-2
+        assert abs(kept - plain) < 1e-6 * abs(plain), (boundary, kept, plain)
+        assert abs(gone) < 1e-6 * abs(plain), (boundary, gone)
+    far = 2.0  # radians from every boundary: w = exp(-(2 / 0.5)^2 ...) underflows the window
+    for phi in (phi_p + far, 3.0 * math.pi / 2.0 + 0.4):
+        if min(abs(phi - b) for b in (math.pi + phi_p, math.pi - phi_p, 2 * math.pi - phi_p)) < 1.0:
+            continue
+        base = _finite(n, phi, phi_p, k, L, 1.0, 1.0)
+        assert abs(_finite(n, phi, phi_p, k, L, trunc, 0.3) - trunc * base) < 1e-9 * abs(base)
+
+
+# reference tests/deterministic/test_diffraction_single_wedge.py:41-66 (edge
+# ids and lengths of the PEC wedge at 3 GHz; pure geometry)
+WEDGE_EDGE_IDS = (0, 1, 2, 5)
+WEDGE_LENGTHS = (3.650281668, 4.744879723, 3.768759727, 4.046976566)
+
+
+def test_wedge_edge_records_follow_reference_numbering():
+    """Public edge ids of the two-structure wedge: per-structure vertex-pair
+    records, the second copy of the shared edge (3) merged into 0. The
+    vertical-only policy keeps 0 and the slanted outline edges 2 and 5, which
+    is the reference's three rows (test_diffraction_single_wedge.py:88-140)."""
+    from witwin.channel import workloads
+    from witwin.channel.scene.compiler import compile as compile_scene
+
+    c = compile_scene(workloads.wedge_diffraction_scene(), reference_frequency_hz=3.0e9)
+    assert list(c.edge_public_id) == [0, 1, 2, 4, 5]
+    e = c.edges
+    dz = np.abs(np.asarray(e["dz"]))
+    assert dz[0] == pytest.approx(1.0) and float(e["n"][0]) == pytest.approx(1.5)  # shared wedge, 270 deg exterior
+    assert dz[1] == pytest.approx(0.0, abs=1e-7) and dz[3] == pytest.approx(0.0, abs=1e-7)  # horizontal 1 and 4
+    assert dz[2] == pytest.approx(3.0 / math.sqrt(13.0)) and dz[4] == pytest.approx(3.0 / math.sqrt(13.0))
+    v = compile_scene(workloads.wedge_diffraction_scene("vertical_only"), reference_frequency_hz=3.0e9)
+    assert list(v.edge_public_id) == [0, 2, 5]
+
+
+def test_oracle_wedge_path_lengths_match_reference_golden():
+    """The oracle's diffraction rows of the reference wedge fixture: one row per
+    public edge id 0, 1, 2, 5 with the reference path lengths (rtol 1e-5)."""
+    from witwin.channel import workloads
+    from witwin.channel.scene.compiler import compile as compile_scene
+
+    c = compile_scene(workloads.wedge_diffraction_scene(), reference_frequency_hz=3.0e9)
+    orc = oracle.Oracle(c)
+    rows = orc.pair_paths(tx=(0.0, -1.0, 0.5), rx=(3.0, 1.0, 0.5), frequency=3.0e9, components=4, max_depth=1,
+                          chains=np.zeros((0, 5), np.int32), chain_depth=np.zeros(0, np.int32))
+    lengths = {int(c.edge_public_id[p.edge]): p.length for p in rows}
+    for eid, expect in zip(WEDGE_EDGE_IDS, WEDGE_LENGTHS):
+        assert lengths[eid] == pytest.approx(expect, rel=1e-5)
+
+
+def test_unsupported_ad_modes_raise_before_any_launch():
+    """ad_mode='jvp' is validated by Config (the reference accepts it) but this
+    build has no forward-mode radiomap: solve raises instead of silently
+    returning an undifferentiated forward; vjp with max_paths truncation
+    raises too (the adjoint differentiates every path of a cell)."""
+    from witwin.channel import workloads
+    from witwin.channel.deterministic import Config, solve
+
+    scene = workloads.two_ray_scene(shape=(4, 4))
+    with pytest.raises(NotImplementedError, match="jvp"):
+        solve(scene, Config(ad_mode="jvp"), reference_frequency_hz=3.5e9)
+    with pytest.raises(NotImplementedError, match="max_paths"):
+        solve(scene, Config(ad_mode="vjp", max_paths=3), reference_frequency_hz=3.5e9)
+
+
+def test_build_info_names_the_hip_backend():
+    from witwin.channel import build_info
+
+    info = build_info()
+    assert info["backend"] == "hip" and info["uses_hip_native"]
+    assert not info["uses_rayd_native"] and not info["cuda_available"]

@@ -51,13 +51,6 @@ struct ChainSet {
     const double* images;   // [K, kMaxDepth, 3]
 };
 
-struct SolveParams {
-    float frequency_hz;
-    int components;   // ComponentBits
-    int max_depth;    // reflection / transmission chain cap (<= kMaxDepth)
-    int coherent;
-};
-
 struct PathRecord {
     int component;
     int depth;
@@ -294,8 +287,8 @@ __device__ inline bool diffraction_geometry(const Edge& e, const V3<float>& tx, 
 
 // Diffracted complex 3-vector at the receiver (unit excitation), K-P dyadic
 // E^d = [-D_s b0 b0' - D_h phi phi'] . E^i(Q) sqrt(s'/(s (s+s'))) e^{-jks},
-// E^i(Q) = (p_tx - (p_tx.s')s') e^{-jks'} / (2ks'), times the finite-edge
-// truncation factor. Face reflection coefficients (Luebbers) are the slab
+// E^i(Q) = (p_tx - (p_tx.s')s') e^{-jks'} / (2ks'), with the finite-edge
+// truncation inside the coefficients (utd_wedge). Face reflection coefficients (Luebbers) are the slab
 // coefficients of each face at its grazing angle: soft <- TE, hard <- TM.
 __device__ inline C3<float> diffraction_field(const SceneView& s, const Edge& e, const DiffGeometry& g,
                                               const V3<float>& tx, const V3<float>& rx, const V3<float>& tx_pol,
@@ -325,20 +318,21 @@ __device__ inline C3<float> diffraction_field(const SceneView& s, const Edge& e,
                         s.mat_thickness[m0], freq, r0_s, r0_h);
     slab_fresnel<float>((float)fabs(sin(npi - g.phi)), s.mat_eps[mn], s.mat_sigma[mn], s.mat_mu[mn],
                         s.mat_gain[mn], s.mat_thickness[mn], freq, rn_s, rn_h);
-    const UtdCoefficients D = utd_wedge((double)e.n, g.phi, g.phi_p, sin_b0, k_d, L, r0_s, rn_s, r0_h, rn_h);
-    // spreading and incident amplitude: sqrt(s'/(s(s+s'))) / (2 k s')
-    const double amp = sqrt(g.sp / (g.s * (g.s + g.sp))) / (2.0 * k_d * g.sp);
-    // finite-edge truncation factor
+    // finite-edge truncation factor and corner blend, folded into the terms
     const double kappa = g.rho_p * g.rho_p / (g.sp * g.sp * g.sp) + g.rho * g.rho / (g.s * g.s * g.s);
     const double sigma = sqrt(k_d * kappa / kPi);
     double f1r, f1i, f0r, f0i;
     fresnel_complex(sigma * ((double)e.length - g.tstar), f1r, f1i);
     fresnel_complex(sigma * (0.0 - g.tstar), f0r, f0i);
     const double dr = f1r - f0r, di = -(f1i - f0i);  // conj(difference)
-    const double tr = 0.5 * dr - 0.5 * di, ti = 0.5 * di + 0.5 * dr;
-    // total propagation scalar: amp * T * e^{-jk(s+s')}
+    const double u = sigma * fmax(-g.tstar, g.tstar - (double)e.length);
+    const FiniteEdge fe{0.5 * dr - 0.5 * di, 0.5 * di + 0.5 * dr, 1.0 / (1.0 + exp(u * 10.0))};
+    const UtdCoefficients D = utd_wedge((double)e.n, g.phi, g.phi_p, sin_b0, k_d, L, r0_s, rn_s, r0_h, rn_h, fe);
+    // spreading and incident amplitude: sqrt(s'/(s(s+s'))) / (2 k s')
+    const double amp = sqrt(g.sp / (g.s * (g.s + g.sp))) / (2.0 * k_d * g.sp);
+    // total propagation scalar: amp * e^{-jk(s+s')}
     const float ph = neg_kd_phase(k_d, g.s + g.sp);
-    const Cx<float> prop = Cx<float>{(float)(amp * tr), (float)(amp * ti)} * cexpj(ph);
+    const Cx<float> prop = Cx<float>{(float)amp, 0.0f} * cexpj(ph);
     const Cx<float> ds{D.ds_re, D.ds_im}, dh{D.dh_re, D.dh_im};
     const Cx<float> cb_o = scale(ds, -e_b) * prop;
     const Cx<float> cp_o = scale(dh, -e_p) * prop;

@@ -130,8 +130,8 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
     const int rx_index = q.rx_shared ? cell - q.cell_offset[t] : cell;
     const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
     const V3<float> rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
-    const float freq = q.frequency_hz;
-    const double k_d = (double)freq * kTwoPi / kC0;
+    const float freq = (float)q.frequency_hz;
+    const double k_d = q.frequency_hz * kTwoPi / kC0;
     const float k = (float)k_d;
     const int comps = q.components;
     const int max_depth = q.max_depth;
@@ -706,8 +706,8 @@ diff_record_kernel(SceneView s, wc_solve q, const unsigned* __restrict__ masks, 
     if (!ctrl[2]) return;
     const int total = ctrl[0];
     const int cells = q.total_cells;
-    const float freq = q.frequency_hz;
-    const double k_d = (double)freq * kTwoPi / kC0;
+    const float freq = (float)q.frequency_hz;
+    const double k_d = q.frequency_hz * kTwoPi / kC0;
     while (true) {
         int chunk = 0;
         if (lane == 0) chunk = atomicAdd(ctrl + 1, 1);
@@ -1671,7 +1671,7 @@ static int check_solve(const wc_solve* q) {
     if (!q) return wc_fail("solve is null");
     if (q->tx_count <= 0 && q->total_cells > 0) return wc_fail("solve has cells but no transmitters");
     if (q->max_depth < 0 || q->max_depth > kMaxDepth) return wc_fail("max_depth must be in [0, 5]");
-    if (!(q->frequency_hz > 0.0f)) return wc_fail("frequency_hz must be positive");
+    if (!(q->frequency_hz > 0.0)) return wc_fail("frequency_hz must be positive");
     if (q->total_cells > 0 && (!q->tx_pos || !q->tx_pol || !q->tx_power || !q->rx_pos || !q->rx_pol ||
                                !q->cell_offset))
         return wc_fail("solve endpoint pointers are null");

@@ -186,31 +186,74 @@ template <typename R> struct UtdTangent;
 template <int N> struct UtdTangent<Dual<N>> {
     using D = Dual<N>;
     __device__ static Cx<D> term(double n, const D& beta, int sgn, const D& kL) {
+        return term_at(n, (double)beta.v, beta, sgn, kL);
+    }
+    // the term at the double argument beta_v with the tangents of beta_t
+    __device__ static Cx<D> term_at(double n, double beta_v, const D& beta_t, int sgn, const D& kL) {
         double re, im, br, bi, kr, ki;
-        utd_term_partials(n, (double)beta.v, sgn, (double)kL.v, re, im, br, bi, kr, ki);
+        utd_term_partials(n, beta_v, sgn, (double)kL.v, re, im, br, bi, kr, ki);
         Cx<D> out;
         out.re.v = (float)re;
         out.im.v = (float)im;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            out.re.d[i] = (float)(br * beta.d[i] + kr * kL.d[i]);
-            out.im.d[i] = (float)(bi * beta.d[i] + ki * kL.d[i]);
+            out.re.d[i] = (float)(br * beta_t.d[i] + kr * kL.d[i]);
+            out.im.d[i] = (float)(bi * beta_t.d[i] + ki * kL.d[i]);
         }
         return out;
     }
     // the same term with the wedge index n carrying tangents too
     __device__ static Cx<D> term_n(const D& n, const D& beta, int sgn, const D& kL) {
+        return term_n_at(n, (double)beta.v, beta, sgn, kL);
+    }
+    __device__ static Cx<D> term_n_at(const D& n, double beta_v, const D& beta_t, int sgn, const D& kL) {
         double re, im, br, bi, kr, ki, nr, ni;
-        utd_term_partials((double)n.v, (double)beta.v, sgn, (double)kL.v, re, im, br, bi, kr, ki, &nr, &ni);
+        utd_term_partials((double)n.v, beta_v, sgn, (double)kL.v, re, im, br, bi, kr, ki, &nr, &ni);
         Cx<D> out;
         out.re.v = (float)re;
         out.im.v = (float)im;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            out.re.d[i] = (float)(br * beta.d[i] + kr * kL.d[i] + nr * n.d[i]);
-            out.im.d[i] = (float)(bi * beta.d[i] + ki * kL.d[i] + ni * n.d[i]);
+            out.re.d[i] = (float)(br * beta_t.d[i] + kr * kL.d[i] + nr * n.d[i]);
+            out.im.d[i] = (float)(bi * beta_t.d[i] + ki * kL.d[i] + ni * n.d[i]);
         }
         return out;
+    }
+    // E1d assembly of one term t (utd_term_e1d): t T for an inactive term;
+    // for the active one (t - odd_w) T + odd_w gamma with the mirrored term
+    // evaluated at 2 b* - beta in double. which: 0 -> b* = -pi, 1 -> +pi,
+    // 2 -> pi, 3 -> (2n - 1) pi, whose tangent 2 pi dn is carried when
+    // n_tangent is set.
+    __device__ static Cx<D> mend(const Cx<D>& t, bool active, const D& n, bool n_tangent, const D& beta, int sgn,
+                                 int which, const D& kL, const Cx<D>& trunc, const D& gamma) {
+        if (!active) return t * trunc;
+        const double nv = (double)n.v;
+        const double bstar = which == 0 ? -kPi : (which == 3 ? (2.0 * nv - 1.0) * kPi : kPi);
+        D delta, mirror_t;
+        delta.v = (float)((double)beta.v - bstar);
+        mirror_t.v = 0.0f;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            const float db = (which == 3 && n_tangent) ? (float)kTwoPi * n.d[i] : 0.0f;
+            delta.d[i] = beta.d[i] - db;
+            mirror_t.d[i] = 2.0f * db - beta.d[i];
+        }
+        const D dw = w_sqrt(make_dual<N>((float)kTwoPi) / kL) * 4.0f;
+        const D x = delta / dw;
+        const D w = w_exp(-(x * x));
+        if (!(w.v > 1.0e-20f)) return t * trunc;
+        const double mirror_v = 2.0 * bstar - (double)beta.v;
+        const Cx<D> tm = n_tangent ? term_n_at(n, mirror_v, mirror_t, sgn, kL)
+                                   : term_at(nv, mirror_v, mirror_t, sgn, kL);
+        const Cx<D> odd = scale(t - tm, w * 0.5f);
+        return (t - odd) * trunc + scale(odd, gamma);
+    }
+    // corner blend gamma = 1 / (1 + exp(u / 0.1)), u = sg max(-t*, t* - l)
+    __device__ static D corner_blend(const D& sg, const D& tstar, const D& length) {
+        const D a = -tstar, b = tstar - length;
+        const D u = sg * (a.v >= b.v ? a : b);
+        if (u.v * 10.0f > 80.0f) return make_dual<N>(0.0f);
+        return make_dual<N>(1.0f) / (w_exp(u * 10.0f) + 1.0f);
     }
     // Fr(x) = C(x) + jS(x), dFr/dx = e^{j pi x^2 / 2}
     __device__ static Cx<D> fresnel(const D& x) {
@@ -296,16 +339,7 @@ __device__ inline C3<R> diffraction_tangent(const SceneView& s, const Edge& e, c
                     lift<R>(s.mat_thickness[mn]), freq, rn_s, rn_h);
     const double nn = (double)e.n;
     const R kL = k * L;
-    const Cx<R> t1 = UtdTangent<R>::term(nn, phi - phi_p, +1, kL);
-    const Cx<R> t2 = UtdTangent<R>::term(nn, phi - phi_p, -1, kL);
-    const Cx<R> t3 = UtdTangent<R>::term(nn, phi + phi_p, -1, kL);
-    const Cx<R> t4 = UtdTangent<R>::term(nn, phi + phi_p, +1, kL);
-    // prefactor -e^{-j pi/4} / (2 n sqrt(2 pi k) sin b0)
-    const R pre_mag = lift<R>(1.0f) / (w_sqrt(k * (float)kTwoPi) * sin_b0 * (2.0f * e.n));
-    const Cx<R> pre = scale(Cx<R>{lift<R>(-(float)cos(-0.25 * kPi)), lift<R>(-(float)sin(-0.25 * kPi))}, pre_mag);
-    const Cx<R> ds = pre * (t1 + t2 + r0_s * t3 + rn_s * t4);
-    const Cx<R> dh = pre * (t1 + t2 + r0_h * t3 + rn_h * t4);
-    const R amp = w_sqrt(sp / (so * (so + sp))) / (k * sp * 2.0f);
+    // finite-edge truncation factor and corner blend (E1d, see utd_term_e1d)
     const R kappa = rho_p * rho_p / (sp * sp * sp) + rho * rho / (so * so * so);
     const R sg = w_sqrt(k * kappa * (float)(1.0 / kPi));
     const Cx<R> f1 = UtdTangent<R>::fresnel(sg * (lift<R>(e.length) - tstar));
@@ -313,8 +347,25 @@ __device__ inline C3<R> diffraction_tangent(const SceneView& s, const Edge& e, c
     const Cx<R> dfr = f1 - f0;
     const Cx<R> dconj{dfr.re, -dfr.im};
     const Cx<R> trunc = cxc<R>(0.5f, 0.5f) * dconj;
+    const R gamma = UtdTangent<R>::corner_blend(sg, tstar, lift<R>(e.length));
+    const UtdBoundaries bd = utd_boundaries(nn, (double)val(phi), (double)val(phi_p));
+    const R n_r = lift<R>(e.n);
+    const Cx<R> t1 = UtdTangent<R>::mend(UtdTangent<R>::term(nn, phi - phi_p, +1, kL), !bd.incident_up, n_r, false,
+                                         phi - phi_p, +1, 0, kL, trunc, gamma);
+    const Cx<R> t2 = UtdTangent<R>::mend(UtdTangent<R>::term(nn, phi - phi_p, -1, kL), bd.incident_up, n_r, false,
+                                         phi - phi_p, -1, 1, kL, trunc, gamma);
+    const Cx<R> t3 = UtdTangent<R>::mend(UtdTangent<R>::term(nn, phi + phi_p, -1, kL), bd.reflect_0, n_r, false,
+                                         phi + phi_p, -1, 2, kL, trunc, gamma);
+    const Cx<R> t4 = UtdTangent<R>::mend(UtdTangent<R>::term(nn, phi + phi_p, +1, kL), !bd.reflect_0, n_r, false,
+                                         phi + phi_p, +1, 3, kL, trunc, gamma);
+    // prefactor -e^{-j pi/4} / (2 n sqrt(2 pi k) sin b0)
+    const R pre_mag = lift<R>(1.0f) / (w_sqrt(k * (float)kTwoPi) * sin_b0 * (2.0f * e.n));
+    const Cx<R> pre = scale(Cx<R>{lift<R>(-(float)cos(-0.25 * kPi)), lift<R>(-(float)sin(-0.25 * kPi))}, pre_mag);
+    const Cx<R> ds = pre * (t1 + t2 + r0_s * t3 + rn_s * t4);
+    const Cx<R> dh = pre * (t1 + t2 + r0_h * t3 + rn_h * t4);
+    const R amp = w_sqrt(sp / (so * (so + sp))) / (k * sp * 2.0f);
     const R ph = PhaseOf<R>::make(k, so + sp, k_d, length_d);
-    const Cx<R> prop = scale(trunc, amp) * cexpj(ph);
+    const Cx<R> prop = scale(cexpj(ph), amp);
     arrival = sov;
     if (faces) {
         const Cx<float> pp = cval(pre) * cval(prop);
@@ -474,15 +525,6 @@ __device__ inline C3<R> diffraction_tangent_edge(const SceneView& s, const Edge&
     slab_fresnel<R>(cosn, lift<R>(s.mat_eps[mn]), lift<R>(s.mat_sigma[mn]), lift<R>(s.mat_mu[mn]),
                     lift<R>(s.mat_gain[mn]), lift<R>(s.mat_thickness[mn]), fr, rn_s, rn_h);
     const R kL = k * L;
-    const Cx<R> t1 = UtdTangent<R>::term_n(g.n, phi - phi_p, +1, kL);
-    const Cx<R> t2 = UtdTangent<R>::term_n(g.n, phi - phi_p, -1, kL);
-    const Cx<R> t3 = UtdTangent<R>::term_n(g.n, phi + phi_p, -1, kL);
-    const Cx<R> t4 = UtdTangent<R>::term_n(g.n, phi + phi_p, +1, kL);
-    const R pre_mag = lift<R>(1.0f) / (w_sqrt(k * (float)kTwoPi) * sin_b0 * (g.n * 2.0f));
-    const Cx<R> pre = scale(Cx<R>{lift<R>(-(float)cos(-0.25 * kPi)), lift<R>(-(float)sin(-0.25 * kPi))}, pre_mag);
-    const Cx<R> ds = pre * (t1 + t2 + r0_s * t3 + rn_s * t4);
-    const Cx<R> dh = pre * (t1 + t2 + r0_h * t3 + rn_h * t4);
-    const R amp = w_sqrt(sp / (so * (so + sp))) / (k * sp * 2.0f);
     const R kappa = rho_p * rho_p / (sp * sp * sp) + rho * rho / (so * so * so);
     const R sg = w_sqrt(k * kappa * (float)(1.0 / kPi));
     const Cx<R> f1 = UtdTangent<R>::fresnel(sg * (g.length - tstar));
@@ -490,8 +532,23 @@ __device__ inline C3<R> diffraction_tangent_edge(const SceneView& s, const Edge&
     const Cx<R> dfr = f1 - f0;
     const Cx<R> dconj{dfr.re, -dfr.im};
     const Cx<R> trunc = cxc<R>(0.5f, 0.5f) * dconj;
+    const R gamma = UtdTangent<R>::corner_blend(sg, tstar, g.length);
+    const UtdBoundaries bd = utd_boundaries((double)val(g.n), (double)val(phi), (double)val(phi_p));
+    const Cx<R> t1 = UtdTangent<R>::mend(UtdTangent<R>::term_n(g.n, phi - phi_p, +1, kL), !bd.incident_up, g.n, true,
+                                         phi - phi_p, +1, 0, kL, trunc, gamma);
+    const Cx<R> t2 = UtdTangent<R>::mend(UtdTangent<R>::term_n(g.n, phi - phi_p, -1, kL), bd.incident_up, g.n, true,
+                                         phi - phi_p, -1, 1, kL, trunc, gamma);
+    const Cx<R> t3 = UtdTangent<R>::mend(UtdTangent<R>::term_n(g.n, phi + phi_p, -1, kL), bd.reflect_0, g.n, true,
+                                         phi + phi_p, -1, 2, kL, trunc, gamma);
+    const Cx<R> t4 = UtdTangent<R>::mend(UtdTangent<R>::term_n(g.n, phi + phi_p, +1, kL), !bd.reflect_0, g.n, true,
+                                         phi + phi_p, +1, 3, kL, trunc, gamma);
+    const R pre_mag = lift<R>(1.0f) / (w_sqrt(k * (float)kTwoPi) * sin_b0 * (g.n * 2.0f));
+    const Cx<R> pre = scale(Cx<R>{lift<R>(-(float)cos(-0.25 * kPi)), lift<R>(-(float)sin(-0.25 * kPi))}, pre_mag);
+    const Cx<R> ds = pre * (t1 + t2 + r0_s * t3 + rn_s * t4);
+    const Cx<R> dh = pre * (t1 + t2 + r0_h * t3 + rn_h * t4);
+    const R amp = w_sqrt(sp / (so * (so + sp))) / (k * sp * 2.0f);
     const R ph = PhaseOf<R>::make(k, so + sp, k_d, length_d);
-    const Cx<R> prop = scale(trunc, amp) * cexpj(ph);
+    const Cx<R> prop = scale(cexpj(ph), amp);
     arrival = sov;
     return c3_axis(b0_o, (ds * prop) * Cx<R>{-e_b, lift<R>(0.f)}) +
            c3_axis(phi_o, (dh * prop) * Cx<R>{-e_p, lift<R>(0.f)});

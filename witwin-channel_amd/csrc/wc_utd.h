@@ -4,7 +4,8 @@
 // faces and the finite-edge Fresnel truncation factor
 //   T = (0.5 + 0.5j) * conj(Fr(sigma*(t_max - t*)) - Fr(sigma*(t_min - t*))),
 //   sigma = sqrt(k*kappa/pi), kappa = rho'^2/R'^3 + rho^2/R^3
-// (reference docs/dev/audit/utd-continuity-fix-design.md:91-140).
+// (reference docs/dev/audit/utd-continuity-fix-design.md:91-140), applied
+// inside the four-term assembly with the odd-part corner mend (E1d, :160-205).
 // The transition function is evaluated in double precision: near a shadow
 // boundary its argument kL*a goes to zero and the float cot*F product loses
 // the digits that carry the geometrical-optics compensation.
@@ -142,19 +143,73 @@ struct UtdCoefficients {
     float ds_re, ds_im, dh_re, dh_im;
 };
 
+// Finite-edge parameters of the E1d assembly: the truncation factor
+// T = (0.5 + 0.5j) conj(Fr(sigma (l - t*)) - Fr(sigma (0 - t*))) and the corner
+// blend gamma = 1 / (1 + exp(u / 0.1)) of the stationary point's exit
+// coordinate u = sigma max(-t*, t* - l) (u < 0 inside the edge).
+struct FiniteEdge {
+    double t_re, t_im, gamma;
+};
+
+// Nearest geometrical-optics boundaries of the two cot*F groups: the incident
+// group beta = phi - phi' has boundaries -pi (term 1) and +pi (term 2), the
+// reflection group beta = phi + phi' has pi (term 3, face 0) and (2n - 1) pi
+// (term 4, face n). Only the term singular at the nearer boundary of its
+// group is mended.
+struct UtdBoundaries {
+    bool incident_up;  // term 2 (+pi) active, else term 1 (-pi)
+    bool reflect_0;    // term 3 (pi) active, else term 4 ((2n-1) pi)
+};
+__device__ inline UtdBoundaries utd_boundaries(double n, double phi, double phi_p) {
+    const double br = phi + phi_p;
+    return {phi - phi_p >= 0.0, fabs(br - kPi) <= fabs(br - (2.0 * n - 1.0) * kPi)};
+}
+
+// One cot*F term in the E1d assembly (reference
+// docs/dev/audit/utd-continuity-fix-design.md:160-205): the active term is
+// split about its boundary b* into odd_w = (t(beta) - t(2 b* - beta)) / 2 * w,
+// w = exp(-((beta - b*) / delta_w)^2), delta_w = 4 sqrt(2 pi / kL), and enters
+// as (t - odd_w) T + odd_w gamma, so the step-carrying odd part keeps factor
+// gamma (1 inside the edge) and only the smooth background is truncated; an
+// inactive term enters as t T. Where w < 1e-20 the mirrored evaluation is
+// skipped (its contribution is below double rounding).
+__device__ inline void utd_term_e1d(double n, double beta, int sgn, double kL, bool active, double bstar,
+                                    const FiniteEdge& fe, double& re, double& im) {
+    double tr, ti;
+    utd_term(n, beta, sgn, kL, tr, ti);
+    double orr = 0.0, oi = 0.0;
+    if (active) {
+        const double dw = 4.0 * sqrt(2.0 * kPi / kL);
+        const double d = (beta - bstar) / dw;
+        const double w = exp(-d * d);
+        if (w > 1.0e-20) {
+            double mr, mi;
+            utd_term(n, 2.0 * bstar - beta, sgn, kL, mr, mi);
+            orr = 0.5 * (tr - mr) * w;
+            oi = 0.5 * (ti - mi) * w;
+        }
+    }
+    const double ar = tr - orr, ai = ti - oi;
+    re = ar * fe.t_re - ai * fe.t_im + orr * fe.gamma;
+    im = ar * fe.t_im + ai * fe.t_re + oi * fe.gamma;
+}
+
 // D_s, D_h of a wedge with exterior angle n*pi at observation angle phi and
 // incidence angle phi_p (both measured from the 0-face into the exterior),
-// skew angle beta0 (sin_beta0 > 0), spherical-wave distance parameter L.
+// skew angle beta0 (sin_beta0 > 0), spherical-wave distance parameter L, with
+// the finite-edge factors folded into the four terms (utd_term_e1d).
 // r0_s/r0_h, rn_s/rn_h are the 0-face and n-face reflection coefficients for
 // the soft (E parallel to the edge) and hard polarizations (PEC: -1 / +1).
 __device__ inline UtdCoefficients utd_wedge(double n, double phi, double phi_p, double sin_beta0, double k, double L,
-                                Cx<float> r0_s, Cx<float> rn_s, Cx<float> r0_h, Cx<float> rn_h) {
+                                Cx<float> r0_s, Cx<float> rn_s, Cx<float> r0_h, Cx<float> rn_h,
+                                const FiniteEdge& fe) {
     const double kL = k * L;
+    const UtdBoundaries b = utd_boundaries(n, phi, phi_p);
     double t1r, t1i, t2r, t2i, t3r, t3i, t4r, t4i;
-    utd_term(n, phi - phi_p, +1, kL, t1r, t1i);   // cot((pi + (phi-phi'))/2n) F(kL a+(phi-phi'))
-    utd_term(n, phi - phi_p, -1, kL, t2r, t2i);   // cot((pi - (phi-phi'))/2n) F(kL a-(phi-phi'))
-    utd_term(n, phi + phi_p, -1, kL, t3r, t3i);   // 0-face reflection term
-    utd_term(n, phi + phi_p, +1, kL, t4r, t4i);   // n-face reflection term
+    utd_term_e1d(n, phi - phi_p, +1, kL, !b.incident_up, -kPi, fe, t1r, t1i);  // cot((pi + b-)/2n) F(kL a+(b-))
+    utd_term_e1d(n, phi - phi_p, -1, kL, b.incident_up, kPi, fe, t2r, t2i);    // cot((pi - b-)/2n) F(kL a-(b-))
+    utd_term_e1d(n, phi + phi_p, -1, kL, b.reflect_0, kPi, fe, t3r, t3i);      // 0-face reflection term
+    utd_term_e1d(n, phi + phi_p, +1, kL, !b.reflect_0, (2.0 * n - 1.0) * kPi, fe, t4r, t4i);  // n-face term
     // prefactor -e^{-j pi/4} / (2 n sqrt(2 pi k) sin beta0)
     const double scale = 1.0 / (2.0 * n * sqrt(2.0 * kPi * k) * fmax(sin_beta0, 1.0e-6));
     const double p_re = -cos(-0.25 * kPi) * scale;
@@ -165,10 +220,10 @@ __device__ inline UtdCoefficients utd_wedge(double n, double phi, double phi_p, 
         o_re = p_re * s_re - p_im * s_im;
         o_im = p_re * s_im + p_im * s_re;
     };
-    double a, b, c, d;
-    assemble(r0_s, rn_s, a, b);
+    double a, bb, c, d;
+    assemble(r0_s, rn_s, a, bb);
     assemble(r0_h, rn_h, c, d);
-    return {(float)a, (float)b, (float)c, (float)d};
+    return {(float)a, (float)bb, (float)c, (float)d};
 }
 
 }  // namespace wc

@@ -250,8 +250,8 @@ __device__ __forceinline__ void setup_path(const SceneView& s, const wc_solve& q
     const V3<float> rx = ld3(q.rx_pos + 3 * rx_index);
     ps.rx = rx;
     ps.rx_pol = ld3(q.rx_pol + 3 * (q.rx_pol_stride ? rx_index : 0));
-    ps.freq = q.frequency_hz;
-    ps.k_d = (double)ps.freq * kTwoPi / kC0;
+    ps.freq = (float)q.frequency_hz;
+    ps.k_d = q.frequency_hz * kTwoPi / kC0;
     const int comp = desc >> 24;
     ps.comp = comp;
     int cand = desc & 0xFFFFFF;

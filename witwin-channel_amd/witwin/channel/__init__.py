@@ -18,9 +18,12 @@ def build_info() -> dict:
         "target": lib.wc_build_target().decode(),
         "abi_version": int(lib.wc_abi_version()),
         "library": str(_native.library_path()),
-        "uses_rayd_native": True,
+        # the reference's RayD / CUDA / OptiX backends are not used: every
+        # solver runs on the HIP library above
+        "uses_rayd_native": False,
         "uses_path_native": True,
-        "cuda_available": True,
+        "uses_hip_native": True,
+        "cuda_available": False,
         "optix_available": False,
     }
 

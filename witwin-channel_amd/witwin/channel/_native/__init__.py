@@ -25,6 +25,7 @@ c_float = ctypes.c_float
 c_size_t = ctypes.c_size_t
 c_void_p = ctypes.c_void_p
 c_char_p = ctypes.c_char_p
+c_double = ctypes.c_double
 c_double_p = ctypes.POINTER(ctypes.c_double)
 
 
@@ -58,7 +59,7 @@ class WcSolve(ctypes.Structure):
         ("rx_pos", c_void_p), ("rx_pol", c_void_p), ("rx_pol_stride", c_int), ("cell_offset", c_void_p),
         ("rx_shared", c_int), ("total_cells", c_int),
         ("chain_offset", c_void_p), ("chain_groups", c_void_p), ("chain_images", c_void_p),
-        ("frequency_hz", c_float), ("components", c_int), ("max_depth", c_int), ("coherent", c_int),
+        ("frequency_hz", c_double), ("components", c_int), ("max_depth", c_int), ("coherent", c_int),
         ("lanes_per_cell", c_int), ("cell_order", c_void_p), ("max_tx_chains", c_int),
     ]
 

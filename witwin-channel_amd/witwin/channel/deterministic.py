@@ -195,6 +195,44 @@ def build_path_table(rows: dict[str, Any], *, include_fields: bool = True) -> Pa
     )
 
 
+def accumulate_rows(rows: dict[str, Any], num_tx: int, num_rx: int, coherent: bool,
+                    tx_power: torch.Tensor) -> tuple[torch.Tensor, ...]:
+    """Per-cell sums of explicit rows (wc_accumulate_flat: slot map and
+    coherent / incoherent finalize of reference deterministic.cu:829-970) plus
+    the grid diffraction override's vector sum over the rows' field vectors
+    (reference deterministic.py:870-882). Returns power_total [T*R],
+    field_total [T*R] c64, component_power [4, T*R] (los, reflection,
+    diffraction, transmission), component_field [4, T*R] c64 and the vector
+    diffraction power [T*R]."""
+    device = rows["tx_id"].device
+    count = int(rows["tx_id"].shape[0])
+    cells = num_tx * num_rx
+    cp = torch.zeros((6, cells), dtype=torch.float32, device=device)
+    cf = torch.zeros((6, cells), dtype=torch.complex64, device=device)
+    pt = torch.zeros(cells, dtype=torch.float32, device=device)
+    ft = torch.zeros(cells, dtype=torch.complex64, device=device)
+    diff = torch.zeros(cells, dtype=torch.float32, device=device)
+    if count:
+        field = rows["path_field"].to(torch.complex64)
+        args = (torch.ones(count, dtype=torch.uint8, device=device), rows["tx_id"].to(torch.int32).contiguous(),
+                rows["rx_id"].to(torch.int32).contiguous(), rows["component_id"].to(torch.int32).contiguous(),
+                rows["path_gain"].to(torch.float32).contiguous(), field.real.contiguous(), field.imag.contiguous())
+        work = torch.empty(int(_native.call("wc_accumulate_workspace_bytes", count, num_tx, num_rx)),
+                           dtype=torch.uint8, device=device)
+        p = _native.ptr
+        _native.call("wc_accumulate_flat", count, *[p(a) for a in args], num_tx, num_rx, 1 if coherent else 0,
+                     p(cp), p(torch.view_as_real(cf)), p(pt), p(torch.view_as_real(ft)), p(work), work.numel(),
+                     _native.stream_handle(device))
+        is_diff = rows["component_id"] == 2
+        if bool(is_diff.any()):
+            cell = rows["tx_id"][is_diff].long() * num_rx + rows["rx_id"][is_diff].long()
+            amp = tx_power.to(device)[rows["tx_id"][is_diff].long()].clamp_min(0.0).sqrt()
+            vec = torch.zeros((cells, 3), dtype=torch.complex64, device=device)
+            vec.index_add_(0, cell, rows["field_xyz"][is_diff] * amp[:, None])
+            diff = vec.abs().square().sum(dim=-1)
+    return pt, ft, cp[:4], cf[:4], diff
+
+
 def _validate(config: Config) -> None:
     unsupported = set(config.components) - {"los", "reflection", "diffraction", "transmission"}
     if unsupported:
@@ -203,6 +241,11 @@ def _validate(config: Config) -> None:
         raise NotImplementedError("coupled reflection-diffraction paths are outside this build's hot path")
     if config.isb_boundary_taper:
         raise NotImplementedError("isb_boundary_taper is outside this build's hot path")
+    if config.ad_mode == "jvp":
+        raise NotImplementedError("deterministic ad_mode='jvp' is not available in this build; use ad_mode='vjp'")
+    if config.ad_mode == "vjp" and config.max_paths is not None:
+        raise NotImplementedError("ad_mode='vjp' differentiates every path of a cell; max_paths truncation is "
+                                  "not supported with it")
     if "reflection" in config.components and config.max_depth < 1:
         raise RuntimeError("deterministic reflection requires max_depth >= 1")
     if "diffraction" in config.components:
@@ -229,7 +272,7 @@ def _metadata(config: Config, path_count: int, counts: dict[str, int], launches:
         "accumulation_strategy": "coherent" if config.coherent else "incoherent",
         "components": status,
         "counts": {"path_count": path_count, "valid_path_count": path_count, "components": counts},
-        "capability": {"rayd_native": True, "path_native": True, "cuda_available": True,
+        "capability": {"rayd_native": False, "path_native": True, "hip_native": True, "cuda_available": False,
                        "optix_available": False},
         "kernel": {"primitive": "deterministic_solver", "forward_launch_count": launches,
                    "accumulation_strategy": "per_cell_float64_registers", "scheduling_strategy": "fused_hip",
@@ -247,8 +290,8 @@ def _metadata(config: Config, path_count: int, counts: dict[str, int], launches:
 
 def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: torch.Tensor | None = None,
                     material_params: Any = None) -> Result:
-    device = _native.require_gpu()
     _validate(config)
+    device = _native.require_gpu()
     ep = compiled.endpoints
     if len(ep.rx_position) == 0 or len(ep.tx_position) == 0:
         raise RuntimeError("deterministic solve requires at least one transmitter and one receiver")
@@ -283,15 +326,32 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
     path_count = int(counts_t.sum().item())
     paths = None
     counts = {"los": 0, "reflection": 0, "diffraction": 0}
-    if config.export_paths:
+    truncated = False
+    if config.export_paths or config.max_paths is not None:
         rows = engine.export_rows(dscene, inputs, width=max(depth, 1))
         keep = select_max_paths(rows, config.max_paths, config.max_paths_scope, num_tx)
+        truncated = bool((~keep).any().item())
         rows = {k: v[keep] if isinstance(v, torch.Tensor) else v for k, v in rows.items()}
-        paths = build_path_table(rows, include_fields=config.return_field)
-        path_count = int(paths.tx_id.shape[0])
-        cid = paths.component_id
+        if truncated:
+            # the reference truncates before fields and accumulation
+            # (propagation/enumerated.py:788-792): the maps sum the kept rows only
+            power_total, field_total, comp_p4, comp_f4, diff_t = accumulate_rows(
+                rows, num_tx, num_rx, config.coherent, inputs.tx_power)
+            power_total = power_total.reshape(num_tx, num_rx)
+            field_total = field_total.reshape(num_tx, num_rx)
+            component_power = {name: comp_p4[_SLOTS[name]].reshape(num_tx, num_rx) for name in names}
+            component_fields = {name: comp_f4[_SLOTS[name]].reshape(num_tx, num_rx) for name in names}
+            if ep.layout_kind == "grid" and "diffraction" in config.components:
+                exact = diff_t.reshape(num_tx, num_rx)
+                if not config.coherent:
+                    power_total = power_total - component_power["diffraction"] + exact
+                component_power["diffraction"] = exact
+        path_count = int(rows["tx_id"].shape[0])
+        cid = rows["component_id"]
         for name, c in (("los", 0), ("reflection", 1), ("diffraction", 2), ("transmission", 5)):
             counts[name] = int((cid == c).sum().item())
+        if config.export_paths:
+            paths = build_path_table(rows, include_fields=config.return_field)
     metadata = _metadata(config, path_count, counts, launches=2 if config.export_paths else 1)
     path_gain = apply_layout(power_total, compiled)
     field = apply_layout(field_total, compiled) if config.return_field else \
@@ -309,7 +369,7 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
             "diffraction_accumulation": ("exact_vector_coherent_paths" if ep.layout_kind == "grid"
                                          else "scalar_path_accumulation"),
             "path_planning": {"max_paths": config.max_paths, "max_paths_scope": config.max_paths_scope,
-                              "candidate_count": inputs.chains.count, "truncated": False},
+                              "candidate_count": inputs.chains.count, "truncated": truncated},
         }
     if bool((out.failure != 0).any().item()):
         raise RuntimeError("deterministic solve exceeded a native capacity (failure bits "

@@ -45,6 +45,7 @@ class RadiomapFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, eps_r, sigma_e, frequency, tx_position, vertices, state: RadiomapState):  # type: ignore[override]
+        apply_parameters(state, eps_r, sigma_e, frequency, tx_position, vertices)
         if state.out is None:
             state.out = engine.forward_with_topology(state.dscene, state.inputs, capacity=state.topology_capacity)
             state.topology_capacity = state.out.topology_capacity
@@ -97,6 +98,35 @@ class RadiomapFunction(torch.autograd.Function):
                      p(grad_tx), stream)
         grad_vertices = geometry_vjp(state.dscene, inputs, view) if ctx.needs_input_grad[4] else None
         return grad_params[:m], grad_params[m:2 * m], grad_params[2 * m], grad_tx, grad_vertices, None
+
+
+def apply_parameters(state: RadiomapState, eps_r, sigma_e, frequency, tx_position, vertices) -> None:
+    """Write the differentiable inputs' current values into the device scene
+    and the solve before a forward: first-layer eps_r / sigma_e of every
+    material (scalar view and layer table), the carrier frequency, the
+    transmitter positions (the reflection chains keep their group sequences;
+    their image points are recomputed) and, when a vertex table other than
+    the scene's own is given, the vertices (device refit, same topology)."""
+    d, inputs = state.dscene, state.inputs
+    m = state.material_count
+    first = d.layer_offset[:m].long()
+    eps = eps_r.detach().to(device=d.device, dtype=torch.float32).reshape(-1)[:m]
+    sig = sigma_e.detach().to(device=d.device, dtype=torch.float32).reshape(-1)[:m]
+    d.mat_eps[:m].copy_(eps)
+    d.mat_sigma[:m].copy_(sig)
+    d.layer_eps.index_copy_(0, first, eps)
+    d.layer_sigma.index_copy_(0, first, sig)
+    inputs.view.frequency_hz = float(frequency.detach()) if isinstance(frequency, torch.Tensor) else float(frequency)
+    txp = tx_position.detach().to(device=d.device, dtype=torch.float32).reshape(inputs.tx_pos.shape)
+    if txp.data_ptr() != inputs.tx_pos.data_ptr():
+        inputs.tx_pos.copy_(txp)
+        chains = inputs.chains
+        if chains.count:
+            _native.call("wc_chain_images", d.ref(), _native.ptr(inputs.tx_pos), _native.ptr(chains.tx),
+                         _native.ptr(chains.groups), _native.ptr(chains.depth), chains.count,
+                         _native.ptr(chains.images), _native.stream_handle(d.device))
+    if vertices is not None and vertices.data_ptr() != d.vertices.data_ptr():
+        d.refit(vertices)
 
 
 def geometry_vjp(dscene, inputs: engine.SolveInputs, view) -> torch.Tensor:

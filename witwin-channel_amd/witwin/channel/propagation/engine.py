@@ -397,6 +397,24 @@ def forward_with_topology(dscene: DeviceScene, inputs: SolveInputs, *, capacity:
     return out
 
 
+def public_edge_ids(dscene: DeviceScene, rows: dict[str, torch.Tensor]) -> None:
+    """Rewrite the edge ids of diffraction rows (``edge_id``, ``primitive_id``
+    and the first sequence slot) from positions in the device edge table to
+    the public edge ids of the compiled scene. The table is ordered by public
+    id, so the canonical row order is unchanged."""
+    if dscene.edge_count == 0 or rows["edge_id"].numel() == 0:
+        return
+    table = dscene.edge_public_id
+    diff = rows["component_id"] == 2
+    for name in ("edge_id", "primitive_id"):
+        col = rows[name]
+        rows[name] = torch.where(diff & (col >= 0), table[col.clamp(min=0).long()], col)
+    seq = rows["primitive_sequence"]
+    if seq.numel():
+        first = seq[:, 0]
+        seq[:, 0] = torch.where(diff & (first >= 0), table[first.clamp(min=0).long()], first)
+
+
 ROW_FIELDS_INT = ("tx_id", "rx_id", "depth", "component_id", "primitive_id", "edge_id", "material_id")
 ROW_FIELDS_FLOAT = ("path_length_m", "delay_s", "path_gain")
 
@@ -455,6 +473,7 @@ def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int) -> dict
     perm = perm[:total].long()
     keep_b = keep[:total].bool()
     out = {name: value[:total][perm][keep_b] for name, value in rows.items()}
+    public_edge_ids(dscene, out)
     out["pair_counts_canonical"] = canon_counts
     out["cell_order_canonical"] = canon
     return out

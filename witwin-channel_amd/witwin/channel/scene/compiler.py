@@ -86,6 +86,11 @@ class CompiledScene:
     edge_policy: str = "all"
     # [E, 4] int32 vertex ids (a, b, c0, c1) of every edge record (see edge_geometry)
     edge_vertices: np.ndarray = field(default_factory=lambda: np.zeros((0, 4), np.int32))
+    # [E] int32 public id of every edge record: the id path rows report (record_ids)
+    edge_public_id: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    # [F] structure of every face and [S] first vertex row of every structure
+    face_structure: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int64))
+    structure_vertex_base: np.ndarray = field(default_factory=lambda: np.zeros(1, np.int64))
     device: "DeviceScene | None" = None
     cache: dict[str, Any] = field(default_factory=dict)
 
@@ -287,21 +292,64 @@ def _edge_rows(geometry: dict[str, np.ndarray], ids: np.ndarray) -> np.ndarray:
     return rows
 
 
+def record_ids(faces: np.ndarray, face_structure: np.ndarray | None,
+               structure_vertex_base: np.ndarray | None) -> dict[tuple[int, int, int], int]:
+    """Public edge id of every (structure, local vertex a, local vertex b), a < b.
+
+    The reference numbers its edge records structure by structure: each
+    structure contributes one record per distinct vertex pair of its faces,
+    in ascending (lower, higher) local vertex id order, after the records of
+    the structures before it (reference scene/resources.py:79-148 exports the
+    records; their ids are what path rows carry as edge_id, e.g. the wedge of
+    tests/deterministic/test_diffraction_single_wedge.py:41-50 yields ids
+    0, 1, 2, 5 with record 3 the second copy of the shared edge). Without
+    structure tables every face belongs to structure 0.
+    """
+    faces = np.asarray(faces, dtype=np.int64).reshape(-1, 3)
+    if face_structure is None:
+        face_structure = np.zeros(faces.shape[0], dtype=np.int64)
+        structure_vertex_base = np.zeros(1, dtype=np.int64)
+    face_structure = np.asarray(face_structure, dtype=np.int64)
+    base = np.asarray(structure_vertex_base, dtype=np.int64)
+    ids: dict[tuple[int, int, int], int] = {}
+    cursor = 0
+    for s in range(base.shape[0]):
+        local = faces[face_structure == s] - base[s]
+        if local.size == 0:
+            continue
+        pairs = np.concatenate([local[:, [0, 1]], local[:, [1, 2]], local[:, [2, 0]]], axis=0)
+        pairs = np.unique(np.sort(pairs, axis=1), axis=0)
+        for r, (a, b) in enumerate(pairs.tolist()):
+            ids[(s, a, b)] = cursor + r
+        cursor += pairs.shape[0]
+    return ids
+
+
 def build_edge_table(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray, face_material: np.ndarray,
-                     policy: str = "all") -> tuple[np.ndarray, np.ndarray]:
+                     policy: str = "all", face_structure: np.ndarray | None = None,
+                     structure_vertex_base: np.ndarray | None = None,
+                     return_ids: bool = False) -> tuple[np.ndarray, ...]:
     """Diffraction edge table and the vertex ids of every edge.
 
     Every triangle edge is keyed by its welded end points. An edge shared by
     two faces of different, non-coplanar planes is a wedge whose exterior is the
     side the lower-id face's winding normal points to; an edge of exactly one
     face is a half-plane (n = 2). Edges between coplanar faces and edges with
-    more than two faces are not diffracting. Ids follow first appearance
-    (faces ascending, local edges v0v1, v1v2, v2v0). ``policy="vertical_only"``
-    keeps edges with |dz|/length > 0.7. Returns (records, [E, 4] int32 vertex
-    ids (a, b, c0, c1), see edge_geometry).
+    more than two faces are not diffracting. A wedge whose faces belong to two
+    structures merges the two boundary records of the reference into the one
+    with the lower id (reference scene/resources.py:286-404). Edges are ordered
+    by their public ids (``record_ids``; the lower id of a merged pair).
+    ``policy="vertical_only"`` keeps edges with |dz|/length > 0.7. Returns
+    (records, [E, 4] int32 vertex ids (a, b, c0, c1), see edge_geometry) and,
+    with ``return_ids``, the [E] int32 public ids.
     """
     v = vertices.astype(np.float64)
     q = np.rint(v / EDGE_WELD_QUANTIZATION).astype(np.int64)
+    public = record_ids(faces, face_structure, structure_vertex_base)
+    fstruct = (np.zeros(faces.shape[0], np.int64) if face_structure is None
+               else np.asarray(face_structure, dtype=np.int64))
+    vbase = (np.zeros(1, np.int64) if structure_vertex_base is None
+             else np.asarray(structure_vertex_base, dtype=np.int64))
     table: dict[tuple, list] = {}
     order: list[tuple] = []
     for f in range(faces.shape[0]):
@@ -314,31 +362,43 @@ def build_edge_table(vertices: np.ndarray, faces: np.ndarray, face_group: np.nda
                 table[key] = []
                 order.append(key)
             table[key].append((f, i0, i1, i2))
-    cand_v, cand_i = [], []
+
+    def entry_id(f: int, i0: int, i1: int) -> int:
+        s = int(fstruct[f])
+        la, lb = i0 - int(vbase[s]), i1 - int(vbase[s])
+        return public[(s, min(la, lb), max(la, lb))]
+
+    cand_v, cand_i, cand_p = [], [], []
     for key in order:
         entries = table[key]
         if len(entries) > 2:
             continue
         f0, i0, i1, i2 = entries[0]
         if len(entries) == 2:
-            f1, _, _, j2 = entries[1]
+            f1, j0, j1, j2 = entries[1]
             if face_group[f1] == face_group[f0]:
                 continue
             cand_v.append((i0, i1, i2, j2))
             cand_i.append((f0, f1, int(face_group[f0]), int(face_group[f1]), int(face_material[f0]),
                            int(face_material[f1])))
+            cand_p.append(min(entry_id(f0, i0, i1), entry_id(f1, j0, j1)))
         else:
             cand_v.append((i0, i1, i2, -1))
             cand_i.append((f0, -1, int(face_group[f0]), -1, int(face_material[f0]), -1))
+            cand_p.append(entry_id(f0, i0, i1))
     ev = np.asarray(cand_v, dtype=np.int64).reshape(-1, 4)
     ids = np.asarray(cand_i, dtype=np.int64).reshape(-1, 6)
+    pid = np.asarray(cand_p, dtype=np.int64).reshape(-1)
     g = edge_geometry(v, ev)
     keep = g["length"] >= 1e-9
     if policy == "vertical_only":
         keep &= np.abs(g["d"][:, 2]) > 0.7
     keep &= ~(g["wedge"] & ((np.abs(g["alpha"] - math.pi) < COPLANAR_ANGLE_TOLERANCE) | (g["alpha"] < 1e-6)))
-    g = {k: a[keep] for k, a in g.items()}
-    return _edge_rows(g, ids[keep]), ev[keep].astype(np.int32)
+    sel = np.nonzero(keep)[0]
+    sel = sel[np.argsort(pid[sel], kind="stable")]
+    g = {k: a[sel] for k, a in g.items()}
+    out = (_edge_rows(g, ids[sel]), ev[sel].astype(np.int32))
+    return out + (pid[sel].astype(np.int32),) if return_ids else out
 
 
 def build_edges(vertices: np.ndarray, faces: np.ndarray, face_group: np.ndarray, face_material: np.ndarray,
@@ -398,7 +458,7 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
         else float(reference_frequency_hz)
     if not frequency > 0.0:
         raise ValueError("reference_frequency_hz must be positive")
-    vert_chunks, face_chunks, mat_ids, surf_ids, vert_leaves = [], [], [], [], []
+    vert_chunks, face_chunks, mat_ids, surf_ids, vert_leaves, face_struct, vbase = [], [], [], [], [], [], []
     unique: list[PhysicalMaterial] = []
     lookup: dict[int, int] = {}
     base = 0
@@ -413,6 +473,8 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
         vert_chunks.append(v)
         vert_leaves.append((mesh.vertices, base))
         face_chunks.append(f + base)
+        face_struct.append(np.full(f.shape[0], index, dtype=np.int64))
+        vbase.append(base)
         mat_ids.append(np.full(f.shape[0], lookup[id(mat)], dtype=np.int32))
         sid = structure.surface_id if structure.surface_id is not None else index
         surf_ids.append(np.full(f.shape[0], int(sid), dtype=np.int64))
@@ -443,13 +505,18 @@ def compile(scene: Scene, *, reference_frequency_hz: Any) -> CompiledScene:  # n
                                                              face_material)
     policy = getattr(scene.metadata.get("imported_edge_policy"), "edge_selection_mode", "all") \
         if scene.metadata else "all"
-    edges, edge_vertices = build_edge_table(vertices, faces, face_group, face_material, policy)
+    face_structure = np.concatenate(face_struct) if face_struct else np.zeros(0, np.int64)
+    structure_vertex_base = np.asarray(vbase if vbase else [0], dtype=np.int64)
+    edges, edge_vertices, edge_public = build_edge_table(
+        vertices, faces, face_group, face_material, policy, face_structure=face_structure,
+        structure_vertex_base=structure_vertex_base, return_ids=True)
     compiled = CompiledScene(
         vertices=vertices, faces=faces, face_material=face_material, face_surface=face_surface,
         face_group=face_group, group_plane=planes, group_face_offset=offsets, group_faces=members,
         group_material=group_material, representative_faces=representative, materials=materials,
         edges=edges, frequency_hz=frequency, endpoints=compile_endpoints(scene), edge_policy=policy,
-        edge_vertices=edge_vertices,
+        edge_vertices=edge_vertices, edge_public_id=edge_public, face_structure=face_structure,
+        structure_vertex_base=structure_vertex_base,
     )
     compiled.cache["ad_leaves"] = ad_leaves
     return compiled
@@ -559,6 +626,9 @@ class DeviceScene:
         self.layer_mu = t(mats.layer_mu_r, torch.float32)
         self.group_face = t(compiled.representative_faces, torch.int32)
         self.edge_vertices = t(compiled.edge_vertices.reshape(-1, 4), torch.int32)
+        public = compiled.edge_public_id if compiled.edge_public_id.shape[0] == compiled.edge_count \
+            else np.arange(compiled.edge_count)
+        self.edge_public_id = t(public.reshape(-1), torch.int32)
         self.group_count = compiled.group_count
         self.edge_count = compiled.edge_count
         self._csr: tuple[torch.Tensor, torch.Tensor] | None = None

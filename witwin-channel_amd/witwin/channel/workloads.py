@@ -12,6 +12,10 @@
   z = 1.5 m covering the block (256 x 256 at 2 m by default).
 * ``batched_transmitters`` — seeded transmitter positions over the streets
   with log-uniform receiver-set sizes.
+* ``wedge_diffraction_scene`` — two perfectly conducting triangles (two
+  structures) meeting along the vertical edge x = 2, y = 0, z in [-1, 2],
+  transmitter (0, -1, 0.5), receiver (3, 1, 0.5), used at 3 GHz (the geometry
+  of the reference fixture tests/support/scenes.py:172-215).
 * ``heightfield_scene`` — seeded fBm terrain (128 x 128 vertices at 2 m
   pitch, relief 20 m, 32,258 triangles) with 40 random boxes (480
   triangles), 4 materials (soil, concrete, glass, brick), one transmitter
@@ -168,6 +172,22 @@ def batched_transmitters(seed: int = 3, tx_count: int = 64, min_rx: int = 16, ma
     sizes = np.floor(np.exp(rng.uniform(math.log(min_rx), math.log(max_rx), size=tx_count))).astype(np.int64)
     rx = np.concatenate([street_points(rng, int(n)) for n in sizes], axis=0)
     return tx, sizes, rx
+
+
+def wedge_diffraction_scene(edge_policy: str | None = None, tx=(0.0, -1.0, 0.5), rx=(3.0, 1.0, 0.5)) -> Scene:
+    """PEC wedge of two single-triangle structures sharing one vertical edge
+    (geometry of reference tests/support/scenes.py:172-215). Public edge ids:
+    0 the shared vertical wedge, 1 and 2 the horizontal and slanted outline
+    edges of the first structure, 3 the second structure's copy of the shared
+    edge (merged into 0), 4 and 5 its horizontal and slanted outline edges."""
+    pec = PhysicalMaterial.perfect_conductor()
+    face_a = Structure(Mesh(torch.tensor([[2.0, 0.0, -1.0], [2.0, 0.0, 2.0], [2.0, 2.0, -1.0]]),
+                            torch.tensor([[0, 1, 2]])), pec, name="wedge-a", surface_id=2)
+    face_b = Structure(Mesh(torch.tensor([[2.0, 0.0, -1.0], [2.0, 0.0, 2.0], [4.0, 0.0, -1.0]]),
+                            torch.tensor([[0, 2, 1]])), pec, name="wedge-b", surface_id=3)
+    metadata = {} if edge_policy is None else {
+        "imported_edge_policy": SimpleNamespace(edge_selection_mode=edge_policy)}
+    return Scene(structures=[face_a, face_b], endpoints=[transmitter(tx), receiver(rx)], metadata=metadata)
 
 
 SOIL = dict(eps_r=15.0, sigma_e=0.035)
