@@ -422,6 +422,24 @@ int wc_accumulate_flat(int count, const unsigned char* valid, const int* tx_id, 
                        float* component_field, float* power_total, float* field_total, void* workspace,
                        size_t workspace_bytes, void* stream);
 
+/* Allocation-free variant of wc_accumulate_flat. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as wc_accumulate_flat without the workspace: the
+ * sort keys, row ids, histogram and segment starts live in a fixed
+ * device-resident scratch array of wc_accumulate_noalloc_max_bytes() bytes
+ * (64 MiB, one per GPU, held until the process ends). Calls whose
+ * wc_accumulate_workspace_bytes(count, num_tx, num_rx) exceeds that fail with
+ * a message and launch nothing. It calls no allocator. Every call on one GPU
+ * shares that GPU's scratch: calls that overlap in time on one GPU (two
+ * streams, or two host threads) race and must be serialized by the caller. */
+size_t wc_accumulate_noalloc_max_bytes(void);
+int wc_accumulate_flat_noalloc(int count, const unsigned char* valid, const int* tx_id, const int* rx_id,
+                               const int* component_id, const float* path_gain, const float* field_real,
+                               const float* field_imag, int num_tx, int num_rx, int coherent,
+                               float* component_power, float* component_field, float* power_total,
+                               float* field_total, void* stream);
+
 /* ------------------------------------------------------------ visibility
  * Segment visibility (reference kernels/geometry.py rayd_visibility_forward
  * :380 -> RayD). visible[i] = 1 when segment source[i] -> target[i] crosses

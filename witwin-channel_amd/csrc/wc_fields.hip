@@ -1,5 +1,7 @@
 // Row-level field operators (one path row per lane) and the deterministic
 // flat accumulator.
+#include <mutex>
+
 #include "wc_api_internal.h"
 #include "wc_paths.h"
 #include "../../include/wcmi.h"
@@ -255,19 +257,17 @@ extern "C" size_t wc_accumulate_workspace_bytes(int count, int num_tx, int num_r
     return align_up(4 * n) * 4 + align_up(4 * radix_sort_hist_ints(count)) + align_up(4 * (cells + 1));
 }
 
-extern "C" int wc_accumulate_flat(int count, const unsigned char* valid, const int* tx_id, const int* rx_id,
-                                  const int* component_id, const float* path_gain, const float* field_real,
-                                  const float* field_imag, int num_tx, int num_rx, int coherent,
-                                  float* component_power, float* component_field, float* power_total,
-                                  float* field_total, void* workspace, size_t workspace_bytes, void* stream) {
-    if (count < 0 || num_tx < 0 || num_rx < 0) return wc_fail("wc_accumulate_flat: sizes must be non-negative");
-    const long long cells_ll = (long long)num_tx * num_rx;
-    if (cells_ll >= 0x7fffffffLL) return wc_fail("wc_accumulate_flat: num_tx * num_rx exceeds int32");
-    const int cells = (int)cells_ll;
-    if (workspace_bytes < wc_accumulate_workspace_bytes(count, num_tx, num_rx))
-        return wc_fail("wc_accumulate_flat: workspace too small");
-    if (cells == 0) return 0;
-    hipStream_t st = (hipStream_t)stream;
+namespace wc {
+namespace {
+
+// Workspace layout of the flat accumulation: sort keys and row ids (two
+// buffers each), the radix histogram, then cells + 1 segment starts. Every
+// byte is written before it is read, so nothing needs clearing.
+int accumulate_flat_launch(int count, const unsigned char* valid, const int* tx_id, const int* rx_id,
+                           const int* component_id, const float* path_gain, const float* field_real,
+                           const float* field_imag, int num_tx, int num_rx, int cells, int coherent,
+                           float* component_power, float* component_field, float* power_total, float* field_total,
+                           void* workspace, hipStream_t st) {
     char* base = (char*)workspace;
     const size_t n = (size_t)std::max(count, 1);
     unsigned int* keys = (unsigned int*)base; base += align_up(4 * n);
@@ -291,4 +291,90 @@ extern "C" int wc_accumulate_flat(int count, const unsigned char* valid, const i
                        power_total, field_total);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_accumulate_flat", err);
+}
+
+// Scratch of the allocation-free accumulation: one copy per GPU, resident
+// until the process ends. Host code never uses this name as an address; it
+// uses the per-device pointer from hipGetSymbolAddress (scratch_address).
+constexpr size_t kAccumulateScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kAccumulateScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept; nullptr after a failure, with
+// the error recorded.
+void* scratch_address() {
+    int device = -1;
+    hipError_t err = hipGetDevice(&device);
+    if (err != hipSuccess) {
+        wc_fail_hip("wc_accumulate_flat_noalloc: hipGetDevice", err);
+        return nullptr;
+    }
+    if (device < 0 || device >= kMaxScratchDevices) {
+        wc_fail("wc_accumulate_flat_noalloc: device ordinal %d out of range", device);
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !ptr) {
+            wc_fail_hip("wc_accumulate_flat_noalloc: hipGetSymbolAddress(scratch)", err);
+            return nullptr;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+
+// 0 and the cell count, or the failure status with the error recorded
+int accumulate_sizes(int count, int num_tx, int num_rx, const char* who, int& cells) {
+    if (count < 0 || num_tx < 0 || num_rx < 0) return wc_fail("%s: sizes must be non-negative", who);
+    const long long cells_ll = (long long)num_tx * num_rx;
+    if (cells_ll >= 0x7fffffffLL) return wc_fail("%s: num_tx * num_rx exceeds int32", who);
+    cells = (int)cells_ll;
+    return 0;
+}
+
+}  // namespace
+}  // namespace wc
+
+extern "C" int wc_accumulate_flat(int count, const unsigned char* valid, const int* tx_id, const int* rx_id,
+                                  const int* component_id, const float* path_gain, const float* field_real,
+                                  const float* field_imag, int num_tx, int num_rx, int coherent,
+                                  float* component_power, float* component_field, float* power_total,
+                                  float* field_total, void* workspace, size_t workspace_bytes, void* stream) {
+    int cells = 0;
+    if (const int st = accumulate_sizes(count, num_tx, num_rx, "wc_accumulate_flat", cells)) return st;
+    if (workspace_bytes < wc_accumulate_workspace_bytes(count, num_tx, num_rx))
+        return wc_fail("wc_accumulate_flat: workspace too small");
+    if (cells == 0) return 0;
+    return accumulate_flat_launch(count, valid, tx_id, rx_id, component_id, path_gain, field_real, field_imag, num_tx,
+                                  num_rx, cells, coherent, component_power, component_field, power_total, field_total,
+                                  workspace, (hipStream_t)stream);
+}
+
+extern "C" size_t wc_accumulate_noalloc_max_bytes(void) { return kAccumulateScratchBytes; }
+
+extern "C" int wc_accumulate_flat_noalloc(int count, const unsigned char* valid, const int* tx_id, const int* rx_id,
+                                          const int* component_id, const float* path_gain, const float* field_real,
+                                          const float* field_imag, int num_tx, int num_rx, int coherent,
+                                          float* component_power, float* component_field, float* power_total,
+                                          float* field_total, void* stream) {
+    int cells = 0;
+    if (const int st = accumulate_sizes(count, num_tx, num_rx, "wc_accumulate_flat_noalloc", cells)) return st;
+    const size_t need = wc_accumulate_workspace_bytes(count, num_tx, num_rx);
+    if (need > kAccumulateScratchBytes)
+        return wc_fail("wc_accumulate_flat_noalloc: %d rows over %d cells need %zu bytes of scratch, more than its "
+                       "fixed %zu bytes; use wc_accumulate_flat with a workspace", count, cells, need,
+                       kAccumulateScratchBytes);
+    void* ws = scratch_address();
+    if (!ws) return 1;
+    if (cells == 0) return 0;
+    return accumulate_flat_launch(count, valid, tx_id, rx_id, component_id, path_gain, field_real, field_imag, num_tx,
+                                  num_rx, cells, coherent, component_power, component_field, power_total, field_total,
+                                  ws, (hipStream_t)stream);
 }

@@ -153,6 +153,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_accumulate_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "wc_accumulate_flat": (c_int, [c_int] + [c_void_p] * 7 + [c_int, c_int, c_int] + [c_void_p] * 5
                            + [c_size_t, c_void_p]),
+    "wc_accumulate_noalloc_max_bytes": (c_size_t, []),
+    "wc_accumulate_flat_noalloc": (c_int, [c_int] + [c_void_p] * 7 + [c_int, c_int, c_int] + [c_void_p] * 5),
     "wc_visibility": (c_int, [ctypes.POINTER(WcScene), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_discover_chains": (c_int, [ctypes.POINTER(WcScene), c_void_p, c_int, c_int, c_int, c_void_p,
                                    c_void_p]),
