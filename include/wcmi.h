@@ -154,6 +154,17 @@ typedef struct wc_solve {
     int max_tx_chains;     /* largest reflection-chain count of one transmitter (all
                               depths); sizes the per-cell chain masks of the tiled
                               radiomap (0: no chain masks) */
+    /* Supertiles of the chain validity cull (optional; NULL / 0: one
+     * transmitter's supertiles are its 16 consecutive tiles by position):
+     * super s covers tiles [super_first_tile[s], super_first_tile[s+1]) of the
+     * tile list, at most 16 tiles of one transmitter; tile_super[tile] is the
+     * super holding a tile. */
+    const int* super_first_tile; /* [super_count + 1] */
+    const int* tile_super;       /* [tile_count] */
+    int super_count;             /* <= ceil(total_cells / 1024) + tx_count */
+    int visibility_mode;  /* 0: by receiver layout (shared grid -> tile loops,
+                             per-transmitter sets -> queued legs); 1: tile loops
+                             (neighbouring receivers per wavefront); 2: queued legs */
 } wc_solve;
 
 /* ------------------------------------------------------------ scene refit

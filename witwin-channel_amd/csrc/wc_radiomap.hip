@@ -417,9 +417,14 @@ constexpr int kSuperCells = 64 * kSuperTiles;
 struct SuperLayout {
     size_t boxes, masks, end;
 };
-__host__ __device__ inline SuperLayout super_layout(size_t base, int cells, int max_tx_chains) {
+// supertiles of one transmitter per 1,024 positions, plus one partial
+// supertile per transmitter at most
+__host__ __device__ inline size_t super_capacity(int cells, int tx_count) {
+    return (size_t)((cells > 0 ? cells : 0) + kSuperCells - 1) / kSuperCells + (size_t)(tx_count > 0 ? tx_count : 0);
+}
+__host__ __device__ inline SuperLayout super_layout(size_t base, int cells, int max_tx_chains, int tx_count) {
     SuperLayout l;
-    const size_t supers = (size_t)((cells > 0 ? cells : 0) + kSuperCells - 1) / kSuperCells;
+    const size_t supers = super_capacity(cells, tx_count);
     l.boxes = align256(base);
     l.masks = align256(l.boxes + supers * 24);
     l.end = l.masks + supers * (size_t)mask_words(max_tx_chains > 0 ? max_tx_chains : 0) * 4;
@@ -427,7 +432,8 @@ __host__ __device__ inline SuperLayout super_layout(size_t base, int cells, int 
 }
 // start of the staged diffraction pass
 __host__ __device__ inline size_t stage_base(int cells, int lanes, int edge_count, int max_tx_chains, int tx_count) {
-    return super_layout(shadow_end(cells, lanes, edge_count, max_tx_chains, tx_count), cells, max_tx_chains).end;
+    return super_layout(shadow_end(cells, lanes, edge_count, max_tx_chains, tx_count), cells, max_tx_chains,
+                        tx_count).end;
 }
 
 // cell evaluated at launch position `pos` (identity without an order table)
@@ -913,15 +919,35 @@ __device__ __forceinline__ float wave_max(float v) {
 // tile pass culls only the chains its supertile kept (a chain that misses the
 // supertile box misses every tile box inside it: chain_reaches_box only
 // grows with the box).
-__global__ void __launch_bounds__(256) super_box_kernel(wc_solve q, int supers, float* __restrict__ boxes) {
+// Supertile s: its transmitter and its position range. With host supertile
+// tables (super_first_tile) a super is up to 16 consecutive tiles of one
+// transmitter; without them (one transmitter) it is positions
+// [1024 s, 1024 s + 1024).
+__device__ __forceinline__ void super_range(const wc_solve& q, const int* tiles, int sup, int& t, int& first,
+                                            int& last) {
+    if (q.super_first_tile) {
+        const int f = q.super_first_tile[sup], l = q.super_first_tile[sup + 1];
+        t = tiles[2 * f];
+        first = tiles[2 * f + 1];
+        last = min(tiles[2 * (l - 1) + 1] + 64, q.cell_offset[t + 1]);
+    } else {
+        t = 0;
+        first = sup * kSuperCells;
+        last = min(first + kSuperCells, q.total_cells);
+    }
+}
+
+__global__ void __launch_bounds__(256) super_box_kernel(wc_solve q, const int* __restrict__ tiles, int supers,
+                                                        float* __restrict__ boxes) {
     const int sup = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (sup >= supers) return;
     const int lane = threadIdx.x & 63;
-    const int first = sup * kSuperCells, last = min(first + kSuperCells, q.total_cells);
+    int t, first, last;
+    super_range(q, tiles, sup, t, first, last);
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int pos = first + lane; pos < last; pos += 64) {
         const int cell = cell_at(q, pos);
-        const V3<float> rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[0] : cell));
+        const V3<float> rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[t] : cell));
         lo[0] = fminf(lo[0], rx.x); lo[1] = fminf(lo[1], rx.y); lo[2] = fminf(lo[2], rx.z);
         hi[0] = fmaxf(hi[0], rx.x); hi[1] = fmaxf(hi[1], rx.y); hi[2] = fmaxf(hi[2], rx.z);
     }
@@ -936,35 +962,42 @@ __global__ void __launch_bounds__(256) super_box_kernel(wc_solve q, int supers, 
         }
 }
 
-__global__ void __launch_bounds__(256) super_cull_kernel(SceneView s, wc_solve q, int supers, int words,
-                                                         const float* __restrict__ boxes,
-                                                         unsigned* __restrict__ super_masks) {
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(256) super_cull_kernel(SceneView s, wc_solve q, const int* __restrict__ tiles,
+                                                         int supers, int words, const float* __restrict__ boxes,
+                                                         unsigned* __restrict__ super_masks, long long wave_base) {
+    const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)supers * words) return;
     const int sup = (int)(wave / words);
     const int chunk = (int)(wave - (long long)sup * words);
     const int lane = threadIdx.x & 63;
+    int t, first, last;
+    super_range(q, tiles, sup, t, first, last);
     const V3<float> lo{boxes[6 * sup], boxes[6 * sup + 1], boxes[6 * sup + 2]};
     const V3<float> hi{boxes[6 * sup + 3], boxes[6 * sup + 4], boxes[6 * sup + 5]};
-    const int base = q.chain_offset[0];
-    const int end = q.chain_offset[min(q.max_depth, kMaxDepth)];
+    const int base = q.chain_offset[t * kMaxDepth];
+    const int end = q.chain_offset[t * kMaxDepth + min(q.max_depth, kMaxDepth)];
     const int c0 = base + chunk * 32;
     const int c1 = min(c0 + 32, end);
     bool reach = false;
-    if (lane < c1 - c0) {
+    if (lane < c1 - c0 && first < last) {
         const int c = c0 + lane;
         int d = 1;
-        while (c >= q.chain_offset[d]) ++d;
+        while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
         reach = chain_reaches_box(s, q, c, d, lo, hi);
     }
     const unsigned bits = (unsigned)__ballot(reach);
     if (lane == 0) super_masks[(size_t)sup * words + chunk] = bits;
 }
 
+// supertile of a tile (see super_range)
+__device__ __forceinline__ int super_of_tile(const wc_solve& q, int tile) {
+    return q.tile_super ? q.tile_super[tile] : tile / kSuperTiles;
+}
+
 __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_solve q, const int* __restrict__ tiles,
                                                                int tile_count, unsigned* __restrict__ masks,
-                                                               int words, const unsigned* __restrict__ super_masks) {
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+                                                               int words, const unsigned* __restrict__ super_masks, long long wave_base) {
+    const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)tile_count * words) return;
     const int tile = (int)(wave / words);
     const int chunk = (int)(wave - (long long)tile * words);
@@ -977,7 +1010,7 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
     // chains the tile's supertile kept (all when there is no supertile cull;
     // with it the masks were zero-filled first, so a tile whose supertile kept
     // none of the 32 chains has nothing to write)
-    const unsigned kept = super_masks ? super_masks[(size_t)(tile / kSuperTiles) * words + chunk] : ~0u;
+    const unsigned kept = super_masks ? super_masks[(size_t)super_of_tile(q, tile) * words + chunk] : ~0u;
     if (kept == 0u) return;
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     V3<float> rx{0.0f, 0.0f, 0.0f};
@@ -1021,6 +1054,205 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
     masks[(size_t)cell * words + chunk] = bits;
 }
 
+template <int D>
+__device__ __forceinline__ bool chain_lands(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
+                                            const V3<float>& rx);
+template <int D>
+__device__ __forceinline__ bool chain_legs(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
+                                           const V3<float>& rx);
+
+// Chain validity behind a supertile cull, tile loops: one wavefront per
+// (tile, segment of 64 chain words). Its lanes load the segment's 64 supertile
+// words at once and the wavefront visits only the chunks (32 chains) its
+// supertile kept: the tile cull of those chains against the tile's receiver
+// box, then every receiver tests the survivors exactly. The bits equal
+// chain_visibility_kernel's; mask words of chunks with no surviving chain stay
+// as the zero fill left them.
+__global__ void __launch_bounds__(256) chain_visibility_seg_kernel(SceneView s, wc_solve q,
+                                                                   const int* __restrict__ tiles, int tile_count,
+                                                                   unsigned* __restrict__ masks, int words,
+                                                                   const unsigned* __restrict__ super_masks,
+                                                                   long long wave_base) {
+    const int segs = (words + 63) >> 6;
+    const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= (long long)tile_count * segs) return;
+    const int tile = (int)(wave / segs);
+    const int seg = (int)(wave - (long long)tile * segs);
+    const int t = tiles[2 * tile];
+    const int lane = threadIdx.x & 63;
+    const int base = q.chain_offset[t * kMaxDepth];
+    const int end = q.chain_offset[t * kMaxDepth + min(q.max_depth, kMaxDepth)];
+    const int tx_words = mask_words(end - base);
+    const int w = seg * 64 + lane;
+    const unsigned sword = w < tx_words ? super_masks[(size_t)super_of_tile(q, tile) * words + w] : 0u;
+    unsigned long long chunks = __ballot(sword != 0u);
+    if (chunks == 0ull) return;
+    const int pos = tiles[2 * tile + 1] + lane;
+    const bool live = pos < q.cell_offset[t + 1];
+    const int cell = live ? cell_at(q, pos) : 0;
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    V3<float> rx{0.0f, 0.0f, 0.0f};
+    if (live) rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[t] : cell));
+    const V3<float> lo{wave_min(live ? rx.x : INFINITY), wave_min(live ? rx.y : INFINITY),
+                       wave_min(live ? rx.z : INFINITY)};
+    const V3<float> hi{wave_max(live ? rx.x : -INFINITY), wave_max(live ? rx.y : -INFINITY),
+                       wave_max(live ? rx.z : -INFINITY)};
+    int d = 1;
+    while (chunks) {
+        const int b = __ffsll((long long)chunks) - 1;
+        chunks &= chunks - 1;
+        const int chunk = seg * 64 + b;
+        const unsigned kept = (unsigned)__shfl((int)sword, b);
+        const int c0 = base + chunk * 32;
+        const int c1 = min(c0 + 32, end);
+        bool reach = false;
+        if (lane < c1 - c0 && ((kept >> lane) & 1u)) {
+            const int c = c0 + lane;
+            int dd = 1;
+            while (c >= q.chain_offset[t * kMaxDepth + dd]) ++dd;
+            reach = chain_reaches_box(s, q, c, dd, lo, hi);
+        }
+        unsigned todo = (unsigned)__ballot(reach);
+        if (todo == 0u) continue;
+        unsigned bits = 0;
+        while (todo) {
+            const int bb = __ffs(todo) - 1;
+            todo &= todo - 1;
+            const int c = c0 + bb;
+            while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+            if (!live) continue;
+            bool ok;
+            switch (d) {
+                case 1: ok = chain_valid<1>(s, q, c, tx, rx); break;
+                case 2: ok = chain_valid<2>(s, q, c, tx, rx); break;
+                case 3: ok = chain_valid<3>(s, q, c, tx, rx); break;
+                case 4: ok = chain_valid<4>(s, q, c, tx, rx); break;
+                default: ok = chain_valid<5>(s, q, c, tx, rx); break;
+            }
+            if (ok) bits |= 1u << bb;
+        }
+        if (live) masks[(size_t)cell * words + chunk] = bits;
+    }
+}
+
+// The same bits with the legs queued: per (tile, segment) wavefront as
+// chain_visibility_seg_kernel, but each lane first back-projects its receiver
+// through the surviving chains of a chunk (no traversal), and the (receiver,
+// chain) pairs that land in their faces queue up across chunks; their D+1 legs
+// are traced 64 pairs per round with every lane busy, and each visible pair
+// sets its bit with an atomic OR into the zero-filled masks. A wavefront's
+// queue lives in its own LDS rows (wave barriers only).
+__global__ void __launch_bounds__(256) chain_visibility_seg_queue_kernel(SceneView s, wc_solve q,
+                                                                         const int* __restrict__ tiles,
+                                                                         int tile_count, unsigned* __restrict__ masks,
+                                                                         int words,
+                                                                         const unsigned* __restrict__ super_masks,
+                                                                         long long wave_base) {
+    __shared__ unsigned queue[4][128];
+    __shared__ float4 lane_rx[4][64];
+    const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int segs = (words + 63) >> 6;
+    const long long wave = wave_base + (long long)blockIdx.x * 4 + wid;
+    if (wave >= (long long)tile_count * segs) return;
+    const int tile = (int)(wave / segs);
+    const int seg = (int)(wave - (long long)tile * segs);
+    const int t = tiles[2 * tile];
+    const int base = q.chain_offset[t * kMaxDepth];
+    const int end = q.chain_offset[t * kMaxDepth + min(q.max_depth, kMaxDepth)];
+    const int tx_words = mask_words(end - base);
+    const int w = seg * 64 + lane;
+    const unsigned sword = w < tx_words ? super_masks[(size_t)super_of_tile(q, tile) * words + w] : 0u;
+    unsigned long long chunks = __ballot(sword != 0u);
+    if (chunks == 0ull) return;
+    const int first = tiles[2 * tile + 1];
+    const int pos = first + lane;
+    const bool live = pos < q.cell_offset[t + 1];
+    const int cell = live ? cell_at(q, pos) : 0;
+    const V3<float> tx = ld3(q.tx_pos + 3 * t);
+    V3<float> rx{0.0f, 0.0f, 0.0f};
+    if (live) rx = ld3(q.rx_pos + 3 * (q.rx_shared ? cell - q.cell_offset[t] : cell));
+    const V3<float> lo{wave_min(live ? rx.x : INFINITY), wave_min(live ? rx.y : INFINITY),
+                       wave_min(live ? rx.z : INFINITY)};
+    const V3<float> hi{wave_max(live ? rx.x : -INFINITY), wave_max(live ? rx.y : -INFINITY),
+                       wave_max(live ? rx.z : -INFINITY)};
+    lane_rx[wid][lane] = make_float4(rx.x, rx.y, rx.z, __int_as_float(cell));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int queued = 0;  // wave-uniform
+    // one round: the first 64 queued pairs are traced, the rest move down
+    auto round = [&]() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const long long item = lane < queued ? (long long)queue[wid][lane] : -1;
+        const long long carry = lane + 64 < queued ? (long long)queue[wid][lane + 64] : -1;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (carry >= 0) queue[wid][lane] = (unsigned)carry;
+        queued = queued > 64 ? queued - 64 : 0;
+        if (item >= 0) {
+            const int o = (int)(item & 63), b = (int)((item >> 6) & 63), bb = (int)(item >> 12);
+            const float4 r4 = lane_rx[wid][o];
+            const V3<float> orx{r4.x, r4.y, r4.z};
+            const int chunk = seg * 64 + b;
+            const int c = base + chunk * 32 + bb;
+            int d = 1;
+            while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+            bool ok;
+            switch (d) {
+                case 1: ok = chain_legs<1>(s, q, c, tx, orx); break;
+                case 2: ok = chain_legs<2>(s, q, c, tx, orx); break;
+                case 3: ok = chain_legs<3>(s, q, c, tx, orx); break;
+                case 4: ok = chain_legs<4>(s, q, c, tx, orx); break;
+                default: ok = chain_legs<5>(s, q, c, tx, orx); break;
+            }
+            if (ok) atomicOr(&masks[(size_t)__float_as_int(r4.w) * words + chunk], 1u << bb);
+        }
+    };
+    int d = 1;
+    while (chunks) {
+        const int b = __ffsll((long long)chunks) - 1;
+        chunks &= chunks - 1;
+        const int chunk = seg * 64 + b;
+        const unsigned kept = (unsigned)__shfl((int)sword, b);
+        const int c0 = base + chunk * 32;
+        const int c1 = min(c0 + 32, end);
+        bool reach = false;
+        if (lane < c1 - c0 && ((kept >> lane) & 1u)) {
+            const int c = c0 + lane;
+            int dd = 1;
+            while (c >= q.chain_offset[t * kMaxDepth + dd]) ++dd;
+            reach = chain_reaches_box(s, q, c, dd, lo, hi);
+        }
+        unsigned todo = (unsigned)__ballot(reach);
+        while (todo) {
+            const int bb = __ffs(todo) - 1;
+            todo &= todo - 1;
+            const int c = c0 + bb;
+            while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
+            bool lands = false;
+            if (live) {
+                switch (d) {
+                    case 1: lands = chain_lands<1>(s, q, c, tx, rx); break;
+                    case 2: lands = chain_lands<2>(s, q, c, tx, rx); break;
+                    case 3: lands = chain_lands<3>(s, q, c, tx, rx); break;
+                    case 4: lands = chain_lands<4>(s, q, c, tx, rx); break;
+                    default: lands = chain_lands<5>(s, q, c, tx, rx); break;
+                }
+            }
+            const unsigned long long ball = __ballot(lands);
+            if (lands)
+                queue[wid][queued + (int)__popcll(ball & ((1ull << lane) - 1ull))] =
+                    (unsigned)(lane | (b << 6) | (bb << 12));
+            queued += (int)__popcll(ball);
+            if (queued >= 64) round();
+        }
+    }
+    while (queued > 0) round();
+}
+
 // Chain validity for per-transmitter point sets (scattered receivers: a
 // wavefront's cells gain little from testing the same chain together). After
 // the tile cull, each lane back-projects its cell through the surviving
@@ -1049,12 +1281,13 @@ __device__ __forceinline__ bool chain_legs(const SceneView& s, const wc_solve& q
 
 __global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s, wc_solve q,
                                                                      const int* __restrict__ tiles, int tile_count,
-                                                                     unsigned* __restrict__ masks, int words) {
+                                                                     unsigned* __restrict__ masks, int words,
+                                                                     const unsigned* __restrict__ super_masks, long long wave_base) {
     __shared__ unsigned short queue[4][128];
     __shared__ float4 lane_rx[4][64];
     __shared__ unsigned found[4][64];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * 4 + wid;
+    const long long wave = wave_base + (long long)blockIdx.x * 4 + wid;
     const bool wave_live = wave < (long long)tile_count * words;
     const int tile = wave_live ? (int)(wave / words) : 0;
     const int chunk = wave_live ? (int)(wave - (long long)tile * words) : 0;
@@ -1074,9 +1307,13 @@ __global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s
     const int end = q.chain_offset[t * kMaxDepth + min(q.max_depth, kMaxDepth)];
     const int c0 = base + chunk * 32;
     const int c1 = min(c0 + 32, end);
-    // 1. tile cull, one chain per lane
+    // 1. tile cull, one chain per lane, of the chains the tile's supertile
+    // kept (a wave whose supertile kept none runs through with no work: the
+    // block barriers below need every wave)
+    const unsigned kept = (super_masks && wave_live) ? super_masks[(size_t)super_of_tile(q, tile) * words + chunk]
+                                                     : ~0u;
     bool reach = false;
-    if (wave_live && lane < c1 - c0) {
+    if (wave_live && lane < c1 - c0 && ((kept >> lane) & 1u)) {
         const int c = c0 + lane;
         int d = 1;
         while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
@@ -1330,8 +1567,8 @@ __host__ __device__ inline bool fan_plane_misses(const FanPlane& p, const Tri& t
 // chip). Only the triangles that reach an edge's fan plane take the
 // double-precision section test; the P lanes of an edge then OR their bins.
 __global__ void __launch_bounds__(256) edge_shadow_kernel(SceneView s, wc_solve q, int lanes_per_edge,
-                                                          unsigned long long* __restrict__ shadow) {
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+                                                          unsigned long long* __restrict__ shadow, long long wave_base) {
+    const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int edges = s.edge_count;
     const int per_wave = 64 / lanes_per_edge;
     const int groups = (edges + per_wave - 1) / per_wave;
@@ -1363,8 +1600,8 @@ __global__ void __launch_bounds__(256) edge_shadow_kernel(SceneView s, wc_solve 
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VIS_WAVES_PER_EU)))
 diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, int words, long long waves,
-                       const unsigned long long* __restrict__ shadow) {
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+                       const unsigned long long* __restrict__ shadow, long long wave_base) {
+    const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= waves) return;
     const int chunk = (int)(wave % words);
     const long long pos64 = (wave / words) * 64 + (threadIdx.x & 63);
@@ -1404,14 +1641,14 @@ diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, in
 // those of diff_visibility_kernel.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VIS_WAVES_PER_EU)))
 diff_visibility_queue_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, int words, long long waves,
-                       const unsigned long long* __restrict__ shadow) {
+                       const unsigned long long* __restrict__ shadow, long long wave_base) {
     // per wavefront: the candidate queue (lane | edge << 6), each lane's
     // receiver and transmitter, and the visible bits
     __shared__ unsigned short queue[4][128];
     __shared__ float4 lane_rx[4][64];
     __shared__ unsigned found[4][64];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * 4 + wid;
+    const long long wave = wave_base + (long long)blockIdx.x * 4 + wid;
     // waves past the end take part in the barriers with no work
     const bool wave_live = wave < waves;
     const int chunk = wave_live ? (int)(wave % words) : 0;
@@ -1677,6 +1914,11 @@ static int check_solve(const wc_solve* q) {
         return wc_fail("solve endpoint pointers are null");
     if ((q->components & kCompRefl) && q->max_depth > 0 && (!q->chain_offset || !q->chain_groups || !q->chain_images))
         return wc_fail("reflection requested without chain tables");
+    if (q->visibility_mode < 0 || q->visibility_mode > 2) return wc_fail("visibility_mode must be 0, 1 or 2");
+    if (q->super_count < 0 || (size_t)q->super_count > super_capacity(q->total_cells, q->tx_count))
+        return wc_fail("super_count %d exceeds ceil(total_cells / 1024) + tx_count", q->super_count);
+    if (q->super_count > 0 && (!q->super_first_tile || !q->tile_super))
+        return wc_fail("super_count > 0 needs super_first_tile and tile_super");
     return 0;
 }
 
@@ -1691,9 +1933,41 @@ __global__ void __launch_bounds__(256) fill_zero_kernel(unsigned* p, size_t n) {
         p[i] = 0u;
 }
 
+// Launch-size guard of the wavefront-indexed kernels: a grid covers at most
+// kWavesPerLaunch wavefronts (4 per 256-thread block, 2^28 threads), larger
+// wave counts run as consecutive launches with their first wave passed as
+// wave_base. Every launch's status is kept (the first failure is reported by
+// the entry point even when later launches succeed).
+constexpr long long kWavesPerLaunch = 1LL << 22;
+#define WC_LAUNCH(...)                 \
+    do {                               \
+        hipLaunchKernelGGL(__VA_ARGS__); \
+        note_launch();                 \
+    } while (0)
+thread_local hipError_t t_launch_error = hipSuccess;
+
+static void note_launch() {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess && t_launch_error == hipSuccess) t_launch_error = e;
+}
+
+template <class Launch>
+static void for_wave_chunks(long long waves, Launch&& launch) {
+    for (long long base = 0; base < waves; base += kWavesPerLaunch) {
+        const long long n = std::min(kWavesPerLaunch, waves - base);
+        launch(dim3((unsigned)((n + 3) / 4)), base);
+        note_launch();
+    }
+}
+
 // chain words (32 chains each) from which the supertile cull pays for its
 // two extra launches
 constexpr int kSuperMinWords = 64;
+
+// the tile loops (neighbouring receivers per wavefront) or the queued legs
+__host__ inline bool coherent_tiles(const wc_solve& q) {
+    return q.visibility_mode == 1 || (q.visibility_mode == 0 && q.rx_shared) || q.lanes_per_cell != 1;
+}
 
 static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o, const int* tiles,
                                     int tile_count, hipStream_t st) {
@@ -1701,36 +1975,58 @@ static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const
     const int words = mask_words(q.max_tx_chains);
     unsigned* masks = chain_masks(o.workspace, q, s.edge_count);
     const long long waves = (long long)tile_count * words;
-    // a shared receiver grid tests each chain coherently across the tile;
-    // per-transmitter point sets take the queued form
-    if (q.rx_shared || q.lanes_per_cell != 1) {
-        // one transmitter with many chains: cull chains per supertile first
-        // (tiles are then the 64-position blocks in order, 16 per supertile)
-        const unsigned* super_masks = nullptr;
-        if (q.tx_count == 1 && q.lanes_per_cell == 1 && words >= kSuperMinWords) {
-            const SuperLayout l = super_layout(
-                shadow_end(q.total_cells, q.lanes_per_cell, s.edge_count, q.max_tx_chains, q.tx_count),
-                q.total_cells, q.max_tx_chains);
-            char* ws = static_cast<char*>(o.workspace);
-            float* boxes = reinterpret_cast<float*>(ws + l.boxes);
-            unsigned* sm = reinterpret_cast<unsigned*>(ws + l.masks);
-            const int supers = (q.total_cells + kSuperCells - 1) / kSuperCells;
-            hipLaunchKernelGGL(super_box_kernel, dim3((unsigned)((supers + 3) / 4)), dim3(256), 0, st, q, supers,
-                               boxes);
-            const long long sw = (long long)supers * words;
-            hipLaunchKernelGGL(super_cull_kernel, dim3((unsigned)((sw + 3) / 4)), dim3(256), 0, st, s, q, supers,
-                               words, boxes, sm);
-            const size_t mask_count = (size_t)q.total_cells * (size_t)words;
-            hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((mask_count + 255) / 256, 65536)),
-                               dim3(256), 0, st, masks, mask_count);
-            super_masks = sm;
+    // many chains: cull them per supertile first (host supertile tables, or
+    // for one transmitter the 1,024-position blocks in order); the tile pass
+    // then culls only what its supertile kept, and a tile whose supertile kept
+    // nothing of a chunk writes nothing, so the masks are zero-filled first
+    const unsigned* super_masks = nullptr;
+    const bool tables = q.super_first_tile && q.tile_super && q.super_count > 0;
+    if (q.lanes_per_cell == 1 && words >= kSuperMinWords && (tables || q.tx_count == 1)) {
+        const SuperLayout l = super_layout(
+            shadow_end(q.total_cells, q.lanes_per_cell, s.edge_count, q.max_tx_chains, q.tx_count), q.total_cells,
+            q.max_tx_chains, q.tx_count);
+        char* ws = static_cast<char*>(o.workspace);
+        float* boxes = reinterpret_cast<float*>(ws + l.boxes);
+        unsigned* sm = reinterpret_cast<unsigned*>(ws + l.masks);
+        wc_solve qs = q;
+        if (!tables) {
+            qs.super_first_tile = nullptr;
+            qs.tile_super = nullptr;
         }
-        hipLaunchKernelGGL(chain_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q,
-                           tiles, tile_count, masks, words, super_masks);
+        const int supers = tables ? q.super_count : (q.total_cells + kSuperCells - 1) / kSuperCells;
+        hipLaunchKernelGGL(super_box_kernel, dim3((unsigned)((supers + 3) / 4)), dim3(256), 0, st, qs, tiles, supers,
+                           boxes);
+        note_launch();
+        for_wave_chunks((long long)supers * words, [&](dim3 grid, long long base) {
+            hipLaunchKernelGGL(super_cull_kernel, grid, dim3(256), 0, st, s, qs, tiles, supers, words, boxes, sm, base);
+        });
+        const size_t mask_count = (size_t)q.total_cells * (size_t)words;
+        hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((mask_count + 255) / 256, 65536)),
+                           dim3(256), 0, st, masks, mask_count);
+        note_launch();
+        super_masks = sm;
+        if (coherent_tiles(q))
+            for_wave_chunks((long long)tile_count * ((words + 63) / 64), [&](dim3 grid, long long base) {
+                hipLaunchKernelGGL(chain_visibility_seg_kernel, grid, dim3(256), 0, st, s, qs, tiles, tile_count,
+                                   masks, words, super_masks, base);
+            });
+        else
+            for_wave_chunks((long long)tile_count * ((words + 63) / 64), [&](dim3 grid, long long base) {
+                hipLaunchKernelGGL(chain_visibility_seg_queue_kernel, grid, dim3(256), 0, st, s, qs, tiles,
+                                   tile_count, masks, words, super_masks, base);
+            });
+        return;
     }
+    if (coherent_tiles(q))
+        for_wave_chunks(waves, [&](dim3 grid, long long base) {
+            hipLaunchKernelGGL(chain_visibility_kernel, grid, dim3(256), 0, st, s, q, tiles, tile_count, masks, words,
+                               super_masks, base);
+        });
     else
-        hipLaunchKernelGGL(chain_visibility_queue_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s,
-                           q, tiles, tile_count, masks, words);
+        for_wave_chunks(waves, [&](dim3 grid, long long base) {
+            hipLaunchKernelGGL(chain_visibility_queue_kernel, grid, dim3(256), 0, st, s, q, tiles, tile_count, masks,
+                               words, super_masks, base);
+        });
 }
 
 // Compute-unit count of the current device, queried once per device and
@@ -1769,20 +2065,25 @@ static void launch_diff_visibility(const SceneView& s, const wc_solve& q, const 
         const int lanes_per_edge = 64;
         const int per_wave = 64 / lanes_per_edge;
         const long long shadow_waves = (long long)q.tx_count * ((s.edge_count + per_wave - 1) / per_wave);
-        hipLaunchKernelGGL(edge_shadow_kernel, dim3((unsigned)((shadow_waves + 3) / 4)), dim3(256), 0, st, s, q,
-                           lanes_per_edge, shadow);
+        for_wave_chunks(shadow_waves, [&](dim3 grid, long long base) {
+            hipLaunchKernelGGL(edge_shadow_kernel, grid, dim3(256), 0, st, s, q, lanes_per_edge, shadow, base);
+        });
     }
     const long long waves = ((long long)q.total_cells + 63) / 64 * words;
     // a receiver grid shared by the transmitters keeps neighbouring receivers
     // in a wavefront (the edge-by-edge loop traces coherently: 0.84 vs 0.97 ms
     // on the 256 x 256 canyon); per-transmitter point sets take the queue
     // (1.16 vs 1.26 ms on 64 transmitters with scattered street points)
-    if (q.rx_shared)
-        hipLaunchKernelGGL(diff_visibility_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q, masks,
-                           words, waves, (const unsigned long long*)shadow);
+    const unsigned long long* sh = shadow;
+    if (coherent_tiles(q))
+        for_wave_chunks(waves, [&](dim3 grid, long long base) {
+            hipLaunchKernelGGL(diff_visibility_kernel, grid, dim3(256), 0, st, s, q, masks, words, waves, sh, base);
+        });
     else
-        hipLaunchKernelGGL(diff_visibility_queue_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, s, q,
-                           masks, words, waves, (const unsigned long long*)shadow);
+        for_wave_chunks(waves, [&](dim3 grid, long long base) {
+            hipLaunchKernelGGL(diff_visibility_queue_kernel, grid, dim3(256), 0, st, s, q, masks, words, waves, sh,
+                               base);
+        });
 }
 
 // Launch stages: bit 0 chain validity (tiled only), bit 1 the
@@ -1811,6 +2112,8 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
     if (solve->total_cells == 0 || stages == 0) return 0;
     const SceneView s = *reinterpret_cast<const SceneView*>(scene);
     hipStream_t st = (hipStream_t)stream;
+    (void)hipGetLastError();
+    t_launch_error = hipSuccess;
     if ((diffraction || chains) && !out->workspace)
         return wc_fail("%s: workspace (wc_radiomap_workspace_bytes) is required with diffraction or reflection", who);
     if (tiles) {
@@ -1821,12 +2124,12 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
         // a fill kernel rather than a memset node keeps the launches graph-capturable
         const bool main = (stages & kStageMain) != 0, diff = (stages & kStageDiff) != 0;
         if (main && diff)
-            hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 2);
+            WC_LAUNCH(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter, 2);
         else if (main || diff)
-            hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter + (main ? 0 : 1), 1);
+            WC_LAUNCH(zero_int_kernel, dim3(1), dim3(64), 0, st, tile_counter + (main ? 0 : 1), 1);
         if (stages & kStageChains) launch_chain_visibility(s, *solve, *out, tiles, tile_count, st);
         if (main)
-            hipLaunchKernelGGL(radiomap_main_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
+            WC_LAUNCH(radiomap_main_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter);
         if (stages & kStageEdges) launch_diff_visibility(s, *solve, *out, st);
         if (diff) {
@@ -1841,29 +2144,31 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
                 int* ctrl = reinterpret_cast<int*>(ws + l.ctrl);
                 DiffRecord* records = reinterpret_cast<DiffRecord*>(ws + l.records);
                 const unsigned* masks = reinterpret_cast<const unsigned*>(ws + partial_bytes(n, width));
-                hipLaunchKernelGGL(diff_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, *solve, masks,
+                WC_LAUNCH(diff_count_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, *solve, masks,
                                    words, offsets);
-                hipLaunchKernelGGL(diff_scan_kernel, dim3(1), dim3(1024), 0, st, offsets, n,
+                WC_LAUNCH(diff_scan_kernel, dim3(1), dim3(1024), 0, st, offsets, n,
                                    diff_record_capacity(n, s.edge_count), ctrl);
-                hipLaunchKernelGGL(diff_record_kernel, dim3(cached_cus * 4), dim3(256), 0, st, s, *solve, masks,
+                WC_LAUNCH(diff_record_kernel, dim3(cached_cus * 4), dim3(256), 0, st, s, *solve, masks,
                                    words, offsets, ctrl, records);
-                hipLaunchKernelGGL(diff_sum_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, *solve, *out,
+                WC_LAUNCH(diff_sum_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, *solve, *out,
                                    offsets, ctrl, records);
                 staged = ctrl;
             }
-            hipLaunchKernelGGL(radiomap_diff_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
+            WC_LAUNCH(radiomap_diff_tiles, dim3(blocks), dim3(256), 0, st, s, *solve, *out, tiles,
                                tile_count, tile_counter + 1, staged);
         }
     } else {
         if (stages & kStageMain)
-            hipLaunchKernelGGL(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
+            WC_LAUNCH(radiomap_kernel<kPassMain>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
                                st, s, *solve, *out);
         if (stages & kStageEdges) launch_diff_visibility(s, *solve, *out, st);
         if (stages & kStageDiff)
-            hipLaunchKernelGGL(radiomap_kernel<kPassDiff>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
+            WC_LAUNCH(radiomap_kernel<kPassDiff>, dim3(blocks_for(solve->total_cells, 256)), dim3(256), 0,
                                st, s, *solve, *out);
     }
-    const hipError_t err = hipGetLastError();
+    note_launch();
+    const hipError_t err = t_launch_error;
+    t_launch_error = hipSuccess;
     return err == hipSuccess ? 0 : wc_fail_hip(who, err);
 }
 

@@ -61,6 +61,8 @@ class WcSolve(ctypes.Structure):
         ("chain_offset", c_void_p), ("chain_groups", c_void_p), ("chain_images", c_void_p),
         ("frequency_hz", c_double), ("components", c_int), ("max_depth", c_int), ("coherent", c_int),
         ("lanes_per_cell", c_int), ("cell_order", c_void_p), ("max_tx_chains", c_int),
+        ("super_first_tile", c_void_p), ("tile_super", c_void_p), ("super_count", c_int),
+        ("visibility_mode", c_int),
     ]
 
 

@@ -220,13 +220,14 @@ class MeanDbStep:
     """
 
     def __init__(self, dscene, inputs: engine.SolveInputs, material_count: int,
-                 topology_capacity: int = DEFAULT_TOPOLOGY_CAPACITY, vertices: bool = False) -> None:
+                 topology_capacity: int = DEFAULT_TOPOLOGY_CAPACITY, vertices: bool = False,
+                 workspace: torch.Tensor | None = None) -> None:
         self.dscene = dscene
         self.inputs = inputs
         self.m = material_count
         self.capacity = topology_capacity
         device = dscene.device
-        self.out = engine.forward_with_topology(dscene, inputs, capacity=topology_capacity)
+        self.out = engine.forward_with_topology(dscene, inputs, capacity=topology_capacity, workspace=workspace)
         self.capacity = self.out.topology_capacity
         power = self.out.power_total
         lit = power > 0
@@ -262,8 +263,17 @@ class MeanDbStep:
             self.grad_vertices = torch.empty((int(dscene.vertices.shape[0]), 3), dtype=torch.float32, device=device)
             self.csr = dscene.vertex_csr()
 
-    def launch(self) -> None:
-        engine.radiomap_forward(self.dscene, self.inputs, topology_capacity=self.capacity, out=self.out)
+    def launch(self, reuse_visibility: bool = False) -> None:
+        """One step. ``reuse_visibility``: the chain-validity and edge-visibility
+        masks in the workspace are those of this solve's geometry (left by a
+        step of the same transmitters and receivers at another carrier), so
+        only the field passes run; the outputs are identical."""
+        if reuse_visibility:
+            engine.radiomap_pass(self.dscene, self.inputs, self.out, 2)
+            if int(self.inputs.view.components) & 4:
+                engine.radiomap_pass(self.dscene, self.inputs, self.out, 4)
+        else:
+            engine.radiomap_forward(self.dscene, self.inputs, topology_capacity=self.capacity, out=self.out)
         power = self.out.power_total
         torch.div(self.scale, torch.clamp_min(power, 1.0e-38), out=self.grad_power)
         self.adjoint()

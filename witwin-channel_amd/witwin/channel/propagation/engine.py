@@ -97,23 +97,24 @@ def unique_prefixes(rows: np.ndarray, depth: int) -> np.ndarray:
     return np.unique(pref, axis=0).astype(np.int32)
 
 
-def plan_chains(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tensor, max_depth: int, *,
-                reflection: bool, ray_count: int = DISCOVERY_RAY_COUNT,
-                discovered: np.ndarray | None = None) -> ChainPlan:
-    """Candidate reflection chains of every transmitter (sorted by depth, then
-    lexicographically by group sequence)."""
-    device = tx_pos.device
+def chain_sequences(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tensor, max_depth: int, *,
+                    reflection: bool, ray_count: int = DISCOVERY_RAY_COUNT,
+                    discovered: np.ndarray | None = None) -> list[list[np.ndarray]]:
+    """Candidate group sequences of every transmitter: ``[T][d - 1]`` int32
+    arrays [K_d, d] for depths 1..5 (empty above ``max_depth``), each sorted
+    lexicographically — exhaustive when the count is small, otherwise the
+    distinct prefixes of the transmitter's discovery rays (reference
+    reflection.py:186-205, 541-569)."""
     t = int(tx_pos.shape[0])
     g = compiled.group_count
     depths = range(1, max_depth + 1) if reflection and g > 0 else range(0)
     needs_rays = [d for d in depths if not is_exhaustive(g, d)]
     if needs_rays and discovered is None:
         discovered = discover(dscene, tx_pos, max(needs_rays), ray_count)
-    blocks = []
-    offsets = np.zeros(t * MAX_DEPTH + 1, dtype=np.int32)
-    cursor = 0
     exhaustive_cache: dict[int, np.ndarray] = {}
+    out = []
     for ti in range(t):
+        per = []
         for d in range(1, MAX_DEPTH + 1):
             seqs = np.zeros((0, d), np.int32)
             if d in depths:
@@ -123,6 +124,24 @@ def plan_chains(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tens
                     if d not in exhaustive_cache:
                         exhaustive_cache[d] = exhaustive_sequences(g, d)
                     seqs = exhaustive_cache[d]
+            per.append(seqs)
+        out.append(per)
+    return out
+
+
+def plan_from_sequences(dscene: DeviceScene, tx_pos: torch.Tensor, sequences: list[list[np.ndarray]],
+                        discovered_depths: tuple[int, ...] = ()) -> ChainPlan:
+    """ChainPlan of the transmitters ``tx_pos`` [T, 3] from their candidate
+    sequences (``chain_sequences``): chains CSR by (transmitter, depth) with
+    their double-precision image points."""
+    device = tx_pos.device
+    t = int(tx_pos.shape[0])
+    blocks = []
+    offsets = np.zeros(t * MAX_DEPTH + 1, dtype=np.int32)
+    cursor = 0
+    for ti in range(t):
+        for d in range(1, MAX_DEPTH + 1):
+            seqs = sequences[ti][d - 1]
             padded = np.full((seqs.shape[0], MAX_DEPTH), -1, dtype=np.int32)
             padded[:, :d] = seqs
             blocks.append((padded, ti, d))
@@ -146,8 +165,21 @@ def plan_chains(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tens
                                                                                          dtype=torch.int32,
                                                                                          device=device),
         images=images, depth=depth_t, tx=tx_t, groups_host=groups, offset_host=offsets,
-        discovered_depths=tuple(needs_rays),
+        discovered_depths=tuple(discovered_depths),
     )
+
+
+def plan_chains(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tensor, max_depth: int, *,
+                reflection: bool, ray_count: int = DISCOVERY_RAY_COUNT,
+                discovered: np.ndarray | None = None) -> ChainPlan:
+    """Candidate reflection chains of every transmitter (sorted by depth, then
+    lexicographically by group sequence)."""
+    g = compiled.group_count
+    depths = range(1, max_depth + 1) if reflection and g > 0 else range(0)
+    needs_rays = tuple(d for d in depths if not is_exhaustive(g, d))
+    seqs = chain_sequences(compiled, dscene, tx_pos, max_depth, reflection=reflection, ray_count=ray_count,
+                           discovered=discovered)
+    return plan_from_sequences(dscene, tx_pos, seqs, needs_rays)
 
 
 @dataclass
@@ -183,6 +215,28 @@ def make_tiles(cell_offset: np.ndarray, tile_cells: int = TILE) -> np.ndarray:
     if not parts:
         return np.zeros((0, 2), dtype=np.int32)
     return np.concatenate(parts, axis=0).astype(np.int32)
+
+
+SUPER_TILES = 16  # tiles per supertile of the chain-validity cull
+
+
+def super_tables(tiles: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Supertiles of a tile list (wc_solve.super_first_tile / tile_super):
+    runs of at most ``SUPER_TILES`` consecutive tiles of one transmitter.
+    Returns ([S + 1] first tile of each supertile, [tiles] supertile of each
+    tile)."""
+    tiles = np.asarray(tiles).reshape(-1, 2)
+    n = tiles.shape[0]
+    if n == 0:
+        return np.zeros(1, np.int32), np.zeros(0, np.int32)
+    tx = tiles[:, 0].astype(np.int64)
+    start = np.ones(n, dtype=bool)
+    start[1:] = tx[1:] != tx[:-1]
+    run_start = np.maximum.accumulate(np.where(start, np.arange(n), 0))
+    first = start | ((np.arange(n) - run_start) % SUPER_TILES == 0)
+    firsts = np.nonzero(first)[0]
+    tile_super = (np.cumsum(first) - 1).astype(np.int32)
+    return np.concatenate([firsts, [n]]).astype(np.int32), tile_super
 
 
 def max_tx_chains(chains: ChainPlan, tx_count: int) -> int:
@@ -243,7 +297,7 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
                  rx_pos: np.ndarray | None = None, rx_pol: np.ndarray | None = None,
                  rx_count: np.ndarray | None = None, ray_count: int = DISCOVERY_RAY_COUNT,
                  chains: ChainPlan | None = None, lanes_per_cell: int = LANES_PER_CELL,
-                 spatial_order: bool = True) -> SolveInputs:
+                 spatial_order: bool = True, visibility_mode: int = 0) -> SolveInputs:
     """Upload endpoint tables. Without ``rx_count`` every transmitter shares
     all receivers; with it, receivers are the concatenation of per-transmitter
     sets of the given sizes (uneven batched solves). ``lanes_per_cell`` lanes
@@ -285,6 +339,12 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
              if spatial_order and offsets[-1] > 0 else None)
     tiles_np = make_tiles(offsets, TILE // lanes_per_cell)
     tiles = torch.as_tensor(tiles_np if tiles_np.size else np.zeros((1, 2), np.int32)).to(device)
+    # supertiles of the chain cull hold 16 tiles of 64 positions (one lane per cell)
+    super_first, tile_super = super_tables(tiles_np if lanes_per_cell == 1 else np.zeros((0, 2), np.int32))
+    super_first_t = torch.as_tensor(super_first).to(device)
+    tile_super_t = torch.as_tensor(tile_super if tile_super.size else np.zeros(1, np.int32)).to(device)
+    if lanes_per_cell != 1:
+        super_first_t = None
     inputs = SolveInputs(tx_pos_t, f32(tx_pol).reshape(-1, 3), f32(tx_power).reshape(-1), f32(rx_pos).reshape(-1, 3),
                          f32(rx_pol).reshape(-1, 3), cell_offset, shared, chains, None, tiles,  # type: ignore[arg-type]
                          int(tiles_np.shape[0]), counts, order)
@@ -296,7 +356,11 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
         chain_groups=p(chains.groups), chain_images=p(chains.images), frequency_hz=float(frequency_hz),
         components=int(components), max_depth=int(max_depth), coherent=1 if coherent else 0,
         lanes_per_cell=int(lanes_per_cell), cell_order=p(order), max_tx_chains=max_tx_chains(chains, t),
+        super_first_tile=p(super_first_t), tile_super=p(tile_super_t) if super_first_t is not None else None,
+        super_count=int(super_first.shape[0] - 1) if super_first_t is not None else 0,
+        visibility_mode=int(visibility_mode),
     )
+    inputs.__dict__["_super_tables"] = (super_first_t, tile_super_t)
     return inputs
 
 
@@ -314,8 +378,19 @@ class RadiomapOutputs:
     workspace: torch.Tensor          # per-cell partial sums between the two radiomap passes
 
 
+def workspace_bytes(dscene: DeviceScene, inputs: SolveInputs) -> int:
+    """Bytes of the radiomap workspace of one solve (wc_radiomap_workspace_bytes)."""
+    return max(int(_native.call("wc_radiomap_workspace_bytes", inputs.total_cells, int(inputs.view.lanes_per_cell),
+                                int(dscene.view.edge_count), int(inputs.view.max_tx_chains),
+                                int(inputs.view.tx_count))), 32)
+
+
 def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capacity: int = 0,
-                     tiled: bool = True, out: RadiomapOutputs | None = None) -> RadiomapOutputs:
+                     tiled: bool = True, out: RadiomapOutputs | None = None,
+                     workspace: torch.Tensor | None = None) -> RadiomapOutputs:
+    """Fused radiomap forward. ``workspace`` (uint8, at least
+    ``workspace_bytes``) lets solves that run one after another share one
+    buffer; by default each output set owns its own."""
     device = dscene.device
     cells = inputs.total_cells
     width = int(inputs.view.lanes_per_cell)
@@ -335,13 +410,11 @@ def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capac
                       if topology_capacity > 0 else None),
             topology_capacity=topology_capacity,
             failure=torch.zeros(1, dtype=torch.int32, device=device),
-            workspace=torch.empty(max(int(_native.call("wc_radiomap_workspace_bytes", cells,
-                                                       int(inputs.view.lanes_per_cell),
-                                                       int(dscene.view.edge_count),
-                                                       int(inputs.view.max_tx_chains),
-                                                       int(inputs.view.tx_count))), 32),
-                                  dtype=torch.uint8, device=device),
+            workspace=(workspace if workspace is not None else
+                       torch.empty(workspace_bytes(dscene, inputs), dtype=torch.uint8, device=device)),
         )
+        if out.workspace.numel() < workspace_bytes(dscene, inputs):
+            raise ValueError("radiomap workspace is smaller than wc_radiomap_workspace_bytes")
     p = _native.ptr
     view = _out_view(out)
     counter = _tile_counter(inputs, device)
@@ -381,17 +454,17 @@ def radiomap_pass(dscene: DeviceScene, inputs: SolveInputs, out: RadiomapOutputs
 
 
 def forward_with_topology(dscene: DeviceScene, inputs: SolveInputs, *, capacity: int = 64,
-                          tiled: bool = True) -> RadiomapOutputs:
+                          tiled: bool = True, workspace: torch.Tensor | None = None) -> RadiomapOutputs:
     """Forward pass that stores every cell's path descriptors. When a cell
     holds more paths than ``capacity`` (failure bit 2) the descriptor buffer is
     regrown to the next power of two above the largest cell count and the
     forward is rerun, so the adjoint always sees the complete topology."""
-    out = radiomap_forward(dscene, inputs, topology_capacity=capacity, tiled=tiled)
+    out = radiomap_forward(dscene, inputs, topology_capacity=capacity, tiled=tiled, workspace=workspace)
     if int(out.failure.item()) & 2:
         worst = int(out.path_count.max().item())
         # a single lane of a cell's lane group may hold all of its paths
         grown = int(inputs.view.lanes_per_cell) * (1 << max(worst - 1, 1).bit_length())
-        out = radiomap_forward(dscene, inputs, topology_capacity=grown, tiled=tiled)
+        out = radiomap_forward(dscene, inputs, topology_capacity=grown, tiled=tiled, workspace=workspace)
         if int(out.failure.item()) & 2:
             raise RuntimeError("path descriptor capacity still exceeded after regrowth")
     return out

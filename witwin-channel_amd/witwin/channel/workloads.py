@@ -271,3 +271,87 @@ def heightfield_scene(seed: int = 4, size: int = 128, pitch: float = 2.0, relief
     tx = (HEIGHTFIELD_TX_OFFSET[0], HEIGHTFIELD_TX_OFFSET[1], top + HEIGHTFIELD_TX_OFFSET[2])
     return Scene(structures=structures, endpoints=[transmitter(tx), grid(origin, shape, spacing)],
                  metadata={"imported_edge_policy": SimpleNamespace(edge_selection_mode="vertical_only")})
+
+
+# eight materials of the scaled city: terrain soil, then seven wall materials
+# (relative permittivity, conductivity S/m, slab thickness m)
+CITY_MATERIALS = (
+    ("soil", 15.0, 0.035, 5.0),
+    ("concrete", 5.31, 0.0326, 0.2),
+    ("glass", 6.27, 0.0043, 0.02),
+    ("brick", 3.75, 0.038, 0.25),
+    ("wood", 1.99, 0.0047, 0.05),
+    ("plasterboard", 2.94, 0.0116, 0.015),
+    ("marble", 7.074, 0.0055, 0.05),
+    ("metal", 1.0, 1.0e7, 0.005),
+)
+
+
+def scaled_city(seed: int = 5, size: int = 256, pitch: float = 2.0, relief: float = 20.0, boxes: int = 400,
+                tx_count: int = 128, side_range: tuple[int, int] = (64, 512)) -> SimpleNamespace:
+    """Seeded city of ``boxes`` random boxes on a ``size`` x ``size`` fBm
+    heightfield (``pitch`` m, ``relief`` m; 256 x 256 gives 130,050 terrain
+    triangles and 4,800 box triangles), eight materials (``CITY_MATERIALS``),
+    ``tx_count`` transmitters 3-30 m above the terrain outside every box, and
+    one square receiver grid per transmitter whose side is drawn uniformly
+    from ``side_range`` and which spans the terrain 1.5 m above its highest
+    point. Diffraction edges follow the vertical-only policy.
+
+    Returns ``scene`` (all structures and transmitters, plus the first
+    transmitter's grid so the scene compiles) and the batched endpoint
+    tables: ``tx`` [T, 3], ``sides`` [T], ``rx_count`` [T] (= sides^2) and
+    ``rx`` [sum, 3] (each transmitter's grid, row-major).
+    """
+    rng = np.random.default_rng(seed + 2000)
+    heights = fbm_heights(seed, size, relief=relief)
+    extent = (size - 1) * pitch
+    mats = [PhysicalMaterial(eps_r=e, sigma_e=s, thickness_m=t, name=n) for n, e, s, t in CITY_MATERIALS]
+    tv, tf = heightfield_mesh(heights, pitch)
+    structures = [Structure(Mesh(torch.from_numpy(tv), torch.from_numpy(tf)), mats[0], name="terrain",
+                            surface_id=0)]
+    placed: list[tuple[float, float, float, float]] = []
+    attempts = 0
+    while len(placed) < boxes:
+        attempts += 1
+        if attempts > 200_000:
+            raise RuntimeError("scaled_city: could not place every box")
+        w, l = rng.uniform(6.0, 16.0, size=2)
+        x0 = rng.uniform(2.0, extent - 2.0 - w)
+        y0 = rng.uniform(2.0, extent - 2.0 - l)
+        box = (x0 - 1.5, y0 - 1.5, x0 + w + 1.5, y0 + l + 1.5)
+        if any(box[0] < p[2] and p[0] < box[2] and box[1] < p[3] and p[1] < box[3] for p in placed):
+            continue
+        placed.append(box)
+        i0, i1 = int(x0 // pitch), int(math.ceil((x0 + w) / pitch))
+        j0, j1 = int(y0 // pitch), int(math.ceil((y0 + l) / pitch))
+        under = heights[i0:i1 + 1, j0:j1 + 1]
+        base = float(under.min()) - 1.0
+        top = float(under.max()) + float(rng.uniform(6.0, 30.0))
+        v, f, _ = box_mesh(x0, y0, x0 + w, y0 + l, top - base)
+        v[:, 2] += base
+        mat = mats[1 + int(rng.integers(0, len(mats) - 1))]
+        structures.append(Structure(Mesh(torch.from_numpy(v), torch.from_numpy(f)), mat, name=f"box-{len(placed)}",
+                                    surface_id=len(placed)))
+    tx = []
+    while len(tx) < tx_count:
+        x, y = rng.uniform(4.0, extent - 4.0, size=2)
+        if any(p[0] <= x <= p[2] and p[1] <= y <= p[3] for p in placed):
+            continue
+        i, j = int(round(x / pitch)), int(round(y / pitch))
+        tx.append((x, y, float(heights[min(i, size - 1), min(j, size - 1)]) + float(rng.uniform(3.0, 30.0))))
+    tx = np.asarray(tx, dtype=np.float32)
+    sides = rng.integers(side_range[0], side_range[1] + 1, size=tx_count).astype(np.int64)
+    z = float(heights.max()) + 1.5
+    grids = []
+    for s in sides:
+        spacing = extent / float(s)
+        axis = (np.arange(s, dtype=np.float32) + 0.5) * np.float32(spacing)
+        gx, gy = np.meshgrid(axis, axis, indexing="ij")
+        grids.append(np.stack([gx.reshape(-1), gy.reshape(-1), np.full(gx.size, z, np.float32)], axis=1))
+    rx = np.concatenate(grids, axis=0).astype(np.float32)
+    first = grid((0.5 * extent / sides[0], 0.5 * extent / sides[0], z), (int(sides[0]), int(sides[0])),
+                 extent / float(sides[0]))
+    scene = Scene(structures=structures, endpoints=[transmitter(tuple(map(float, p))) for p in tx] + [first],
+                  metadata={"imported_edge_policy": SimpleNamespace(edge_selection_mode="vertical_only")})
+    return SimpleNamespace(scene=scene, tx=tx, sides=sides, rx_count=sides * sides, rx=rx, extent=extent,
+                           receiver_z=z, frequencies=(3.5e9, 28.0e9))
