@@ -1,20 +1,24 @@
 """Radiomap forward+VJP throughput on one MI355X.
 
-Workload (the value line): seeded urban canyon (8 x 8 box buildings, 770
-triangles, 3 materials incl. ground), one transmitter at a street
-intersection (53.231, 52.013, 25) m,
-28 GHz, up to 3 specular bounces + first-order UTD diffraction + line of sight,
-256 x 256 receiver grid at 2 m, z = 1.5 m. One step = radiomap forward (fused
-discovery + fields + float64 coherent cell sums, path descriptors stored) +
-the fixed-topology VJP of L = mean dB path gain over lit cells w.r.t. every
-material's eps_r and sigma, the frequency and the transmitter position, with
-inputs resident in HBM. ``value`` = cells / step time of the batched form
-(persistent wave-tiled launches captured once in a HIP graph and replayed).
+Workload of the value line: the scaled city (seed 5): 400 random boxes on a
+256 x 256 fBm heightfield (134,850 triangles), 8 materials, 128 transmitters
+3-30 m above the terrain, each with its own square receiver grid of side
+U{64..512} spanning the terrain (12,417,953 cells in all), 5-bounce specular
+reflection + first-order UTD diffraction + thin-sheet transmission + line of
+sight, at 3.5 GHz and 28 GHz. One step = for every transmitter and both
+carriers the radiomap forward (path descriptors stored) + the fixed-topology
+VJP of L = mean dB path gain over every lit (cell, carrier) w.r.t. every
+material's eps_r and sigma, both carrier frequencies and every transmitter
+position, with inputs resident in HBM. ``value`` = cells x carriers / step
+time of the batched form (transmitter batches sized by the autotuner, one
+persistent tiled launch per kernel and batch, the whole step captured once
+in a HIP graph and replayed). The plain form is the per-transmitter,
+per-carrier loop of the same kernels, launched eagerly.
 
-Also reported: the plain form (the same kernels launched eagerly per step),
-a 64-transmitter batched solve with uneven receiver sets (log-uniform 16..4096
-street points per transmitter) against a per-transmitter loop, the roofline of
-the dominant kernel, and the FP64 CPU oracle on the same cells.
+Also reported: the urban canyon (1 Tx, 256 x 256 grid), a 64-transmitter
+point-set batch on it, the differentiable heightfield with vertex gradients,
+the roofline of the dominant kernel and the FP64 CPU oracle on a bounded
+sample of the scaled city.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -128,11 +132,155 @@ def copy_calibration(nbytes: int = 256 << 20, reps: int = 3) -> int:
     return 0
 
 
+def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: bool = True,
+                        launch_shape: str | None = None) -> dict:
+    """The value workload: batched fwd+VJP over the scaled city (see the
+    module docstring); returns the measurements for the JSON line."""
+    import numpy as np
+    import torch
+
+    from witwin.channel import workloads
+    from witwin.channel.components import component_bits
+    from witwin.channel.propagation import batching, engine
+    from witwin.channel.scene.compiler import compile as compile_scene, device_scene
+
+    device = torch.device("cuda", 0)
+    depth = 5
+    bits = component_bits({"los", "reflection", "diffraction", "transmission"})
+    city = workloads.scaled_city(seed=5)
+    compiled = compile_scene(city.scene, reference_frequency_hz=city.frequencies[-1])
+    dscene = device_scene(compiled, device)
+    tx_t = torch.as_tensor(city.tx).to(device)
+    t0 = time.perf_counter()
+    seqs = batching.chain_sequences_batched(compiled, dscene, tx_t, depth, True)
+    torch.cuda.synchronize()
+    planning_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    if launch_shape:
+        mode, size = (int(x) for x in launch_shape.split(","))
+        shape = batching.LaunchShape(mode, size, "given")
+    else:
+        shape = batching.autotune(compiled, dscene, city.tx, city.rx, city.rx_count, components=bits,
+                                  max_depth=depth, frequencies=city.frequencies, sequences=seqs)
+    tune_s = time.perf_counter() - t0
+    run = batching.BatchedMeanDb(compiled, dscene, city.tx, city.rx, city.rx_count, components=bits,
+                                 max_depth=depth, frequencies=city.frequencies, shape=shape, sequences=seqs)
+    run.capture()
+    ms_batched, per_step = timed(run.step, steps, warmup, stream)
+    cells = run.cells * len(city.frequencies)
+    grads = {"d_eps_r": run.grad_params[:run.material_count].cpu().tolist(),
+             "d_sigma": run.grad_params[run.material_count:].cpu().tolist(),
+             "d_frequency": run.grad_frequency.cpu().tolist(),
+             "d_tx_norm_mean": float(run.grad_tx.norm(dim=1).mean())}
+
+    # kernels of the heaviest unit (first batch, first carrier), each alone
+    # with HIP events on its stream
+    _, members, unit = run.units[0]
+    inp = unit.inputs
+    p = lambda k: (lambda: engine.radiomap_pass(dscene, inp, unit.out, k))  # noqa: E731
+
+    def before(k):
+        return lambda: [engine.radiomap_pass(dscene, inp, unit.out, j) for j in range(1, k)]
+
+    unit_ms = {"chain_validity": kernel_time(p(1), stream), "fields_main": kernel_time(p(2), stream, before=before(2)),
+               "edge_visibility": kernel_time(p(3), stream, before=before(3)),
+               "fields_diffraction": kernel_time(p(4), stream, before=before(4))}
+    unit.launch()
+    unit_ms["adjoint_main"] = kernel_time(lambda: unit.adjoint(1), stream)
+    unit_ms["adjoint_diffraction"] = kernel_time(lambda: unit.adjoint(2), stream, before=lambda: unit.adjoint(1))
+    unit_cells = inp.total_cells
+    scene_bytes = compiled.face_count * 48 + int(dscene.node_count) * 64
+    # the dominant kernel, the chain validity pass, over every batch (first
+    # carrier; the second reuses its masks): per cell it reads the receiver
+    # and its order entry and writes one bit per chain of its transmitter
+    # (mask words padded to the batch's largest count); the scene and the
+    # chain tables (groups + float64 images) once per launch
+    first = [u for u in run.units if u[0] == 0]
+
+    def all_chain_passes():
+        for _, _, u in first:
+            engine.radiomap_pass(dscene, u.inputs, u.out, 1)
+
+    dom_ms = kernel_time(all_chain_passes, stream, reps=3) / len(first)
+    dom_bytes = sum(u.inputs.total_cells * (16 + 4 * ((int(u.inputs.view.max_tx_chains) + 31) // 32)) + scene_bytes
+                    + int(u.inputs.chains.count) * (4 * 5 + 8 * 15) for _, _, u in first) / len(first)
+    paths = run.paths
+
+    ms_plain, identical = None, None
+    if plain_loop:
+        plain = batching.BatchedMeanDb(compiled, dscene, city.tx, city.rx, city.rx_count, components=bits,
+                                       max_depth=depth, frequencies=city.frequencies,
+                                       shape=batching.LaunchShape(shape.visibility_mode, 1), sequences=seqs)
+        plain.reuse_visibility = False
+        ms_plain, _ = timed(plain.launch, 2, 1, stream)
+        identical = all(torch.equal(a, b) for fi in range(len(city.frequencies))
+                        for a, b in zip(run.power_maps(fi), plain.power_maps(fi)))
+        del plain
+
+    cpu_line = None
+    if cpu:
+        from tests.gpu_helpers import DB_TOL
+        import oracle
+
+        t = 0
+        rng = np.random.default_rng(0)
+        n = int(city.rx_count[t])
+        sample = np.sort(rng.choice(n, 64, replace=False))
+        per = seqs[t]
+        groups = np.concatenate([np.pad(x, ((0, 0), (0, 5 - x.shape[1])), constant_values=-1) for x in per if x.size],
+                                axis=0).astype(np.int32)
+        chain_depth = (groups >= 0).sum(axis=1).astype(np.int32)
+        orc = oracle.Oracle(compiled)
+        c0 = time.perf_counter()
+        ref = orc.radiomap(tx=city.tx[t], rx=city.rx[:n][sample], frequency=city.frequencies[-1], components=bits,
+                           max_depth=depth, chains=groups, chain_depth=chain_depth, perturb=2e-4)
+        cpu_s = time.perf_counter() - c0
+        p_gpu = run.power_maps(len(city.frequencies) - 1)[t].cpu().numpy()[sample].astype(np.float64)
+        p_ref = ref["record"][:, 0]
+        amb = ref["ambiguous"].astype(bool)
+        lit = (p_gpu > 0) & (p_ref > 0)
+        err = np.abs(10 * np.log10(p_gpu[lit] / p_ref[lit]))
+        stable = lit & ~amb
+        err_s = np.abs(10 * np.log10(p_gpu[stable] / p_ref[stable]))
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu_line = {
+            "value": len(sample) * 5 / cpu_s, "unit": "cells/s (forward only, each cell evaluated 5 times: at its "
+                                                    "receiver and with receiver/transmitter moved by 0.2 mm)",
+            "cores": threads, "kind": "port",
+            "sample": f"64 random cells of transmitter 0's {int(city.sides[t])}x{int(city.sides[t])} grid at 28 GHz "
+                      f"(its {groups.shape[0]} candidate chains, 1600 edges, 134,850 triangles brute force), "
+                      f"FP64 oracle, OpenMP over cells, {cpu_s:.1f} s",
+            "cells_lit_both": int(lit.sum()), "cells_lit_one_only": int(((p_gpu > 0) != (p_ref > 0)).sum()),
+            "ambiguous_cells": int(amb.sum()),
+            "median_db_error_vs_gpu": float(np.median(err)) if err.size else 0.0,
+            "max_db_error_vs_gpu": float(err.max()) if err.size else 0.0,
+            "cells_above_0.01_db": int((err > DB_TOL).sum()),
+            "max_db_error_topology_stable_cells": float(err_s.max()) if err_s.size else 0.0,
+            "stable_cells_above_0.01_db": int((err_s > DB_TOL).sum()),
+        }
+
+    return {
+        "cells": cells, "ms_batched": ms_batched, "ms_plain": ms_plain, "per_step_ms": per_step,
+        "planning_s": planning_s, "tune_s": tune_s, "shape": shape.as_dict(), "paths": paths,
+        "identical": identical, "grads": grads, "unit_ms": unit_ms, "unit_cells": unit_cells,
+        "unit_tx": int(len(members)), "dom_ms": dom_ms, "dom_bytes": dom_bytes, "cpu": cpu_line,
+        "tx": int(city.tx.shape[0]), "grid_cells": int(run.cells), "triangles": compiled.face_count,
+        "edges": compiled.edge_count, "chains": int(sum(sum(x.shape[0] for x in per) for per in seqs)),
+        "workspace_gb": run.workspace_bytes / 1e9, "frequencies": list(city.frequencies),
+        "batches": len(run.batches),
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
-    parser.add_argument("--steps", type=int, default=20)
-    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--steps", type=int, default=10)
+    parser.add_argument("--warmup", type=int, default=2)
+    parser.add_argument("--canyon-steps", type=int, default=20)
+    parser.add_argument("--value-only", action="store_true",
+                        help="only the value workload (no plain loop, canyon, heightfield or CPU oracle)")
+    parser.add_argument("--launch-shape", default=None,
+                        help="visibility_mode,batch_size for the value workload instead of the autotuner")
     parser.add_argument("--grid", type=int, default=256)
     parser.add_argument("--tx-batch", type=int, default=64)
     parser.add_argument("--no-cpu", action="store_true")
@@ -159,6 +307,14 @@ def main() -> int:
 
     device = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(device)
+    city = scaled_city_section(args.steps, args.warmup, stream, cpu=not args.no_cpu and not args.value_only,
+                               plain_loop=not args.value_only, launch_shape=args.launch_shape)
+    torch.cuda.empty_cache()
+    if args.value_only:
+        print(json.dumps({"value_only": True, "cells_per_s": city["cells"] / (city["ms_batched"] * 1e-3),
+                          "ms_per_step": city["ms_batched"], "launch_shape": city["shape"],
+                          "chain_pass_ms_per_batch": city["dom_ms"], "heaviest_batch_ms": city["unit_ms"]}))
+        return 0
     frequency = 28.0e9
     depth = 3
     bits = component_bits({"los", "reflection", "diffraction"})
@@ -177,9 +333,9 @@ def main() -> int:
     if bool((step.out.failure != 0).any()):
         raise RuntimeError("topology capacity exceeded")
     path_count = int(step.out.path_count.sum().item())
-    ms_plain, _ = timed(step.launch, max(5, args.steps // 2), 2, stream)
+    ms_plain, _ = timed(step.launch, max(5, args.canyon_steps // 2), 2, stream)
     step.capture()
-    ms_batched, _ = timed(step.step, args.steps, args.warmup, stream)
+    ms_batched, _ = timed(step.step, args.canyon_steps, 3, stream)
     value = cells / (ms_batched * 1e-3)
 
     # each kernel of the step timed alone (HIP events on its stream); the
@@ -231,7 +387,7 @@ def main() -> int:
                                       rx_pos=rx_pos, rx_pol=rx_pol, rx_count=sizes)
         bstep = MeanDbStep(dscene, binputs, m)
         bstep.capture()
-        ms_b, _ = timed(bstep.step, max(5, args.steps // 2), 2, stream)
+        ms_b, _ = timed(bstep.step, max(5, args.canyon_steps // 2), 2, stream)
         offsets = np.concatenate([[0], np.cumsum(sizes)])
         singles = []
         for t in range(tx_pos.shape[0]):
@@ -267,7 +423,7 @@ def main() -> int:
         if bool((h_step.out.failure != 0).any()):
             raise RuntimeError("topology capacity exceeded (heightfield)")
         h_step.capture()
-        h_ms, _ = timed(h_step.step, max(5, args.steps // 2), 2, stream)
+        h_ms, _ = timed(h_step.step, max(5, args.canyon_steps // 2), 2, stream)
         h_geo_ms = kernel_time(h_step.geometry, stream)
         h_fwd_ms = kernel_time(lambda: engine.radiomap_forward(h_dscene, h_inputs, topology_capacity=h_step.capacity,
                                                                out=h_step.out), stream)
@@ -302,7 +458,7 @@ def main() -> int:
                    "cells_per_s": h_cells / (h_ms * 1e-3), "forward_ms": h_fwd_ms, "vertex_vjp_ms": h_geo_ms,
                    "geometry_update": {"device_refit_ms": refit_ms, "host_recompile_and_build_ms": rebuild_ms}}
 
-    cpu = None
+    canyon_cpu = None
     if not args.no_cpu:
         from tests.gpu_helpers import Run, run_oracle
 
@@ -312,20 +468,25 @@ def main() -> int:
         c0 = time.perf_counter()
         ref = run_oracle(run, 0, sample)
         cpu_s = time.perf_counter() - c0
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         p_gpu = step.out.power_total.cpu().numpy()[sample]
         p_ref = ref["record"][:, 0]
         both = (p_gpu > 0) & (p_ref > 0)
         err = np.abs(10 * np.log10(p_gpu[both] / p_ref[both]))
-        cpu = {"value": len(sample) / cpu_s, "unit": "cells/s (forward only)", "cores": threads, "kind": "port",
-               "sample": f"{len(sample)} random cells of the same 256x256 radiomap; FP64 oracle with brute-force "
-                         f"visibility, OpenMP over cells, {cpu_s:.1f} s",
-               "median_db_error_vs_gpu": float(np.median(err)) if err.size else 0.0,
-               "p99_db_error_vs_gpu": float(np.percentile(err, 99)) if err.size else 0.0}
+        canyon_cpu = {"cells_per_s": len(sample) / cpu_s, "sample": f"{len(sample)} random cells, {cpu_s:.1f} s",
+                      "cells_lit_one_only": int(((p_gpu > 0) != (p_ref > 0)).sum()),
+                      "median_db_error_vs_gpu": float(np.median(err)) if err.size else 0.0,
+                      "max_db_error_vs_gpu": float(err.max()) if err.size else 0.0,
+                      "cells_above_0.01_db": int((err > 0.01).sum())}
 
-    workload_name = (f"urban canyon 8x8 boxes, {compiled.face_count} tris, 1 Tx at street intersection "
-                     f"(53.231,52.013,25), {args.grid}x{args.grid} grid at 2 m, 28 GHz, depth 3 refl + UTD diff + "
-                     "LoS, loss mean dB over lit cells")
+    canyon_name = (f"urban canyon 8x8 boxes, {compiled.face_count} tris, 1 Tx at street intersection "
+                   f"(53.231,52.013,25), {args.grid}x{args.grid} grid at 2 m, 28 GHz, depth 3 refl + UTD diff + "
+                   "LoS, loss mean dB over lit cells")
+    workload_name = (f"scaled city seed 5: 400 boxes + 256x256 fBm heightfield ({city['triangles']} tris), 8 materials, "
+                     f"{city['tx']} Tx with grids 64^2..512^2 ({city['grid_cells']} cells), 5-bounce reflection + "
+                     "UTD diffraction + transmission + LoS at 3.5 and 28 GHz, loss mean dB over lit cells, VJP w.r.t. "
+                     "materials, frequencies and Tx positions")
+    value = city["cells"] / (city["ms_batched"] * 1e-3)
+    achieved = city["dom_bytes"] / (city["dom_ms"] * 1e-3) / 1e9
     line = {
         "metric": "radiomap cells/s fwd+VJP (batched vs plain, 1 GPU) at <=0.01 dB err vs FP64",
         "value": value,
@@ -333,33 +494,51 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_batched,
+        "ms_per_step": city["ms_batched"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 geometry/fields/tangents, f64 phase, UTD transition and cell sums",
-        "data": "synthetic seeded urban canyon (seed 2)",
+        "data": "synthetic seeded scaled city (seed 5)",
         "config": {
             "workload": workload_name,
             "pass": "forward+vjp",
-            "cells": cells,
-            "paths": path_count,
-            "chains": inputs.chains.count,
-            "plain_ms_per_step": ms_plain,
-            "plain_cells_per_s": cells / (ms_plain * 1e-3),
-            "batched_over_plain": ms_plain / ms_batched,
-            "forward_ms": fwd_ms,
-            "kernel_ms": {**{k: v[0] for k, v in kernels.items()}, "vjp_reduce_kernel": reduce_ms},
-            "paths_reflection_pass": n_main,
-            "paths_diffraction": n_diff,
-            "chain_planning_s": plan_s,
+            "cells": city["cells"],
+            "cells_per_carrier": city["grid_cells"],
+            "carriers_hz": city["frequencies"],
+            "paths": city["paths"],
+            "chains": city["chains"],
+            "edges": city["edges"],
+            "plain_ms_per_step": city["ms_plain"],
+            "plain_cells_per_s": city["cells"] / (city["ms_plain"] * 1e-3) if city["ms_plain"] else None,
+            "batched_over_plain": city["ms_plain"] / city["ms_batched"] if city["ms_plain"] else None,
+            "batched_maps_bit_identical_to_plain": city["identical"],
+            "launch_shape": city["shape"],
+            "batches": city["batches"],
+            "autotune_s": city["tune_s"],
+            "chain_planning_s": city["planning_s"],
+            "workspace_gb": city["workspace_gb"],
+            "gradients": city["grads"],
+            "heaviest_batch": {"tx": city["unit_tx"], "cells": city["unit_cells"], "kernel_ms": city["unit_ms"]},
+            "canyon": {
+                "workload": canyon_name, "cells": cells, "paths": path_count, "chains": inputs.chains.count,
+                "ms_per_step": ms_batched, "cells_per_s": cells / (ms_batched * 1e-3),
+                "plain_ms_per_step": ms_plain, "forward_ms": fwd_ms,
+                "kernel_ms": {**{k: v[0] for k, v in kernels.items()}, "vjp_reduce_kernel": reduce_ms},
+                "paths_reflection_pass": n_main, "paths_diffraction": n_diff, "chain_planning_s": plan_s,
+                "dominant_kernel": dominant, "dominant_hbm_frac": (dom_bytes / (dom_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
+                "oracle_check": canyon_cpu,
+            },
             "multi_tx": batch,
             "heightfield": terrain,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_bytes(dominant, workload_name),
-                     "kernel": dominant, "kernel_ms": dom_ms, "algorithmic_bytes_per_launch": dom_bytes},
-        "cpu_baseline": cpu,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_bytes("chain_visibility_seg_kernel", workload_name),
+                     "kernel": "chain_visibility_seg_kernel (with its pass's super_box, super_cull and fill_zero "
+                               "launches; average over the batches)",
+                     "kernel_ms": city["dom_ms"], "algorithmic_bytes_per_launch": city["dom_bytes"]},
+        "cpu_baseline": city["cpu"],
     }
     print(json.dumps(line))
     return 0
