@@ -182,12 +182,16 @@ def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: 
     def before(k):
         return lambda: [engine.radiomap_pass(dscene, inp, unit.out, j) for j in range(1, k)]
 
-    unit_ms = {"chain_validity": kernel_time(p(1), stream), "fields_main": kernel_time(p(2), stream, before=before(2)),
-               "edge_visibility": kernel_time(p(3), stream, before=before(3)),
-               "fields_diffraction": kernel_time(p(4), stream, before=before(4))}
-    unit.launch()
-    unit_ms["adjoint_main"] = kernel_time(lambda: unit.adjoint(1), stream)
-    unit_ms["adjoint_diffraction"] = kernel_time(lambda: unit.adjoint(2), stream, before=lambda: unit.adjoint(1))
+    unit_ms = None
+    if plain_loop:
+        # (skipped with --value-only, so a kernel trace of that run averages every batch evenly)
+        unit_ms = {"chain_validity": kernel_time(p(1), stream),
+                   "fields_main": kernel_time(p(2), stream, before=before(2)),
+                   "edge_visibility": kernel_time(p(3), stream, before=before(3)),
+                   "fields_diffraction": kernel_time(p(4), stream, before=before(4))}
+        unit.launch()
+        unit_ms["adjoint_main"] = kernel_time(lambda: unit.adjoint(1), stream)
+        unit_ms["adjoint_diffraction"] = kernel_time(lambda: unit.adjoint(2), stream, before=lambda: unit.adjoint(1))
     unit_cells = inp.total_cells
     scene_bytes = compiled.face_count * 48 + int(dscene.node_count) * 64
     # the dominant kernel, the chain validity pass, over every batch (first
@@ -277,6 +281,8 @@ def main() -> int:
     parser.add_argument("--steps", type=int, default=10)
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--canyon-steps", type=int, default=20)
+    parser.add_argument("--canyon-only", action="store_true",
+                        help="skip the value workload; print the urban canyon measurements only")
     parser.add_argument("--value-only", action="store_true",
                         help="only the value workload (no plain loop, canyon, heightfield or CPU oracle)")
     parser.add_argument("--launch-shape", default=None,
@@ -307,9 +313,11 @@ def main() -> int:
 
     device = torch.device("cuda", 0)
     stream = torch.cuda.current_stream(device)
-    city = scaled_city_section(args.steps, args.warmup, stream, cpu=not args.no_cpu and not args.value_only,
-                               plain_loop=not args.value_only, launch_shape=args.launch_shape)
-    torch.cuda.empty_cache()
+    city = None
+    if not args.canyon_only:
+        city = scaled_city_section(args.steps, args.warmup, stream, cpu=not args.no_cpu and not args.value_only,
+                                   plain_loop=not args.value_only, launch_shape=args.launch_shape)
+        torch.cuda.empty_cache()
     if args.value_only:
         print(json.dumps({"value_only": True, "cells_per_s": city["cells"] / (city["ms_batched"] * 1e-3),
                           "ms_per_step": city["ms_batched"], "launch_shape": city["shape"],
@@ -478,6 +486,18 @@ def main() -> int:
                       "max_db_error_vs_gpu": float(err.max()) if err.size else 0.0,
                       "cells_above_0.01_db": int((err > 0.01).sum())}
 
+    canyon = {
+        "cells": cells, "paths": path_count, "chains": inputs.chains.count,
+        "ms_per_step": ms_batched, "cells_per_s": cells / (ms_batched * 1e-3),
+        "plain_ms_per_step": ms_plain, "forward_ms": fwd_ms,
+        "kernel_ms": {**{k: v[0] for k, v in kernels.items()}, "vjp_reduce_kernel": reduce_ms},
+        "paths_reflection_pass": n_main, "paths_diffraction": n_diff, "chain_planning_s": plan_s,
+        "dominant_kernel": dominant, "dominant_hbm_frac": (dom_bytes / (dom_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
+        "oracle_check": canyon_cpu,
+    }
+    if city is None:
+        print(json.dumps({"canyon_only": True, **canyon, "multi_tx": batch, "heightfield": terrain}))
+        return 0
     canyon_name = (f"urban canyon 8x8 boxes, {compiled.face_count} tris, 1 Tx at street intersection "
                    f"(53.231,52.013,25), {args.grid}x{args.grid} grid at 2 m, 28 GHz, depth 3 refl + UTD diff + "
                    "LoS, loss mean dB over lit cells")
@@ -520,15 +540,7 @@ def main() -> int:
             "workspace_gb": city["workspace_gb"],
             "gradients": city["grads"],
             "heaviest_batch": {"tx": city["unit_tx"], "cells": city["unit_cells"], "kernel_ms": city["unit_ms"]},
-            "canyon": {
-                "workload": canyon_name, "cells": cells, "paths": path_count, "chains": inputs.chains.count,
-                "ms_per_step": ms_batched, "cells_per_s": cells / (ms_batched * 1e-3),
-                "plain_ms_per_step": ms_plain, "forward_ms": fwd_ms,
-                "kernel_ms": {**{k: v[0] for k, v in kernels.items()}, "vjp_reduce_kernel": reduce_ms},
-                "paths_reflection_pass": n_main, "paths_diffraction": n_diff, "chain_planning_s": plan_s,
-                "dominant_kernel": dominant, "dominant_hbm_frac": (dom_bytes / (dom_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
-                "oracle_check": canyon_cpu,
-            },
+            "canyon": {"workload": canyon_name, **canyon},
             "multi_tx": batch,
             "heightfield": terrain,
         },

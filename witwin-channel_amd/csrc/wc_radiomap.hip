@@ -18,11 +18,33 @@ __device__ __forceinline__ int slot_of(int component) {
     }
 }
 
-// Next chain of [c, c1) whose bit is set in `mask` (bit c - base), or c1.
-__device__ __forceinline__ int next_set_chain(const unsigned* mask, int base, int c, int c1) {
+// Chain-validity bits of one evaluation position: chunk k (chains
+// 32 k .. 32 k + 31 of the transmitter) is the word words[k * stride]
+// (chunk-major, position-indexed, so a wavefront's 64 positions read and write
+// 256 consecutive bytes); `touched` is the tile's chunk bitmap: bit k is set
+// when some position of the tile has a set bit in chunk k. Words of untouched
+// chunks are never read (and not kept zero).
+struct ChainMaskView {
+    const unsigned* words;
+    size_t stride;
+    const unsigned* touched;
+};
+
+// Next chain of [c, c1) whose bit is set (bit c - base), or c1.
+__device__ __forceinline__ int next_set_chain(const ChainMaskView& m, int base, int c, int c1) {
     while (c < c1) {
         const int rel = c - base;
-        const unsigned word = mask[rel >> 5] >> (rel & 31);
+        const int chunk = rel >> 5;
+        const unsigned tw = m.touched[chunk >> 5] >> (chunk & 31);
+        if (tw == 0u) {  // no touched chunk up to the end of this bitmap word
+            c = base + (((chunk >> 5) + 1) << 10);
+            continue;
+        }
+        if (!(tw & 1u)) {  // jump to the next touched chunk
+            c = base + ((chunk + __ffs(tw) - 1) << 5);
+            continue;
+        }
+        const unsigned word = m.words[(size_t)chunk * m.stride] >> (rel & 31);
         if (word) {
             const int hit = c + __ffs(word) - 1;
             return hit < c1 ? hit : c1;
@@ -39,7 +61,7 @@ template <int D, class Sink, bool kMasked>
 __device__ __forceinline__ void reflect_depth(const SceneView& s, const wc_solve& q, int t, const V3<float>& tx,
                                               const V3<float>& rx, const V3<float>& tx_pol, const V3<float>& rx_pol,
                                               float sqrt_power, float freq, double k_d, int sub, int stride,
-                                              Sink& sink, const unsigned* chain_mask) {
+                                              Sink& sink, const ChainMaskView& chain_mask) {
     const int base = q.chain_offset[t * kMaxDepth];
     const int c0 = q.chain_offset[t * kMaxDepth + D - 1];
     const int c1 = q.chain_offset[t * kMaxDepth + D];
@@ -123,7 +145,7 @@ enum EvalPass : int { kPassAll = 0, kPassMain = 1, kPassDiff = 2 };
 template <class Sink, int kPass, bool kMasked = false>
 __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell, int sub, int stride, Sink& sink,
                           const unsigned* diff_mask = nullptr, int mask_words = 0,
-                          const unsigned* chain_mask = nullptr) {
+                          const ChainMaskView& chain_mask = ChainMaskView{nullptr, 0, nullptr}) {
     const V3<float> tx = ld3(q.tx_pos + 3 * t);
     const V3<float> tx_pol = ld3(q.tx_pol + 3 * t);
     const float sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
@@ -372,9 +394,26 @@ __host__ __device__ __forceinline__ unsigned* chain_masks(void* workspace, const
 __host__ __device__ __forceinline__ size_t chain_mask_bytes(int cells, int max_tx_chains) {
     return (size_t)cells * (size_t)mask_words(max_tx_chains > 0 ? max_tx_chains : 0) * 4;
 }
-__host__ __device__ __forceinline__ size_t shadow_offset(int cells, int lanes, int edge_count, int max_tx_chains) {
-    const size_t o = partial_bytes(cells, lanes) + diff_mask_bytes(cells, edge_count) +
-                     chain_mask_bytes(cells, max_tx_chains);
+// after the chain masks: one chunk bitmap per tile (touched_words(max chains)
+// words each, see ChainMaskView), for at most ceil(cells * L / 64) + T tiles;
+// the shadow table then takes tx_count as it is passed here
+__host__ __device__ __forceinline__ int touched_words(int max_tx_chains) {
+    return (mask_words(max_tx_chains > 0 ? max_tx_chains : 0) + 31) >> 5;
+}
+__host__ __device__ __forceinline__ size_t tile_capacity(int cells, int lanes, int tx_count) {
+    return ((size_t)(cells > 0 ? cells : 0) * (size_t)lanes + 63) / 64 + (size_t)(tx_count > 0 ? tx_count : 0);
+}
+__host__ __device__ __forceinline__ size_t touched_offset(int cells, int lanes, int edge_count, int max_tx_chains) {
+    return partial_bytes(cells, lanes) + diff_mask_bytes(cells, edge_count) + chain_mask_bytes(cells, max_tx_chains);
+}
+__host__ __device__ __forceinline__ unsigned* touched_bitmaps(void* workspace, const wc_solve& q, int edge_count) {
+    return reinterpret_cast<unsigned*>(static_cast<char*>(workspace) +
+                                       touched_offset(q.total_cells, q.lanes_per_cell, edge_count, q.max_tx_chains));
+}
+__host__ __device__ __forceinline__ size_t shadow_offset(int cells, int lanes, int edge_count, int max_tx_chains,
+                                                         int tx_count) {
+    const size_t o = touched_offset(cells, lanes, edge_count, max_tx_chains) +
+                     tile_capacity(cells, lanes, tx_count) * (size_t)touched_words(max_tx_chains) * 4;
     return (o + 7) & ~(size_t)7;
 }
 
@@ -404,7 +443,7 @@ __host__ __device__ inline DiffStageLayout diff_stage_layout(size_t base, int ce
     return l;
 }
 __host__ __device__ inline size_t shadow_end(int cells, int lanes, int edge_count, int max_tx_chains, int tx_count) {
-    return shadow_offset(cells, lanes, edge_count, max_tx_chains) +
+    return shadow_offset(cells, lanes, edge_count, max_tx_chains, tx_count) +
            (size_t)(tx_count > 0 ? tx_count : 0) * (size_t)(edge_count > 0 ? edge_count : 0) * 8;
 }
 
@@ -499,7 +538,8 @@ __device__ __forceinline__ void reduce_slot(AccumSink& a, int sl, int width) {
 // masks (chain_visibility_kernel ran first); the plain launch traces them.
 template <int kPass, bool kMasked>
 __device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve& q, const wc_radiomap_out& o,
-                                              int t, int cell, bool active, int sub, int width) {
+                                              int t, int cell, bool active, int sub, int width,
+                                              int tile = 0, int pos = 0) {
     AccumSink a;
     a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
     CellPartial* partial = reinterpret_cast<CellPartial*>(o.workspace);
@@ -524,10 +564,13 @@ __device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve
             eval_cell<AccumSink, kPass>(s, q, t, cell, sub, width, a, mask, words);
         } else {
             // without reflection chains the masked chain loops are empty
-            const unsigned* cmask =
-                kMasked && q.max_tx_chains > 0
-                    ? chain_masks(o.workspace, q, s.edge_count) + (size_t)cell * mask_words(q.max_tx_chains)
-                    : nullptr;
+            ChainMaskView cmask{nullptr, 0, nullptr};
+            if (kMasked && q.max_tx_chains > 0) {
+                cmask.words = chain_masks(o.workspace, q, s.edge_count) + pos;
+                cmask.stride = (size_t)q.total_cells;
+                cmask.touched = touched_bitmaps(o.workspace, q, s.edge_count) +
+                                (size_t)tile * touched_words(q.max_tx_chains);
+            }
             eval_cell<AccumSink, kPass, kMasked>(s, q, t, cell, sub, width, a, nullptr, 0, cmask);
         }
     }
@@ -609,7 +652,8 @@ __device__ __forceinline__ void radiomap_tiles(const SceneView& s, const wc_solv
         const int t = tiles[2 * tile];
         const int pos = tiles[2 * tile + 1] + local;
         const bool active = pos < q.cell_offset[t + 1];
-        radiomap_lane<kPass, true>(s, q, o, t, cell_at(q, active ? pos : q.cell_offset[t]), active, sub, width);
+        radiomap_lane<kPass, true>(s, q, o, t, cell_at(q, active ? pos : q.cell_offset[t]), active, sub, width, tile,
+                                   pos);
     }
 }
 
@@ -996,7 +1040,8 @@ __device__ __forceinline__ int super_of_tile(const wc_solve& q, int tile) {
 
 __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_solve q, const int* __restrict__ tiles,
                                                                int tile_count, unsigned* __restrict__ masks,
-                                                               int words, const unsigned* __restrict__ super_masks, long long wave_base) {
+                                                               int words, const unsigned* __restrict__ super_masks, long long wave_base,
+                                                               unsigned* __restrict__ touched) {
     const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)tile_count * words) return;
     const int tile = (int)(wave / words);
@@ -1033,9 +1078,9 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
         reach = chain_reaches_box(s, q, c, d, lo, hi);
     }
     unsigned todo = (unsigned)__ballot(reach);
-    if (!live) return;
     unsigned bits = 0;
     int d = 1;
+    if (!live) todo = 0u;
     while (todo) {
         const int b = __ffs(todo) - 1;
         todo &= todo - 1;
@@ -1051,7 +1096,9 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
         }
         if (ok) bits |= 1u << b;
     }
-    masks[(size_t)cell * words + chunk] = bits;
+    if (live) masks[(size_t)chunk * q.total_cells + pos] = bits;
+    if (__ballot(bits != 0u) != 0ull && lane == 0)
+        atomicOr(&touched[(size_t)tile * ((words + 31) >> 5) + (chunk >> 5)], 1u << (chunk & 31));
 }
 
 template <int D>
@@ -1072,7 +1119,8 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_kernel(SceneView s, 
                                                                    const int* __restrict__ tiles, int tile_count,
                                                                    unsigned* __restrict__ masks, int words,
                                                                    const unsigned* __restrict__ super_masks,
-                                                                   long long wave_base) {
+                                                                   long long wave_base,
+                                                                   unsigned* __restrict__ touched) {
     const int segs = (words + 63) >> 6;
     const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)tile_count * segs) return;
@@ -1131,7 +1179,9 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_kernel(SceneView s, 
             }
             if (ok) bits |= 1u << bb;
         }
-        if (live) masks[(size_t)cell * words + chunk] = bits;
+        if (live) masks[(size_t)chunk * q.total_cells + pos] = bits;
+        if (__ballot(bits != 0u) != 0ull && lane == 0)
+            atomicOr(&touched[(size_t)tile * ((words + 31) >> 5) + (chunk >> 5)], 1u << (chunk & 31));
     }
 }
 
@@ -1147,7 +1197,8 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_queue_kernel(SceneVi
                                                                          int tile_count, unsigned* __restrict__ masks,
                                                                          int words,
                                                                          const unsigned* __restrict__ super_masks,
-                                                                         long long wave_base) {
+                                                                         long long wave_base,
+                                                                         unsigned* __restrict__ touched) {
     __shared__ unsigned queue[4][128];
     __shared__ float4 lane_rx[4][64];
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1175,7 +1226,7 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_queue_kernel(SceneVi
                        wave_min(live ? rx.z : INFINITY)};
     const V3<float> hi{wave_max(live ? rx.x : -INFINITY), wave_max(live ? rx.y : -INFINITY),
                        wave_max(live ? rx.z : -INFINITY)};
-    lane_rx[wid][lane] = make_float4(rx.x, rx.y, rx.z, __int_as_float(cell));
+    lane_rx[wid][lane] = make_float4(rx.x, rx.y, rx.z, __int_as_float(pos));
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1208,7 +1259,10 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_queue_kernel(SceneVi
                 case 4: ok = chain_legs<4>(s, q, c, tx, orx); break;
                 default: ok = chain_legs<5>(s, q, c, tx, orx); break;
             }
-            if (ok) atomicOr(&masks[(size_t)__float_as_int(r4.w) * words + chunk], 1u << bb);
+            if (ok) {
+                atomicOr(&masks[(size_t)chunk * q.total_cells + __float_as_int(r4.w)], 1u << bb);
+                atomicOr(&touched[(size_t)tile * ((words + 31) >> 5) + (chunk >> 5)], 1u << (chunk & 31));
+            }
         }
     };
     int d = 1;
@@ -1227,6 +1281,10 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_queue_kernel(SceneVi
             reach = chain_reaches_box(s, q, c, dd, lo, hi);
         }
         unsigned todo = (unsigned)__ballot(reach);
+        if (todo == 0u) continue;
+        // the chunk's words start at zero; the queued legs OR their bits in
+        if (live) masks[(size_t)chunk * q.total_cells + pos] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         while (todo) {
             const int bb = __ffs(todo) - 1;
             todo &= todo - 1;
@@ -1282,7 +1340,8 @@ __device__ __forceinline__ bool chain_legs(const SceneView& s, const wc_solve& q
 __global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s, wc_solve q,
                                                                      const int* __restrict__ tiles, int tile_count,
                                                                      unsigned* __restrict__ masks, int words,
-                                                                     const unsigned* __restrict__ super_masks, long long wave_base) {
+                                                                     const unsigned* __restrict__ super_masks, long long wave_base,
+                                                                     unsigned* __restrict__ touched) {
     __shared__ unsigned short queue[4][128];
     __shared__ float4 lane_rx[4][64];
     __shared__ unsigned found[4][64];
@@ -1386,7 +1445,10 @@ __global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s
         }
     }
     __syncthreads();
-    if (live) masks[(size_t)cell * words + chunk] = found[wid][lane];
+    const unsigned bits = live ? found[wid][lane] : 0u;
+    if (live) masks[(size_t)chunk * q.total_cells + pos] = bits;
+    if (__ballot(bits != 0u) != 0ull && lane == 0 && wave_live)
+        atomicOr(&touched[(size_t)tile * ((words + 31) >> 5) + (chunk >> 5)], 1u << (chunk & 31));
 }
 
 // Visibility of every (cell, edge) diffraction candidate: bit (ei & 31) of
@@ -1974,11 +2036,17 @@ static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const
     if (q.max_tx_chains <= 0 || !(q.components & kCompRefl) || q.max_depth <= 0) return;
     const int words = mask_words(q.max_tx_chains);
     unsigned* masks = chain_masks(o.workspace, q, s.edge_count);
+    unsigned* touched = touched_bitmaps(o.workspace, q, s.edge_count);
     const long long waves = (long long)tile_count * words;
+    // every tile's chunk bitmap starts empty; the mask words of a chunk are
+    // written (or zeroed, then OR-ed) only where the bitmap marks the chunk
+    const size_t touched_count = (size_t)tile_count * (size_t)touched_words(q.max_tx_chains);
+    hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((touched_count + 255) / 256, 65536)),
+                       dim3(256), 0, st, touched, touched_count);
+    note_launch();
     // many chains: cull them per supertile first (host supertile tables, or
     // for one transmitter the 1,024-position blocks in order); the tile pass
-    // then culls only what its supertile kept, and a tile whose supertile kept
-    // nothing of a chunk writes nothing, so the masks are zero-filled first
+    // then visits only what its supertile kept
     const unsigned* super_masks = nullptr;
     const bool tables = q.super_first_tile && q.tile_super && q.super_count > 0;
     if (q.lanes_per_cell == 1 && words >= kSuperMinWords && (tables || q.tx_count == 1)) {
@@ -2000,32 +2068,28 @@ static void launch_chain_visibility(const SceneView& s, const wc_solve& q, const
         for_wave_chunks((long long)supers * words, [&](dim3 grid, long long base) {
             hipLaunchKernelGGL(super_cull_kernel, grid, dim3(256), 0, st, s, qs, tiles, supers, words, boxes, sm, base);
         });
-        const size_t mask_count = (size_t)q.total_cells * (size_t)words;
-        hipLaunchKernelGGL(fill_zero_kernel, dim3((unsigned)std::min<size_t>((mask_count + 255) / 256, 65536)),
-                           dim3(256), 0, st, masks, mask_count);
-        note_launch();
         super_masks = sm;
         if (coherent_tiles(q))
             for_wave_chunks((long long)tile_count * ((words + 63) / 64), [&](dim3 grid, long long base) {
                 hipLaunchKernelGGL(chain_visibility_seg_kernel, grid, dim3(256), 0, st, s, qs, tiles, tile_count,
-                                   masks, words, super_masks, base);
+                                   masks, words, super_masks, base, touched);
             });
         else
             for_wave_chunks((long long)tile_count * ((words + 63) / 64), [&](dim3 grid, long long base) {
                 hipLaunchKernelGGL(chain_visibility_seg_queue_kernel, grid, dim3(256), 0, st, s, qs, tiles,
-                                   tile_count, masks, words, super_masks, base);
+                                   tile_count, masks, words, super_masks, base, touched);
             });
         return;
     }
     if (coherent_tiles(q))
         for_wave_chunks(waves, [&](dim3 grid, long long base) {
             hipLaunchKernelGGL(chain_visibility_kernel, grid, dim3(256), 0, st, s, q, tiles, tile_count, masks, words,
-                               super_masks, base);
+                               super_masks, base, touched);
         });
     else
         for_wave_chunks(waves, [&](dim3 grid, long long base) {
             hipLaunchKernelGGL(chain_visibility_queue_kernel, grid, dim3(256), 0, st, s, q, tiles, tile_count, masks,
-                               words, super_masks, base);
+                               words, super_masks, base, touched);
         });
 }
 
@@ -2059,7 +2123,7 @@ static void launch_diff_visibility(const SceneView& s, const wc_solve& q, const 
     if (pairs > 0 && pairs * (long long)s.face_count <= kShadowBudget) {
         shadow = reinterpret_cast<unsigned long long*>(
             static_cast<char*>(o.workspace) + shadow_offset(q.total_cells, q.lanes_per_cell, s.edge_count,
-                                                            q.max_tx_chains));
+                                                            q.max_tx_chains, q.tx_count));
         // 64 lanes per edge: on the urban canyon with 64 transmitters, 8 lanes
         // per edge (fewer triangle fetches) measured slower
         const int lanes_per_edge = 64;
@@ -2118,6 +2182,9 @@ static int radiomap_launch(const wc_scene* scene, const wc_solve* solve, const w
         return wc_fail("%s: workspace (wc_radiomap_workspace_bytes) is required with diffraction or reflection", who);
     if (tiles) {
         if (!tile_counter || tile_count <= 0) return wc_fail("tiled launch needs tile_counter and tile_count");
+        if ((size_t)tile_count > tile_capacity(solve->total_cells, width, solve->tx_count))
+            return wc_fail("%s: tile_count %d exceeds ceil(total_cells * lanes_per_cell / 64) + tx_count", who,
+                           tile_count);
         const int cached_cus = device_cus();
         int blocks = std::min((tile_count + 3) / 4, cached_cus * 8);
         blocks = std::max(blocks, 1);
