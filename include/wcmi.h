@@ -140,7 +140,9 @@ typedef struct wc_solve {
     const int* chain_groups;
     const double* chain_images;
     double frequency_hz; /* carrier (Hz); the phase -k L uses it in double */
-    int components;  /* bit 0 los, 1 reflection, 2 diffraction, 3 transmission */
+    int components;  /* bit 0 los, 1 reflection, 2 diffraction, 3 transmission; bit 4: a
+                        transmitter-receiver segment crossing more than max_depth walls sets
+                        failure bit 4 (wc_radiomap_out.failure) instead of carrying no path */
     int max_depth;
     int coherent;
     int lanes_per_cell; /* L: lanes sharing one cell's candidates (power of two, 1..64);

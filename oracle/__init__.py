@@ -102,6 +102,7 @@ def load() -> ctypes.CDLL:
         lib.oracle_transition.argtypes = [d, vp]
         lib.oracle_fresnel.argtypes = [d, vp]
         lib.oracle_utd.argtypes = [d, d, d, d, d, d, vp, vp]
+        lib.oracle_discover_all.argtypes = [ctypes.POINTER(OracleScene), vp, i, i, vp]
         lib.oracle_utd_finite.argtypes = [d, d, d, d, d, d, vp, vp, d, vp]
         lib.oracle_free_space.argtypes = [vp, vp, d, vp, vp, d, vp]
         lib.oracle_reflection_sequence.argtypes = [i, vp, vp, vp, vp, d, vp, vp, vp, vp, vp, vp, vp, d, vp]
@@ -286,6 +287,13 @@ class Oracle:
         buf = (OraclePath * cap)()
         count = self.lib.oracle_pair_paths(ctypes.byref(self.scene), ctypes.byref(q), buf, cap)
         return [buf[i] for i in range(min(count, cap))]
+
+    def discover_all(self, tx, ray_count: int, max_depth: int) -> np.ndarray:
+        """[ray_count, max_depth] group hit per bounce of every discovery ray."""
+        t = _c(tx, np.float64)
+        out = np.full((ray_count, max_depth), -1, np.int32)
+        self.lib.oracle_discover_all(ctypes.byref(self.scene), _p(t), int(ray_count), int(max_depth), _p(out))
+        return out
 
     def discover(self, tx, ray_count: int, max_depth: int, rays: np.ndarray | None = None) -> np.ndarray:
         t = _c(tx, np.float64)

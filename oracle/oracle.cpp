@@ -764,6 +764,12 @@ void oracle_discover_ray(const Scene* sp, const double* tx, int i, int ray_count
     for (; b < max_depth; ++b) out[b] = -1;
 }
 
+// Every discovery ray of one transmitter (OpenMP over rays): out [ray_count, max_depth].
+void oracle_discover_all(const Scene* sp, const double* tx, int ray_count, int max_depth, int* out) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < ray_count; ++i) oracle_discover_ray(sp, tx, i, ray_count, max_depth, out + (size_t)i * max_depth);
+}
+
 // Pair topology signature used for the ambiguity probe.
 static bool same_topology(const std::vector<OraclePath>& a, const std::vector<OraclePath>& b) {
     if (a.size() != b.size()) return false;

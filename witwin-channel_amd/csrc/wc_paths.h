@@ -15,6 +15,10 @@
 namespace wc {
 
 enum ComponentBits : int { kCompLos = 1, kCompRefl = 2, kCompDiff = 4, kCompTrans = 8 };
+// wc_solve.components flag: a transmitter-receiver segment that crosses more
+// walls than max_depth sets failure bit 4 instead of carrying no path
+constexpr int kStrictPenetration = 16;
+constexpr int kFailPenetration = 4;
 enum ComponentId : int { kCidLos = 0, kCidRefl = 1, kCidDiff = 2, kCidTrans = 5 };
 
 // accumulation slot of a component id: los 0, reflection 1, diffraction 2, transmission 3

@@ -191,6 +191,8 @@ __device__ void eval_cell(const SceneView& s, const wc_solve& q, int t, int cell
         rec.dep[0] = dir.x; rec.dep[1] = dir.y; rec.dep[2] = dir.z;
         sink.path(rec);
     }
+    if (kMain && sub == 0 && (comps & kCompTrans) && (comps & kStrictPenetration) && nh > max_depth)
+        sink.overflow();
     for (int d = 1; d <= max_depth; ++d) {
         if (kMain && (comps & kCompRefl)) {
             switch (d) {
@@ -280,6 +282,10 @@ struct AccumSink {
     int* topo = nullptr;
     int cap = 0;
     float sqrt_power = 1.0f;
+    int* failure = nullptr;
+    __device__ void overflow() {
+        if (failure) atomicOr(failure, kFailPenetration);
+    }
     __device__ void path(const PathRecord& r) {
         const int slot = slot_of(r.component);
         const double re = r.path_field.re, im = r.path_field.im;
@@ -300,6 +306,7 @@ struct AccumSink {
 };
 
 struct CountSink {
+    __device__ void overflow() {}
     static constexpr bool kNeedsRows = false;
     int count = 0;
     __device__ void path(const PathRecord&) { ++count; }
@@ -314,6 +321,7 @@ struct CellPartial {
 static_assert(sizeof(CellPartial) == 32, "CellPartial layout");
 
 struct EmitSink {
+    __device__ void overflow() {}
     static constexpr bool kNeedsRows = true;
     const wc_path_rows* rows;
     int row;
@@ -542,6 +550,7 @@ __device__ __forceinline__ void radiomap_lane(const SceneView& s, const wc_solve
                                               int tile = 0, int pos = 0) {
     AccumSink a;
     a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    a.failure = o.failure;
     CellPartial* partial = reinterpret_cast<CellPartial*>(o.workspace);
     const size_t lane_slot = (size_t)cell * width + sub;
     int cap_sub = 0;
@@ -830,6 +839,7 @@ __global__ void __launch_bounds__(256) diff_sum_kernel(wc_solve q, wc_radiomap_o
     const int t = tx_of_cell(q, pos);
     AccumSink a;
     a.sqrt_power = sqrtf(fmaxf(q.tx_power[t], 0.0f));
+    a.failure = o.failure;
     const CellPartial p = reinterpret_cast<const CellPartial*>(o.workspace)[cell];
     a.s_re = p.s_re;
     a.s_im = p.s_im;

@@ -298,6 +298,8 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
     dscene = device_scene(compiled, device)
     bits = component_bits(config.components)
     depth = config.max_depth if bits & (2 | 4 | 8) else 0
+    if bits & 8:
+        bits |= 16  # transmission penetration overflow fails loudly, as in the reference
     num_tx = int(ep.tx_position.shape[0])
     num_rx = int(ep.rx_position.shape[0])
     inputs = engine.solve_inputs(compiled, dscene, components=bits, max_depth=depth, coherent=config.coherent,
@@ -371,9 +373,14 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
             "path_planning": {"max_paths": config.max_paths, "max_paths_scope": config.max_paths_scope,
                               "candidate_count": inputs.chains.count, "truncated": truncated},
         }
-    if bool((out.failure != 0).any().item()):
-        raise RuntimeError("deterministic solve exceeded a native capacity (failure bits "
-                           f"{int(out.failure.item())})")
+    failure = int(out.failure.item())
+    if failure & 4:
+        # reference transmission.py: a segment crossing more walls than the
+        # hit capacity (max_depth) fails loudly
+        raise RuntimeError("deterministic transmission: a transmitter-receiver segment crosses more walls than "
+                           f"max_depth={config.max_depth} (penetration overflow)")
+    if failure:
+        raise RuntimeError(f"deterministic solve exceeded a native capacity (failure bits {failure})")
     return Result(path_gain=path_gain, field=field, component_power=comp_p, component_fields=comp_f, paths=paths,
                   metadata=metadata, diagnostics=diagnostics)
 

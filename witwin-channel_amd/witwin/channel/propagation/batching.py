@@ -175,16 +175,23 @@ class BatchedMeanDb:
         else:
             self.launch()
 
-    def power_maps(self, frequency_index: int = 0) -> list[torch.Tensor]:
-        """Per-transmitter path-gain vectors (input transmitter order)."""
+    def maps(self, frequency_index: int = 0, name: str = "power_total") -> list[torch.Tensor]:
+        """Per-transmitter slices (input transmitter order) of one per-cell
+        output of the radiomap (``power_total``, ``field_total``,
+        ``path_count``) at one carrier."""
         maps: list[torch.Tensor | None] = [None] * self.tx_count
         for fi, members, step in self.units:
             if fi != frequency_index:
                 continue
+            values = getattr(step.out, name)
             off = step.inputs.cell_offset.cpu().numpy()
             for k, i in enumerate(members):
-                maps[int(i)] = step.out.power_total[int(off[k]):int(off[k + 1])]
+                maps[int(i)] = values[int(off[k]):int(off[k + 1])]
         return maps  # type: ignore[return-value]
+
+    def power_maps(self, frequency_index: int = 0) -> list[torch.Tensor]:
+        """Per-transmitter path-gain vectors (input transmitter order)."""
+        return self.maps(frequency_index, "power_total")
 
 
 def chain_sequences_batched(compiled, dscene, tx_pos: torch.Tensor, max_depth: int, reflection: bool,
