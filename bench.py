@@ -275,6 +275,61 @@ def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: 
     }
 
 
+def batched_paths_section(stream) -> dict:
+    """Batched explicit-path solve (urban canyon + 3 layered street walls, 64
+    transmitters with 16..4096 street points each, LoS / reflection /
+    transmission to depth 3 + UTD, 28 GHz) and its channel frequency response
+    on 1,024 subcarriers over 100 MHz."""
+    import numpy as np
+    import torch
+
+    from witwin.channel import workloads
+    from witwin.channel.components import component_bits
+    from witwin.channel.path import Config, paths_cfr, solve_batched
+    from witwin.channel.propagation import engine
+    from witwin.channel.scene.compiler import compile as compile_scene, device_scene
+
+    device = torch.device("cuda", 0)
+    comps = {"los", "reflection", "diffraction", "transmission"}
+    w = workloads.batched_path_workload(seed=3, tx_count=64)
+    compiled = compile_scene(w.scene, reference_frequency_hz=w.frequency)
+    dscene = device_scene(compiled, device)
+    bits = component_bits(comps)
+    t = w.tx.shape[0]
+    pol = np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (w.rx.shape[0], 1))
+    t0 = time.perf_counter()
+    inputs = engine.solve_inputs(compiled, dscene, components=bits, max_depth=3, coherent=True,
+                                 frequency_hz=w.frequency, tx_pos=w.tx, tx_pol=pol[:t],
+                                 tx_power=np.ones(t, np.float32), rx_pos=w.rx, rx_pol=pol, rx_count=w.rx_count)
+    torch.cuda.synchronize()
+    planning_s = time.perf_counter() - t0
+    cfg = Config(components=comps, max_depth=3)
+    paths = solve_batched(compiled, cfg, w.tx, w.rx, w.rx_count, dscene=dscene, inputs=inputs)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        paths = solve_batched(compiled, cfg, w.tx, w.rx, w.rx_count, dscene=dscene, inputs=inputs)
+    torch.cuda.synchronize()
+    solve_ms = (time.perf_counter() - t0) * 1e3 / reps
+    a, tau, valid = paths.padded()
+    freqs = torch.as_tensor(w.subcarriers)
+    cfr_ms = kernel_time(lambda: paths_cfr(a, tau, valid, freqs, normalize_delays=False), stream)
+    pairs, width = int(a.shape[0]), int(a.shape[1])
+    cfr_bytes = pairs * 1024 * 8 + pairs * width * (8 + 4 + 1)
+    return {
+        "workload": f"urban canyon + 3 layered street walls ({compiled.face_count} tris), {t} Tx with street point "
+                    f"sets 16..4096 ({pairs} pairs), LoS/reflection/transmission depth 3 + UTD, 28 GHz; CFR on "
+                    "1024 subcarriers over 100 MHz",
+        "pairs": pairs, "paths": paths.path_count, "chain_planning_s": planning_s,
+        "solve_ms": solve_ms, "paths_per_s": paths.path_count / (solve_ms * 1e-3),
+        "cfr_ms": cfr_ms, "cfr_padded_paths_per_pair": width,
+        "cfr_roofline": {"bound": "hbm", "achieved_gbs": cfr_bytes / (cfr_ms * 1e-3) / 1e9, "peak_gbs": HBM_PEAK_GBS,
+                         "frac": cfr_bytes / (cfr_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes": cfr_bytes},
+    }
+
+
 def main() -> int:
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -466,6 +521,7 @@ def main() -> int:
                    "cells_per_s": h_cells / (h_ms * 1e-3), "forward_ms": h_fwd_ms, "vertex_vjp_ms": h_geo_ms,
                    "geometry_update": {"device_refit_ms": refit_ms, "host_recompile_and_build_ms": rebuild_ms}}
 
+    paths_batch = None if args.no_batch else batched_paths_section(stream)
     canyon_cpu = None
     if not args.no_cpu:
         from tests.gpu_helpers import Run, run_oracle
@@ -496,7 +552,8 @@ def main() -> int:
         "oracle_check": canyon_cpu,
     }
     if city is None:
-        print(json.dumps({"canyon_only": True, **canyon, "multi_tx": batch, "heightfield": terrain}))
+        print(json.dumps({"canyon_only": True, **canyon, "multi_tx": batch, "batched_paths": paths_batch,
+                          "heightfield": terrain}))
         return 0
     canyon_name = (f"urban canyon 8x8 boxes, {compiled.face_count} tris, 1 Tx at street intersection "
                    f"(53.231,52.013,25), {args.grid}x{args.grid} grid at 2 m, 28 GHz, depth 3 refl + UTD diff + "
@@ -542,6 +599,7 @@ def main() -> int:
             "heaviest_batch": {"tx": city["unit_tx"], "cells": city["unit_cells"], "kernel_ms": city["unit_ms"]},
             "canyon": {"workload": canyon_name, **canyon},
             "multi_tx": batch,
+            "batched_paths": paths_batch,
             "heightfield": terrain,
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

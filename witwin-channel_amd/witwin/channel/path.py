@@ -721,6 +721,112 @@ def _truncate(result: PathResult, max_paths: int) -> PathResult:
                    field_direction=cut(result.field_direction))
 
 
+@dataclass
+class BatchedPaths:
+    """Explicit paths of transmitters with their own receiver sets, flat in
+    canonical order: pairs are (local receiver, transmitter) with pair index
+    rx * T + tx, rows of a pair contiguous (``pair_offset`` CSR over the
+    ``pair_count`` pairs in that order) and ordered inside a pair as in
+    ``solve``. ``a`` is the complex coefficient (unit transmit power), ``tau``
+    the delay, theta / phi the departure (t) and arrival (r) angles."""
+
+    rows: dict[str, Any]
+    pair_offset: torch.Tensor   # [pairs + 1] int64
+    pair_tx: torch.Tensor       # [pairs] int64
+    pair_rx: torch.Tensor       # [pairs] int64 (index in the transmitter's receiver set)
+    a: torch.Tensor
+    tau: torch.Tensor
+    theta_t: torch.Tensor
+    phi_t: torch.Tensor
+    theta_r: torch.Tensor
+    phi_r: torch.Tensor
+
+    @property
+    def path_count(self) -> int:
+        return int(self.tau.shape[0])
+
+    @property
+    def pair_count(self) -> int:
+        return int(self.pair_tx.shape[0])
+
+    def padded(self) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+        """(a [pairs, P, 1], tau [pairs, P], valid [pairs, P]) padded to the
+        largest path count of a pair."""
+        device = self.tau.device
+        counts = (self.pair_offset[1:] - self.pair_offset[:-1])
+        width = int(counts.max().item()) if counts.numel() else 0
+        pairs = self.pair_count
+        a = torch.zeros((pairs, max(width, 1), 1), dtype=torch.complex64, device=device)
+        tau = torch.zeros((pairs, max(width, 1)), dtype=torch.float32, device=device)
+        valid = torch.zeros((pairs, max(width, 1)), dtype=torch.bool, device=device)
+        if self.path_count:
+            pair_of_row = torch.repeat_interleave(torch.arange(pairs, device=device), counts)
+            slot = torch.arange(self.path_count, device=device) - self.pair_offset[:-1][pair_of_row]
+            a[pair_of_row, slot, 0] = self.a
+            tau[pair_of_row, slot] = self.tau
+            valid[pair_of_row, slot] = True
+        return a, tau, valid
+
+    def cfr(self, frequencies: torch.Tensor, *, normalize_delays: bool = False) -> torch.Tensor:
+        """[pairs, F] c64 channel frequency response of every pair
+        (``wc_paths_cfr``: sum over its paths of a exp(-j 2 pi f (tau - tau0)),
+        tau0 = 0 unless ``normalize_delays``)."""
+        a, tau, valid = self.padded()
+        return paths_cfr(a, tau, valid, frequencies, normalize_delays=normalize_delays)[:, 0, :]
+
+
+def solve_batched(compiled: CompiledScene, config: Config, tx_positions: np.ndarray, rx_positions: np.ndarray,
+                  rx_count: np.ndarray, *, dscene=None, inputs=None) -> BatchedPaths:
+    """Explicit paths of many transmitters, each with its own receiver set
+    (the consecutive sets of sizes ``rx_count`` in ``rx_positions``), in one
+    batched launch set: the row count, the rows and their canonical order for
+    every (transmitter, receiver) pair of the batch (the reference solves the
+    full transmitter x receiver product, path.py:1551-1648)."""
+    device = _native.require_gpu()
+    _validate(config)
+    if dscene is None:
+        dscene = device_scene(compiled, device)
+    bits = component_bits(config.components)
+    depth = config.max_depth if bits & (2 | 4 | 8) else 0
+    tx_positions = np.ascontiguousarray(tx_positions, dtype=np.float32).reshape(-1, 3)
+    rx_positions = np.ascontiguousarray(rx_positions, dtype=np.float32).reshape(-1, 3)
+    t = tx_positions.shape[0]
+    pol = lambda n: np.tile(np.array([[0.0, 0.0, 1.0]], np.float32), (n, 1))  # noqa: E731
+    if inputs is None:
+        inputs = engine.solve_inputs(compiled, dscene, components=bits, max_depth=depth, coherent=True,
+                                     frequency_hz=compiled.frequency_hz, tx_pos=tx_positions, tx_pol=pol(t),
+                                     tx_power=np.ones(t, np.float32), rx_pos=rx_positions,
+                                     rx_pol=pol(rx_positions.shape[0]), rx_count=rx_count)
+    rows = engine.export_rows(dscene, inputs, width=max(depth, 1))
+    if config.max_paths is not None:
+        keep = _select_per_pair(rows, config.max_paths, t)
+        rows = {k: v[keep] if isinstance(v, torch.Tensor) else v for k, v in rows.items()}
+    counts = np.asarray(rows["pair_counts_canonical"], dtype=np.int64)
+    cells = rows["cell_order_canonical"]
+    rx_count = np.asarray(rx_count, dtype=np.int64)
+    cell_tx = np.repeat(np.arange(t), rx_count)
+    cell_rx = np.arange(cell_tx.shape[0]) - np.concatenate([[0], np.cumsum(rx_count)])[cell_tx]
+    if config.max_paths is not None:
+        counts = np.minimum(counts, config.max_paths)
+    offset = torch.as_tensor(np.concatenate([[0], np.cumsum(counts)])).to(device)
+    theta_t, phi_t = endpoint_angles(rows["departure_direction"])
+    theta_r, phi_r = endpoint_angles(-rows["arrival_direction"])
+    return BatchedPaths(rows=rows, pair_offset=offset, pair_tx=torch.as_tensor(cell_tx[cells]).to(device),
+                        pair_rx=torch.as_tensor(cell_rx[cells]).to(device), a=rows["coefficient"].to(torch.complex64),
+                        tau=rows["delay_s"].to(torch.float32), theta_t=theta_t, phi_t=phi_t, theta_r=theta_r,
+                        phi_r=phi_r)
+
+
+def _select_per_pair(rows: dict[str, Any], max_paths: int, num_tx: int) -> torch.Tensor:
+    pair = rows["rx_id"].long() * num_tx + rows["tx_id"].long()
+    count = int(pair.shape[0])
+    if count == 0:
+        return torch.zeros(0, dtype=torch.bool, device=pair.device)
+    _, inverse, counts = torch.unique_consecutive(pair, return_inverse=True, return_counts=True)
+    starts = torch.cumsum(counts, 0) - counts
+    return (torch.arange(count, device=pair.device) - starts[inverse]) < max_paths
+
+
 def solve(scene, config: Config, *, reference_frequency_hz) -> PathResult:  # type: ignore[no-untyped-def]
     """Solve canonical paths and pack them per (receiver, transmitter) pair."""
 
@@ -728,5 +834,5 @@ def solve(scene, config: Config, *, reference_frequency_hz) -> PathResult:  # ty
     return solve_compiled(compiled, config)
 
 
-__all__ = ["Config", "InteractionType", "PathResult", "RaggedPathSoA", "BeamformedPathResult", "endpoint_angles",
-           "from_rows", "paths_cfr", "solve"]
+__all__ = ["BatchedPaths", "Config", "InteractionType", "PathResult", "RaggedPathSoA", "BeamformedPathResult",
+           "endpoint_angles", "from_rows", "paths_cfr", "solve", "solve_batched"]

@@ -355,3 +355,38 @@ def scaled_city(seed: int = 5, size: int = 256, pitch: float = 2.0, relief: floa
                   metadata={"imported_edge_policy": SimpleNamespace(edge_selection_mode="vertical_only")})
     return SimpleNamespace(scene=scene, tx=tx, sides=sides, rx_count=sides * sides, rx=rx, extent=extent,
                            receiver_z=z, frequencies=(3.5e9, 28.0e9))
+
+
+# thin walls of the batched path workload: (x0, y0, x1, y1) footprint in the
+# street network, height, and the layer stack (thickness m, eps_r, sigma S/m)
+STREET_WALLS = (
+    ((52.0, 20.0, 52.0, 40.0), 8.0, ((0.12, 5.31, 0.0326),)),
+    ((105.0, 97.0, 125.0, 97.0), 10.0, ((0.01, 6.27, 0.0043), (0.05, 1.0, 0.0), (0.01, 6.27, 0.0043))),
+    ((187.0, 150.0, 187.0, 175.0), 6.0, ((0.02, 2.94, 0.0116), (0.1, 1.99, 0.0047))),
+)
+
+
+def street_wall(x0: float, y0: float, x1: float, y1: float, height: float, layers, surface_id: int) -> Structure:
+    """Vertical thin-sheet wall on the segment (x0, y0)-(x1, y1), 0..height m,
+    with a layered material (1-3 layers)."""
+    v = np.array([[x0, y0, 0.0], [x1, y1, 0.0], [x0, y0, height], [x1, y1, height]], dtype=np.float32)
+    f = np.array([[0, 1, 2], [1, 3, 2]], dtype=np.int64)
+    mat = PhysicalMaterial(layers=tuple(MaterialLayer(thickness_m=t, eps_r=e, sigma_e=s) for t, e, s in layers),
+                           name=f"wall-{surface_id}")
+    return Structure(Mesh(torch.from_numpy(v), torch.from_numpy(f)), mat, name=f"street-wall-{surface_id}",
+                     surface_id=surface_id)
+
+
+def batched_path_workload(seed: int = 3, tx_count: int = 64) -> SimpleNamespace:
+    """The urban canyon (seed 2) plus the three layered street walls of
+    ``STREET_WALLS``; ``tx_count`` transmitters over the streets at 3-30 m,
+    each with a street-point receiver set of log-uniform size 16..4096
+    (``batched_transmitters``). Carrier 28 GHz; the channel frequency response
+    is evaluated on 1,024 subcarriers spanning 100 MHz around it."""
+    scene = urban_canyon_scene(seed=2, shape=(2, 2), spacing=2.0)
+    walls = [street_wall(*seg, h, layers, 900 + k) for k, (seg, h, layers) in enumerate(STREET_WALLS)]
+    scene = Scene(structures=list(scene.structures) + walls, endpoints=list(scene.endpoints))
+    tx, sizes, rx = batched_transmitters(seed=seed, tx_count=tx_count)
+    spacing = 100.0e6 / 1024
+    subcarriers = 28.0e9 + (np.arange(1024) - 512) * spacing
+    return SimpleNamespace(scene=scene, tx=tx, rx_count=sizes, rx=rx, frequency=28.0e9, subcarriers=subcarriers)
