@@ -99,10 +99,8 @@ def traffic_bytes(kernel: str, workload: str):
     path = ROOT / "profiles" / "hbm_traffic.json"
     if not path.exists():
         return None
-    entry = json.loads(path.read_text()).get("kernels", {}).get(kernel)
-    if not entry or entry.get("workload") != workload:
-        return None
-    return entry.get("bytes_per_launch")
+    entry = json.loads(path.read_text()).get("workloads", {}).get(workload, {}).get("kernels", {}).get(kernel)
+    return entry.get("bytes_per_launch") if entry else None
 
 
 def descriptor_split(out) -> tuple[int, int]:
@@ -196,18 +194,20 @@ def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: 
     scene_bytes = compiled.face_count * 48 + int(dscene.node_count) * 64
     # the dominant kernel, the chain validity pass, over every batch (first
     # carrier; the second reuses its masks): per cell it reads the receiver
-    # and its order entry and writes one bit per chain of its transmitter
-    # (mask words padded to the batch's largest count); the scene and the
-    # chain tables (groups + float64 images) once per launch
+    # and its order entry and writes one bit per chain of its own
+    # transmitter; the scene and the chain tables (groups + float64 images)
+    # once per launch
     first = [u for u in run.units if u[0] == 0]
+    tx_chains = np.array([sum(x.shape[0] for x in per) for per in seqs], np.int64)
+    cell_bytes = city.rx_count.astype(np.int64) * (16 + 4 * ((tx_chains + 31) // 32))
 
     def all_chain_passes():
         for _, _, u in first:
             engine.radiomap_pass(dscene, u.inputs, u.out, 1)
 
     dom_ms = kernel_time(all_chain_passes, stream, reps=3) / len(first)
-    dom_bytes = sum(u.inputs.total_cells * (16 + 4 * ((int(u.inputs.view.max_tx_chains) + 31) // 32)) + scene_bytes
-                    + int(u.inputs.chains.count) * (4 * 5 + 8 * 15) for _, _, u in first) / len(first)
+    dom_bytes = sum(int(cell_bytes[np.asarray(m)].sum()) + scene_bytes + int(u.inputs.chains.count) * (4 * 5 + 8 * 15)
+                    for _, m, u in first) / len(first)
     paths = run.paths
 
     ms_plain, identical = None, None
@@ -418,6 +418,18 @@ def main() -> int:
     vjp1_ms = kernel_time(lambda: step.adjoint(2), stream, before=lambda: step.adjoint(1))
     reduce_ms = kernel_time(step.reduce, stream)
     n_main, n_diff = descriptor_split(step.out)
+    # explicit export of the same solve (counts, device pair offsets, emit,
+    # in-pair canonical order, identity compaction, gather of every field),
+    # wall time including its two total reads
+    engine.export_rows(dscene, inputs, width=depth)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        rows = engine.export_rows(dscene, inputs, width=depth)
+    torch.cuda.synchronize()
+    export_ms = (time.perf_counter() - t0) * 1e3 / 3
+    export_paths = int(rows["tx_id"].shape[0])
+    del rows
     edges = int(dscene.view.edge_count)
     chains = int(inputs.chains.count)
     scene_bytes = compiled.face_count * 48 + int(dscene.node_count) * 64
@@ -548,6 +560,7 @@ def main() -> int:
         "plain_ms_per_step": ms_plain, "forward_ms": fwd_ms,
         "kernel_ms": {**{k: v[0] for k, v in kernels.items()}, "vjp_reduce_kernel": reduce_ms},
         "paths_reflection_pass": n_main, "paths_diffraction": n_diff, "chain_planning_s": plan_s,
+        "export": {"ms": export_ms, "paths": export_paths, "paths_per_s": export_paths / (export_ms * 1e-3)},
         "dominant_kernel": dominant, "dominant_hbm_frac": (dom_bytes / (dom_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
         "oracle_check": canyon_cpu,
     }
@@ -584,6 +597,7 @@ def main() -> int:
             "cells_per_carrier": city["grid_cells"],
             "carriers_hz": city["frequencies"],
             "paths": city["paths"],
+            "paths_per_s": city["paths"] / (city["ms_batched"] * 1e-3),
             "chains": city["chains"],
             "edges": city["edges"],
             "plain_ms_per_step": city["ms_plain"],
@@ -604,7 +618,7 @@ def main() -> int:
         },
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_bytes("chain_visibility_seg_kernel", workload_name),
+                     "traffic": traffic_bytes("chain_validity_pass", workload_name),
                      "kernel": "chain_visibility_seg_kernel (with its pass's super_box, super_cull and fill_zero "
                                "launches; average over the batches)",
                      "kernel_ms": city["dom_ms"], "algorithmic_bytes_per_launch": city["dom_bytes"]},

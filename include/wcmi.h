@@ -382,6 +382,74 @@ int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const int* row_o
 int wc_paths_canonical_order(const wc_path_rows* rows, const int* pair_offset, int pair_count, int* permutation,
                              unsigned char* keep, void* stream);
 
+/* Device-side canonical compaction (reference canonical compact,
+ * native/channel/kernels/evaluated_paths.cu:355-384 (max_paths), 513-965;
+ * pair offsets there come from the topology stages' scans, paths.cu:1808-2007).
+ * Workspace of the three calls below for up to `cells` pairs. */
+size_t wc_paths_compact_workspace_bytes(int cells);
+
+/* Canonical pair order: pair_cell [cells] receives the cell (transmitter t's
+ * receiver r, cells laid out by cell_offset [T + 1]) at each canonical position,
+ * receiver-major then transmitter (a stable radix sort of r * T + t; max_rx
+ * bounds r, and max_rx * T must fit 32 bits). Depends on the layout only. */
+int wc_paths_pair_order(const int* cell_offset, int tx_count, int cells, int max_rx, int* pair_cell, void* workspace,
+                        void* stream);
+
+/* Row offsets of the export: pair_offset [cells + 1] = exclusive scan of
+ * path_count [cells] taken in canonical order (path_count[pair_cell[i]]),
+ * row_offset [cells] = the same offsets by cell (wc_paths_emit's input), and
+ * *total (device int64) = the row count; the caller checks it fits int32. */
+int wc_paths_row_offsets(const int* path_count, const int* pair_cell, int cells, int* row_offset, int* pair_offset,
+                         long long* total, void* workspace, void* stream);
+
+/* Canonical order, identity de-duplication and per-pair cap in one call:
+ * permutation / keep as wc_paths_canonical_order; each pair then keeps its
+ * identity winners in sorted order, at most max_paths_per_pair of them (< 0:
+ * no cap); kept_offset [pair_count + 1] is the scan of the kept counts,
+ * source_row [rows] lists the kept rows' indices in the emitted table in final
+ * order (the caller gathers every field through it), *kept_total (device
+ * int64) the kept row count and *dropped (device) the identity winners the cap
+ * removed. */
+int wc_paths_compact(const wc_path_rows* rows, const int* pair_offset, int pair_count, int max_paths_per_pair,
+                     int* permutation, unsigned char* keep, int* kept_offset, int* source_row, long long* kept_total,
+                     int* dropped, void* workspace, void* stream);
+
+/* ------------------------------------------------------------ chain planning
+ * Candidate reflection chains from discovery rays, on the device (reference
+ * witwin/channel/propagation/reflection.py:186-205, 541-569: the distinct
+ * group-sequence prefixes of each transmitter's rays, per depth, sorted).
+ * rays [T, R, max_depth] are wc_discover_chains' output (-1 = escaped).
+ * Per depth d = 1, 2, ...: wc_plan_level sorts the rays' d-prefixes and
+ * counts the distinct ones into *prefix_count (device int64); the caller
+ * allocates that many entries per array and wc_plan_assign numbers them in
+ * (transmitter, lexicographic) order: prefix_parent (id at depth d - 1, or
+ * the transmitter for d = 1), prefix_group (d-th group), prefix_tx, and adds
+ * them to counts [T, WC_PLAN_DEPTHS] (zeroed by the d = 1 level call).
+ * parent_tx is depth d - 1's prefix_tx (unused for d = 1). The workspace
+ * (wc_plan_workspace_bytes(T, R)) carries the rays' prefix ids from one
+ * depth to the next, so the depths run in order on one workspace.
+ * wc_plan_table writes depth d's prefixes as chain rows (groups [K, 5],
+ * padded with -1, their transmitter and depth) at row row_start[tx] + id;
+ * wc_plan_exhaustive writes one depth's exhaustive sequence list
+ * [count, d] for every transmitter at rows row_start[tx] + j. */
+#define WC_PLAN_DEPTHS 5
+typedef struct wc_plan_levels {
+    const int* prefix_parent[WC_PLAN_DEPTHS];
+    const int* prefix_group[WC_PLAN_DEPTHS];
+    const int* prefix_tx[WC_PLAN_DEPTHS];
+} wc_plan_levels;
+
+size_t wc_plan_workspace_bytes(int tx_count, int ray_count);
+int wc_plan_level(const int* rays, int tx_count, int ray_count, int max_depth, int d, int group_count, int* counts,
+                  long long* prefix_count, void* workspace, void* stream);
+int wc_plan_assign(const int* rays, int tx_count, int ray_count, int max_depth, int d, int group_count,
+                   const int* parent_tx, int* prefix_parent, int* prefix_group, int* prefix_tx, int* counts,
+                   void* workspace, void* stream);
+int wc_plan_table(const wc_plan_levels* levels, int d, int prefix_count, const int* row_start, int* groups,
+                  int* chain_tx, int* chain_depth, void* stream);
+int wc_plan_exhaustive(const int* sequences, int count, int d, int tx_count, const int* row_start, int* groups,
+                       int* chain_tx, int* chain_depth, void* stream);
+
 /* ------------------------------------------------------------ channel frequency response
  * CFR of padded per-pair path sets, replacing PathResult.cfr
  * (witwin/channel/path.py:646-656) and BeamformedPathResult.cfr (:824-832):

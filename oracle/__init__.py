@@ -35,6 +35,11 @@ visibility. Parity status of each restated function, i.e. what pins it:
                          deterministic.cu:829-970 on hand-built rows  -> pinned
 * canonical order ...... sort-order golden vector of
                          test_reflection_multibounce.py:25-80  -> pinned
+* canonical compact .... (canonical_compact) the sort-order golden vector above,
+                         hand-worked identity groups (shortest row kept, first
+                         of equal shortest, a longer duplicate dropped, a
+                         diffraction row keyed by its edge) and the per-pair
+                         cap  -> pinned
 * discovery rays ....... restated ray-by-ray; compared with the GPU chain set  -> pinned by equality
 """
 
@@ -305,3 +310,82 @@ class Oracle:
                                          _p(row))
             out[j] = row
         return out
+
+
+# ----------------------------------------------------------------- canonical compact
+
+def _event(component: int, depth: int, slot: int, width: int) -> int:
+    """Event type of a row's slot (reference evaluated_paths.cu:128-153)."""
+    if component == 1 and slot < depth:
+        return 1
+    if component == 2 and depth > 0 and slot == 0:
+        return 2
+    if component == 5 and slot < depth:
+        return 4
+    if component == 6 and depth > 0 and slot == 0:
+        return 8
+    if width >= 2 and depth >= 2 and slot < 2:
+        if component == 3:
+            return 1 if slot == 0 else 2
+        if component == 4:
+            return 2 if slot == 0 else 1
+        if component == 7:
+            return 2
+    return 0
+
+
+def _identity(rows: dict, i: int, width: int) -> tuple:
+    c, d = int(rows["component_id"][i]), int(rows["depth"][i])
+    out = []
+    for j in range(width):
+        t = _event(c, d, j, width)
+        obj = 0
+        if t:
+            obj = int(rows["edge_id"][i]) if (c == 2 and j == 0) else int(rows["primitive_sequence"][i, j])
+        out.append((t, obj))
+    return tuple(out)
+
+
+def canonical_compact(rows: dict, pair_offset: np.ndarray, cap: int | None = None) -> dict:
+    """Canonical compact of emitted rows, pair by pair (reference
+    native/channel/kernels/evaluated_paths.cu): stable sort on (depth,
+    component, primitive sequence, primitive, edge) (:654-700); identity groups
+    (per slot: event type and object id, :126-310; runs of the sorted pair,
+    :216-320) keep their first strictly shortest row, none when a length is
+    NaN; at most ``cap`` identity winners per pair (:355-384).
+    rows: numpy arrays (``primitive_sequence`` [K, width]). Returns the sorted
+    permutation, keep flags in sorted order, the kept rows' offsets per pair and
+    their source row indices."""
+    width = rows["primitive_sequence"].shape[1]
+    pairs = len(pair_offset) - 1
+    perm = np.zeros(int(pair_offset[-1]), np.int64)
+    keep = np.zeros(int(pair_offset[-1]), np.uint8)
+    kept_offset = np.zeros(pairs + 1, np.int64)
+    source = []
+    dropped = 0
+    for p in range(pairs):
+        b, e = int(pair_offset[p]), int(pair_offset[p + 1])
+        key = lambda i: (int(rows["depth"][i]), int(rows["component_id"][i]),  # noqa: E731
+                         *(int(x) for x in rows["primitive_sequence"][i]), int(rows["primitive_id"][i]),
+                         int(rows["edge_id"][i]))
+        order = sorted(range(b, e), key=key)  # Python's sort is stable
+        perm[b:e] = order
+        s = 0
+        winners = []
+        while s < len(order):
+            ident = _identity(rows, order[s], width)
+            t = s + 1
+            while t < len(order) and _identity(rows, order[t], width) == ident:
+                t += 1
+            lengths = [float(rows["path_length_m"][order[x]]) for x in range(s, t)]
+            if not any(np.isnan(lengths)):  # a NaN length leaves its group without a winner
+                win = s + lengths.index(min(lengths))
+                keep[b + win] = 1
+                winners.append(order[win])
+            s = t
+        kept = winners if cap is None else winners[:cap]
+        dropped += len(winners) - len(kept)
+        source.extend(kept)
+        kept_offset[p + 1] = kept_offset[p] + len(kept)
+    return {"perm": perm, "keep": keep, "kept_offset": kept_offset, "source_row": np.asarray(source, np.int64),
+            "dropped": dropped}

@@ -215,6 +215,55 @@ def test_oracle_canonical_sort_contract():
     assert seqs[order].tolist() == [[3, 9], [4, 1], [4, 2]]
 
 
+def _compact_rows(spec):
+    """Rows from (depth, component, sequence, primitive, edge, length) tuples."""
+    return {"depth": np.array([r[0] for r in spec], np.int32),
+            "component_id": np.array([r[1] for r in spec], np.int32),
+            "primitive_sequence": np.array([r[2] for r in spec], np.int32),
+            "primitive_id": np.array([r[3] for r in spec], np.int32),
+            "edge_id": np.array([r[4] for r in spec], np.int32),
+            "path_length_m": np.array([r[5] for r in spec], np.float32)}
+
+
+def test_oracle_canonical_compact_known_answers():
+    """Hand-worked canonical compact of three pairs (reference
+    evaluated_paths.cu:126-320, 654-700): the sort-order vector [[4,2],[4,1],
+    [3,9]] -> [[3,9],[4,1],[4,2]] after the line-of-sight row; a longer
+    duplicate of [4,1] dropped for the shorter one; two equal-length
+    duplicates keep the first in sorted order; diffraction rows are identified
+    by their edge; a NaN length leaves its group without a winner; an empty
+    pair in between."""
+    spec = [
+        # pair 0 (rows 0-4)
+        (2, 1, [4, 2], 2, -1, 10.0),
+        (2, 1, [4, 1], 1, -1, 11.0),
+        (2, 1, [3, 9], 9, -1, 12.0),
+        (0, 0, [-1, -1], -1, -1, 5.0),
+        (2, 1, [4, 1], 1, -1, 10.5),
+        # pair 2 (rows 5-9); pair 1 is empty
+        (1, 2, [6, -1], 6, 7, 3.0),
+        (1, 2, [6, -1], 6, 8, 4.0),
+        (1, 1, [6, -1], 6, -1, 3.5),
+        (1, 1, [6, -1], 6, -1, 3.5),
+        (1, 5, [3, -1], 3, -1, float("nan")),
+    ]
+    rows = _compact_rows(spec)
+    out = oracle.canonical_compact(rows, np.array([0, 5, 5, 10]))
+    # pair 0 sorted: LoS (3), [3,9] (2), [4,1] (1), [4,1] (4), [4,2] (0); rows 1 and 4 share an identity
+    assert out["perm"][:5].tolist() == [3, 2, 1, 4, 0]
+    assert out["keep"][:5].tolist() == [1, 1, 0, 1, 1]
+    # pair 2 sorted: reflection d1 (7, 8: equal lengths, first kept), diffraction edges 7 then 8, transmission (NaN)
+    assert out["perm"][5:].tolist() == [7, 8, 5, 6, 9]
+    assert out["keep"][5:].tolist() == [1, 0, 1, 1, 0]
+    assert out["kept_offset"].tolist() == [0, 4, 4, 7]
+    assert out["source_row"].tolist() == [3, 2, 4, 0, 7, 5, 6]
+    assert out["dropped"] == 0
+    capped = oracle.canonical_compact(rows, np.array([0, 5, 5, 10]), cap=2)
+    assert capped["kept_offset"].tolist() == [0, 2, 2, 4]
+    assert capped["source_row"].tolist() == [3, 2, 7, 5]
+    assert capped["dropped"] == 3
+
+
 # ----------------------------------------------------------------- host compile
 
 def test_coplanar_groups_golden():

@@ -1819,74 +1819,6 @@ __global__ void __launch_bounds__(256) emit_kernel(SceneView s, wc_solve q, cons
     }
 }
 
-// Canonical order inside each pair segment: (depth, component, sequence
-// columns 0..w-1, primitive, edge), stable; then the first strictly-shortest
-// row of each identity group (tx, rx, per-slot event type and object id) is kept
-// (reference native/channel/kernels/evaluated_paths.cu:126-310, 654-700).
-__device__ __forceinline__ bool row_less(const wc_path_rows& r, int a, int b) {
-    if (r.depth[a] != r.depth[b]) return r.depth[a] < r.depth[b];
-    if (r.component_id[a] != r.component_id[b]) return r.component_id[a] < r.component_id[b];
-    for (int j = 0; j < r.width; ++j) {
-        const int x = r.primitive_sequence[a * r.width + j], y = r.primitive_sequence[b * r.width + j];
-        if (x != y) return x < y;
-    }
-    if (r.primitive_id[a] != r.primitive_id[b]) return r.primitive_id[a] < r.primitive_id[b];
-    if (r.edge_id[a] != r.edge_id[b]) return r.edge_id[a] < r.edge_id[b];
-    return false;
-}
-
-__device__ __forceinline__ int event_type(int component, int depth, int slot) {
-    if (component == 1 && slot < depth) return 1;
-    if (component == 2 && depth > 0 && slot == 0) return 2;
-    if (component == 5 && slot < depth) return 4;
-    if (component == 6 && depth > 0 && slot == 0) return 8;
-    return 0;
-}
-
-__device__ __forceinline__ bool same_identity(const wc_path_rows& r, int a, int b) {
-    for (int j = 0; j < r.width; ++j) {
-        const int ta = event_type(r.component_id[a], r.depth[a], j);
-        const int tb = event_type(r.component_id[b], r.depth[b], j);
-        if (ta != tb) return false;
-        if (ta == 0) continue;
-        const int oa = (r.component_id[a] == 2 && j == 0) ? r.edge_id[a] : r.primitive_sequence[a * r.width + j];
-        const int ob = (r.component_id[b] == 2 && j == 0) ? r.edge_id[b] : r.primitive_sequence[b * r.width + j];
-        if (oa != ob) return false;
-    }
-    return true;
-}
-
-__global__ void canonical_kernel(wc_path_rows rows, const int* __restrict__ pair_offset, int pair_count,
-                                 int* __restrict__ perm, unsigned char* __restrict__ keep) {
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < pair_count; p += blockDim.x * gridDim.x) {
-        const int begin = pair_offset[p], end = pair_offset[p + 1];
-        for (int i = begin; i < end; ++i) perm[i] = i;
-        for (int i = begin + 1; i < end; ++i) {
-            const int v = perm[i];
-            int j = i;
-            while (j > begin && row_less(rows, v, perm[j - 1])) {
-                perm[j] = perm[j - 1];
-                --j;
-            }
-            perm[j] = v;
-        }
-        int g = begin;
-        while (g < end) {
-            int h = g + 1;
-            while (h < end && same_identity(rows, perm[g], perm[h])) ++h;
-            float best = rows.path_length_m[perm[g]];
-            for (int i = g + 1; i < h; ++i) best = fminf(best, rows.path_length_m[perm[i]]);
-            bool taken = false;
-            for (int i = g; i < h; ++i) {
-                const bool win = !taken && rows.path_length_m[perm[i]] == best;
-                keep[i] = win ? 1 : 0;
-                taken = taken || win;
-            }
-            g = h;
-        }
-    }
-}
-
 // ------------------------------------------------------------------ discovery
 __global__ void discover_kernel(SceneView s, const float* __restrict__ tx_pos, int tx_count, int ray_count,
                                 int max_depth, int* __restrict__ chains) {
@@ -2299,16 +2231,6 @@ extern "C" int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const
                        *solve, row_offset, path_count, *rows);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_paths_emit", err);
-}
-
-extern "C" int wc_paths_canonical_order(const wc_path_rows* rows, const int* pair_offset, int pair_count,
-                                        int* permutation, unsigned char* keep, void* stream) {
-    if (!rows || !pair_offset || !permutation || !keep) return wc_fail("wc_paths_canonical_order: null argument");
-    if (pair_count <= 0) return 0;
-    hipLaunchKernelGGL(canonical_kernel, dim3(blocks_for(pair_count, 256)), dim3(256), 0, (hipStream_t)stream, *rows,
-                       pair_offset, pair_count, permutation, keep);
-    const hipError_t err = hipGetLastError();
-    return err == hipSuccess ? 0 : wc_fail_hip("wc_paths_canonical_order", err);
 }
 
 extern "C" int wc_visibility(const wc_scene* scene, int count, const float* source, const float* target,

@@ -82,6 +82,10 @@ class WcPathRows(ctypes.Structure):
         "departure_direction")]
 
 
+class WcPlanLevels(ctypes.Structure):
+    _fields_ = [("prefix_parent", c_void_p * 5), ("prefix_group", c_void_p * 5), ("prefix_tx", c_void_p * 5)]
+
+
 class WcFixedRows(ctypes.Structure):
     _fields_ = [
         ("count", c_int), ("width", c_int), ("component", c_void_p), ("depth", c_void_p), ("faces", c_void_p),
@@ -144,6 +148,18 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                               ctypes.POINTER(WcPathRows), c_void_p]),
     "wc_paths_canonical_order": (c_int, [ctypes.POINTER(WcPathRows), c_void_p, c_int, c_void_p, c_void_p,
                                          c_void_p]),
+    "wc_paths_compact_workspace_bytes": (c_size_t, [c_int]),
+    "wc_plan_workspace_bytes": (c_size_t, [c_int, c_int]),
+    "wc_plan_level": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "wc_plan_assign": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int] + [c_void_p] * 7),
+    "wc_plan_table": (c_int, [ctypes.POINTER(WcPlanLevels), c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                              c_void_p]),
+    "wc_plan_exhaustive": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p]),
+    "wc_paths_pair_order": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "wc_paths_row_offsets": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "wc_paths_compact": (c_int, [ctypes.POINTER(WcPathRows), c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_paths_cfr": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, ctypes.c_double,
                              ctypes.c_double, c_int, c_int, c_void_p, c_void_p]),
     "wc_utd_special_eval": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p]),

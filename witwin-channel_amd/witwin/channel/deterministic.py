@@ -155,22 +155,6 @@ def phase_from_field(field: torch.Tensor) -> torch.Tensor:
     return phase.to(torch.float32)
 
 
-def select_max_paths(rows: dict[str, Any], max_paths: int | None, scope: str, num_tx: int) -> torch.Tensor:
-    """Row mask keeping the first ``max_paths`` rows globally or per pair, in
-    canonical order (reference evaluated_paths.cu:355-384)."""
-    count = int(rows["tx_id"].shape[0])
-    device = rows["tx_id"].device
-    if max_paths is None:
-        return torch.ones(count, dtype=torch.bool, device=device)
-    if scope == "global":
-        return torch.arange(count, device=device) < max_paths
-    pair = rows["rx_id"].long() * num_tx + rows["tx_id"].long()
-    _, inverse, counts = torch.unique_consecutive(pair, return_inverse=True, return_counts=True)
-    starts = torch.cumsum(counts, 0) - counts
-    rank = torch.arange(count, device=device) - starts[inverse]
-    return rank < max_paths
-
-
 def build_path_table(rows: dict[str, Any], *, include_fields: bool = True) -> PathTable:
     path_field = rows["path_field"] if include_fields else torch.zeros_like(rows["path_field"])
     count = int(rows["tx_id"].shape[0])
@@ -330,10 +314,14 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
     counts = {"los": 0, "reflection": 0, "diffraction": 0}
     truncated = False
     if config.export_paths or config.max_paths is not None:
-        rows = engine.export_rows(dscene, inputs, width=max(depth, 1))
-        keep = select_max_paths(rows, config.max_paths, config.max_paths_scope, num_tx)
-        truncated = bool((~keep).any().item())
-        rows = {k: v[keep] if isinstance(v, torch.Tensor) else v for k, v in rows.items()}
+        per_pair = config.max_paths if config.max_paths_scope == "per_pair" else None
+        rows = engine.export_rows(dscene, inputs, width=max(depth, 1), max_paths_per_pair=per_pair)
+        truncated = rows["dropped"] > 0
+        if config.max_paths is not None and config.max_paths_scope == "global":
+            count = int(rows["tx_id"].shape[0])
+            truncated = count > config.max_paths
+            rows = {k: (v[:config.max_paths] if isinstance(v, torch.Tensor) and k not in ("pair_offset", "pair_cell")
+                        else v) for k, v in rows.items()}
         if truncated:
             # the reference truncates before fields and accumulation
             # (propagation/enumerated.py:788-792): the maps sum the kept rows only

@@ -694,7 +694,7 @@ def solve_compiled(compiled: CompiledScene, config: Config) -> PathResult:
     depth = config.max_depth if bits & (2 | 4 | 8) else 0
     inputs = engine.solve_inputs(compiled, dscene, components=bits, max_depth=depth, coherent=True,
                                  frequency_hz=compiled.frequency_hz)
-    rows = engine.export_rows(dscene, inputs, width=max(depth, 1))
+    rows = engine.export_rows(dscene, inputs, width=max(depth, 1), max_paths_per_pair=config.max_paths)
     tx_pos = torch.as_tensor(ep.tx_position, dtype=torch.float32).to(device)
     rx_pos = torch.as_tensor(ep.rx_position, dtype=torch.float32).to(device)
     count = int(rows["tx_id"].shape[0])
@@ -797,34 +797,19 @@ def solve_batched(compiled: CompiledScene, config: Config, tx_positions: np.ndar
                                      frequency_hz=compiled.frequency_hz, tx_pos=tx_positions, tx_pol=pol(t),
                                      tx_power=np.ones(t, np.float32), rx_pos=rx_positions,
                                      rx_pol=pol(rx_positions.shape[0]), rx_count=rx_count)
-    rows = engine.export_rows(dscene, inputs, width=max(depth, 1))
-    if config.max_paths is not None:
-        keep = _select_per_pair(rows, config.max_paths, t)
-        rows = {k: v[keep] if isinstance(v, torch.Tensor) else v for k, v in rows.items()}
-    counts = np.asarray(rows["pair_counts_canonical"], dtype=np.int64)
-    cells = rows["cell_order_canonical"]
-    rx_count = np.asarray(rx_count, dtype=np.int64)
-    cell_tx = np.repeat(np.arange(t), rx_count)
-    cell_rx = np.arange(cell_tx.shape[0]) - np.concatenate([[0], np.cumsum(rx_count)])[cell_tx]
-    if config.max_paths is not None:
-        counts = np.minimum(counts, config.max_paths)
-    offset = torch.as_tensor(np.concatenate([[0], np.cumsum(counts)])).to(device)
+    rows = engine.export_rows(dscene, inputs, width=max(depth, 1), max_paths_per_pair=config.max_paths)
+    offset = rows.pop("pair_offset")
+    cells = rows.pop("pair_cell")
+    rows.pop("dropped")
+    # transmitter and receiver of each canonical pair, from the cell layout
+    cell_offset = inputs.cell_offset.to(torch.int64)
+    pair_tx = torch.searchsorted(cell_offset, cells, right=True) - 1
+    pair_rx = cells - cell_offset[pair_tx]
     theta_t, phi_t = endpoint_angles(rows["departure_direction"])
     theta_r, phi_r = endpoint_angles(-rows["arrival_direction"])
-    return BatchedPaths(rows=rows, pair_offset=offset, pair_tx=torch.as_tensor(cell_tx[cells]).to(device),
-                        pair_rx=torch.as_tensor(cell_rx[cells]).to(device), a=rows["coefficient"].to(torch.complex64),
-                        tau=rows["delay_s"].to(torch.float32), theta_t=theta_t, phi_t=phi_t, theta_r=theta_r,
-                        phi_r=phi_r)
-
-
-def _select_per_pair(rows: dict[str, Any], max_paths: int, num_tx: int) -> torch.Tensor:
-    pair = rows["rx_id"].long() * num_tx + rows["tx_id"].long()
-    count = int(pair.shape[0])
-    if count == 0:
-        return torch.zeros(0, dtype=torch.bool, device=pair.device)
-    _, inverse, counts = torch.unique_consecutive(pair, return_inverse=True, return_counts=True)
-    starts = torch.cumsum(counts, 0) - counts
-    return (torch.arange(count, device=pair.device) - starts[inverse]) < max_paths
+    return BatchedPaths(rows=rows, pair_offset=offset, pair_tx=pair_tx, pair_rx=pair_rx,
+                        a=rows["coefficient"].to(torch.complex64), tau=rows["delay_s"].to(torch.float32),
+                        theta_t=theta_t, phi_t=phi_t, theta_r=theta_r, phi_r=phi_r)
 
 
 def solve(scene, config: Config, *, reference_frequency_hz) -> PathResult:  # type: ignore[no-untyped-def]

@@ -478,6 +478,8 @@ def evaluate(compiled_scene: CompiledScene, request: PropagationRequest) -> Prop
                                      rx_pos=sinks.positions_m.detach().cpu().numpy(),
                                      rx_pol=sinks.polarizations.detach().cpu().numpy())
         rows = engine.export_rows(dscene, inputs, width=width)
+        for extra in ("pair_offset", "pair_cell", "dropped"):
+            rows.pop(extra)
         # (source, sink) candidates: the straight segment, every reflection
         # chain of the source and every edge
         candidates = pair_count * (1 + (int(dscene.view.edge_count) if bits & 4 else 0))

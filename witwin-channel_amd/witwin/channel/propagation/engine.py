@@ -199,6 +199,7 @@ class SolveInputs:
     tile_count: int
     rx_count: np.ndarray   # receivers per transmitter (host)
     cell_order: torch.Tensor | None = None  # evaluation position -> cell
+    pair_cell: torch.Tensor | None = None   # canonical pair position -> cell (built on first export)
 
     @property
     def total_cells(self) -> int:
@@ -492,29 +493,54 @@ ROW_FIELDS_INT = ("tx_id", "rx_id", "depth", "component_id", "primitive_id", "ed
 ROW_FIELDS_FLOAT = ("path_length_m", "delay_s", "path_gain")
 
 
-def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int) -> dict[str, Any]:
+def pair_order(dscene: DeviceScene, inputs: SolveInputs) -> torch.Tensor:
+    """Cell at each canonical pair position (receiver-major, transmitter-minor),
+    sorted on the device once per solve inputs (it depends on the layout only)."""
+    if inputs.pair_cell is None:
+        cells = inputs.total_cells
+        device = dscene.device
+        pair_cell = torch.empty(max(cells, 1), dtype=torch.int32, device=device)
+        if cells:
+            ws = torch.empty(int(_native.call("wc_paths_compact_workspace_bytes", cells)), dtype=torch.uint8,
+                             device=device)
+            max_rx = int(np.max(inputs.rx_count)) if len(inputs.rx_count) else 0
+            _native.call("wc_paths_pair_order", _native.ptr(inputs.cell_offset), int(inputs.view.tx_count), cells,
+                         max_rx, _native.ptr(pair_cell), _native.ptr(ws), _native.stream_handle(device))
+        inputs.pair_cell = pair_cell[:cells]
+    return inputs.pair_cell
+
+
+def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int,
+                max_paths_per_pair: int | None = None) -> dict[str, Any]:
     """Explicit path rows in canonical order: pairs rx-major, tx-minor; inside a
-    pair (depth, component, primitive sequence, primitive, edge)."""
+    pair (depth, component, primitive sequence, primitive, edge), identity
+    duplicates dropped, at most ``max_paths_per_pair`` rows per pair. Counting,
+    pair offsets, the in-pair order and the compaction all run on the device;
+    the host reads two totals (rows emitted, rows kept) to size the tables.
+    ``pair_offset`` [cells + 1] (int64) and ``pair_cell`` [cells] (cell of each
+    canonical pair) describe the kept rows' segmentation; ``dropped`` counts
+    identity winners removed by the cap."""
 
     device = dscene.device
     cells = inputs.total_cells
-    t = int(inputs.view.tx_count)
     stream = _native.stream_handle(device)
+    p = _native.ptr
     counts = torch.empty(max(cells, 1), dtype=torch.int32, device=device)
-    _native.call("wc_paths_count", dscene.ref(), ctypes.byref(inputs.view), _native.ptr(counts), stream)
-    counts_h = counts[:cells].cpu().numpy().astype(np.int64)
-    offsets_cell = np.concatenate([[0], np.cumsum(inputs.rx_count)]).astype(np.int64)
-    # canonical pair order: rx-major then tx
-    cell_tx = np.repeat(np.arange(t), inputs.rx_count)
-    cell_rx = np.arange(cells) - offsets_cell[cell_tx]
-    canon = np.lexsort((cell_tx, cell_rx))          # cells sorted by (rx, tx)
-    canon_counts = counts_h[canon]
-    starts = np.zeros(cells, dtype=np.int64)
-    starts[canon] = np.cumsum(canon_counts) - canon_counts
-    total = int(counts_h.sum())
+    _native.call("wc_paths_count", dscene.ref(), ctypes.byref(inputs.view), p(counts), stream)
+    pair_cell = pair_order(dscene, inputs)
+    ws = torch.empty(int(_native.call("wc_paths_compact_workspace_bytes", max(cells, 1))), dtype=torch.uint8,
+                     device=device)
+    row_offset = torch.empty(max(cells, 1), dtype=torch.int32, device=device)
+    emitted_offset = torch.empty(cells + 1, dtype=torch.int32, device=device)
+    totals = torch.zeros(2, dtype=torch.int64, device=device)
+    _native.call("wc_paths_row_offsets", p(counts), p(pair_cell), cells, p(row_offset), p(emitted_offset),
+                 p(totals), p(ws), stream)
+    total = int(totals[0].item())
+    if total >= 2 ** 31:
+        raise RuntimeError(f"path export: {total} rows exceed the int32 row index")
     w = max(int(width), 1)
-    rows: dict[str, torch.Tensor] = {}
     k = max(total, 1)
+    rows: dict[str, torch.Tensor] = {}
     for name in ROW_FIELDS_INT:
         rows[name] = torch.empty(k, dtype=torch.int32, device=device)
     for name in ROW_FIELDS_FLOAT:
@@ -528,25 +554,26 @@ def export_rows(dscene: DeviceScene, inputs: SolveInputs, *, width: int) -> dict
     rows["field_xyz"] = torch.empty((k, 3), dtype=torch.complex64, device=device)
     rows["arrival_direction"] = torch.empty((k, 3), dtype=torch.float32, device=device)
     rows["departure_direction"] = torch.empty((k, 3), dtype=torch.float32, device=device)
-    p = _native.ptr
     view = _native.WcPathRows(width=w, **{name: p(rows[name]) for name in (
         *ROW_FIELDS_INT, *ROW_FIELDS_FLOAT, "primitive_sequence", "material_sequence", "interaction_positions",
         "interaction_normals", "path_field", "coefficient", "field_xyz", "arrival_direction",
         "departure_direction")})
-    row_offset = torch.as_tensor(starts.astype(np.int32)).to(device)
     if total:
         _native.call("wc_paths_emit", dscene.ref(), ctypes.byref(inputs.view), p(row_offset), p(counts),
                      ctypes.byref(view), stream)
-    pair_offset_h = np.concatenate([[0], np.cumsum(canon_counts)]).astype(np.int32)
-    pair_offset = torch.as_tensor(pair_offset_h).to(device)
     perm = torch.empty(k, dtype=torch.int32, device=device)
     keep = torch.empty(k, dtype=torch.uint8, device=device)
-    if total:
-        _native.call("wc_paths_canonical_order", ctypes.byref(view), p(pair_offset), cells, p(perm), p(keep), stream)
-    perm = perm[:total].long()
-    keep_b = keep[:total].bool()
-    out = {name: value[:total][perm][keep_b] for name, value in rows.items()}
+    kept_offset = torch.empty(cells + 1, dtype=torch.int32, device=device)
+    source_row = torch.empty(k, dtype=torch.int32, device=device)
+    dropped = torch.zeros(1, dtype=torch.int32, device=device)
+    cap = -1 if max_paths_per_pair is None else int(max_paths_per_pair)
+    _native.call("wc_paths_compact", ctypes.byref(view), p(emitted_offset), cells, cap, p(perm), p(keep),
+                 p(kept_offset), p(source_row), p(totals[1:]), p(dropped), p(ws), stream)
+    kept_total, dropped_count = (int(x) for x in torch.cat([totals[1:], dropped.to(torch.int64)]).tolist())
+    index = source_row[:kept_total].long()
+    out: dict[str, Any] = {name: value.index_select(0, index) for name, value in rows.items()}
     public_edge_ids(dscene, out)
-    out["pair_counts_canonical"] = canon_counts
-    out["cell_order_canonical"] = canon
+    out["pair_offset"] = kept_offset.to(torch.int64)
+    out["pair_cell"] = pair_cell.to(torch.int64)
+    out["dropped"] = dropped_count
     return out
