@@ -198,7 +198,7 @@ def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: 
     # transmitter; the scene and the chain tables (groups + float64 images)
     # once per launch
     first = [u for u in run.units if u[0] == 0]
-    tx_chains = np.array([sum(x.shape[0] for x in per) for per in seqs], np.int64)
+    tx_chains = np.asarray(seqs.chain_counts, np.int64)
     cell_bytes = city.rx_count.astype(np.int64) * (16 + 4 * ((tx_chains + 31) // 32))
 
     def all_chain_passes():
@@ -230,7 +230,7 @@ def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: 
         rng = np.random.default_rng(0)
         n = int(city.rx_count[t])
         sample = np.sort(rng.choice(n, 64, replace=False))
-        per = seqs[t]
+        per = seqs.host(t)
         groups = np.concatenate([np.pad(x, ((0, 0), (0, 5 - x.shape[1])), constant_values=-1) for x in per if x.size],
                                 axis=0).astype(np.int32)
         chain_depth = (groups >= 0).sum(axis=1).astype(np.int32)
@@ -269,7 +269,7 @@ def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: 
         "identical": identical, "grads": grads, "unit_ms": unit_ms, "unit_cells": unit_cells,
         "unit_tx": int(len(members)), "dom_ms": dom_ms, "dom_bytes": dom_bytes, "cpu": cpu_line,
         "tx": int(city.tx.shape[0]), "grid_cells": int(run.cells), "triangles": compiled.face_count,
-        "edges": compiled.edge_count, "chains": int(sum(sum(x.shape[0] for x in per) for per in seqs)),
+        "edges": compiled.edge_count, "chains": int(seqs.count),
         "workspace_gb": run.workspace_bytes / 1e9, "frequencies": list(city.frequencies),
         "batches": len(run.batches),
     }

@@ -167,6 +167,9 @@ typedef struct wc_solve {
     int visibility_mode;  /* 0: by receiver layout (shared grid -> tile loops,
                              per-transmitter sets -> queued legs); 1: tile loops
                              (neighbouring receivers per wavefront); 2: queued legs */
+    const float* chain_cones; /* [K, WC_CONE_FLOATS] receiver cones of the chains
+                                 (wc_chain_cones) or NULL: the chain culls then test
+                                 each chain's cone before its last-bounce faces */
 } wc_solve;
 
 /* ------------------------------------------------------------ scene refit
@@ -413,6 +416,19 @@ int wc_paths_row_offsets(const int* path_count, const int* pair_cell, int cells,
 int wc_paths_compact(const wc_path_rows* rows, const int* pair_offset, int pair_count, int max_paths_per_pair,
                      int* permutation, unsigned char* keep, int* kept_offset, int* source_row, long long* kept_total,
                      int* dropped, void* workspace, void* stream);
+
+/* ------------------------------------------------------------ chain cones
+ * Receiver cone of every chain: a conservative bound, as planes, of the
+ * receivers whose back-projection can land inside the convex hull of every
+ * group of the chain (the hull of group g: hull_points [H, 3] float64 rows
+ * hull_offset[g] .. hull_offset[g + 1], in the group's plane). groups [K, 5],
+ * images [K, 5, 3] and depth [K] as in the chain plan; cones [K,
+ * WC_CONE_FLOATS]. Lets the chain culls drop (tile, chain) pairs that no
+ * receiver of the tile can validate. No reference counterpart (the reference
+ * tests every candidate for every receiver). */
+#define WC_CONE_FLOATS 48
+int wc_chain_cones(const wc_scene* scene, const int* hull_offset, const double* hull_points, const int* groups,
+                   const double* images, const int* depth, int count, float* cones, void* stream);
 
 /* ------------------------------------------------------------ chain planning
  * Candidate reflection chains from discovery rays, on the device (reference

@@ -1,12 +1,14 @@
-// CPU check of the tile cull of reflection chains (chain_reaches_box) on the
-// device code itself: for every 8 x 8 tile of a receiver grid and every
-// chain, a chain the cull drops must not back-project into faces of all its
-// groups (solve_chain without the leg tests) for any receiver of the tile.
+// CPU check of the tile cull of reflection chains (chain_reaches_box, with
+// and without the receiver cones of chain_cone) on the device code itself:
+// for every 8 x 8 tile of a receiver grid and every chain, a chain the cull
+// drops must not back-project into faces of all its groups (solve_chain
+// without the leg tests) for any receiver of the tile.
 // Input directory: counts.bin [G, F, K, side], plane.bin (G x 4 float64),
 // goff.bin / gfaces.bin (group CSR), tris.bin (Tri rows by face), chains.bin
 // (K x 5 int32, -1 padded), images.bin (K x 5 x 3 float64), rx.bin (side^2
-// float xyz), tx.bin (float xyz). Prints one summary line; exit status 1
-// when a dropped chain was valid.
+// float xyz), tx.bin (float xyz), hulloff.bin (G + 1 int32) / hullpts.bin
+// (H x 3 float64): group hulls. Prints one summary line; exit status 1 when a
+// dropped chain was valid.
 #include "hip/hip_runtime.h"
 #include "../../witwin-channel_amd/csrc/wc_radiomap.hip"
 
@@ -55,10 +57,12 @@ int main(int argc, char** argv) {
     const auto images = load<double>(dir + "/images.bin");
     const auto rx = load<float>(dir + "/rx.bin");
     const auto txv = load<float>(dir + "/tx.bin");
+    const auto hoff = load<int>(dir + "/hulloff.bin");
+    const auto hpts = load<double>(dir + "/hullpts.bin");
     if (counts.size() < 4 || txv.size() < 3) return 2;
     const int G = counts[0], F = counts[1], K = counts[2], side = counts[3];
     if ((int)plane.size() != 4 * G || (int)tris.size() != F || (int)chains.size() != 5 * K ||
-        (int)images.size() != 15 * K || (int)rx.size() != 3 * side * side)
+        (int)images.size() != 15 * K || (int)rx.size() != 3 * side * side || (int)hoff.size() != G + 1)
         return 2;
     SceneView s{};
     s.group_plane = plane.data();
@@ -70,8 +74,21 @@ int main(int argc, char** argv) {
     wc_solve q{};
     q.chain_groups = chains.data();
     q.chain_images = images.data();
+    std::vector<float> cones((size_t)K * kConeFloats);
+    std::vector<int> depth(K);
+    long bounded = 0;
+    for (int c = 0; c < K; ++c) {
+        int d = 0;
+        while (d < 5 && chains[5 * c + d] >= 0) ++d;
+        depth[c] = d;
+        chain_cone(plane.data(), hoff.data(), hpts.data(), &chains[5 * c], &images[15 * c], d,
+                   &cones[(size_t)kConeFloats * c]);
+        bounded += cones[(size_t)kConeFloats * c + 3] > 0.0f;
+    }
+    wc_solve qc = q;
+    qc.chain_cones = cones.data();
     const V3<float> tx{txv[0], txv[1], txv[2]};
-    long culled = 0, kept = 0, kept_valid = 0, wrong = 0;
+    long culled = 0, kept = 0, kept_valid = 0, wrong = 0, culled_faces = 0, cone_only = 0;
     for (int bi = 0; bi < side; bi += 8)
         for (int bj = 0; bj < side; bj += 8) {
             std::vector<V3<float>> cells;
@@ -85,10 +102,18 @@ int main(int argc, char** argv) {
                     hi = {fmaxf(hi.x, c.x), fmaxf(hi.y, c.y), fmaxf(hi.z, c.z)};
                 }
             for (int c = 0; c < K; ++c) {
-                const int depth = chains[5 * c + 1] >= 0 ? 2 : 1;
-                const bool reach = chain_reaches_box(s, q, c, depth, lo, hi);
-                const bool any = depth == 1 ? valid_somewhere<1>(s, &chains[5 * c], &images[15 * c], tx, cells)
-                                            : valid_somewhere<2>(s, &chains[5 * c], &images[15 * c], tx, cells);
+                const int d = depth[c];
+                const bool faces_reach = chain_reaches_box(s, q, c, d, lo, hi);
+                const bool reach = chain_reaches_box(s, qc, c, d, lo, hi);
+                const int* g = &chains[5 * c];
+                const double* im = &images[15 * c];
+                const bool any = d == 1   ? valid_somewhere<1>(s, g, im, tx, cells)
+                                 : d == 2 ? valid_somewhere<2>(s, g, im, tx, cells)
+                                 : d == 3 ? valid_somewhere<3>(s, g, im, tx, cells)
+                                 : d == 4 ? valid_somewhere<4>(s, g, im, tx, cells)
+                                          : valid_somewhere<5>(s, g, im, tx, cells);
+                culled_faces += !faces_reach;
+                cone_only += faces_reach && !reach;
                 if (reach) {
                     ++kept;
                     kept_valid += any;
@@ -98,6 +123,7 @@ int main(int argc, char** argv) {
                 }
             }
         }
-    std::printf("culled %ld kept %ld kept_valid %ld wrong %ld\n", culled, kept, kept_valid, wrong);
+    std::printf("chains %d bounded %ld culled %ld kept %ld kept_valid %ld wrong %ld culled_faces %ld cone_only %ld\n",
+                K, bounded, culled, kept, kept_valid, wrong, culled_faces, cone_only);
     return wrong ? 1 : 0;
 }

@@ -64,9 +64,13 @@ def chain_checker(tmp_path_factory) -> Path:
     return _build(tmp_path_factory, "chain_cull_check")
 
 
-def _dump_chains(scene, out: Path, seed: int, depth1: int, depth2: int) -> None:
-    """Scene tables, a random sample of depth-1 and depth-2 group chains with
-    their transmitter images, and the scene's square receiver grid."""
+def _dump_chains(scene, out: Path, seed: int, depth1: int, depth2: int, discovered_depth: int = 0) -> None:
+    """Scene tables, a random sample of depth-1 and depth-2 group chains plus
+    (``discovered_depth`` > 0) the distinct prefixes up to that depth of the
+    FP64 oracle's discovery rays, with their transmitter images, the group
+    hulls and the scene's square receiver grid."""
+    from witwin.channel.scene.compiler import group_hulls
+
     compiled = compile_scene(scene, reference_frequency_hz=3.5e9)
     out.mkdir(parents=True, exist_ok=True)
     g = compiled.group_count
@@ -78,10 +82,26 @@ def _dump_chains(scene, out: Path, seed: int, depth1: int, depth2: int) -> None:
     chains[:len(one), 0] = one
     chains[len(one):, :2] = two
     tx = compiled.endpoints.tx_position[0].astype(np.float64)
+    if discovered_depth:
+        import oracle
+
+        rays = oracle.Oracle(compiled).discover_all(tx, 16384, discovered_depth)
+        found = []
+        for d in range(1, discovered_depth + 1):
+            pref = rays[:, :d]
+            pref = np.unique(pref[(pref >= 0).all(axis=1)], axis=0)
+            block = np.full((len(pref), 5), -1, np.int32)
+            block[:, :d] = pref
+            found.append(block)
+        chains = np.concatenate([chains] + found, axis=0)
+    hull_offset, hull_points = group_hulls(compiled.vertices, compiled.faces, compiled.group_face_offset,
+                                           compiled.group_faces, compiled.group_plane)
+    hull_offset.astype(np.int32).tofile(out / "hulloff.bin")
+    hull_points.astype(np.float64).tofile(out / "hullpts.bin")
     n, off = compiled.group_plane[:, :3], compiled.group_plane[:, 3]
     images = np.zeros((len(chains), 5, 3))
     p = np.repeat(tx[None], len(chains), 0)
-    for j in range(2):
+    for j in range(5):
         grp = chains[:, j]
         ok = grp >= 0
         gg = np.where(ok, grp, 0)
@@ -136,10 +156,13 @@ def test_chain_tile_cull_drops_no_valid_chain(chain_checker, tmp_path, which):
         scene = workloads.urban_canyon_scene(seed=2, shape=(24, 24), spacing=14.0)
     else:
         scene = workloads.heightfield_scene(seed=4, size=40, boxes=10, shape=(24, 24))
-    _dump_chains(scene, tmp_path / which, seed=0, depth1=1500, depth2=1500)
+    _dump_chains(scene, tmp_path / which, seed=0, depth1=1500, depth2=1500, discovered_depth=4)
     c = _run(chain_checker, tmp_path / which)
     assert c["wrong"] == 0
     assert c["culled"] > c["kept"] // 4 and c["kept_valid"] > 0
+    # the receiver cones bound most chains and drop (tile, chain) pairs the
+    # last-bounce face test keeps
+    assert c["bounded"] > c["chains"] // 2 and c["cone_only"] > 0
 
 
 @pytest.fixture(scope="module")

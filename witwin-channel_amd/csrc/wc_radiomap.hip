@@ -1,11 +1,13 @@
 // Fused radiomap, path export and discovery kernels behind the C ABI.
 #include "wc_api_internal.h"
+#include "wc_cone.h"
 #include "wc_paths.h"
 #include "../../include/wcmi.h"
 
 namespace wc {
 
 static_assert(sizeof(SceneView) == sizeof(wc_scene), "wc_scene must mirror SceneView");
+static_assert(kConeFloats == WC_CONE_FLOATS, "cone record size");
 static_assert(sizeof(Edge) == sizeof(wc_edge), "wc_edge must mirror Edge");
 
 __device__ __forceinline__ int slot_of(int component) {
@@ -901,6 +903,8 @@ __device__ __forceinline__ bool chain_valid(const SceneView& s, const wc_solve& 
 // receiver of the box (margin 1e-3 m plus 1e-5 of the coordinates).
 __device__ inline bool chain_reaches_box(const SceneView& s, const wc_solve& q, int c, int D, const V3<float>& lo,
                                          const V3<float>& hi) {
+    // the chain's receiver cone first (nested bounces, convex group hulls)
+    if (q.chain_cones && !cone_reaches_box(q.chain_cones + (size_t)c * kConeFloats, lo, hi)) return false;
     const int g = q.chain_groups[c * kMaxDepth + D - 1];
     const double* im = q.chain_images + ((size_t)c * kMaxDepth + D - 1) * 3;
     double off;
@@ -1819,6 +1823,16 @@ __global__ void __launch_bounds__(256) emit_kernel(SceneView s, wc_solve q, cons
     }
 }
 
+// ------------------------------------------------------------------ chain cones
+__global__ void chain_cone_kernel(SceneView s, const int* __restrict__ hull_offset,
+                                  const double* __restrict__ hull_points, const int* __restrict__ groups,
+                                  const double* __restrict__ images, const int* __restrict__ depth, int count,
+                                  float* __restrict__ cones) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < count; c += blockDim.x * gridDim.x)
+        chain_cone(s.group_plane, hull_offset, hull_points, groups + (size_t)kMaxDepth * c,
+                   images + (size_t)kMaxDepth * 3 * c, depth[c], cones + (size_t)kConeFloats * c);
+}
+
 // ------------------------------------------------------------------ discovery
 __global__ void discover_kernel(SceneView s, const float* __restrict__ tx_pos, int tx_count, int ray_count,
                                 int max_depth, int* __restrict__ chains) {
@@ -2231,6 +2245,21 @@ extern "C" int wc_paths_emit(const wc_scene* scene, const wc_solve* solve, const
                        *solve, row_offset, path_count, *rows);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_paths_emit", err);
+}
+
+extern "C" int wc_chain_cones(const wc_scene* scene, const int* hull_offset, const double* hull_points,
+                              const int* groups, const double* images, const int* depth, int count, float* cones,
+                              void* stream) {
+    if (int rc = check_scene(scene)) return rc;
+    if (count < 0) return wc_fail("wc_chain_cones: count must be non-negative");
+    if (count == 0) return 0;
+    if (!hull_offset || !hull_points || !groups || !images || !depth || !cones)
+        return wc_fail("wc_chain_cones: null argument");
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    hipLaunchKernelGGL(chain_cone_kernel, dim3(blocks_for(count, 64)), dim3(64), 0, (hipStream_t)stream, s,
+                       hull_offset, hull_points, groups, images, depth, count, cones);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_chain_cones", err);
 }
 
 extern "C" int wc_visibility(const wc_scene* scene, int count, const float* source, const float* target,

@@ -62,7 +62,7 @@ class WcSolve(ctypes.Structure):
         ("frequency_hz", c_double), ("components", c_int), ("max_depth", c_int), ("coherent", c_int),
         ("lanes_per_cell", c_int), ("cell_order", c_void_p), ("max_tx_chains", c_int),
         ("super_first_tile", c_void_p), ("tile_super", c_void_p), ("super_count", c_int),
-        ("visibility_mode", c_int),
+        ("visibility_mode", c_int), ("chain_cones", c_void_p),
     ]
 
 
@@ -150,6 +150,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                          c_void_p]),
     "wc_paths_compact_workspace_bytes": (c_size_t, [c_int]),
     "wc_plan_workspace_bytes": (c_size_t, [c_int, c_int]),
+    "wc_chain_cones": (c_int, [ctypes.POINTER(WcScene), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                               c_void_p, c_void_p]),
     "wc_plan_level": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_plan_assign": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int] + [c_void_p] * 7),
     "wc_plan_table": (c_int, [ctypes.POINTER(WcPlanLevels), c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,

@@ -118,15 +118,16 @@ def apply_parameters(state: RadiomapState, eps_r, sigma_e, frequency, tx_positio
     d.layer_sigma.index_copy_(0, first, sig)
     inputs.view.frequency_hz = float(frequency.detach()) if isinstance(frequency, torch.Tensor) else float(frequency)
     txp = tx_position.detach().to(device=d.device, dtype=torch.float32).reshape(inputs.tx_pos.shape)
+    moved = False
     if txp.data_ptr() != inputs.tx_pos.data_ptr():
         inputs.tx_pos.copy_(txp)
-        chains = inputs.chains
-        if chains.count:
-            _native.call("wc_chain_images", d.ref(), _native.ptr(inputs.tx_pos), _native.ptr(chains.tx),
-                         _native.ptr(chains.groups), _native.ptr(chains.depth), chains.count,
-                         _native.ptr(chains.images), _native.stream_handle(d.device))
+        moved = True
     if vertices is not None and vertices.data_ptr() != d.vertices.data_ptr():
         d.refit(vertices)
+        moved = True
+    if moved:
+        # images (and receiver cones) follow the transmitters and the group planes
+        engine.refresh_chain_geometry(d, inputs)
 
 
 def geometry_vjp(dscene, inputs: engine.SolveInputs, view) -> torch.Tensor:

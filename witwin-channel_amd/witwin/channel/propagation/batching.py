@@ -38,7 +38,7 @@ from typing import Any
 import numpy as np
 import torch
 
-from witwin.channel.propagation import engine
+from witwin.channel.propagation import engine, planning
 
 AUTOTUNE_CACHE = Path(os.environ.get("WITWIN_AMD_CACHE", Path.home() / ".cache" / "witwin_channel_amd")) \
     / "autotune.json"
@@ -74,7 +74,7 @@ class BatchedMeanDb:
 
     def __init__(self, compiled, dscene, tx_pos: np.ndarray, rx_pos: np.ndarray, rx_count: np.ndarray, *,
                  components: int, max_depth: int, frequencies: tuple[float, ...], shape: LaunchShape,
-                 sequences: list[list[np.ndarray]] | None = None, tx_pol: np.ndarray | None = None,
+                 sequences: "planning.DeviceSequences | None" = None, tx_pol: np.ndarray | None = None,
                  rx_pol: np.ndarray | None = None, tx_power: np.ndarray | None = None,
                  coherent: bool = True) -> None:
         from witwin.channel.kernels.deterministic import MeanDbStep
@@ -96,7 +96,7 @@ class BatchedMeanDb:
         if sequences is None:
             sequences = chain_sequences_batched(compiled, dscene, tx_t, max_depth, bool(components & 2))
         self.planning_s = time.perf_counter() - t0
-        chains = np.array([sum(int(s.shape[0]) for s in per) for per in sequences], dtype=np.int64)
+        chains = np.asarray(sequences.chain_counts, dtype=np.int64)
         self.chain_counts = chains
         # transmitters with similar chain counts share a batch (the masks are
         # padded to the largest count of the batch)
@@ -109,7 +109,7 @@ class BatchedMeanDb:
             rx_b = np.concatenate([rx_pos[s] for s in sl], axis=0)
             rxp_b = np.concatenate([rx_pol[s] for s in sl], axis=0)
             tx_b = torch.as_tensor(np.ascontiguousarray(tx_pos[members], dtype=np.float32)).to(self.device)
-            plan = engine.plan_from_sequences(dscene, tx_b, [sequences[i] for i in members])
+            plan = engine.plan_from_device(dscene, tx_b, sequences.subset(members))
             for fi, f in enumerate(self.frequencies):
                 inputs_all.append((fi, members, engine.solve_inputs(
                     compiled, dscene, components=components, max_depth=max_depth, coherent=coherent,
@@ -194,15 +194,12 @@ class BatchedMeanDb:
         return self.maps(frequency_index, "power_total")
 
 
-def chain_sequences_batched(compiled, dscene, tx_pos: torch.Tensor, max_depth: int, reflection: bool,
-                            chunk: int = 16) -> list[list[np.ndarray]]:
-    """chain_sequences for many transmitters, discovering ``chunk`` at a time
-    (bounds the discovery buffer to chunk x 262,144 x depth words)."""
-    out: list[list[np.ndarray]] = []
-    for a in range(0, int(tx_pos.shape[0]), chunk):
-        out.extend(engine.chain_sequences(compiled, dscene, tx_pos[a:a + chunk].contiguous(), max_depth,
-                                          reflection=reflection))
-    return out
+def chain_sequences_batched(compiled, dscene, tx_pos: torch.Tensor, max_depth: int,
+                            reflection: bool) -> "planning.DeviceSequences":
+    """Candidate chains of every transmitter of ``tx_pos``, planned on the
+    device in one pass (discovery rays of all transmitters, per-depth distinct
+    prefixes, one chain table)."""
+    return engine.chain_sequences_device(compiled, dscene, tx_pos, max_depth, reflection=reflection)
 
 
 def _signature(compiled, tx_count: int, cells: int, components: int, max_depth: int, chains: int) -> str:
@@ -216,7 +213,7 @@ def _free_bytes(device) -> int:
 
 
 def autotune(compiled, dscene, tx_pos: np.ndarray, rx_pos: np.ndarray, rx_count: np.ndarray, *, components: int,
-             max_depth: int, frequencies: tuple[float, ...], sequences: list[list[np.ndarray]],
+             max_depth: int, frequencies: tuple[float, ...], sequences: "planning.DeviceSequences",
              memory_fraction: float = 0.45, use_cache: bool = True) -> LaunchShape:
     """Pick the launch shape of a batched solve by measurement, cached per
     workload signature.
@@ -231,7 +228,7 @@ def autotune(compiled, dscene, tx_pos: np.ndarray, rx_pos: np.ndarray, rx_count:
     """
     t = int(tx_pos.shape[0])
     rx_count = np.asarray(rx_count, dtype=np.int64)
-    chains = np.array([sum(int(s.shape[0]) for s in per) for per in sequences], dtype=np.int64)
+    chains = np.asarray(sequences.chain_counts, dtype=np.int64)
     sig = _signature(compiled, t, int(rx_count.sum()), components, max_depth, int(chains.sum()))
     if use_cache and AUTOTUNE_CACHE.exists():
         try:
@@ -247,7 +244,7 @@ def autotune(compiled, dscene, tx_pos: np.ndarray, rx_pos: np.ndarray, rx_count:
     def subset(members: np.ndarray):
         sl = [slice(int(offsets[i]), int(offsets[i + 1])) for i in members]
         return (tx_pos[members], np.concatenate([rx_pos[s] for s in sl], axis=0), rx_count[members],
-                [sequences[i] for i in members])
+                sequences.subset(members))
 
     def trial(members: np.ndarray, shape: LaunchShape, reps: int = 2) -> float:
         txs, rxs, cnt, seqs = subset(members)

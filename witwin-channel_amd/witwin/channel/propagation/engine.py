@@ -30,6 +30,7 @@ import numpy as np
 import torch
 
 from witwin.channel import _native
+from witwin.channel.propagation import planning
 from witwin.channel.scene.compiler import CompiledScene, DeviceScene
 
 MAX_DEPTH = 5
@@ -51,27 +52,25 @@ class ChainPlan:
     images: torch.Tensor        # [K,5,3] float64
     depth: torch.Tensor         # [K] int32
     tx: torch.Tensor            # [K] int32
-    groups_host: np.ndarray
     offset_host: np.ndarray
     discovered_depths: tuple[int, ...]
+    _groups_host: np.ndarray | None = None
+    cones: torch.Tensor | None = None   # [K, CONE_FLOATS] receiver cones (chain_cones)
 
     @property
     def count(self) -> int:
-        return int(self.groups_host.shape[0])
+        return int(self.offset_host[-1])
+
+    @property
+    def groups_host(self) -> np.ndarray:
+        """[K, 5] chain groups on the host (copied on first use)."""
+        if self._groups_host is None:
+            k = self.count
+            self._groups_host = self.groups[:k].cpu().numpy() if k else np.zeros((0, MAX_DEPTH), np.int32)
+        return self._groups_host
 
 
-def exhaustive_sequences(group_count: int, depth: int) -> np.ndarray:
-    """All adjacent-distinct group sequences of one depth, lexicographic."""
-    if group_count <= 0:
-        return np.zeros((0, depth), np.int32)
-    seqs = np.arange(group_count, dtype=np.int32)[:, None]
-    for _ in range(depth - 1):
-        nxt = np.arange(group_count, dtype=np.int32)
-        a = np.repeat(seqs, group_count, axis=0)
-        b = np.tile(nxt, seqs.shape[0])[:, None]
-        keep = a[:, -1] != b[:, 0]
-        seqs = np.concatenate([a[keep], b[keep]], axis=1)
-    return seqs
+exhaustive_sequences = planning.exhaustive_sequences
 
 
 def is_exhaustive(group_count: int, depth: int) -> bool:
@@ -105,12 +104,12 @@ def chain_sequences(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.
     lexicographically — exhaustive when the count is small, otherwise the
     distinct prefixes of the transmitter's discovery rays (reference
     reflection.py:186-205, 541-569)."""
+    if discovered is None:
+        seqs = chain_sequences_device(compiled, dscene, tx_pos, max_depth, reflection=reflection, ray_count=ray_count)
+        return [seqs.host(t) for t in range(len(seqs))]
     t = int(tx_pos.shape[0])
     g = compiled.group_count
     depths = range(1, max_depth + 1) if reflection and g > 0 else range(0)
-    needs_rays = [d for d in depths if not is_exhaustive(g, d)]
-    if needs_rays and discovered is None:
-        discovered = discover(dscene, tx_pos, max(needs_rays), ray_count)
     exhaustive_cache: dict[int, np.ndarray] = {}
     out = []
     for ti in range(t):
@@ -118,7 +117,7 @@ def chain_sequences(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.
         for d in range(1, MAX_DEPTH + 1):
             seqs = np.zeros((0, d), np.int32)
             if d in depths:
-                if d in needs_rays:
+                if not is_exhaustive(g, d):
                     seqs = unique_prefixes(discovered[ti], d)
                 else:
                     if d not in exhaustive_cache:
@@ -164,9 +163,69 @@ def plan_from_sequences(dscene: DeviceScene, tx_pos: torch.Tensor, sequences: li
         offset=torch.as_tensor(offsets).to(device), groups=groups_t if k else torch.full((1, MAX_DEPTH), -1,
                                                                                          dtype=torch.int32,
                                                                                          device=device),
-        images=images, depth=depth_t, tx=tx_t, groups_host=groups, offset_host=offsets,
-        discovered_depths=tuple(discovered_depths),
+        images=images, depth=depth_t, tx=tx_t, offset_host=offsets, discovered_depths=tuple(discovered_depths),
+        _groups_host=groups,
     )
+
+
+CONE_FLOATS = 48  # WC_CONE_FLOATS
+
+
+def chain_cones(dscene: DeviceScene, chains: ChainPlan) -> torch.Tensor | None:
+    """Receiver cones of the plan's chains (wc_chain_cones over the scene's
+    group hulls), computed on first use; None without chains."""
+    k = chains.count
+    if k == 0:
+        return None
+    if chains.cones is None:
+        hull_offset, hull_points = dscene.hulls()
+        cones = torch.empty((k, CONE_FLOATS), dtype=torch.float32, device=dscene.device)
+        _native.call("wc_chain_cones", dscene.ref(), _native.ptr(hull_offset), _native.ptr(hull_points),
+                     _native.ptr(chains.groups), _native.ptr(chains.images), _native.ptr(chains.depth), k,
+                     _native.ptr(cones), _native.stream_handle(dscene.device))
+        chains.cones = cones
+    return chains.cones
+
+
+def refresh_chain_geometry(dscene: DeviceScene, inputs: "SolveInputs") -> None:
+    """Recompute the chains' image points (transmitters or vertices moved) and,
+    when the solve uses them, their receiver cones."""
+    chains = inputs.chains
+    if not chains.count:
+        return
+    _native.call("wc_chain_images", dscene.ref(), _native.ptr(inputs.tx_pos), _native.ptr(chains.tx),
+                 _native.ptr(chains.groups), _native.ptr(chains.depth), chains.count, _native.ptr(chains.images),
+                 _native.stream_handle(dscene.device))
+    if inputs.view.chain_cones:
+        chains.cones = None
+        inputs.view.chain_cones = _native.ptr(chain_cones(dscene, chains))
+
+
+def chain_sequences_device(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tensor, max_depth: int, *,
+                           reflection: bool, ray_count: int = DISCOVERY_RAY_COUNT) -> "planning.DeviceSequences":
+    """Candidate group sequences of every transmitter, planned on the device
+    (discovery, per-depth distinct prefixes, chain table; see
+    propagation/planning.py): the same sets and order as ``chain_sequences``."""
+    g = compiled.group_count
+    return planning.device_sequences(g, dscene, tx_pos, max_depth, reflection=reflection,
+                                     exhaustive=lambda d: is_exhaustive(g, d), ray_count=ray_count)
+
+
+def plan_from_device(dscene: DeviceScene, tx_pos: torch.Tensor, sequences: "planning.DeviceSequences") -> ChainPlan:
+    """ChainPlan of the transmitters ``tx_pos`` [T, 3] from their device
+    sequences (``chain_sequences_device`` or its ``subset``): the table is used
+    in place and only the image points are computed."""
+    device = tx_pos.device
+    k = sequences.count
+    images = torch.zeros((max(k, 1), MAX_DEPTH, 3), dtype=torch.float64, device=device)
+    if k:
+        _native.call("wc_chain_images", dscene.ref(), _native.ptr(tx_pos), _native.ptr(sequences.chain_tx),
+                     _native.ptr(sequences.groups), _native.ptr(sequences.chain_depth), k, _native.ptr(images),
+                     _native.stream_handle(device))
+    offsets = sequences.offset_host.astype(np.int32)
+    return ChainPlan(offset=torch.as_tensor(offsets).to(device), groups=sequences.groups, images=images,
+                     depth=sequences.chain_depth, tx=sequences.chain_tx, offset_host=offsets,
+                     discovered_depths=tuple(sequences.discovered_depths))
 
 
 def plan_chains(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tensor, max_depth: int, *,
@@ -174,6 +233,9 @@ def plan_chains(compiled: CompiledScene, dscene: DeviceScene, tx_pos: torch.Tens
                 discovered: np.ndarray | None = None) -> ChainPlan:
     """Candidate reflection chains of every transmitter (sorted by depth, then
     lexicographically by group sequence)."""
+    if discovered is None:
+        return plan_from_device(dscene, tx_pos, chain_sequences_device(compiled, dscene, tx_pos, max_depth,
+                                                                       reflection=reflection, ray_count=ray_count))
     g = compiled.group_count
     depths = range(1, max_depth + 1) if reflection and g > 0 else range(0)
     needs_rays = tuple(d for d in depths if not is_exhaustive(g, d))
@@ -298,13 +360,16 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
                  rx_pos: np.ndarray | None = None, rx_pol: np.ndarray | None = None,
                  rx_count: np.ndarray | None = None, ray_count: int = DISCOVERY_RAY_COUNT,
                  chains: ChainPlan | None = None, lanes_per_cell: int = LANES_PER_CELL,
-                 spatial_order: bool = True, visibility_mode: int = 0) -> SolveInputs:
+                 spatial_order: bool = True, visibility_mode: int = 0, cones: bool = False) -> SolveInputs:
     """Upload endpoint tables. Without ``rx_count`` every transmitter shares
     all receivers; with it, receivers are the concatenation of per-transmitter
     sets of the given sizes (uneven batched solves). ``lanes_per_cell`` lanes
     split each cell's reflection chains and diffraction edges, so a tile holds
     64 / lanes_per_cell cells. ``spatial_order`` evaluates cells in the
-    Z-order of their receivers (outputs are unchanged, bit for bit)."""
+    Z-order of their receivers (outputs are unchanged, bit for bit).
+    ``cones`` lets the chain culls test each chain's receiver cone first
+    (outputs unchanged, bit for bit: the cone only drops chains the exact
+    test rejects)."""
     if lanes_per_cell not in (1, 2, 4, 8, 16, 32, 64):
         raise ValueError("lanes_per_cell must be a power of two in [1, 64]")
     device = dscene.device
@@ -360,6 +425,7 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
         super_first_tile=p(super_first_t), tile_super=p(tile_super_t) if super_first_t is not None else None,
         super_count=int(super_first.shape[0] - 1) if super_first_t is not None else 0,
         visibility_mode=int(visibility_mode),
+        chain_cones=p(chain_cones(dscene, chains)) if cones and components & 2 else None,
     )
     inputs.__dict__["_super_tables"] = (super_first_t, tile_super_t)
     return inputs

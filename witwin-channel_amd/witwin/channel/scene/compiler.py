@@ -585,6 +585,69 @@ def wide_node_levels(nodes: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     return order, offsets
 
 
+def group_hulls(vertices: np.ndarray, faces: np.ndarray, group_face_offset: np.ndarray, group_faces: np.ndarray,
+                group_plane: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Convex hull of every coplanar group in its plane: ([G + 1] int32 CSR
+    offsets, [H, 3] float64 points on the plane, counter-clockwise about the
+    normal). A one-triangle group is its triangle; larger groups take the
+    monotone-chain hull of their vertices in an in-plane basis."""
+    g = int(group_plane.shape[0])
+    n = group_plane[:, :3].astype(np.float64)
+    off = group_plane[:, 3].astype(np.float64)
+    # in-plane basis: u = a unit vector orthogonal to n, w = n x u
+    helper = np.where(np.abs(n[:, :1]) < 0.9, np.array([[1.0, 0.0, 0.0]]), np.array([[0.0, 1.0, 0.0]]))
+    u = np.cross(n, helper)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    w = np.cross(n, u)
+    origin = n * off[:, None]
+    counts = np.diff(group_face_offset)
+    hull_pts: list[np.ndarray] = [np.zeros((0, 3))] * g
+    v64 = vertices.astype(np.float64)
+
+    def to_plane(gi: int, pts: np.ndarray) -> np.ndarray:
+        d = pts - origin[gi]
+        return np.stack([d @ u[gi], d @ w[gi]], axis=1)
+
+    def from_plane(gi: int, uv: np.ndarray) -> np.ndarray:
+        return origin[gi] + uv[:, :1] * u[gi] + uv[:, 1:2] * w[gi]
+
+    def monotone(uv: np.ndarray) -> np.ndarray:
+        pts = np.unique(uv, axis=0)
+        if pts.shape[0] < 3:
+            return pts
+
+        def half(seq):
+            out: list[np.ndarray] = []
+            for p in seq:
+                while len(out) >= 2 and ((out[-1][0] - out[-2][0]) * (p[1] - out[-2][1])
+                                         - (out[-1][1] - out[-2][1]) * (p[0] - out[-2][0])) <= 0.0:
+                    out.pop()
+                out.append(p)
+            return out
+
+        lower = half(pts)
+        upper = half(pts[::-1])
+        return np.array(lower[:-1] + upper[:-1])
+
+    single = np.nonzero(counts == 1)[0]
+    if single.size:
+        tri = v64[faces[group_faces[group_face_offset[single]]]]  # [S, 3, 3]
+        for k, gi in enumerate(single):
+            uv = to_plane(gi, tri[k])
+            area = (uv[1, 0] - uv[0, 0]) * (uv[2, 1] - uv[0, 1]) - (uv[1, 1] - uv[0, 1]) * (uv[2, 0] - uv[0, 0])
+            if area < 0.0:
+                uv = uv[::-1]
+            hull_pts[gi] = from_plane(gi, uv)
+    for gi in np.nonzero(counts > 1)[0]:
+        members = group_faces[group_face_offset[gi]:group_face_offset[gi + 1]]
+        pts = v64[faces[members].reshape(-1)]
+        hull_pts[gi] = from_plane(gi, monotone(to_plane(gi, pts)))
+    sizes = np.array([h.shape[0] for h in hull_pts], dtype=np.int64)
+    offset = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    points = np.concatenate(hull_pts, axis=0) if g else np.zeros((0, 3))
+    return offset, np.ascontiguousarray(points, dtype=np.float64)
+
+
 class DeviceScene:
     """HBM copy of a compiled scene plus its BVH, and the wc_scene view.
 
@@ -632,6 +695,7 @@ class DeviceScene:
         self.group_count = compiled.group_count
         self.edge_count = compiled.edge_count
         self._csr: tuple[torch.Tensor, torch.Tensor] | None = None
+        self._hulls: tuple[torch.Tensor, torch.Tensor] | None = None
         edge_bytes = compiled.edges.view(np.uint8).reshape(-1) if compiled.edge_count else np.zeros(0, np.uint8)
         self.edges = torch.as_tensor(edge_bytes.copy()).to(device)
         f = compiled.face_count
@@ -698,6 +762,7 @@ class DeviceScene:
 
         v = vertices.detach().to(device=self.device, dtype=torch.float32).reshape(self.vertices.shape)
         self.vertices.copy_(v)
+        self._hulls = None  # group hulls follow the vertices
         failure = torch.zeros(1, dtype=torch.int32, device=self.device)
         p = _native.ptr
         levels = np.ascontiguousarray(self._refit_offsets, dtype=np.int32)
@@ -708,6 +773,17 @@ class DeviceScene:
         _native.call("wc_scene_refit", self.ref(), ctypes.byref(refit), _native.stream_handle(self.device))
         if int(failure.item()):
             raise RuntimeError("wc_scene_refit: a node box could not be quantized; rebuild the scene")
+
+    def hulls(self) -> tuple[torch.Tensor, torch.Tensor]:
+        """Device copy of group_hulls for the current vertices (built on first
+        use, rebuilt after a refit)."""
+        if self._hulls is None:
+            offset, points = group_hulls(self.vertices.cpu().numpy(), self.faces.cpu().numpy(),
+                                         self.group_face_offset.cpu().numpy(), self.group_faces.cpu().numpy(),
+                                         self.group_plane.cpu().numpy())
+            self._hulls = (torch.as_tensor(offset).to(self.device),
+                           torch.as_tensor(points if points.size else np.zeros((1, 3))).to(self.device))
+        return self._hulls
 
     def vertex_csr(self) -> tuple[torch.Tensor, torch.Tensor]:
         """Device copy of vertex_csr for this scene (built on first use)."""
