@@ -360,7 +360,7 @@ def solve_inputs(compiled: CompiledScene, dscene: DeviceScene, *, components: in
                  rx_pos: np.ndarray | None = None, rx_pol: np.ndarray | None = None,
                  rx_count: np.ndarray | None = None, ray_count: int = DISCOVERY_RAY_COUNT,
                  chains: ChainPlan | None = None, lanes_per_cell: int = LANES_PER_CELL,
-                 spatial_order: bool = True, visibility_mode: int = 0, cones: bool = False) -> SolveInputs:
+                 spatial_order: bool = True, visibility_mode: int = 0, cones: bool = True) -> SolveInputs:
     """Upload endpoint tables. Without ``rx_count`` every transmitter shares
     all receivers; with it, receivers are the concatenation of per-transmitter
     sets of the given sizes (uneven batched solves). ``lanes_per_cell`` lanes
