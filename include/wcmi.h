@@ -292,6 +292,32 @@ typedef struct wc_adjoint {
     float* partials;
 } wc_adjoint;
 
+/* Tangent of the radiomap along one direction of its parameters (forward
+ * mode; reference ad_mode="jvp", the jvp companions of
+ * witwin/channel/kernels/deterministic.py:156-183 and
+ * native/channel/kernels/fields.cu:2183): d_eps_r / d_sigma_e [M] (first
+ * layer of each material) and d_tx_pos [T, 3] may be NULL (zero). primal
+ * carries the forward outputs and the stored topology (its grad_* and
+ * partials fields are unused). Outputs: tangents of power_total [cells],
+ * field_total [cells] c64, component_power [4, cells], component_field
+ * [4, cells] c64 and (optional) the diffraction power sum_axis
+ * |diffraction_vector|^2 [cells]. */
+typedef struct wc_tangent_in {
+    const float* d_eps_r;
+    const float* d_sigma_e;
+    float d_frequency;
+    const float* d_tx_pos;
+} wc_tangent_in;
+typedef struct wc_tangent_out {
+    float* d_power_total;
+    float* d_field_total;
+    float* d_component_power;
+    float* d_component_field;
+    float* d_diffraction_power;
+} wc_tangent_out;
+int wc_radiomap_jvp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* primal,
+                    const wc_tangent_in* tangent, const wc_tangent_out* out, void* stream);
+
 int wc_vjp_partial_stride(int material_count);
 int wc_vjp_partial_rows(int tile_count);
 /* Deterministic reduction of the partial rows (float64 sums in a fixed
@@ -416,6 +442,51 @@ int wc_paths_row_offsets(const int* path_count, const int* pair_cell, int cells,
 int wc_paths_compact(const wc_path_rows* rows, const int* pair_offset, int pair_count, int max_paths_per_pair,
                      int* permutation, unsigned char* keep, int* kept_offset, int* source_row, long long* kept_total,
                      int* dropped, void* workspace, void* stream);
+
+/* ------------------------------------------------------------ flat accumulation, reference arithmetic
+ * deterministic_accumulate_flat / _fwd64 / _backward / _jvp of the reference
+ * (witwin/channel/kernels/deterministic.py:23-390,
+ * native/channel/kernels/deterministic.cu:829-1262): rows scatter into six
+ * slots per (tx, rx) cell (component 0/1/2 -> 0/1/2, 5 -> 3, 6 -> 4
+ * scattering, 3/4/7 -> 5 coupled); coherent cells square the summed field of
+ * the field slots and add the scattering power, incoherent cells sum powers
+ * with sqrt(P) as pseudo-field. Arithmetic in the row precision (dtype 0:
+ * float32, 1: float64), per-slot sums in row order. Outputs: power_total
+ * [T*R], field_total_real/imag [T*R], component_power and
+ * component_field_real/imag [6, T*R].
+ * wc_accumulate_rows_jvp: rows carry the tangents (any of the three may be
+ * NULL = zero) and `forward` the primal outputs; writes the output tangents.
+ * wc_accumulate_rows_backward: cotangents of the outputs (any NULL = zero)
+ * -> per-row gradients of path_gain / field_real / field_imag. */
+typedef struct wc_accum_rows {
+    int count;
+    int num_tx;
+    int num_rx;
+    int coherent;
+    int scattering_coherent; /* scattering_combine_domain: 0 power, 1 coherent */
+    int dtype;               /* 0 float32, 1 float64 */
+    const unsigned char* valid;
+    const int* tx_id;
+    const int* rx_id;
+    const int* component_id;
+    const void* path_gain;
+    const void* field_real;
+    const void* field_imag;
+} wc_accum_rows;
+typedef struct wc_accum_out {
+    void* power_total;
+    void* field_total_real;
+    void* field_total_imag;
+    void* component_power;
+    void* component_field_real;
+    void* component_field_imag;
+} wc_accum_out;
+size_t wc_accumulate_rows_workspace_bytes(int count, int num_tx, int num_rx);
+int wc_accumulate_rows(const wc_accum_rows* rows, const wc_accum_out* out, void* workspace, void* stream);
+int wc_accumulate_rows_jvp(const wc_accum_rows* tangents, const wc_accum_out* forward, const wc_accum_out* out,
+                           void* workspace, void* stream);
+int wc_accumulate_rows_backward(const wc_accum_rows* rows, const wc_accum_out* forward, const wc_accum_out* grad_out,
+                                void* grad_path_gain, void* grad_field_real, void* grad_field_imag, void* stream);
 
 /* ------------------------------------------------------------ chain cones
  * Receiver cone of every chain: a conservative bound, as planes, of the

@@ -32,7 +32,11 @@ visibility. Parity status of each restated function, i.e. what pins it:
                          tree  -> partial (edge semantics and lossy-wedge
                          coefficients: parity unpinned)
 * accumulation ......... slot map and coherent/incoherent finalize of
-                         deterministic.cu:829-970 on hand-built rows  -> pinned
+                         deterministic.cu:829-970 on hand-built rows; the
+                         reference-arithmetic restatement (accumulate_flat)
+                         reproduces the reference's four frozen SHA-256
+                         output hashes (tests/ad/test_deterministic_accum_ad.py:46-129:
+                         float32/float64, coherent/incoherent)  -> pinned
 * canonical order ...... sort-order golden vector of
                          test_reflection_multibounce.py:25-80  -> pinned
 * canonical compact .... (canonical_compact) the sort-order golden vector above,
@@ -389,3 +393,75 @@ def canonical_compact(rows: dict, pair_offset: np.ndarray, cap: int | None = Non
         kept_offset[p + 1] = kept_offset[p] + len(kept)
     return {"perm": perm, "keep": keep, "kept_offset": kept_offset, "source_row": np.asarray(source, np.int64),
             "dropped": dropped}
+
+
+# ----------------------------------------------------------------- flat accumulation
+
+def _fma_round(a: float, b: float, c: float, dtype) -> float:
+    """a*b + c rounded once to ``dtype`` (exact rational intermediate)."""
+    from fractions import Fraction
+
+    exact = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    if dtype == np.float64:
+        return float(exact)
+    # round-to-nearest-even to float32 from the exact value: float64 first can
+    # double-round, so pick the nearer of the two float32 neighbours exactly
+    f = np.float32(float(exact))
+    lo, hi = (np.nextafter(f, np.float32(-np.inf)), f) if Fraction(float(f)) > exact else \
+        (f, np.nextafter(f, np.float32(np.inf)))
+    dl, dh = exact - Fraction(float(lo)), Fraction(float(hi)) - exact
+    if dl < dh:
+        return lo
+    if dh < dl:
+        return hi
+    return lo if (lo.view(np.int32) & 1) == 0 else hi
+
+
+def accumulate_flat(valid, tx_id, rx_id, component_id, path_gain, field_real, field_imag, *, num_tx: int,
+                    num_rx: int, coherent: bool, dtype=np.float32) -> dict:
+    """The reference's flat accumulation in ``dtype`` arithmetic (reference
+    native/channel/kernels/deterministic.cu:829-970): rows into six slots per
+    cell (0/1/2 -> 0/1/2, 5 -> 3, 6 -> 4 scattering, 3/4/7 -> 5), per-slot
+    sums in row order, coherent |z|^2 as one fused multiply-add (what the
+    reference's compiler emits), scattering a power-domain slot."""
+    dt = np.float32 if dtype == np.float32 else np.float64
+
+    def slot(c):
+        if 0 <= c < 3:
+            return c
+        if c in (3, 4, 7):
+            return 5
+        return {5: 3, 6: 4}.get(int(c), -1)
+
+    cells = num_tx * num_rx
+    cp, cr, ci = (np.zeros((6, cells), dt) for _ in range(3))
+    for i in range(len(tx_id)):
+        s, t, r = slot(int(component_id[i])), int(tx_id[i]), int(rx_id[i])
+        if not valid[i] or s < 0 or t < 0 or r < 0 or t >= num_tx or r >= num_rx:
+            continue
+        c = t * num_rx + r
+        cp[s, c] = dt(cp[s, c] + dt(path_gain[i]))
+        cr[s, c] = dt(cr[s, c] + dt(field_real[i]))
+        ci[s, c] = dt(ci[s, c] + dt(field_imag[i]))
+    pt, fr, fi = (np.zeros(cells, dt) for _ in range(3))
+    for c in range(cells):
+        rs = is_ = ps = sc = dt(0)
+        for s in range(6):
+            a, b = cr[s, c], ci[s, c]
+            if coherent:
+                if s == 4:
+                    sc = dt(sc + cp[s, c])
+                    continue
+                cp[s, c] = _fma_round(a, a, dt(b * b), dt)
+            else:
+                ps = dt(ps + cp[s, c])
+            rs, is_ = dt(rs + a), dt(is_ + b)
+        if coherent:
+            fr[c], fi[c] = rs, is_
+            pt[c] = dt(_fma_round(rs, rs, dt(is_ * is_), dt) + sc)
+        else:
+            pt[c], fr[c], fi[c] = ps, dt(np.sqrt(max(ps, dt(0)))), dt(0)
+    shape2, shape3 = (num_tx, num_rx), (6, num_tx, num_rx)
+    return {"power_total": pt.reshape(shape2), "field_total_real": fr.reshape(shape2),
+            "field_total_imag": fi.reshape(shape2), "component_power": cp.reshape(shape3),
+            "component_field_real": cr.reshape(shape3), "component_field_imag": ci.reshape(shape3)}

@@ -264,6 +264,53 @@ def test_oracle_canonical_compact_known_answers():
     assert capped["dropped"] == 3
 
 
+# reference tests/ad/test_deterministic_accum_ad.py:38-82: 12 rows over a 2 x 2
+# grid covering every slot (a same-slot collision, transmission, scattering,
+# two coupled rows, an out-of-range tx and rx) and the frozen SHA-256 of the
+# six outputs for each (dtype, coherent)
+ACCUM_TX = (0, 0, 0, 1, 1, 0, -1, 0, 0, 0, 1, 1)
+ACCUM_RX = (0, 0, 1, 1, 0, 1, 0, 3, 0, 0, 1, 1)
+ACCUM_CID = (0, 1, 1, 2, 0, 5, 1, 2, 0, 6, 3, 4)
+ACCUM_SHA256 = {
+    ("float32", True): "29c4e0ef1406dfa4e145967dc088535730e59e9538fee1ab4c6fcfe42a380fbe",
+    ("float32", False): "d68266d382aeb6ae7fdf6479660e29eea039065bb7645eeafcc7aad25ebf303d",
+    ("float64", True): "74d28d783c3f5d6613a305eeb66f63644e3373000a6d36272b22409813bfe7e7",
+    ("float64", False): "70ca8b66f1bbc6a97fb39582ca7cd02cc1037a55a9e9891e4b3a5261453b669e",
+}
+ACCUM_FIELDS = ("power_total", "field_total_real", "field_total_imag", "component_power", "component_field_real",
+                "component_field_imag")
+
+
+def accum_fixture():
+    """(gain, real, imag) float64 rows of the reference fixture (CPU generator
+    seeded 3: rand + 0.25, randn, randn)."""
+    import torch
+
+    g = torch.Generator(device="cpu").manual_seed(3)
+    n = len(ACCUM_TX)
+    gain = torch.rand((n,), generator=g, dtype=torch.float64) + 0.25
+    real = torch.randn((n,), generator=g, dtype=torch.float64)
+    imag = torch.randn((n,), generator=g, dtype=torch.float64)
+    return gain.numpy(), real.numpy(), imag.numpy()
+
+
+def accum_sha256(out) -> str:
+    import hashlib
+
+    return hashlib.sha256(b"".join(np.ascontiguousarray(np.asarray(out[k])).tobytes()
+                                   for k in ACCUM_FIELDS)).hexdigest()
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("coherent", [True, False])
+def test_oracle_accumulate_reproduces_reference_hashes(dtype, coherent):
+    gain, real, imag = accum_fixture()
+    dt = np.float32 if dtype == "float32" else np.float64
+    out = oracle.accumulate_flat(np.ones(len(ACCUM_TX), bool), ACCUM_TX, ACCUM_RX, ACCUM_CID, gain.astype(dt),
+                                 real.astype(dt), imag.astype(dt), num_tx=2, num_rx=2, coherent=coherent, dtype=dt)
+    assert accum_sha256(out) == ACCUM_SHA256[(dtype, coherent)]
+
+
 # ----------------------------------------------------------------- host compile
 
 def test_coplanar_groups_golden():
@@ -736,18 +783,15 @@ def test_oracle_wedge_path_lengths_match_reference_golden():
 
 
 def test_unsupported_ad_modes_raise_before_any_launch():
-    """ad_mode='jvp' is validated by Config (the reference accepts it) but this
-    build has no forward-mode radiomap: solve raises instead of silently
-    returning an undifferentiated forward; vjp with max_paths truncation
-    raises too (the adjoint differentiates every path of a cell)."""
+    """max_paths truncation with either AD mode raises before any launch (the
+    adjoint and the tangents differentiate every path of a cell)."""
     from witwin.channel import workloads
     from witwin.channel.deterministic import Config, solve
 
     scene = workloads.two_ray_scene(shape=(4, 4))
-    with pytest.raises(NotImplementedError, match="jvp"):
-        solve(scene, Config(ad_mode="jvp"), reference_frequency_hz=3.5e9)
-    with pytest.raises(NotImplementedError, match="max_paths"):
-        solve(scene, Config(ad_mode="vjp", max_paths=3), reference_frequency_hz=3.5e9)
+    for mode in ("vjp", "jvp"):
+        with pytest.raises(NotImplementedError, match="max_paths"):
+            solve(scene, Config(ad_mode=mode, max_paths=3), reference_frequency_hz=3.5e9)
 
 
 def test_build_info_names_the_hip_backend():

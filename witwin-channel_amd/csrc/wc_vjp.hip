@@ -662,6 +662,161 @@ __global__ void __launch_bounds__(256) vjp_reduce_kernel(const float* __restrict
     }
 }
 
+// ------------------------------------------------------------------ JVP
+// Tangent of the radiomap along one direction of its parameters (per
+// material first-layer eps_r and sigma, the carrier frequency, every
+// transmitter position): each stored path is recomputed with one-tangent dual
+// numbers seeded by the direction, and its path-field tangent is summed per
+// cell and slot in float64, then mapped through the forward's finalize
+// (coherent |S|^2 or incoherent sums).
+using D1 = Dual<1>;
+
+__device__ __forceinline__ D1 jv(float v, float t) {
+    D1 r = make_dual<1>(v);
+    r.d[0] = t;
+    return r;
+}
+
+struct JvpDirection {
+    const float* d_eps;   // [M] or null
+    const float* d_sig;   // [M] or null
+    float d_freq;
+    const float* d_tx;    // [T, 3] or null
+};
+
+// path field value and tangent (times sqrt power), its slot and, for a
+// diffraction path, the field 3-vector value and tangent (times sqrt power)
+__device__ void path_jvp(const SceneView& s, const wc_solve& q, const wc_adjoint& ga, int t, int cell, int desc,
+                         const JvpDirection& dir, Cx<float>& pf, Cx<float>& dpf, C3<float>& vec, C3<float>& dvec,
+                         int& slot) {
+    const int comp = desc >> 24;
+    PathSetup ps;
+    if (comp == kCidDiff) setup_path<kPhaseDiff>(s, q, ga, t, cell, desc, ps);
+    else setup_path<kPhaseMain>(s, q, ga, t, cell, desc, ps);
+    const PathParams& pp = ps.pp;
+    const V3<float> dt = dir.d_tx ? ld3(dir.d_tx + 3 * t) : v3c<float>(0.f, 0.f, 0.f);
+    const V3<D1> txd{jv(ps.tx.x, dt.x), jv(ps.tx.y, dt.y), jv(ps.tx.z, dt.z)};
+    const D1 fd = jv(ps.freq, dir.d_freq);
+    auto d_of = [&](const float* a, int m) { return (a && m >= 0) ? a[m] : 0.0f; };
+    V3<D1> arrival;
+    C3<D1> field;
+    if (comp == kCidDiff) {
+        const Edge e = s.edges[ps.cand];
+        const D1 eps[2] = {lift<D1>(pp.eps[0]), lift<D1>(pp.eps[1])};
+        const D1 sig[2] = {lift<D1>(pp.sig[0]), lift<D1>(pp.sig[1])};
+        DiffFaceTerms faces;
+        field = diffraction_tangent(s, e, txd, ps.rx, ps.tx_pol, fd, eps, sig, ps.k_d, ps.length_d, arrival, &faces);
+        // the face materials enter through the faces' slab coefficients
+        for (int k = 0; k < 2; ++k) {
+            const int m = pp.material[k];
+            Cx<Dual<2>> rs, rh;
+            slab_fresnel<Dual<2>>(lift<Dual<2>>(faces.cos_face[k]), seed<2>(pp.eps[k], 0), seed<2>(pp.sig[k], 1),
+                                  lift<Dual<2>>(s.mat_mu[m]), lift<Dual<2>>(s.mat_gain[m]),
+                                  lift<Dual<2>>(s.mat_thickness[m]), lift<Dual<2>>(ps.freq), rs, rh);
+            const float de = d_of(dir.d_eps, m), ds = d_of(dir.d_sig, m);
+            const Cx<float> drs{rs.re.d[0] * de + rs.re.d[1] * ds, rs.im.d[0] * de + rs.im.d[1] * ds};
+            const Cx<float> drh{rh.re.d[0] * de + rh.re.d[1] * ds, rh.im.d[0] * de + rh.im.d[1] * ds};
+            const C3<float> df = c3_scale(faces.d_r[2 * k], drs) + c3_scale(faces.d_r[2 * k + 1], drh);
+            field.x.re.d[0] += df.x.re; field.x.im.d[0] += df.x.im;
+            field.y.re.d[0] += df.y.re; field.y.im.d[0] += df.y.im;
+            field.z.re.d[0] += df.z.re; field.z.im.d[0] += df.z.im;
+        }
+    } else {
+        D1 eps[kMaxDepth], sig[kMaxDepth];
+        const int nm = (pp.count - 4) / 2;
+        for (int k = 0; k < nm; ++k) {
+            eps[k] = jv(pp.eps[k], d_of(dir.d_eps, pp.material[k]));
+            sig[k] = jv(pp.sig[k], d_of(dir.d_sig, pp.material[k]));
+        }
+        if (comp == kCidLos) field = los_tangent(txd, ps.rx, ps.tx_pol, fd, ps.k_d, ps.length_d, arrival);
+        else if (comp == kCidRefl)
+            field = reflection_tangent(s, ps.depth, ps.groups, ps.faces, txd, ps.rx, ps.tx_pol, fd, eps, sig, ps.k_d,
+                                       ps.length_d, arrival);
+        else
+            field = transmission_tangent(s, ps.depth, ps.walls, txd, ps.rx, ps.tx_pol, fd, eps, sig, ps.k_d,
+                                         ps.length_d, arrival);
+    }
+    const Cx<D1> coef = project_receiver(field, arrival, lift3<D1>(ps.rx_pol));
+    const float sp = ps.sqrt_power;
+    pf = {coef.re.v * sp, coef.im.v * sp};
+    dpf = {coef.re.d[0] * sp, coef.im.d[0] * sp};
+    vec = {{field.x.re.v * sp, field.x.im.v * sp}, {field.y.re.v * sp, field.y.im.v * sp},
+           {field.z.re.v * sp, field.z.im.v * sp}};
+    dvec = {{field.x.re.d[0] * sp, field.x.im.d[0] * sp}, {field.y.re.d[0] * sp, field.y.im.d[0] * sp},
+            {field.z.re.d[0] * sp, field.z.im.d[0] * sp}};
+    slot = ps.slot_id;
+}
+
+// one thread per cell: its stored descriptors in order
+__global__ void __launch_bounds__(256) jvp_kernel(SceneView s, wc_solve q, wc_adjoint ga, JvpDirection dir,
+                                                  float* __restrict__ d_power, float* __restrict__ d_field,
+                                                  float* __restrict__ d_cpower, float* __restrict__ d_cfield,
+                                                  float* __restrict__ d_dpower) {
+    const int cells = q.total_cells;
+    for (int cell = blockIdx.x * blockDim.x + threadIdx.x; cell < cells; cell += blockDim.x * gridDim.x) {
+        int lo = 0, hi = q.tx_count - 1;  // transmitter: last t with cell_offset[t] <= cell
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (q.cell_offset[mid] <= cell) lo = mid;
+            else hi = mid - 1;
+        }
+        const int t = lo;
+        double ds_re[4] = {0, 0, 0, 0}, ds_im[4] = {0, 0, 0, 0}, dp_inc[4] = {0, 0, 0, 0};
+        double dv[6] = {0, 0, 0, 0, 0, 0};
+        const int width = q.lanes_per_cell;
+        const int cap_sub = ga.topology_capacity / width;
+        int remaining = ga.path_count[cell];
+        for (int sub = 0; sub < width; ++sub) {
+            const int* seg = ga.topology + (size_t)cell * ga.topology_capacity + (size_t)sub * cap_sub;
+            const int n = width == 1 ? min(remaining, cap_sub) : cap_sub;
+            for (int j = 0; j < n; ++j) {
+                const int desc = seg[j];
+                if (desc == -1) break;
+                Cx<float> pf, dpf;
+                C3<float> vec, dvec;
+                int sl;
+                path_jvp(s, q, ga, t, cell, desc, dir, pf, dpf, vec, dvec, sl);
+                if (sl < 0 || sl > 3) continue;
+                ds_re[sl] += dpf.re;
+                ds_im[sl] += dpf.im;
+                dp_inc[sl] += 2.0 * ((double)pf.re * dpf.re + (double)pf.im * dpf.im);
+                if ((desc >> 24) == kCidDiff) {
+                    dv[0] += dvec.x.re; dv[1] += dvec.x.im; dv[2] += dvec.y.re;
+                    dv[3] += dvec.y.im; dv[4] += dvec.z.re; dv[5] += dvec.z.im;
+                }
+            }
+        }
+        double s_re = 0.0, s_im = 0.0, p_inc = 0.0;
+        for (int sl = 0; sl < 4; ++sl) {
+            const size_t o = (size_t)sl * cells + cell;
+            const double c_re = ga.component_field[2 * o], c_im = ga.component_field[2 * o + 1];
+            d_cfield[2 * o] = (float)ds_re[sl];
+            d_cfield[2 * o + 1] = (float)ds_im[sl];
+            d_cpower[o] = (float)(q.coherent ? 2.0 * (c_re * ds_re[sl] + c_im * ds_im[sl]) : dp_inc[sl]);
+            s_re += ds_re[sl];
+            s_im += ds_im[sl];
+            p_inc += dp_inc[sl];
+        }
+        if (q.coherent) {
+            const double f_re = ga.field_total[2 * cell], f_im = ga.field_total[2 * cell + 1];
+            d_power[cell] = (float)(2.0 * (f_re * s_re + f_im * s_im));
+            d_field[2 * cell] = (float)s_re;
+            d_field[2 * cell + 1] = (float)s_im;
+        } else {
+            const double p = ga.power_total ? ga.power_total[cell] : 0.0;
+            d_power[cell] = (float)p_inc;
+            d_field[2 * cell] = p > 0.0 ? (float)(p_inc / (2.0 * sqrt(p))) : 0.0f;
+            d_field[2 * cell + 1] = 0.0f;
+        }
+        if (d_dpower) {
+            double acc = 0.0;
+            if (ga.diffraction_vector)
+                for (int j = 0; j < 6; ++j) acc += 2.0 * (double)ga.diffraction_vector[6 * cell + j] * dv[j];
+            d_dpower[cell] = (float)acc;
+        }
+    }
+}
+
 }  // namespace
 }  // namespace wc
 
@@ -883,4 +1038,27 @@ extern "C" int wc_geometry_vertex_grad(const float* vertices, const int* faces, 
                        contrib, csr_offset, csr_slot, vertex_count, grad_vertices);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_geometry_vertex_grad", err);
+}
+
+extern "C" int wc_radiomap_jvp(const wc_scene* scene, const wc_solve* solve, const wc_adjoint* primal,
+                               const wc_tangent_in* tangent, const wc_tangent_out* out, void* stream) {
+    if (!scene || !solve || !primal || !tangent || !out) return wc_fail("wc_radiomap_jvp: null argument");
+    if (!primal->topology || !primal->path_count || primal->topology_capacity <= 0)
+        return wc_fail("wc_radiomap_jvp: the forward topology (descriptors and counts) is required");
+    if (!primal->field_total || !primal->component_field)
+        return wc_fail("wc_radiomap_jvp: the forward fields are required");
+    if (!out->d_power_total || !out->d_field_total || !out->d_component_power || !out->d_component_field)
+        return wc_fail("wc_radiomap_jvp: tangent outputs are required");
+    if (solve->lanes_per_cell < 1 || primal->topology_capacity % solve->lanes_per_cell != 0)
+        return wc_fail("wc_radiomap_jvp: topology_capacity must be a multiple of lanes_per_cell");
+    if (solve->total_cells <= 0) return 0;
+    if (solve->tx_count <= 0) return wc_fail("wc_radiomap_jvp: cells without transmitters");
+    const SceneView s = *reinterpret_cast<const SceneView*>(scene);
+    const JvpDirection dir{tangent->d_eps_r, tangent->d_sigma_e, tangent->d_frequency, tangent->d_tx_pos};
+    const int blocks = (solve->total_cells + 255) / 256;
+    hipLaunchKernelGGL(jvp_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, s, *solve, *primal, dir,
+                       out->d_power_total, out->d_field_total, out->d_component_power, out->d_component_field,
+                       out->d_diffraction_power);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_radiomap_jvp", err);
 }

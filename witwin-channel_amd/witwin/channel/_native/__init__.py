@@ -82,6 +82,28 @@ class WcPathRows(ctypes.Structure):
         "departure_direction")]
 
 
+class WcAccumRows(ctypes.Structure):
+    _fields_ = [("count", c_int), ("num_tx", c_int), ("num_rx", c_int), ("coherent", c_int),
+                ("scattering_coherent", c_int), ("dtype", c_int), ("valid", c_void_p), ("tx_id", c_void_p),
+                ("rx_id", c_void_p), ("component_id", c_void_p), ("path_gain", c_void_p), ("field_real", c_void_p),
+                ("field_imag", c_void_p)]
+
+
+class WcAccumOut(ctypes.Structure):
+    _fields_ = [("power_total", c_void_p), ("field_total_real", c_void_p), ("field_total_imag", c_void_p),
+                ("component_power", c_void_p), ("component_field_real", c_void_p),
+                ("component_field_imag", c_void_p)]
+
+
+class WcTangentIn(ctypes.Structure):
+    _fields_ = [("d_eps_r", c_void_p), ("d_sigma_e", c_void_p), ("d_frequency", c_float), ("d_tx_pos", c_void_p)]
+
+
+class WcTangentOut(ctypes.Structure):
+    _fields_ = [("d_power_total", c_void_p), ("d_field_total", c_void_p), ("d_component_power", c_void_p),
+                ("d_component_field", c_void_p), ("d_diffraction_power", c_void_p)]
+
+
 class WcPlanLevels(ctypes.Structure):
     _fields_ = [("prefix_parent", c_void_p * 5), ("prefix_group", c_void_p * 5), ("prefix_tx", c_void_p * 5)]
 
@@ -132,6 +154,8 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                 c_void_p, c_int, c_void_p]),
     "wc_radiomap_vjp_phase": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
                                       c_void_p, c_int, c_int, c_void_p]),
+    "wc_radiomap_jvp": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), ctypes.POINTER(WcAdjoint),
+                                ctypes.POINTER(WcTangentIn), ctypes.POINTER(WcTangentOut), c_void_p]),
     "wc_vjp_partial_stride": (c_int, [c_int]),
     "wc_vjp_partial_rows": (c_int, [c_int]),
     "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
@@ -174,6 +198,12 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_accumulate_flat": (c_int, [c_int] + [c_void_p] * 7 + [c_int, c_int, c_int] + [c_void_p] * 5
                            + [c_size_t, c_void_p]),
     "wc_accumulate_noalloc_max_bytes": (c_size_t, []),
+    "wc_accumulate_rows_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "wc_accumulate_rows": (c_int, [ctypes.POINTER(WcAccumRows), ctypes.POINTER(WcAccumOut), c_void_p, c_void_p]),
+    "wc_accumulate_rows_jvp": (c_int, [ctypes.POINTER(WcAccumRows), ctypes.POINTER(WcAccumOut),
+                                       ctypes.POINTER(WcAccumOut), c_void_p, c_void_p]),
+    "wc_accumulate_rows_backward": (c_int, [ctypes.POINTER(WcAccumRows), ctypes.POINTER(WcAccumOut),
+                                            ctypes.POINTER(WcAccumOut), c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_accumulate_flat_noalloc": (c_int, [c_int] + [c_void_p] * 7 + [c_int, c_int, c_int] + [c_void_p] * 5),
     "wc_visibility": (c_int, [ctypes.POINTER(WcScene), c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_discover_chains": (c_int, [ctypes.POINTER(WcScene), c_void_p, c_int, c_int, c_int, c_void_p,

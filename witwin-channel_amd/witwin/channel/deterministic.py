@@ -6,7 +6,9 @@ and runs the fused gfx950 radiomap kernel, which discovers every path of every
 (transmitter, cell) pair, evaluates its field and sums the cell coherently in
 float64 registers — no per-path rows are materialized unless
 ``export_paths=True``. ``ad_mode="vjp"`` attaches the fixed-topology adjoint
-(:mod:`witwin.channel.kernels.deterministic`).
+and ``ad_mode="jvp"`` its forward-mode tangents (under
+``torch.autograd.forward_ad``), both in
+:mod:`witwin.channel.kernels.deterministic`.
 
 Result layout follows the reference: ``path_gain``/``field`` are
 ``[T, cols, rows]`` for a single ``ReceiverGrid`` (the flat receiver index is
@@ -225,11 +227,9 @@ def _validate(config: Config) -> None:
         raise NotImplementedError("coupled reflection-diffraction paths are outside this build's hot path")
     if config.isb_boundary_taper:
         raise NotImplementedError("isb_boundary_taper is outside this build's hot path")
-    if config.ad_mode == "jvp":
-        raise NotImplementedError("deterministic ad_mode='jvp' is not available in this build; use ad_mode='vjp'")
-    if config.ad_mode == "vjp" and config.max_paths is not None:
-        raise NotImplementedError("ad_mode='vjp' differentiates every path of a cell; max_paths truncation is "
-                                  "not supported with it")
+    if config.ad_mode in ("vjp", "jvp") and config.max_paths is not None:
+        raise NotImplementedError(f"ad_mode='{config.ad_mode}' differentiates every path of a cell; max_paths "
+                                  "truncation is not supported with it")
     if "reflection" in config.components and config.max_depth < 1:
         raise RuntimeError("deterministic reflection requires max_depth >= 1")
     if "diffraction" in config.components:
@@ -288,7 +288,7 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
     num_rx = int(ep.rx_position.shape[0])
     inputs = engine.solve_inputs(compiled, dscene, components=bits, max_depth=depth, coherent=config.coherent,
                                  frequency_hz=compiled.frequency_hz, tx_pos=tx_position)
-    if config.ad_mode == "vjp":
+    if config.ad_mode in ("vjp", "jvp"):
         from witwin.channel.kernels.deterministic import radiomap_with_vjp
 
         out, tensors = radiomap_with_vjp(compiled, dscene, inputs, material_params=material_params,

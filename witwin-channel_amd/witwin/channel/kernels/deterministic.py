@@ -2,7 +2,9 @@
 
 Forward runs ``wc_radiomap_forward`` with path descriptors stored per cell;
 backward runs ``wc_radiomap_vjp`` (the fixed-topology adjoint that recomputes
-each stored path with dual numbers) and reduces its per-tile partials.
+each stored path with dual numbers) and reduces its per-tile partials; the
+forward-mode ``jvp`` runs ``wc_radiomap_jvp`` (the same recomputation with one
+tangent seeded by the input tangents).
 Differentiable inputs: first-layer ``eps_r`` and ``sigma_e`` of every
 material, the carrier frequency, the transmitter positions and the mesh
 vertices (``wc_radiomap_vjp_geometry`` + ``wc_geometry_vertex_grad``, run
@@ -23,6 +25,9 @@ from typing import Any
 import torch
 
 from witwin.channel import _native
+from witwin.channel.kernels._accumulate import (  # noqa: F401  (reference facade names)
+    deterministic_accumulate_flat, deterministic_accumulate_flat_ad, deterministic_accumulate_flat_backward,
+    deterministic_accumulate_flat_fwd64, deterministic_accumulate_flat_jvp)
 from witwin.channel.propagation import engine
 
 DEFAULT_TOPOLOGY_CAPACITY = 64
@@ -56,6 +61,17 @@ class RadiomapFunction(torch.autograd.Function):
         diff_power = out.diffraction_vector.abs().square().sum(dim=-1)
         ctx.state = state
         return out.power_total, out.field_total, out.component_power, out.component_field, diff_power
+
+    @staticmethod
+    def jvp(ctx, d_eps, d_sigma, d_frequency, d_tx, d_vertices, _d_state):  # type: ignore[override]
+        """Forward-mode tangents of the five outputs (wc_radiomap_jvp): the
+        stored paths recomputed with one-tangent dual numbers seeded by the
+        input tangents (first-layer eps_r / sigma_e, frequency, transmitter
+        positions). Vertex tangents are not available in forward mode."""
+        if d_vertices is not None and bool((d_vertices != 0).any()):
+            raise NotImplementedError("forward-mode tangents of the mesh vertices are not available; use "
+                                      "ad_mode='vjp' for vertex gradients")
+        return radiomap_jvp(ctx.state, d_eps, d_sigma, d_frequency, d_tx)
 
     @staticmethod
     def backward(ctx, g_power, g_field, g_cpower, g_cfield, g_dpower):  # type: ignore[override]
@@ -98,6 +114,44 @@ class RadiomapFunction(torch.autograd.Function):
                      p(grad_tx), stream)
         grad_vertices = geometry_vjp(state.dscene, inputs, view) if ctx.needs_input_grad[4] else None
         return grad_params[:m], grad_params[m:2 * m], grad_params[2 * m], grad_tx, grad_vertices, None
+
+
+def radiomap_jvp(state: RadiomapState, d_eps=None, d_sigma=None, d_frequency=None, d_tx=None):
+    """Tangents of (power_total, field_total, component_power,
+    component_field, diffraction_power) of the last forward of ``state``
+    along the given input tangents (None = zero)."""
+    out = state.out
+    inputs = state.inputs
+    device = out.power_total.device
+    m = state.material_count
+    cells = inputs.total_cells
+
+    def vec(x, n):
+        if x is None:
+            return None
+        return x.detach().to(device=device, dtype=torch.float32).reshape(-1)[:n].contiguous()
+
+    de, ds = vec(d_eps, m), vec(d_sigma, m)
+    dt = vec(d_tx, 3 * int(inputs.view.tx_count))
+    df = 0.0 if d_frequency is None else float(d_frequency.detach().reshape(()))
+    p = _native.ptr
+    d_power = torch.empty(cells, dtype=torch.float32, device=device)
+    d_field = torch.empty(cells, dtype=torch.complex64, device=device)
+    d_cpower = torch.empty((4, cells), dtype=torch.float32, device=device)
+    d_cfield = torch.empty((4, cells), dtype=torch.complex64, device=device)
+    d_dpower = torch.empty(cells, dtype=torch.float32, device=device)
+    primal = _native.WcAdjoint(
+        grad_power_total=None, grad_field_total=None, grad_component_power=None, grad_component_field=None,
+        grad_diffraction_power=None, power_total=p(out.power_total), field_total=p(out.field_total),
+        component_field=p(out.component_field), diffraction_vector=p(out.diffraction_vector),
+        topology=p(out.topology), path_count=p(out.path_count), topology_capacity=out.topology_capacity,
+        partials=None)
+    tin = _native.WcTangentIn(d_eps_r=p(de), d_sigma_e=p(ds), d_frequency=df, d_tx_pos=p(dt))
+    tout = _native.WcTangentOut(d_power_total=p(d_power), d_field_total=p(d_field), d_component_power=p(d_cpower),
+                                d_component_field=p(d_cfield), d_diffraction_power=p(d_dpower))
+    _native.call("wc_radiomap_jvp", state.dscene.ref(), ctypes.byref(inputs.view), ctypes.byref(primal),
+                 ctypes.byref(tin), ctypes.byref(tout), _native.stream_handle(device))
+    return d_power, d_field, d_cpower, d_cfield, d_dpower
 
 
 def apply_parameters(state: RadiomapState, eps_r, sigma_e, frequency, tx_position, vertices) -> None:

@@ -123,10 +123,11 @@ EDGE_DTYPE = np.dtype(
 def _sigma_from(eps_r: Any, sigma_e: float, frequency_hz: float) -> tuple[float, float]:
     """Real permittivity and total conductivity; a complex eps_r carries its
     loss as sigma = -Im(eps_r) * omega * eps0 (reference compiler.py:955-965)."""
+    plain = lambda x: float(x.detach()) if isinstance(x, torch.Tensor) else float(x)  # noqa: E731
     if isinstance(eps_r, complex):
         extra = -eps_r.imag * (2.0 * math.pi * frequency_hz * EPS0)
-        return float(eps_r.real), float(sigma_e) + extra
-    return float(eps_r), float(sigma_e)
+        return float(eps_r.real), plain(sigma_e) + extra
+    return plain(eps_r), plain(sigma_e)
 
 
 def compile_materials(materials: list[PhysicalMaterial], frequency_hz: float) -> MaterialTable:
