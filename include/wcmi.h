@@ -575,6 +575,172 @@ int wc_field_reflection_sequence(int count, int depth, const float* source, cons
                                  float* path_field, float* path_gain, float* path_length, float* delay,
                                  float* direction, void* stream);
 
+/* ------------------------------------------------------------ row operators: transmission, layer stacks, AD
+ * Layered-material tables in CSR form (reference kernels/materials.py:42-70
+ * _validate_layer_csr): material m owns layers layer_offset[m] ..
+ * layer_offset[m] + layer_count[m] - 1, ordered entry -> exit, at most 8
+ * layers. A row that names a material outside the tables, or one whose span
+ * is negative, longer than 8 or past layer_total, gives zeros. */
+typedef struct wc_layer_tables {
+    const int* layer_offset;      /* [material_count] */
+    const int* layer_count;       /* [material_count] */
+    const float* layer_thickness; /* [layer_total] metres */
+    const float* layer_eps;       /* [layer_total] relative permittivity */
+    const float* layer_sigma;     /* [layer_total] conductivity S/m */
+    const float* layer_mu;        /* [layer_total] relative permeability */
+    int material_count;
+    int layer_total;
+} wc_layer_tables;
+
+/* The enumerated-field outputs of one row set (the reference's shared
+ * result schema, kernels/fields.py:85-107): field_vector [N,3] c64,
+ * coefficient / path_field [N] c64, path_gain, path_length_m, delay_s [N],
+ * direction [N,3]. As tangents (the _jvp entry points) they receive
+ * d(output); as cotangents (wc_field_rows_grad) NULL means zero. */
+typedef struct wc_field_rows_out {
+    float* field_vector;
+    float* coefficient;
+    float* path_field;
+    float* path_gain;
+    float* path_length_m;
+    float* delay_s;
+    float* direction;
+} wc_field_rows_out;
+
+typedef struct wc_field_rows_grad {
+    const float* field_vector;
+    const float* coefficient;
+    const float* path_field;
+    const float* path_gain;
+    const float* path_length_m;
+    const float* delay_s;
+    const float* direction;
+} wc_field_rows_grad;
+
+/* Bytes of the workspace the _vjp entry points take: per-layer float64 sums
+ * and per-row frequency partials, reduced in a fixed order. */
+size_t wc_field_rows_workspace_bytes(int rows, int layers);
+
+/* Free-space rows, forward and reverse mode (reference kernels/fields.py:551
+ * field_free_space_backward, :583 field_free_space_jvp). Tangent pointers
+ * may be NULL (zero). grad_source / grad_target may be NULL (not needed);
+ * grad_frequency is one float. */
+int wc_field_free_space_jvp(int count, const float* source, const float* target, const float* tx_power,
+                            const float* tx_polarization, const float* rx_polarization, double frequency_hz,
+                            float tangent_frequency, const float* tangent_source, const float* tangent_target,
+                            const wc_field_rows_out* tangents, void* stream);
+int wc_field_free_space_vjp(int count, const float* source, const float* target, const float* tx_power,
+                            const float* tx_polarization, const float* rx_polarization, double frequency_hz,
+                            const wc_field_rows_grad* grads, float* grad_source, float* grad_target,
+                            float* grad_frequency, void* workspace, void* stream);
+
+/* Reflection-sequence rows (reference kernels/fields.py:180, :607 backward,
+ * :665 jvp): depth D bounces, per-interaction arrays [N, D] (positions and
+ * normals [N, D, 3]). */
+typedef struct wc_reflection_rows {
+    int count;
+    int depth;
+    double frequency_hz;
+    const float* source;
+    const float* target;
+    const float* interaction_positions;
+    const float* interaction_normals;
+    const float* tx_power;
+    const float* tx_polarization;
+    const float* rx_polarization;
+    const float* eps_r;
+    const float* sigma_e;
+    const float* mu_r;
+    const float* gain;
+    const float* thickness;
+} wc_reflection_rows;
+
+typedef struct wc_reflection_rows_gradout {
+    float* grad_source;                /* each may be NULL (not needed) */
+    float* grad_target;
+    float* grad_interaction_positions;
+    float* grad_interaction_normals;
+    float* grad_eps_r;
+    float* grad_sigma_e;
+    float* grad_gain;
+    float* grad_thickness;
+    float* grad_frequency;             /* one float, required */
+} wc_reflection_rows_gradout;
+
+/* tangents: the same layout as rows; each tangent pointer may be NULL (zero);
+ * its count, depth, frequency, tx_power, polarizations and mu_r are unused. */
+int wc_field_reflection_sequence_jvp(const wc_reflection_rows* rows, const wc_reflection_rows* tangents,
+                                     float tangent_frequency, const wc_field_rows_out* out, void* stream);
+int wc_field_reflection_sequence_vjp(const wc_reflection_rows* rows, const wc_field_rows_grad* grads,
+                                     const wc_reflection_rows_gradout* out, void* workspace, void* stream);
+
+/* Straight transmission through thin layered walls (reference
+ * kernels/fields.py:245 field_transmission_sequence, :710 backward, :771
+ * jvp): per valid interaction the layered t_TE / t_TM on the wall's s/p
+ * basis, carrier phase over L - sum(thickness cos theta), amplitude
+ * 1 / (2 k L). Rows with path_valid == 0, or with a valid interaction whose
+ * material is outside the tables, give zeros. interaction_positions of the
+ * reference signature do not enter the field and are not passed. */
+typedef struct wc_transmission_rows {
+    int count;
+    int depth; /* 1..8 */
+    double frequency_hz;
+    const unsigned char* path_valid;        /* [N] */
+    const unsigned char* interaction_valid; /* [N, D] */
+    const int* interaction_material_id;     /* [N, D] */
+    const float* source;
+    const float* target;
+    const float* interaction_normals; /* [N, D, 3] */
+    const float* tx_power;
+    const float* tx_polarization;
+    const float* rx_polarization;
+    wc_layer_tables layers;
+} wc_transmission_rows;
+
+int wc_field_transmission_sequence(const wc_transmission_rows* rows, const wc_field_rows_out* out, void* stream);
+int wc_field_transmission_sequence_jvp(const wc_transmission_rows* rows, const float* tangent_source,
+                                       const float* tangent_target, const float* tangent_normals,
+                                       const float* tangent_layer_thickness, const float* tangent_layer_eps_r,
+                                       const float* tangent_layer_sigma_e, float tangent_frequency,
+                                       const wc_field_rows_out* out, void* stream);
+int wc_field_transmission_sequence_vjp(const wc_transmission_rows* rows, const wc_field_rows_grad* grads,
+                                       float* grad_source, float* grad_target, float* grad_normals,
+                                       float* grad_layer_thickness, float* grad_layer_eps_r,
+                                       float* grad_layer_sigma_e, float* grad_frequency, void* workspace,
+                                       void* stream);
+
+/* Layer stack between vacuum half-spaces (reference kernels/materials.py:151
+ * em_layer_stack_eval, :199 em_layer_stack_backward, :235
+ * em_layer_stack_jvp): amplitude r, t per polarization (backward Rouard
+ * recursion) and R = |r|^2, T = |t|^2. A zero-layer material gives r = 0,
+ * t = 1; a material id outside the tables gives zeros. As cotangents of the
+ * _vjp entry point the twelve pointers are read only, NULL meaning zero. */
+typedef struct wc_layer_stack_out {
+    float* r_te_real;
+    float* r_te_imag;
+    float* r_tm_real;
+    float* r_tm_imag;
+    float* t_te_real;
+    float* t_te_imag;
+    float* t_tm_real;
+    float* t_tm_imag;
+    float* cap_R_te;
+    float* cap_R_tm;
+    float* cap_T_te;
+    float* cap_T_tm;
+} wc_layer_stack_out;
+
+int wc_em_layer_stack_eval(int count, const float* cos_theta, const int* material_id, const wc_layer_tables* layers,
+                           double frequency_hz, const wc_layer_stack_out* out, void* stream);
+int wc_em_layer_stack_jvp(int count, const float* cos_theta, const int* material_id, const wc_layer_tables* layers,
+                          double frequency_hz, const float* tangent_cos_theta, const float* tangent_layer_thickness,
+                          const float* tangent_layer_eps_r, const float* tangent_layer_sigma_e,
+                          float tangent_frequency, const wc_layer_stack_out* out, void* stream);
+int wc_em_layer_stack_vjp(int count, const float* cos_theta, const int* material_id, const wc_layer_tables* layers,
+                          double frequency_hz, const wc_layer_stack_out* grads, float* grad_cos_theta,
+                          float* grad_layer_thickness, float* grad_layer_eps_r, float* grad_layer_sigma_e,
+                          float* grad_frequency, void* workspace, void* stream);
+
 /* ------------------------------------------------------------ accumulation
  * Deterministic flat accumulation (reference deterministic.cu:829-970,
  * witwin/channel/kernels/deterministic.py:23-102): rows are ordered by cell

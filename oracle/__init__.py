@@ -22,6 +22,16 @@ visibility. Parity status of each restated function, i.e. what pins it:
                          for lossless stacks  -> pinned
 * transmission ......... vacuum wall equals line of sight
                          (tests/deterministic/test_transmission.py:88-180)  -> pinned
+* layer stack .......... (layer_stack) characteristic-matrix method, independent of
+                         the Rouard recursion above and equal to it to 1e-9;
+                         normal-incidence impedance formula, vacuum interior
+                         phase, lossless energy balance, Brewster TM null,
+                         split layer, PEC limit, thin-slab TE closed form
+                         (tests/kernels/test_em_layer_stack.py:110-300)  -> pinned
+* transmission rows .... (transmission_sequence) vacuum wall equals free space,
+                         lossy normal incidence scales by |t_TE|, invalid rows
+                         zero (tests/kernels/test_field_transmission_sequence.py:67-172)
+                         -> pinned
 * Fresnel integrals .... scipy.special.fresnel  -> pinned
 * UTD wedge ............ geometrical-optics compensation across the incident
                          and reflection shadow boundaries (total field continuous),
@@ -49,6 +59,7 @@ visibility. Parity status of each restated function, i.e. what pins it:
 
 from __future__ import annotations
 
+import cmath
 import ctypes
 import subprocess
 from pathlib import Path
@@ -465,3 +476,124 @@ def accumulate_flat(valid, tx_id, rx_id, component_id, path_gain, field_real, fi
     return {"power_total": pt.reshape(shape2), "field_total_real": fr.reshape(shape2),
             "field_total_imag": fi.reshape(shape2), "component_power": cp.reshape(shape3),
             "component_field_real": cr.reshape(shape3), "component_field_imag": ci.reshape(shape3)}
+
+
+# ----------------------------------------------------------------- layer stacks and transmission rows
+
+_EPS0 = 8.8541878128e-12
+_MU0 = 1.25663706212e-6
+_C0 = 299792458.0
+
+
+def _passive_sqrt(z: complex) -> complex:
+    """Square root on the passive branch (imaginary part <= 0)."""
+    m = abs(z)
+    return complex(np.sqrt(max(0.5 * (m + z.real), 0.0)), -np.sqrt(max(0.5 * (m - z.real), 0.0)))
+
+
+def layer_stack(layers, cos_theta: float, frequency: float, pol: int) -> tuple[complex, complex]:
+    """(r, t) of a layered wall between vacuum half-spaces by the
+    characteristic-matrix method (Born & Wolf, Principles of Optics, 1.6):
+    each layer contributes [[cos d, j sin d / Y], [j Y sin d, cos d]] with
+    d = kz t, Y the tangential admittance (TE kz / (omega mu), TM
+    omega eps / kz), eps = eps0 eps_r - j sigma / omega; r = (Y0 B - C) /
+    (Y0 B + C), t = 2 Y0 / (Y0 B + C) with [B, C] = M [1, Y0]. ``layers`` is
+    a sequence of (thickness, eps_r, sigma_e, mu_r) entry -> exit; pol 0 = TE,
+    1 = TM. An independent formulation of what the kernels compute by the
+    backward Rouard recursion (reference kernels/materials.py:151
+    em_layer_stack_eval; its test oracle tests/kernels/test_em_layer_stack.py:66-107)."""
+    omega = 2.0 * np.pi * frequency
+    ct = min(max(abs(cos_theta), 1e-6), 1.0)
+    k_par = omega / _C0 * np.sqrt(max(0.0, 1.0 - ct * ct))
+
+    def medium(eps_r, sigma, mu_r):
+        eps = _EPS0 * max(eps_r, 1e-6) - 1j * max(sigma, 0.0) / omega
+        mu = _MU0 * max(mu_r, 1e-6)
+        k = omega * _passive_sqrt(eps * mu)
+        kz = _passive_sqrt(k * k - k_par * k_par)
+        return kz, (kz / (omega * mu) if pol == 0 else omega * eps / kz)
+
+    _, y0 = medium(1.0, 0.0, 1.0)
+    m = np.eye(2, dtype=complex)
+    scale = 0.0  # the product is kept as m e^{scale}: each layer factor is divided by e^{|Im d|}
+    for thickness, eps_r, sigma, mu_r in layers:
+        kz, y = medium(eps_r, sigma, mu_r)
+        d = kz * max(thickness, 0.0)
+        g = abs(d.imag)
+        ep, em = cmath.exp(1j * d - g), cmath.exp(-1j * d - g)
+        cos_d, sin_d = 0.5 * (ep + em), -0.5j * (ep - em)
+        m = m @ np.array([[cos_d, 1j * sin_d / y], [1j * y * sin_d, cos_d]])
+        scale += g
+    b = m[0, 0] + m[0, 1] * y0
+    c = m[1, 0] + m[1, 1] * y0
+    return complex((y0 * b - c) / (y0 * b + c)), complex(2.0 * y0 / (y0 * b + c) * np.exp(-scale))
+
+
+def _unit(v, fallback):
+    n = np.linalg.norm(v)
+    return v / n if n > 1e-6 else np.asarray(fallback, np.float64)
+
+
+def _perp_basis(d, preferred):
+    """Unit vector perpendicular to d, from ``preferred`` or a world axis."""
+    p = preferred - d * np.dot(d, preferred)
+    if np.linalg.norm(p) > 1e-6:
+        return p / np.linalg.norm(p)
+    axis = np.array([1.0, 0.0, 0.0]) if abs(d[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    return _unit(axis - d * np.dot(d, axis), [0, 0, 1])
+
+
+def transmission_sequence(path_valid, src, tgt, normals, material_id, interaction_valid, tx_power, tx_pol, rx_pol,
+                          materials, frequency) -> dict:
+    """Straight transmission rows in float64 (reference
+    tests/ad/_reference_fields.py:269-336, the restatement of the reference's
+    transmission_sequence_kernel): per valid interaction the wall's s/p basis,
+    t_TE / t_TM of :func:`layer_stack`, carrier phase over
+    L - sum(thickness) cos theta, amplitude 1 / (2 k L). ``materials`` is a
+    list of layer lists. Invalid rows, and rows whose valid interaction names
+    no material, give zeros."""
+    n = len(path_valid)
+    out = {"field_vector": np.zeros((n, 3), complex), "coefficient": np.zeros(n, complex),
+           "path_field": np.zeros(n, complex), "path_gain": np.zeros(n), "path_length_m": np.zeros(n),
+           "delay_s": np.zeros(n), "direction": np.zeros((n, 3))}
+    k = 2.0 * np.pi * frequency / _C0
+    ez = np.array([0.0, 0.0, 1.0])
+    for i in range(n):
+        depth = len(interaction_valid[i])
+        if not path_valid[i] or any(interaction_valid[i][w] and not 0 <= material_id[i][w] < len(materials)
+                                    for w in range(depth)):
+            continue
+        a, b = np.asarray(src[i], np.float64), np.asarray(tgt[i], np.float64)
+        length = float(np.linalg.norm(b - a))
+        d = _unit(b - a, ez)
+        tp = np.asarray(tx_pol[i], np.float64)
+        value = (tp - d * np.dot(d, tp)).astype(complex)
+        carrier = length
+        for w in range(depth):
+            if not interaction_valid[i][w]:
+                continue
+            layers = materials[material_id[i][w]]
+            nrm = _unit(np.asarray(normals[i][w], np.float64), ez)
+            if np.dot(d, nrm) > 0.0:
+                nrm = -nrm
+            ct = min(max(abs(float(np.dot(d, nrm))), 1e-6), 1.0)
+            _, t_te = layer_stack(layers, ct, frequency, 0)
+            _, t_tm = layer_stack(layers, ct, frequency, 1)
+            s = np.cross(nrm, d)
+            s = s / np.linalg.norm(s) if np.linalg.norm(s) > 1e-6 else _perp_basis(d, nrm)
+            p = np.cross(s, d)
+            p = p / np.linalg.norm(p) if np.linalg.norm(p) > 1e-6 else _perp_basis(d, s)
+            value = s * (t_te * np.dot(value, s)) + p * (t_tm * np.dot(value, p))
+            carrier -= sum(max(layer[0], 0.0) for layer in layers) * ct
+        field = value * (np.exp(-1j * k * carrier) / (2.0 * k * max(length, 1e-10)))
+        rp = np.asarray(rx_pol[i], np.float64)
+        coef = complex(np.dot(field, rp - d * np.dot(d, rp)))
+        pf = coef * np.sqrt(max(float(tx_power[i]), 0.0))
+        out["field_vector"][i] = field
+        out["coefficient"][i] = coef
+        out["path_field"][i] = pf
+        out["path_gain"][i] = abs(pf) ** 2
+        out["path_length_m"][i] = length
+        out["delay_s"][i] = length / _C0
+        out["direction"][i] = d
+    return out

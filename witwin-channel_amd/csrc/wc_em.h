@@ -95,11 +95,17 @@ WC_HD C3<R> reflect_c3(const C3<R>& value, const V3<R>& incident, const V3<R>& n
 // polarization (pol 0 = TE, 1 = TM). Layer arrays are float; admittances are
 // normalized by the vacuum values so everything stays in float range.
 // (reference tests/ad/_reference_fields.py:196-266, tests/reference/em_oracle.py:204).
+// stack_rt also returns the stack's reflection coefficient seen from the
+// entry side (r_out); thickness_override, like the eps / sigma overrides,
+// replaces the float layer values with R values (tangent-carrying).
 template <typename R>
-WC_HD Cx<R> stack_t(const R& cos_theta, const float* thickness, const float* eps_r, const float* sigma_e,
-                    const float* mu_r, int count, const R& frequency_hz, int pol, const R* eps_override,
-                    const R* sigma_override) {
-    if (count <= 0) return cxc<R>(1.f, 0.f);
+WC_HD Cx<R> stack_rt(const R& cos_theta, const float* thickness, const float* eps_r, const float* sigma_e,
+                     const float* mu_r, int count, const R& frequency_hz, int pol, const R* eps_override,
+                     const R* sigma_override, const R* thickness_override, Cx<R>& r_out) {
+    if (count <= 0) {
+        r_out = cxc<R>(0.f, 0.f);
+        return cxc<R>(1.f, 0.f);
+    }
     const R omega = w_max(frequency_hz * (float)kTwoPi, kSmallEpsF);
     const R ct = w_min(w_max(w_abs(cos_theta), kSmallEpsF), 1.0f);
     const R sin2 = w_max(lift<R>(1.0f) - ct * ct, 0.0f);
@@ -127,7 +133,7 @@ WC_HD Cx<R> stack_t(const R& cos_theta, const float* thickness, const float* eps
     Cx<R> t_total = scale(y_below, 2.0f) / (y_below + y_vac);
     for (int l = count - 1; l >= 0; --l) {
         // phase = exp(-j k0 kz_rel d)
-        const R kd = k0 * fmaxf(thickness[l], 0.0f);
+        const R kd = k0 * (thickness_override ? w_max(thickness_override[l], 0.0f) : lift<R>(fmaxf(thickness[l], 0.0f)));
         const Cx<R> arg = scale(kz_below, kd);  // k0 kz d
         const Cx<R> phase = scale(cexpj(-arg.re), w_exp(arg.im));
         const Cx<R> phase2 = phase * phase;
@@ -147,7 +153,17 @@ WC_HD Cx<R> stack_t(const R& cos_theta, const float* thickness, const float* eps
         kz_below = kz_above;
         y_below = y_above;
     }
+    r_out = r_total;
     return t_total;
+}
+
+template <typename R>
+WC_HD Cx<R> stack_t(const R& cos_theta, const float* thickness, const float* eps_r, const float* sigma_e,
+                    const float* mu_r, int count, const R& frequency_hz, int pol, const R* eps_override,
+                    const R* sigma_override) {
+    Cx<R> r;
+    return stack_rt<R>(cos_theta, thickness, eps_r, sigma_e, mu_r, count, frequency_hz, pol, eps_override,
+                       sigma_override, nullptr, r);
 }
 
 // Complex coefficient seen by the receiver: p_rx,perp . E

@@ -108,6 +108,52 @@ class WcPlanLevels(ctypes.Structure):
     _fields_ = [("prefix_parent", c_void_p * 5), ("prefix_group", c_void_p * 5), ("prefix_tx", c_void_p * 5)]
 
 
+class WcLayerTables(ctypes.Structure):
+    _fields_ = [("layer_offset", c_void_p), ("layer_count", c_void_p), ("layer_thickness", c_void_p),
+                ("layer_eps", c_void_p), ("layer_sigma", c_void_p), ("layer_mu", c_void_p),
+                ("material_count", c_int), ("layer_total", c_int)]
+
+
+_FIELD_ROW_NAMES = ("field_vector", "coefficient", "path_field", "path_gain", "path_length_m", "delay_s",
+                    "direction")
+
+
+class WcFieldRowsOut(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in _FIELD_ROW_NAMES]
+
+
+class WcFieldRowsGrad(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in _FIELD_ROW_NAMES]
+
+
+class WcReflectionRows(ctypes.Structure):
+    _fields_ = [("count", c_int), ("depth", c_int), ("frequency_hz", ctypes.c_double)] + [
+        (name, c_void_p) for name in ("source", "target", "interaction_positions", "interaction_normals",
+                                      "tx_power", "tx_polarization", "rx_polarization", "eps_r", "sigma_e",
+                                      "mu_r", "gain", "thickness")]
+
+
+class WcReflectionRowsGradOut(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in (
+        "grad_source", "grad_target", "grad_interaction_positions", "grad_interaction_normals", "grad_eps_r",
+        "grad_sigma_e", "grad_gain", "grad_thickness", "grad_frequency")]
+
+
+class WcTransmissionRows(ctypes.Structure):
+    _fields_ = [("count", c_int), ("depth", c_int), ("frequency_hz", ctypes.c_double)] + [
+        (name, c_void_p) for name in ("path_valid", "interaction_valid", "interaction_material_id", "source",
+                                      "target", "interaction_normals", "tx_power", "tx_polarization",
+                                      "rx_polarization")] + [("layers", WcLayerTables)]
+
+
+LAYER_STACK_FIELDS = ("r_te_real", "r_te_imag", "r_tm_real", "r_tm_imag", "t_te_real", "t_te_imag", "t_tm_real",
+                      "t_tm_imag", "cap_R_te", "cap_R_tm", "cap_T_te", "cap_T_tm")
+
+
+class WcLayerStackOut(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in LAYER_STACK_FIELDS]
+
+
 class WcFixedRows(ctypes.Structure):
     _fields_ = [
         ("count", c_int), ("width", c_int), ("component", c_void_p), ("depth", c_void_p), ("faces", c_void_p),
@@ -194,6 +240,27 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                     c_void_p]),
     "wc_field_project_complex3": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_field_reflection_sequence": (c_int, [c_int, c_int] + [c_void_p] * 12 + [c_float] + [c_void_p] * 8),
+    "wc_field_rows_workspace_bytes": (c_size_t, [c_int, c_int]),
+    "wc_field_free_space_jvp": (c_int, [c_int] + [c_void_p] * 5 + [ctypes.c_double, c_float, c_void_p, c_void_p,
+                                        ctypes.POINTER(WcFieldRowsOut), c_void_p]),
+    "wc_field_free_space_vjp": (c_int, [c_int] + [c_void_p] * 5 + [ctypes.c_double, ctypes.POINTER(WcFieldRowsGrad)]
+                                + [c_void_p] * 5),
+    "wc_field_reflection_sequence_jvp": (c_int, [ctypes.POINTER(WcReflectionRows), ctypes.POINTER(WcReflectionRows),
+                                                 c_float, ctypes.POINTER(WcFieldRowsOut), c_void_p]),
+    "wc_field_reflection_sequence_vjp": (c_int, [ctypes.POINTER(WcReflectionRows), ctypes.POINTER(WcFieldRowsGrad),
+                                                 ctypes.POINTER(WcReflectionRowsGradOut), c_void_p, c_void_p]),
+    "wc_field_transmission_sequence": (c_int, [ctypes.POINTER(WcTransmissionRows), ctypes.POINTER(WcFieldRowsOut),
+                                               c_void_p]),
+    "wc_field_transmission_sequence_jvp": (c_int, [ctypes.POINTER(WcTransmissionRows)] + [c_void_p] * 6
+                                           + [c_float, ctypes.POINTER(WcFieldRowsOut), c_void_p]),
+    "wc_field_transmission_sequence_vjp": (c_int, [ctypes.POINTER(WcTransmissionRows),
+                                                   ctypes.POINTER(WcFieldRowsGrad)] + [c_void_p] * 9),
+    "wc_em_layer_stack_eval": (c_int, [c_int, c_void_p, c_void_p, ctypes.POINTER(WcLayerTables), ctypes.c_double,
+                                       ctypes.POINTER(WcLayerStackOut), c_void_p]),
+    "wc_em_layer_stack_jvp": (c_int, [c_int, c_void_p, c_void_p, ctypes.POINTER(WcLayerTables), ctypes.c_double]
+                              + [c_void_p] * 4 + [c_float, ctypes.POINTER(WcLayerStackOut), c_void_p]),
+    "wc_em_layer_stack_vjp": (c_int, [c_int, c_void_p, c_void_p, ctypes.POINTER(WcLayerTables), ctypes.c_double,
+                                      ctypes.POINTER(WcLayerStackOut)] + [c_void_p] * 7),
     "wc_accumulate_workspace_bytes": (c_size_t, [c_int, c_int, c_int]),
     "wc_accumulate_flat": (c_int, [c_int] + [c_void_p] * 7 + [c_int, c_int, c_int] + [c_void_p] * 5
                            + [c_size_t, c_void_p]),
