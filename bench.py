@@ -275,6 +275,40 @@ def scaled_city_section(steps: int, warmup: int, stream, cpu: bool, plain_loop: 
     }
 
 
+def public_solve_section(grid: int, frequency: float, depth: int, reps: int = 3) -> dict:
+    """The public call on the urban canyon, end to end: ``deterministic.solve``
+    with ``ad_mode="vjp"`` (scene compile or cache lookup, chain planning,
+    inputs, forward) and ``backward()`` of the mean-dB loss to the transmitter
+    position, wall time with a synchronization at the end. The first call
+    compiles the scene; the later ones find it in the compile cache."""
+    import torch
+
+    from witwin.channel import workloads
+    from witwin.channel.deterministic import Config, solve
+
+    scene = workloads.urban_canyon_scene(seed=2, shape=(grid, grid), spacing=2.0)
+    tx = scene.endpoints[0]
+    tx.position.requires_grad_(True)
+    cfg = Config(components={"los", "reflection", "diffraction"}, max_depth=depth, ad_mode="vjp")
+
+    def once() -> float:
+        tx.position.grad = None
+        t0 = time.perf_counter()
+        res = solve(scene, cfg, reference_frequency_hz=frequency)
+        p = res.path_gain
+        lit = p > 0
+        loss = (10.0 * torch.log10(torch.where(lit, p, torch.ones_like(p)))).sum() / lit.sum().clamp_min(1)
+        loss.backward()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    cold = once()
+    warm = sorted(once() for _ in range(reps))
+    return {"call": "deterministic.solve(ad_mode='vjp') + backward() to the transmitter position",
+            "cold_ms": cold, "warm_ms": warm[len(warm) // 2], "warm_ms_all": warm,
+            "grad_tx": [float(x) for x in tx.position.grad]}
+
+
 def batched_paths_section(stream) -> dict:
     """Batched explicit-path solve (urban canyon + 3 layered street walls, 64
     transmitters with 16..4096 street points each, LoS / reflection /
@@ -430,6 +464,7 @@ def main() -> int:
     export_ms = (time.perf_counter() - t0) * 1e3 / 3
     export_paths = int(rows["tx_id"].shape[0])
     del rows
+    public = public_solve_section(args.grid, frequency, depth)
     edges = int(dscene.view.edge_count)
     chains = int(inputs.chains.count)
     scene_bytes = compiled.face_count * 48 + int(dscene.node_count) * 64
@@ -561,6 +596,7 @@ def main() -> int:
         "kernel_ms": {**{k: v[0] for k, v in kernels.items()}, "vjp_reduce_kernel": reduce_ms},
         "paths_reflection_pass": n_main, "paths_diffraction": n_diff, "chain_planning_s": plan_s,
         "export": {"ms": export_ms, "paths": export_paths, "paths_per_s": export_paths / (export_ms * 1e-3)},
+        "public_solve": public,
         "dominant_kernel": dominant, "dominant_hbm_frac": (dom_bytes / (dom_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
         "oracle_check": canyon_cpu,
     }

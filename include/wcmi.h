@@ -558,9 +558,10 @@ int wc_paths_cfr(const float* a, const float* tau, const unsigned char* valid, i
  *  field_project_complex3 :149, field_reflection_sequence :180;
  *  native/channel/kernels/fields.cu:36-204). Outputs per row: field_vector
  * [N,3] c64, coefficient [N] c64, path_field [N] c64, path_gain [N],
- * path_length [N], delay [N], direction [N,3]. */
+ * path_length [N], delay [N], direction [N,3]. The frequency is float64: the
+ * phase -k L is formed in float64 from it, the Fresnel terms in float32. */
 int wc_field_free_space(int count, const float* source, const float* target, const float* tx_power,
-                        const float* tx_polarization, const float* rx_polarization, float frequency_hz,
+                        const float* tx_polarization, const float* rx_polarization, double frequency_hz,
                         float* field_vector, float* coefficient, float* path_field, float* path_gain,
                         float* path_length, float* delay, float* direction, void* stream);
 
@@ -571,7 +572,7 @@ int wc_field_reflection_sequence(int count, int depth, const float* source, cons
                                  const float* interaction_positions, const float* interaction_normals,
                                  const float* tx_power, const float* tx_polarization, const float* rx_polarization,
                                  const float* eps_r, const float* sigma_e, const float* mu_r, const float* gain,
-                                 const float* thickness, float frequency_hz, float* field_vector, float* coefficient,
+                                 const float* thickness, double frequency_hz, float* field_vector, float* coefficient,
                                  float* path_field, float* path_gain, float* path_length, float* delay,
                                  float* direction, void* stream);
 

@@ -21,7 +21,7 @@ __device__ __forceinline__ void store_c3(float* p, int64_t i, const C3<float>& v
 
 __global__ void free_space_kernel(int count, const float* __restrict__ source, const float* __restrict__ target,
                                   const float* __restrict__ tx_power, const float* __restrict__ tx_pol,
-                                  const float* __restrict__ rx_pol, float freq, float* __restrict__ field_vector,
+                                  const float* __restrict__ rx_pol, double freq, float* __restrict__ field_vector,
                                   float* __restrict__ coefficient, float* __restrict__ path_field,
                                   float* __restrict__ path_gain, float* __restrict__ path_length,
                                   float* __restrict__ delay, float* __restrict__ direction) {
@@ -30,7 +30,7 @@ __global__ void free_space_kernel(int count, const float* __restrict__ source, c
         const V3<double> off = to_d(b) - to_d(a);
         const double dist = norm(off);
         const V3<float> dir = safe_normalize(b - a, v3c<float>(0.f, 0.f, 1.f));
-        const double k_d = (double)freq * kTwoPi / kC0;
+        const double k_d = freq * kTwoPi / kC0;
         const C3<float> v = los_field(dir, ld3(tx_pol + 3 * i), (float)k_d, k_d, dist);
         const Cx<float> coef = project_receiver(v, dir, ld3(rx_pol + 3 * i));
         const Cx<float> pf = scale(coef, sqrtf(fmaxf(tx_power[i], 0.0f)));
@@ -64,7 +64,7 @@ __global__ void reflection_sequence_kernel(int count, int depth, const float* __
                                            const float* __restrict__ tx_pol, const float* __restrict__ rx_pol,
                                            const float* __restrict__ eps_r, const float* __restrict__ sigma_e,
                                            const float* __restrict__ mu_r, const float* __restrict__ gain,
-                                           const float* __restrict__ thickness, float freq,
+                                           const float* __restrict__ thickness, double freq,
                                            float* __restrict__ field_vector, float* __restrict__ coefficient,
                                            float* __restrict__ path_field, float* __restrict__ path_gain,
                                            float* __restrict__ path_length, float* __restrict__ delay,
@@ -83,13 +83,13 @@ __global__ void reflection_sequence_kernel(int count, int depth, const float* __
             total += norm(to_d(hit) - to_d(prev));
             const int m = i * depth + b;
             value = reflect_c3(value, incident, ld3(normals + 3 * m), eps_r[m], sigma_e[m], mu_r[m], gain[m],
-                               thickness[m], freq, outgoing);
+                               thickness[m], (float)freq, outgoing);
             prev = hit;
         }
         const V3<float> tgt = ld3(target + 3 * i);
         const V3<float> fdir = safe_normalize(tgt - prev, outgoing);
         total += norm(to_d(tgt) - to_d(prev));
-        const double k_d = (double)freq * kTwoPi / kC0;
+        const double k_d = freq * kTwoPi / kC0;
         const float amp = 1.0f / (2.0f * (float)k_d * fmaxf((float)total, kEpsF));
         value = c3_scale(value, scale(cexpj(neg_kd_phase(k_d, total)), amp));
         const Cx<float> coef = project_receiver(value, fdir, ld3(rx_pol + 3 * i));
@@ -207,12 +207,12 @@ size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
 using namespace wc;
 
 extern "C" int wc_field_free_space(int count, const float* source, const float* target, const float* tx_power,
-                                   const float* tx_polarization, const float* rx_polarization, float frequency_hz,
+                                   const float* tx_polarization, const float* rx_polarization, double frequency_hz,
                                    float* field_vector, float* coefficient, float* path_field, float* path_gain,
                                    float* path_length, float* delay, float* direction, void* stream) {
     if (count < 0) return wc_fail("wc_field_free_space: count must be non-negative");
     if (count == 0) return 0;
-    if (!(frequency_hz > 0.0f)) return wc_fail("frequency_hz must be positive");
+    if (!(frequency_hz > 0.0)) return wc_fail("frequency_hz must be positive");
     hipLaunchKernelGGL(free_space_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream, count,
                        source, target, tx_power, tx_polarization, rx_polarization, frequency_hz, field_vector,
                        coefficient, path_field, path_gain, path_length, delay, direction);
@@ -236,13 +236,13 @@ extern "C" int wc_field_reflection_sequence(int count, int depth, const float* s
                                             const float* tx_power, const float* tx_polarization,
                                             const float* rx_polarization, const float* eps_r, const float* sigma_e,
                                             const float* mu_r, const float* gain, const float* thickness,
-                                            float frequency_hz, float* field_vector, float* coefficient,
+                                            double frequency_hz, float* field_vector, float* coefficient,
                                             float* path_field, float* path_gain, float* path_length, float* delay,
                                             float* direction, void* stream) {
     if (count < 0) return wc_fail("wc_field_reflection_sequence: count must be non-negative");
     if (depth < 1) return wc_fail("reflection depth must be positive");
     if (count == 0) return 0;
-    if (!(frequency_hz > 0.0f)) return wc_fail("frequency_hz must be positive");
+    if (!(frequency_hz > 0.0)) return wc_fail("frequency_hz must be positive");
     hipLaunchKernelGGL(reflection_sequence_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, (hipStream_t)stream,
                        count, depth, source, target, interaction_positions, interaction_normals, tx_power,
                        tx_polarization, rx_polarization, eps_r, sigma_e, mu_r, gain, thickness, frequency_hz,

@@ -235,11 +235,11 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_paths_cfr": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p, ctypes.c_double,
                              ctypes.c_double, c_int, c_int, c_void_p, c_void_p]),
     "wc_utd_special_eval": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p]),
-    "wc_field_free_space": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_float,
+    "wc_field_free_space": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_double,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p]),
     "wc_field_project_complex3": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
-    "wc_field_reflection_sequence": (c_int, [c_int, c_int] + [c_void_p] * 12 + [c_float] + [c_void_p] * 8),
+    "wc_field_reflection_sequence": (c_int, [c_int, c_int] + [c_void_p] * 12 + [ctypes.c_double] + [c_void_p] * 8),
     "wc_field_rows_workspace_bytes": (c_size_t, [c_int, c_int]),
     "wc_field_free_space_jvp": (c_int, [c_int] + [c_void_p] * 5 + [ctypes.c_double, c_float, c_void_p, c_void_p,
                                         ctypes.POINTER(WcFieldRowsOut), c_void_p]),
