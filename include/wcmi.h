@@ -576,6 +576,34 @@ int wc_field_reflection_sequence(int count, int depth, const float* source, cons
                                  float* path_field, float* path_gain, float* path_length, float* delay,
                                  float* direction, void* stream);
 
+/* ------------------------------------------------------------ coupled reflection-diffraction paths
+ * The radiomap's coupled slot (reference interactions/coupled.py:93-230,
+ * native/channel/kernels/coupled.cu:2279-2710; deterministic.py:120-127
+ * coupled_paths): per (transmitter, receiver) cell every (coplanar group,
+ * edge) candidate as TX -> reflect -> diffract -> RX (component 3) and as
+ * TX -> diffract -> reflect -> RX (component 4), then every ordered edge pair
+ * e1 != e2 as TX -> diffract -> diffract -> RX (component 7). Stationary points
+ * in float64 (image method, Keller points, a bisection along edge 2 for
+ * double diffraction), strictly inside the edges and the reflecting face,
+ * every leg unoccluded; fields composed from the thin-slab Jones reflection and
+ * the first-order UTD wedge coefficient. kinds: 1 R->D, 2 D->R, 4 D->D.
+ * Cells [cell_begin, cell_end) are evaluated; the outputs are indexed by cell:
+ * field [cells] c64 = sum of sqrt(P) x receiver-projected path fields,
+ * power [cells] = sum of path gains, count [cells] = paths found (sums in
+ * float64 in candidate order). The workspace holds 16 B per (cell, candidate)
+ * of the range: wc_coupled_workspace_bytes(cell_end - cell_begin, G, E, kinds). */
+typedef struct wc_coupled_out {
+    float* field;
+    float* power;
+    int* count;
+    int* count_dd; /* [cells] double-diffraction paths among count, or NULL */
+} wc_coupled_out;
+
+long long wc_coupled_candidates_per_cell(int group_count, int edge_count, int kinds);
+size_t wc_coupled_workspace_bytes(int cells, int group_count, int edge_count, int kinds);
+int wc_coupled_paths(const wc_scene* scene, const wc_solve* solve, int cell_begin, int cell_end, int kinds,
+                     const wc_coupled_out* out, void* workspace, size_t workspace_bytes, void* stream);
+
 /* ------------------------------------------------------------ row operators: transmission, layer stacks, AD
  * Layered-material tables in CSR form (reference kernels/materials.py:42-70
  * _validate_layer_csr): material m owns layers layer_offset[m] ..

@@ -129,6 +129,9 @@ def load() -> ctypes.CDLL:
         lib.oracle_pair_paths.argtypes = [ctypes.POINTER(OracleScene), ctypes.POINTER(OracleSolve),
                                           ctypes.POINTER(OraclePath), i]
         lib.oracle_pair_paths.restype = i
+        lib.oracle_coupled_paths.argtypes = [ctypes.POINTER(OracleScene), ctypes.POINTER(OracleSolve), i,
+                                             ctypes.POINTER(OraclePath), i]
+        lib.oracle_coupled_paths.restype = i
         lib.oracle_pair_accumulate.argtypes = [ctypes.POINTER(OraclePath), i, i, d, vp]
         lib.oracle_radiomap.argtypes = [ctypes.POINTER(OracleScene), ctypes.POINTER(OracleSolve), vp, i, i, vp,
                                         vp, i, vp, d, vp]
@@ -306,6 +309,17 @@ class Oracle:
         cap = 4096
         buf = (OraclePath * cap)()
         count = self.lib.oracle_pair_paths(ctypes.byref(self.scene), ctypes.byref(q), buf, cap)
+        return [buf[i] for i in range(min(count, cap))]
+
+    def coupled_paths(self, *, tx, rx, tx_pol=(0, 0, 1), tx_power=1.0, rx_pol=(0, 0, 1), frequency,
+                      kinds: int = 7) -> list[OraclePath]:
+        """Coupled paths of one pair in candidate order (components 3 R->D, 4
+        D->R, 7 D->D; oracle.cpp oracle_coupled_paths)."""
+        q, keep = self._solve(tx, tx_pol, tx_power, rx_pol, frequency, 0, 0, np.zeros((0, MAXD)), np.zeros(0))
+        q.rx[:] = [float(x) for x in rx]
+        cap = 65536
+        buf = (OraclePath * cap)()
+        count = self.lib.oracle_coupled_paths(ctypes.byref(self.scene), ctypes.byref(q), int(kinds), buf, cap)
         return [buf[i] for i in range(min(count, cap))]
 
     def discover_all(self, tx, ray_count: int, max_depth: int) -> np.ndarray:

@@ -355,6 +355,101 @@ int face_containing(const Scene& s, int g, V p, V n) {
     return -1;
 }
 
+// ---------------------------------------------------------------- coupled paths
+// Restatement of the coupled reflection-diffraction candidates (reference
+// interactions/coupled.py:93-230, native/channel/kernels/coupled.cu:2279-2710)
+// in float64 with brute-force visibility: R->D via the image of the
+// transmitter, D->R via the image of the receiver, D->D by bisection of the
+// path length's derivative along edge 2 (edge-1 Keller point in closed form).
+struct EdgeRec {
+    V A, d, t0, n0;
+    double length, n;
+    int group0, group1, mat0, mat1;
+};
+
+EdgeRec edge_rec(const Scene& s, int e) {
+    const float* f = s.edge_f + 14 * e;
+    const int* i = s.edge_i + 6 * e;
+    return {{f[0], f[1], f[2]}, {f[3], f[4], f[5]}, {f[8], f[9], f[10]}, {f[11], f[12], f[13]}, f[6], f[7],
+            i[2], i[3], i[4], i[5]};
+}
+
+struct Keller {
+    V q;
+    double t, sp, so, rp, rr, phip, phi;
+};
+
+// Keller point of end points a (source side) and b on the edge line; false
+// when either end is on the line or the source is inside the wedge
+bool keller(const EdgeRec& e, V a, V b, Keller& k) {
+    const double ta = dot(a - e.A, e.d), tb = dot(b - e.A, e.d);
+    const V pa = a - e.A - e.d * ta, pb = b - e.A - e.d * tb;
+    k.rp = len(pa);
+    k.rr = len(pb);
+    if (k.rp < 1e-6 || k.rr < 1e-6) return false;
+    k.t = ta + (tb - ta) * k.rp / (k.rp + k.rr);
+    k.q = e.A + e.d * k.t;
+    k.sp = len(a - k.q);
+    k.so = len(b - k.q);
+    auto ang = [&](V u) { double x = std::atan2(dot(u, e.n0), dot(u, e.t0)); if (x < 0) x += 2 * PI; return x; };
+    const double npi = e.n * PI;
+    k.phip = ang(pa);
+    if (k.phip >= npi) return false;
+    double phi = ang(pb);
+    if (phi > npi) phi = (phi - npi < 2 * PI - phi) ? npi : 0.0;
+    k.phi = phi;
+    return true;
+}
+
+// K-P wedge diffraction of the incident vector einc at k.q (spreading from
+// its source applied) observed along src -> q -> obs:
+// [-Ds b0 b0' - Dh phi phi'] einc sqrt(sp/(so(so+sp))) e^{-jk so}
+CV wedge(const Scene& s, const EdgeRec& e, const Keller& k, V src, V obs, CV einc, double f, double kw, V& so_dir) {
+    const V si = (k.q - src) * (1.0 / k.sp);
+    const V so = (obs - k.q) * (1.0 / k.so);
+    so_dir = so;
+    const double cb = dot(si, e.d);
+    const double sinb = std::sqrt(std::max(1.0 - cb * cb, 0.0));
+    V phi_i = -cross(e.d, si); phi_i = phi_i * (1.0 / std::max(len(phi_i), 1e-30));
+    const V b0_i = cross(phi_i, si);
+    V phi_o = cross(e.d, so); phi_o = phi_o * (1.0 / std::max(len(phi_o), 1e-30));
+    const V b0_o = cross(phi_o, so);
+    const cd eb = cdot(einc, b0_i), ep = cdot(einc, phi_i);
+    const double L = k.so * k.sp * sinb * sinb / (k.so + k.sp);
+    const double npi = e.n * PI;
+    const int m0 = e.mat0, mn = e.mat1 >= 0 ? e.mat1 : e.mat0;
+    cd r0s, r0h, rns, rnh;
+    slab_fresnel(std::fabs(std::sin(k.phip)), s.mat_eps[m0], s.mat_sigma[m0], s.mat_mu[m0], s.mat_gain[m0],
+                 s.mat_thick[m0], f, r0s, r0h);
+    slab_fresnel(std::fabs(std::sin(npi - k.phi)), s.mat_eps[mn], s.mat_sigma[mn], s.mat_mu[mn], s.mat_gain[mn],
+                 s.mat_thick[mn], f, rns, rnh);
+    const double kappa = k.rp * k.rp / (k.sp * k.sp * k.sp) + k.rr * k.rr / (k.so * k.so * k.so);
+    const double sg = std::sqrt(kw * kappa / PI);
+    double c1, s1, c0, s0;
+    fresnel(sg * (e.length - k.t), c1, s1);
+    fresnel(sg * (0.0 - k.t), c0, s0);
+    const cd trunc = cd(0.5, 0.5) * std::conj(cd(c1 - c0, s1 - s0));
+    const double u = sg * std::max(-k.t, k.t - e.length);
+    const double gamma = 1.0 / (1.0 + std::exp(u / 0.1));
+    cd ds, dh;
+    utd_wedge(e.n, k.phi, k.phip, sinb, kw, L, r0s, rns, r0h, rnh, trunc, gamma, ds, dh);
+    const cd prop = std::sqrt(k.sp / (k.so * (k.so + k.sp))) * std::exp(cd(0.0, -kw * k.so));
+    return cv_axis(b0_o, -ds * eb * prop) + cv_axis(phi_o, -dh * ep * prop);
+}
+
+bool occluded4(const Scene& s, V a, V b, int g0, int g1, int g2, int g3) {
+    const V d = b - a;
+    const double l = len(d);
+    if (l <= 2.0 * BIAS) return false;
+    const double eps = BIAS / l;
+    for (int f = 0; f < s.face_count; ++f) {
+        const int g = s.face_group[f];
+        if (g == g0 || g == g1 || g == g2 || g == g3) continue;
+        if (seg_hits_tri(s, f, a, d, eps, 1.0 - eps, nullptr)) return true;
+    }
+    return false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -691,6 +786,164 @@ int oracle_pair_paths(const Scene* sp, const Solve* qp, OraclePath* out, int cap
             const cd prop = (1.0 / (2.0 * k * std::max(dist, EPS))) * std::exp(cd(0.0, -k * carrier));
             finish(p, value * prop, dir, dir);
             paths.push_back(p);
+        }
+    }
+    const int count = (int)paths.size();
+    for (int i = 0; i < count && i < cap; ++i) out[i] = paths[i];
+    return count;
+}
+
+// Coupled paths of one (transmitter, receiver) pair in candidate order:
+// (group, edge) R->D (component 3), (group, edge) D->R (4), ordered edge
+// pairs D->D (7). kinds: 1 R->D, 2 D->R, 4 D->D. Returns the path count.
+int oracle_coupled_paths(const Scene* sp, const Solve* qp, int kinds, OraclePath* out, int cap) {
+    const Scene& s = *sp;
+    const Solve& q = *qp;
+    const V T{q.tx[0], q.tx[1], q.tx[2]}, X{q.rx[0], q.rx[1], q.rx[2]};
+    const V tpol{q.tx_pol[0], q.tx_pol[1], q.tx_pol[2]}, rpol{q.rx_pol[0], q.rx_pol[1], q.rx_pol[2]};
+    const double f = q.frequency, kw = 2.0 * PI * f / C0;
+    const double sqp = std::sqrt(std::max(q.tx_power, 0.0));
+    const double eps = 1e-6;
+    std::vector<OraclePath> paths;
+    auto emit = [&](int comp, CV v, V arrival, double length, V p0, V p1, int prim0, int edge) {
+        OraclePath p;
+        std::memset(&p, 0, sizeof(p));
+        p.component = comp;
+        p.depth = 2;
+        for (int i = 0; i < MAXD; ++i) { p.prim[i] = -1; p.mat[i] = -1; }
+        p.prim[0] = prim0;
+        p.edge = edge;
+        p.length = length;
+        p.pos[0][0] = p0.x; p.pos[0][1] = p0.y; p.pos[0][2] = p0.z;
+        p.pos[1][0] = p1.x; p.pos[1][1] = p1.y; p.pos[1][2] = p1.z;
+        const cd xs[3] = {v.x, v.y, v.z};
+        for (int i = 0; i < 3; ++i) { p.field[2 * i] = xs[i].real(); p.field[2 * i + 1] = xs[i].imag(); }
+        const cd coef = cdot(v, transverse(arrival, rpol));
+        const cd pf = coef * sqp;
+        p.coefficient[0] = coef.real(); p.coefficient[1] = coef.imag();
+        p.path_field[0] = pf.real(); p.path_field[1] = pf.imag();
+        p.gain = std::norm(pf);
+        p.arrival[0] = arrival.x; p.arrival[1] = arrival.y; p.arrival[2] = arrival.z;
+        if (p.gain >= FLOOR) paths.push_back(p);
+    };
+    auto inside = [&](const EdgeRec& e, double t) { return t > eps && t < e.length - eps; };
+    const int G = s.group_count, E = s.edge_count;
+    for (int dir = 0; dir < 2; ++dir) {
+        if (!(kinds & (1 << dir))) continue;
+        for (int g = 0; g < G; ++g) {
+            double off;
+            const V n = plane_n(s, g, off);
+            for (int ei = 0; ei < E; ++ei) {
+                const EdgeRec e = edge_rec(s, ei);
+                Keller k;
+                if (dir == 0) {  // R->D: image transmitter
+                    const V I = T - n * (2.0 * (dot(n, T) - off));
+                    if (!keller(e, I, X, k) || !inside(e, k.t)) continue;
+                    const double den = dot(n, k.q - I);
+                    if (std::fabs(den) < 1e-12) continue;
+                    const double t = (off - dot(n, I)) / den;
+                    if (!(t > eps && t < 1.0 - eps)) continue;
+                    const V R = I + (k.q - I) * t;
+                    const int face = face_containing(s, g, R, n);
+                    if (face < 0) continue;
+                    if (occluded4(s, T, R, g, -1, -1, -1) || occluded4(s, R, k.q, g, e.group0, e.group1, -1) ||
+                        occluded4(s, k.q, X, e.group0, e.group1, -1, -1))
+                        continue;
+                    const int m = s.face_material[face];
+                    const V d1 = safe_normalize(R - T, {0, 0, 1});
+                    V outg;
+                    CV v = cv_axis(transverse(d1, tpol), cd(1.0, 0.0));
+                    v = reflect(v, d1, n, s.mat_eps[m], s.mat_sigma[m], s.mat_mu[m], s.mat_gain[m], s.mat_thick[m], f,
+                                outg);
+                    const cd c = (1.0 / (2.0 * kw * k.sp)) * std::exp(cd(0.0, -kw * k.sp));
+                    V so;
+                    const CV fld = wedge(s, e, k, I, X, v * c, f, kw, so);
+                    emit(3, fld, so, k.sp + k.so, R, k.q, face, ei);
+                } else {  // D->R: image receiver
+                    const V Xi = X - n * (2.0 * (dot(n, X) - off));
+                    if (!keller(e, T, Xi, k) || !inside(e, k.t)) continue;
+                    const double den = dot(n, Xi - k.q);
+                    if (std::fabs(den) < 1e-12) continue;
+                    const double t = (off - dot(n, k.q)) / den;
+                    if (!(t > eps && t < 1.0 - eps)) continue;
+                    const V R = k.q + (Xi - k.q) * t;
+                    const int face = face_containing(s, g, R, n);
+                    if (face < 0) continue;
+                    if (occluded4(s, T, k.q, e.group0, e.group1, -1, -1) ||
+                        occluded4(s, k.q, R, e.group0, e.group1, g, -1) || occluded4(s, R, X, g, -1, -1, -1))
+                        continue;
+                    const V si = (k.q - T) * (1.0 / k.sp);
+                    const cd c = (1.0 / (2.0 * kw * k.sp)) * std::exp(cd(0.0, -kw * k.sp));
+                    V so;
+                    const CV at_r = wedge(s, e, k, T, Xi, cv_axis(transverse(si, tpol), c), f, kw, so);
+                    const int m = s.face_material[face];
+                    const V inc = safe_normalize(R - k.q, {0, 0, 1});
+                    V outg;
+                    const CV fld = reflect(at_r, inc, n, s.mat_eps[m], s.mat_sigma[m], s.mat_mu[m], s.mat_gain[m],
+                                           s.mat_thick[m], f, outg);
+                    emit(4, fld, safe_normalize(X - R, outg), k.sp + k.so, k.q, R, face, ei);
+                }
+            }
+        }
+    }
+    if (kinds & 4) {
+        for (int a = 0; a < E; ++a) {
+            for (int b = 0; b < E; ++b) {
+                if (a == b) continue;
+                const EdgeRec e1 = edge_rec(s, a), e2 = edge_rec(s, b);
+                if (len(cross(e1.d, e2.d)) < 1e-9 && len(cross(e2.A - e1.A, e1.d)) < 1e-6) continue;
+                // edges of one face group: the middle leg would graze that face
+                if ((e1.group0 >= 0 && (e1.group0 == e2.group0 || e1.group0 == e2.group1)) ||
+                    (e1.group1 >= 0 && (e1.group1 == e2.group0 || e1.group1 == e2.group1)))
+                    continue;
+                auto q1_of = [&](double t2, V& q1, V& q2) {
+                    q2 = e2.A + e2.d * t2;
+                    const double ta = dot(T - e1.A, e1.d), tb = dot(q2 - e1.A, e1.d);
+                    const double ra = len(T - e1.A - e1.d * ta), rb = len(q2 - e1.A - e1.d * tb);
+                    const double t1 = (ra + rb) > 0 ? ta + (tb - ta) * ra / (ra + rb) : ta;
+                    q1 = e1.A + e1.d * t1;
+                    return t1;
+                };
+                auto slope = [&](double t2) {
+                    V q1, q2;
+                    q1_of(t2, q1, q2);
+                    const V mid = q2 - q1, o = q2 - X;
+                    const double lm = len(mid), lo = len(o);
+                    if (!(lm > eps) || !(lo > eps)) return std::nan("");
+                    return dot(e2.d, mid * (1.0 / lm) + o * (1.0 / lo));
+                };
+                double lo = eps, hi = e2.length - eps;
+                if (!(hi > lo)) continue;
+                if (!(slope(lo) < 0.0 && slope(hi) > 0.0)) continue;
+                bool bad = false;
+                for (int it = 0; it < 200 && hi - lo > 1e-13; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    const double sm = slope(mid);
+                    if (std::isnan(sm)) { bad = true; break; }
+                    if (sm > 0.0) hi = mid; else lo = mid;
+                }
+                if (bad) continue;
+                const double t2 = 0.5 * (lo + hi);
+                V Q1, Q2;
+                const double t1 = q1_of(t2, Q1, Q2);
+                if (!inside(e1, t1)) continue;
+                Keller k1, k2;
+                if (!keller(e1, T, Q2, k1) || !keller(e2, Q1, X, k2)) continue;
+                k2.q = Q2;
+                k2.t = t2;
+                k2.sp = len(Q2 - Q1);
+                k2.so = len(X - Q2);
+                if (occluded4(s, T, Q1, e1.group0, e1.group1, -1, -1) ||
+                    occluded4(s, Q1, Q2, e1.group0, e1.group1, e2.group0, e2.group1) ||
+                    occluded4(s, Q2, X, e2.group0, e2.group1, -1, -1))
+                    continue;
+                const V si = (k1.q - T) * (1.0 / k1.sp);
+                const cd c = (1.0 / (2.0 * kw * k1.sp)) * std::exp(cd(0.0, -kw * k1.sp));
+                V so;
+                const CV at_q2 = wedge(s, e1, k1, T, Q2, cv_axis(transverse(si, tpol), c), f, kw, so);
+                const CV fld = wedge(s, e2, k2, Q1, X, at_q2, f, kw, so);
+                emit(7, fld, so, k1.sp + k2.sp + k2.so, Q1, Q2, -1, a * E + b);
+            }
         }
     }
     const int count = (int)paths.size();

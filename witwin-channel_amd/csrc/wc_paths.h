@@ -265,10 +265,11 @@ __device__ __forceinline__ double wedge_angle(const V3<double>& u, const V3<doub
 // With observation = false the receiver's wedge angle phi (used only by the
 // field) is left unset; the existence test and every other member are the
 // same.
-__device__ inline bool diffraction_geometry(const Edge& e, const V3<float>& tx, const V3<float>& rx,
-                                            DiffGeometry& g, bool observation = true) {
+// (diffraction_geometry_d takes double end points: image sources and
+// receivers of the coupled paths keep their full precision.)
+__device__ inline bool diffraction_geometry_d(const Edge& e, const V3<double>& T, const V3<double>& Rr,
+                                              DiffGeometry& g, bool observation = true) {
     const V3<double> A{e.ax, e.ay, e.az}, d{e.dx, e.dy, e.dz};
-    const V3<double> T = to_d(tx), Rr = to_d(rx);
     const double a = dot(T - A, d), b = dot(Rr - A, d);
     const V3<double> pt = T - A - d * a, pr = Rr - A - d * b;
     g.rho_p = norm(pt);
@@ -287,6 +288,11 @@ __device__ inline bool diffraction_geometry(const Edge& e, const V3<float>& tx, 
     if (phi > npi) phi = (phi - npi < kTwoPi - phi) ? npi : 0.0;  // nearest face of the exterior domain
     g.phi = phi;
     return true;
+}
+
+__device__ inline bool diffraction_geometry(const Edge& e, const V3<float>& tx, const V3<float>& rx,
+                                            DiffGeometry& g, bool observation = true) {
+    return diffraction_geometry_d(e, to_d(tx), to_d(rx), g, observation);
 }
 
 // Diffracted complex 3-vector at the receiver (unit excitation), K-P dyadic

@@ -154,6 +154,10 @@ class WcLayerStackOut(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in LAYER_STACK_FIELDS]
 
 
+class WcCoupledOut(ctypes.Structure):
+    _fields_ = [("field", c_void_p), ("power", c_void_p), ("count", c_void_p), ("count_dd", c_void_p)]
+
+
 class WcFixedRows(ctypes.Structure):
     _fields_ = [
         ("count", c_int), ("width", c_int), ("component", c_void_p), ("depth", c_void_p), ("faces", c_void_p),
@@ -240,6 +244,11 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
                                     c_void_p]),
     "wc_field_project_complex3": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_field_reflection_sequence": (c_int, [c_int, c_int] + [c_void_p] * 12 + [ctypes.c_double] + [c_void_p] * 8),
+    "wc_coupled_candidates_per_cell": (ctypes.c_longlong, [c_int, c_int, c_int]),
+    "wc_coupled_workspace_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "wc_coupled_paths": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve), c_int, c_int, c_int,
+                                 ctypes.POINTER(WcCoupledOut),
+                                 c_void_p, c_size_t, c_void_p]),
     "wc_field_rows_workspace_bytes": (c_size_t, [c_int, c_int]),
     "wc_field_free_space_jvp": (c_int, [c_int] + [c_void_p] * 5 + [ctypes.c_double, c_float, c_void_p, c_void_p,
                                         ctypes.POINTER(WcFieldRowsOut), c_void_p]),

@@ -219,12 +219,40 @@ def accumulate_rows(rows: dict[str, Any], num_tx: int, num_rx: int, coherent: bo
     return pt, ft, cp[:4], cf[:4], diff
 
 
+def _add_coupled(dscene, inputs, config: Config, power_total, field_total, component_power, component_fields,
+                 num_tx: int, num_rx: int):
+    """Coupled R->D / D->R / D->D paths into the dedicated coupled slot
+    (reference deterministic.py:265-290, ADR-011 D2: component ids 3, 4 and 7
+    accumulate coherently into one slot that joins the field sum). Coherent:
+    field_total += coupled field, power_total = |field_total|^2; incoherent:
+    power_total += coupled gains, field_total = sqrt(power_total)."""
+    co = engine.coupled_fields(dscene, inputs, candidate_limit=config.coupled_candidate_limit)
+    cf = co.field.reshape(num_tx, num_rx)
+    if config.coherent:
+        total = field_total.to(torch.complex128) + cf.to(torch.complex128)
+        field_total = total.to(torch.complex64)
+        power_total = (total.real.square() + total.imag.square()).to(torch.float32)
+        component_power["coupled"] = (cf.real.double().square() + cf.imag.double().square()).float()
+    else:
+        power_total = (power_total.double() + co.power.reshape(num_tx, num_rx).double()).float()
+        field_total = power_total.clamp_min(0.0).sqrt().to(torch.complex64)
+        component_power["coupled"] = co.power.reshape(num_tx, num_rx)
+    component_fields["coupled"] = cf
+    total_n = int(co.count.sum().item())
+    dd_n = int(co.count_dd.sum().item())
+    return power_total, field_total, {"coupled": total_n, "coupled_double_diffraction": dd_n}
+
+
 def _validate(config: Config) -> None:
     unsupported = set(config.components) - {"los", "reflection", "diffraction", "transmission"}
     if unsupported:
         raise NotImplementedError(f"components {sorted(unsupported)} are outside this build's hot path")
-    if config.coupled_paths:
-        raise NotImplementedError("coupled reflection-diffraction paths are outside this build's hot path")
+    if config.coupled_paths and config.ad_mode in ("vjp", "jvp"):
+        raise NotImplementedError("coupled reflection-diffraction paths are forward only in this build "
+                                  f"(ad_mode='{config.ad_mode}')")
+    if config.coupled_paths and (config.export_paths or config.max_paths is not None):
+        raise NotImplementedError("coupled reflection-diffraction paths enter the radiomap only; export_paths and "
+                                  "max_paths do not carry coupled rows in this build")
     if config.isb_boundary_taper:
         raise NotImplementedError("isb_boundary_taper is outside this build's hot path")
     if config.ad_mode in ("vjp", "jvp") and config.max_paths is not None:
@@ -269,6 +297,12 @@ def _metadata(config: Config, path_count: int, counts: dict[str, int], launches:
     }
     if "transmission" in config.components:
         meta["transmission"] = {"thin_sheet_straight_path_approximation": True, "group_delay": "geometric"}
+    meta["coupled_paths"] = ({"requested": False, "geometry": "not_requested", "coefficient": "not_requested"}
+                             if not config.coupled_paths else
+                             {"requested": True, "geometry": "native_1r1d_reciprocal_plus_dd",
+                              "coefficient": "unified_complex3_jones",
+                              "double_diffraction": {"geometry": "native_dd_cascade", "component_id": 7,
+                                                     "row_count": int(counts.get("coupled_double_diffraction", 0))}})
     return meta
 
 
@@ -310,6 +344,11 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
         component_power["diffraction"] = exact
     counts_t = out.path_count.reshape(num_tx, num_rx)
     path_count = int(counts_t.sum().item())
+    coupled_counts = None
+    if config.coupled_paths:
+        power_total, field_total, coupled_counts = _add_coupled(dscene, inputs, config, power_total, field_total,
+                                                                component_power, component_fields, num_tx, num_rx)
+        path_count += coupled_counts["coupled"]
     paths = None
     counts = {"los": 0, "reflection": 0, "diffraction": 0}
     truncated = False
@@ -342,6 +381,8 @@ def _solve_pipeline(compiled: CompiledScene, config: Config, *, tx_position: tor
             counts[name] = int((cid == c).sum().item())
         if config.export_paths:
             paths = build_path_table(rows, include_fields=config.return_field)
+    if coupled_counts is not None:
+        counts.update(coupled_counts)
     metadata = _metadata(config, path_count, counts, launches=2 if config.export_paths else 1)
     path_gain = apply_layout(power_total, compiled)
     field = apply_layout(field_total, compiled) if config.return_field else \

@@ -491,6 +491,58 @@ def radiomap_forward(dscene: DeviceScene, inputs: SolveInputs, *, topology_capac
     return out
 
 
+MAX_COUPLED_CANDIDATES = 1_000_000  # reference interactions/coupled.py:47
+COUPLED_KINDS = 7  # 1 R->D, 2 D->R, 4 D->D
+
+
+@dataclass
+class CoupledOutputs:
+    field: torch.Tensor   # [cells] c64: sum of the coupled path fields (sqrt(P) x coefficient)
+    power: torch.Tensor   # [cells]: sum of the coupled path gains
+    count: torch.Tensor   # [cells] i32: coupled paths found
+    count_dd: torch.Tensor  # [cells] i32: double-diffraction paths among them
+
+
+def coupled_candidates_per_cell(dscene: DeviceScene, kinds: int = COUPLED_KINDS) -> int:
+    return int(_native.call("wc_coupled_candidates_per_cell", int(dscene.group_count), int(dscene.edge_count),
+                            int(kinds)))
+
+
+def coupled_fields(dscene: DeviceScene, inputs: SolveInputs, *, candidate_limit: int = MAX_COUPLED_CANDIDATES,
+                   kinds: int = COUPLED_KINDS) -> CoupledOutputs:
+    """Coupled reflection-diffraction paths of every cell (components 3, 4
+    and 7; reference interactions/coupled.py:93-230): the cells are streamed
+    in blocks of at most ``candidate_limit`` (capped at 1,000,000)
+    candidates, as the reference streams receiver blocks; a single cell whose
+    candidates exceed the budget raises the reference's error."""
+    device = dscene.device
+    cells = inputs.total_cells
+    per_cell = coupled_candidates_per_cell(dscene, kinds)
+    limit = min(int(candidate_limit), MAX_COUPLED_CANDIDATES)
+    if per_cell > limit:
+        raise RuntimeError(f"coupled reflection-diffraction topology requires {per_cell} candidates, exceeding "
+                           f"coupled_candidate_limit={limit}")
+    out = CoupledOutputs(field=torch.zeros(cells, dtype=torch.complex64, device=device),
+                         power=torch.zeros(cells, dtype=torch.float32, device=device),
+                         count=torch.zeros(cells, dtype=torch.int32, device=device),
+                         count_dd=torch.zeros(cells, dtype=torch.int32, device=device))
+    if per_cell == 0 or cells == 0:
+        return out
+    block = max(1, limit // per_cell)
+    ws_bytes = int(_native.call("wc_coupled_workspace_bytes", min(block, cells), int(dscene.group_count),
+                                int(dscene.edge_count), int(kinds)))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=device)
+    p = _native.ptr
+    view = _native.WcCoupledOut(field=p(torch.view_as_real(out.field)), power=p(out.power), count=p(out.count),
+                                count_dd=p(out.count_dd))
+    stream = _native.stream_handle(device)
+    for begin in range(0, cells, block):
+        end = min(cells, begin + block)
+        _native.call("wc_coupled_paths", dscene.ref(), ctypes.byref(inputs.view), begin, end, int(kinds),
+                     ctypes.byref(view), p(ws), ws_bytes, stream)
+    return out
+
+
 def _out_view(out: RadiomapOutputs) -> _native.WcRadiomapOut:
     p = _native.ptr
     return _native.WcRadiomapOut(

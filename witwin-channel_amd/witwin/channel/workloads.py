@@ -190,6 +190,27 @@ def wedge_diffraction_scene(edge_policy: str | None = None, tx=(0.0, -1.0, 0.5),
     return Scene(structures=[face_a, face_b], endpoints=[transmitter(tx), receiver(rx)], metadata=metadata)
 
 
+def coupled_wall_wedge_scene(material: PhysicalMaterial | None = None, tx=(0.0, -2.0, 1.0), rx=(0.0, 2.0, 5.0),
+                             ground: bool = True, grid_shape: tuple[int, int] | None = None,
+                             grid_origin=(-3.0, -3.0, 3.0), grid_spacing: float = 0.5) -> Scene:
+    """A reflector at z = 0 ([-5, 5]^2) and an L-shaped wedge whose convex edge
+    runs along (2, y, 2), y in [-1, 1]: a horizontal face at z = 2 over
+    x in [2, 4] and a vertical face at x = 2 over z in [2, 4] (geometry of
+    reference tests/support/scenes.py:218-264), PEC unless ``material`` is
+    given. ``ground=False`` drops the reflector; ``grid_shape`` replaces the
+    single receiver by a receiver grid."""
+    mat = PhysicalMaterial.perfect_conductor() if material is None else material
+    v = [[2.0, -1.0, 2.0], [2.0, 1.0, 2.0], [4.0, -1.0, 2.0], [4.0, 1.0, 2.0], [2.0, -1.0, 4.0], [2.0, 1.0, 4.0]]
+    f = [[0, 2, 3], [0, 3, 1], [0, 1, 5], [0, 5, 4]]
+    if ground:
+        v = [[-5.0, -5.0, 0.0], [5.0, -5.0, 0.0], [5.0, 5.0, 0.0], [-5.0, 5.0, 0.0]] + v
+        f = [[0, 1, 2], [0, 2, 3]] + [[a + 4, b + 4, c + 4] for a, b, c in f]
+    wedge = Structure(Mesh(torch.tensor(v, dtype=torch.float32), torch.tensor(f)), mat, name="wall-wedge",
+                      surface_id=0)
+    rx_end = receiver(rx) if grid_shape is None else grid(grid_origin, grid_shape, grid_spacing)
+    return Scene(structures=[wedge], endpoints=[transmitter(tx), rx_end])
+
+
 SOIL = dict(eps_r=15.0, sigma_e=0.035)
 BRICK = dict(eps_r=3.75, sigma_e=0.038)
 
