@@ -55,6 +55,10 @@ visibility. Parity status of each restated function, i.e. what pins it:
                          diffraction row keyed by its edge) and the per-pair
                          cap  -> pinned
 * discovery rays ....... restated ray-by-ray; compared with the GPU chain set  -> pinned by equality
+* coupled paths ........ (Oracle.coupled_paths) R->D / D->R fields equal the image-theory first-order
+                         diffraction over a perfect conductor; every path's length is stationary
+                         (Fermat) with its points on their plane and edges  -> pinned; the D->D
+                         second-edge spreading factor has no such check  -> parity unpinned
 """
 
 from __future__ import annotations
