@@ -841,7 +841,7 @@ typedef struct wc_fixed_rows {
     float* normals;               /* [N, width, 3] out */
 } wc_fixed_rows;
 
-int wc_fixed_rows_evaluate(const wc_scene* scene, const wc_fixed_rows* rows, float frequency_hz, void* stream);
+int wc_fixed_rows_evaluate(const wc_scene* scene, const wc_fixed_rows* rows, double frequency_hz, void* stream);
 
 /* ------------------------------------------------------------ discovery
  * Transmitter-launched specular discovery rays (reference

@@ -67,12 +67,13 @@ __device__ inline ReplayOut replay_reflection(const SceneView& s, const int* fro
     return out;
 }
 
-__global__ void __launch_bounds__(256) fixed_rows_kernel(SceneView s, wc_fixed_rows r, float freq) {
+__global__ void __launch_bounds__(256) fixed_rows_kernel(SceneView s, wc_fixed_rows r, double freq_d) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= r.count) return;
     const V3<float> tx = ld3(r.source + 3 * i), rx = ld3(r.target + 3 * i);
     const V3<float> tx_pol = ld3(r.tx_polarization + 3 * i), rx_pol = ld3(r.rx_polarization + 3 * i);
-    const double k_d = (double)freq * kTwoPi / kC0;
+    const double k_d = freq_d * kTwoPi / kC0;
+    const float freq = (float)freq_d;
     const int comp = r.component[i];
     const int depth = r.depth[i];
     const int w = r.width;
@@ -134,13 +135,13 @@ __global__ void __launch_bounds__(256) fixed_rows_kernel(SceneView s, wc_fixed_r
 
 using namespace wc;
 
-extern "C" int wc_fixed_rows_evaluate(const wc_scene* scene, const wc_fixed_rows* rows, float frequency_hz,
+extern "C" int wc_fixed_rows_evaluate(const wc_scene* scene, const wc_fixed_rows* rows, double frequency_hz,
                                       void* stream) {
     if (!scene || !rows) return wc_fail("wc_fixed_rows_evaluate: null argument");
     if (rows->count < 0 || rows->width < 0 || rows->width > kMaxDepth)
         return wc_fail("wc_fixed_rows_evaluate: invalid sizes (count %d, width %d)", rows->count, rows->width);
     if (rows->count == 0) return 0;
-    if (!(frequency_hz > 0.0f)) return wc_fail("wc_fixed_rows_evaluate: frequency_hz must be positive");
+    if (!(frequency_hz > 0.0)) return wc_fail("wc_fixed_rows_evaluate: frequency_hz must be positive");
     if (!rows->component || !rows->depth || !rows->source || !rows->target || !rows->tx_power ||
         !rows->tx_polarization || !rows->rx_polarization || !rows->valid || !rows->path_field ||
         !rows->field_xyz || !rows->path_length || !rows->delay || !rows->direction)

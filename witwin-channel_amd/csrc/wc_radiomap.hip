@@ -1129,7 +1129,16 @@ __device__ __forceinline__ bool chain_legs(const SceneView& s, const wc_solve& q
 // box, then every receiver tests the survivors exactly. The bits equal
 // chain_visibility_kernel's; mask words of chunks with no surviving chain stay
 // as the zero fill left them.
-__global__ void __launch_bounds__(256) chain_visibility_seg_kernel(SceneView s, wc_solve q,
+// Occupancy target of the segment kernel: 4 waves per SIMD (128 VGPRs, a
+// few spilled) instead of the 3 its 168 registers allow: its leg walks are
+// latency-bound (scaled city chain pass 61.5 -> 50.7 ms per batch; 5 waves
+// gave 51.1 ms).
+#ifndef WC_SEG_WAVES_PER_EU
+#define WC_SEG_WAVES_PER_EU 4
+#endif
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_SEG_WAVES_PER_EU)))
+chain_visibility_seg_kernel(SceneView s, wc_solve q,
                                                                    const int* __restrict__ tiles, int tile_count,
                                                                    unsigned* __restrict__ masks, int words,
                                                                    const unsigned* __restrict__ super_masks,

@@ -477,10 +477,17 @@ __device__ __forceinline__ int wave_inclusive_scan(int v, int lane) {
     return v;
 }
 
-// occupancy target (waves per SIMD) of the adjoint; 1 leaves the register
-// budget to the compiler
+// Occupancy targets (waves per SIMD) of the adjoint phases. The line-of-sight /
+// reflection / transmission phase runs 2 waves (256 VGPRs and more private
+// memory, but its many short paths hide the latency: scaled-city step
+// 1434 -> 1372 ms, canyon 3.26 -> 3.30 ms); the diffraction phase and the
+// geometry adjoint keep the compiler's choice (1: the diffraction adjoint at 2
+// waves took the canyon's 1.13 ms to 1.43 ms).
 #ifndef WC_VJP_WAVES_PER_EU
 #define WC_VJP_WAVES_PER_EU 1
+#endif
+#ifndef WC_VJP_MAIN_WAVES_PER_EU
+#define WC_VJP_MAIN_WAVES_PER_EU 2
 #endif
 
 // Work items of the adjoint per tile: the tile's flattened path list is cut
@@ -514,7 +521,8 @@ static_assert(kVjpMainSlices >= 1 && kVjpMainSlices <= kVjpSlices, "main-phase s
 // kPhaseMain writes the item's partial vector, kPhaseDiff adds to it. With
 // kGeometry the same dealing drives geometry_path instead (no partials).
 template <int kPhase, bool kGeometry = false>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VJP_WAVES_PER_EU)))
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(kPhase == 0 && !kGeometry ? WC_VJP_MAIN_WAVES_PER_EU : WC_VJP_WAVES_PER_EU)))
 vjp_kernel(SceneView s, wc_solve q, wc_adjoint ga, const int* __restrict__ tiles, int tile_count, GeoGrad geo,
            int slices) {
     const int lane = threadIdx.x & 63;

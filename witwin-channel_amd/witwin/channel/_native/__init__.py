@@ -210,7 +210,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_vjp_partial_rows": (c_int, [c_int]),
     "wc_vjp_reduce": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     "wc_scene_refit": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcRefit), c_void_p]),
-    "wc_fixed_rows_evaluate": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcFixedRows), ctypes.c_float,
+    "wc_fixed_rows_evaluate": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcFixedRows), ctypes.c_double,
                                        c_void_p]),
     "wc_radiomap_vjp_geometry": (c_int, [ctypes.POINTER(WcScene), ctypes.POINTER(WcSolve),
                                          ctypes.POINTER(WcAdjoint), c_void_p, c_int, c_void_p, c_void_p,

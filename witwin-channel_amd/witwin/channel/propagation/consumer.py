@@ -422,7 +422,7 @@ def _free_space_field(source: torch.Tensor, target: torch.Tensor, polarization: 
     p = _native.ptr
     if n:
         _native.call("wc_field_free_space", n, p(source), p(target), p(power), p(polarization), p(polarization),
-                     ctypes.c_float(frequency_hz), p(torch.view_as_real(out["field_vector"])),
+                     float(frequency_hz), p(torch.view_as_real(out["field_vector"])),
                      p(torch.view_as_real(out["coefficient"])), p(torch.view_as_real(out["path_field"])),
                      p(out["path_gain"]), p(out["path_length"]), p(out["delay"]), p(out["direction"]),
                      _native.stream_handle(device))
@@ -800,7 +800,7 @@ def _replay_rows(dscene: Any, topology: PropagationTopology, source: torch.Tenso
         field_xyz=p(torch.view_as_real(out["field_xyz"])), path_length=p(out["path_length"]), delay=p(out["delay"]),
         direction=p(out["direction"]), positions=p(out["positions"]) if width else None,
         normals=p(out["normals"]) if width else None)
-    _native.call("wc_fixed_rows_evaluate", dscene.ref(), ctypes.byref(view), ctypes.c_float(frequency_hz),
+    _native.call("wc_fixed_rows_evaluate", dscene.ref(), ctypes.byref(view), float(frequency_hz),
                  _native.stream_handle(device))
     return out
 
