@@ -565,6 +565,16 @@ int wc_field_free_space(int count, const float* source, const float* target, con
                         float* field_vector, float* coefficient, float* path_field, float* path_gain,
                         float* path_length, float* delay, float* direction, void* stream);
 
+/* Line-of-sight export of every (transmitter, receiver) pair, replacing
+ * path_los_export (reference kernels/topology.py:84, los.cu:74-125): pair
+ * index rx * T + tx; path_length = max(|tx - rx|, 1e-6), delay = length / c,
+ * gain = P (lambda / (4 pi d))^2 (|p|^2 - (p . k)^2); path_gain_matrix [T, R].
+ * No visibility test (the caller gates the pairs). */
+int wc_path_los_export(int tx_count, int rx_count, const float* tx_positions, const float* tx_power,
+                       const float* rx_positions, const float* tx_polarizations, double frequency_hz, int* tx_id,
+                       int* rx_id, float* path_length, float* delay, float* path_gain, float* path_gain_matrix,
+                       void* stream);
+
 int wc_field_project_complex3(int count, const float* field_vector, const float* direction,
                               const float* rx_polarization, float* coefficient, float* path_gain, void* stream);
 

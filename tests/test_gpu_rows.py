@@ -615,3 +615,35 @@ def test_ad_forms_carry_the_companion_derivatives(device):
         tangent = fwAD.unpack_dual(res["coefficient"]).tangent.clone()
     ref = F.field_reflection_sequence_jvp(*base, frequency_hz=3.5e9, tangent_eps_r=t_eps)["coefficient"]
     torch.testing.assert_close(tangent, ref)
+
+
+def test_path_los_export_matches_closed_form(device):
+    _say("case: path_los_export, 7 Tx x 33 Rx against the float64 closed form")
+    from witwin.channel.kernels.topology import path_los_export
+
+    rng = np.random.default_rng(31)
+    tx = rng.uniform(-20, 20, (7, 3))
+    rx = rng.uniform(-20, 20, (33, 3))
+    rx[0] = tx[0]  # a coincident pair: the distance clamps at 1e-6 m
+    pol = _units(rng, 7)
+    power = rng.uniform(0.5, 2.0, 7)
+    f = 5.2e9
+    t = lambda a: torch.as_tensor(np.asarray(a, np.float32)).to(device)  # noqa: E731
+    out = path_los_export(t(tx), t(power), t(rx), t(pol), frequency_hz=f)
+    tx32, rx32, pol32, p32 = (np.asarray(a, np.float32).astype(np.float64) for a in (tx, rx, pol, power))
+    pair = np.arange(7 * 33)
+    ti, ri = pair % 7, pair // 7
+    off = tx32[ti] - rx32[ri]
+    d = np.maximum(np.linalg.norm(off, axis=1), 1e-6)
+    k = off / d[:, None]
+    pattern = np.maximum((pol32[ti] ** 2).sum(1) - (pol32[ti] * k).sum(1) ** 2, 0.0)
+    gain = p32[ti] * (C0 / f / (4 * np.pi * d)) ** 2 * pattern
+    assert np.array_equal(out["tx_id"].cpu().numpy(), ti) and np.array_equal(out["rx_id"].cpu().numpy(), ri)
+    np.testing.assert_allclose(out["path_length_m"].cpu().numpy(), d, rtol=1e-6)
+    np.testing.assert_allclose(out["delay_s"].cpu().numpy(), d / C0, rtol=1e-6)
+    # |p|^2 - (p . k)^2 cancels in float32 when p is nearly along k: the bound
+    # carries a term of a few ulp of the uncancelled P (lambda / 4 pi d)^2 |p|^2
+    whole = p32[ti] * (C0 / f / (4 * np.pi * d)) ** 2 * (pol32[ti] ** 2).sum(1)
+    err = np.abs(out["path_gain"].cpu().numpy().astype(np.float64) - gain)
+    assert np.all(err <= 2e-5 * gain + 1e-6 * whole), float((err / (gain + whole)).max())
+    np.testing.assert_array_equal(out["path_gain_matrix"].cpu().numpy()[ti, ri], out["path_gain"].cpu().numpy())

@@ -106,6 +106,36 @@ __global__ void reflection_sequence_kernel(int count, int depth, const float* __
     }
 }
 
+// ---- line-of-sight export of every (transmitter, receiver) pair -------------------
+// pair = rx * T + tx; gain = P (lambda / (4 pi d))^2 (|p|^2 - (p . k)^2), d
+// clamped at 1e-6 m; no visibility test (reference los.cu:74-125).
+__global__ void los_export_kernel(long long pairs, int tx_count, int rx_count, const float* __restrict__ tx_pos,
+                                  const float* __restrict__ tx_power, const float* __restrict__ rx_pos,
+                                  const float* __restrict__ tx_pol, float scale, int* __restrict__ tx_id,
+                                  int* __restrict__ rx_id, float* __restrict__ path_length, float* __restrict__ delay,
+                                  float* __restrict__ path_gain, float* __restrict__ gain_matrix) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < pairs;
+         i += (long long)blockDim.x * gridDim.x) {
+        const int r = (int)(i / tx_count);
+        const int t = (int)(i - (long long)r * tx_count);
+        const V3<float> a = ld3(tx_pos + 3 * t), b = ld3(rx_pos + 3 * r);
+        const V3<float> off = a - b;
+        const float d = fmaxf(norm(off), 1.0e-6f);
+        const V3<float> k = off * (1.0f / d);
+        const V3<float> p = ld3(tx_pol + 3 * t);
+        const float pk = dot(p, k);
+        const float pattern = fmaxf(dot(p, p) - pk * pk, 0.0f);
+        const float g = scale / d;
+        const float gain = tx_power[t] * g * g * pattern;
+        tx_id[i] = t;
+        rx_id[i] = r;
+        path_length[i] = d;
+        delay[i] = d / (float)kC0;
+        path_gain[i] = gain;
+        gain_matrix[(size_t)t * rx_count + r] = gain;
+    }
+}
+
 // ---- deterministic accumulation -------------------------------------------------
 
 __device__ __forceinline__ int accum_slot(int component) {
@@ -218,6 +248,22 @@ extern "C" int wc_field_free_space(int count, const float* source, const float* 
                        coefficient, path_field, path_gain, path_length, delay, direction);
     const hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : wc_fail_hip("wc_field_free_space", err);
+}
+
+extern "C" int wc_path_los_export(int tx_count, int rx_count, const float* tx_positions, const float* tx_power,
+                                  const float* rx_positions, const float* tx_polarizations, double frequency_hz,
+                                  int* tx_id, int* rx_id, float* path_length, float* delay, float* path_gain,
+                                  float* path_gain_matrix, void* stream) {
+    if (tx_count < 0 || rx_count < 0) return wc_fail("wc_path_los_export: counts must be non-negative");
+    if (!(frequency_hz > 0.0)) return wc_fail("frequency_hz must be positive");
+    const long long pairs = (long long)tx_count * rx_count;
+    if (pairs == 0) return 0;
+    const float scale = (float)(kC0 / frequency_hz / (4.0 * kPi));
+    hipLaunchKernelGGL(los_export_kernel, dim3(blocks_for(pairs, 256)), dim3(256), 0, (hipStream_t)stream, pairs,
+                       tx_count, rx_count, tx_positions, tx_power, rx_positions, tx_polarizations, scale, tx_id, rx_id,
+                       path_length, delay, path_gain, path_gain_matrix);
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : wc_fail_hip("wc_path_los_export", err);
 }
 
 extern "C" int wc_field_project_complex3(int count, const float* field_vector, const float* direction,

@@ -242,6 +242,7 @@ _SIGNATURES: dict[str, tuple[Any, list[Any]]] = {
     "wc_field_free_space": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_double,
                                     c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                     c_void_p]),
+    "wc_path_los_export": (c_int, [c_int, c_int] + [c_void_p] * 4 + [ctypes.c_double] + [c_void_p] * 7),
     "wc_field_project_complex3": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "wc_field_reflection_sequence": (c_int, [c_int, c_int] + [c_void_p] * 12 + [ctypes.c_double] + [c_void_p] * 8),
     "wc_coupled_candidates_per_cell": (ctypes.c_longlong, [c_int, c_int, c_int]),
