@@ -23,6 +23,7 @@ typedef void* hipStream_t;
 
 struct float4 { float x, y, z, w; };
 struct int4 { int x, y, z, w; };
+struct uint4 { unsigned x, y, z, w; };
 struct float2 { float x, y; };
 inline float4 make_float4(float x, float y, float z, float w) { return {x, y, z, w}; }
 inline int4 make_int4(int x, int y, int z, int w) { return {x, y, z, w}; }

@@ -535,6 +535,27 @@ def test_wide_node_levels_put_children_before_parents():
                     assert level_of[int(c)] < level_of[int(n)]
 
 
+@pytest.mark.parametrize("which", ["canyon", "heightfield", "one", "two"])
+def test_wide_tree_top_levels_take_the_first_ids(which):
+    """The walk kernels stage wide nodes 0 .. 340 in LDS: they must be the
+    tree's first 341 nodes in breadth-first order from the root (children in
+    slot order), so every walk's top levels come from the staged block."""
+    compiled = _build_case(which)
+    nodes, _, _ = _sah_tables(compiled)
+    bfs = [0]
+    i = 0
+    while i < len(bfs) and len(bfs) < 341:
+        _, _, child = _wide_children(nodes, bfs[i])
+        bfs += [int(c) for c in child if c >= 0][:341 - len(bfs)]
+        i += 1
+    assert bfs == list(range(len(bfs)))
+    reach, i = [0], 0
+    while i < len(reach):
+        reach += [int(c) for c in _wide_children(nodes, reach[i])[2] if c >= 0]
+        i += 1
+    assert len(bfs) == min(341, len(reach))
+
+
 def _slab_np(o, inv, lo, hi, tmin, tmax):
     t0, t1 = (lo - o) * inv, (hi - o) * inv
     tn = max(np.minimum(t0, t1).max(), tmin)

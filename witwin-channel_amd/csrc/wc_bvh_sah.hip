@@ -335,6 +335,40 @@ struct Widener {
     }
 };
 
+// Renumbers a wide tree (emitted depth-first, root at 0) so that its first
+// kTopNodes nodes in breadth-first order take ids 0 .. kTopNodes - 1, the
+// block the walk kernels stage in LDS; every other node keeps its depth-first
+// place after them. Child links are remapped; the tree itself is unchanged.
+void top_levels_first(std::vector<WideNode>& nodes) {
+    const int n = (int)nodes.size();
+    std::vector<int> bfs;
+    bfs.reserve(kTopNodes);
+    bfs.push_back(0);
+    for (size_t i = 0; i < bfs.size() && (int)bfs.size() < kTopNodes; ++i) {
+        const WideNode& w = nodes[bfs[i]];
+        const int count = (int)(w.exps >> 24);
+        const int kids[4] = {w.child.x, w.child.y, w.child.z, w.child.w};
+        for (int k = 0; k < count && (int)bfs.size() < kTopNodes; ++k)
+            if (kids[k] >= 0) bfs.push_back(kids[k]);
+    }
+    std::vector<int> id(n, -1);
+    for (size_t i = 0; i < bfs.size(); ++i) id[bfs[i]] = (int)i;
+    int next = (int)bfs.size();
+    for (int v = 0; v < n; ++v)
+        if (id[v] < 0) id[v] = next++;
+    std::vector<WideNode> renumbered(n);
+    for (int v = 0; v < n; ++v) {
+        WideNode w = nodes[v];
+        const int count = (int)(w.exps >> 24);
+        int kids[4] = {w.child.x, w.child.y, w.child.z, w.child.w};
+        for (int k = 0; k < count; ++k)
+            if (kids[k] >= 0) kids[k] = id[kids[k]];
+        w.child = make_int4(kids[0], kids[1], kids[2], kids[3]);
+        renumbered[id[v]] = w;
+    }
+    nodes.swap(renumbered);
+}
+
 // Collapses a binary tree of `face_count` leaves into wide nodes; returns the
 // wide node count or -1 (malformed tree, or no collapse fits the stack).
 int widen(const Node* bin, int face_count, std::vector<WideNode>& out) {
@@ -356,6 +390,7 @@ int widen(const Node* bin, int face_count, std::vector<WideNode>& out) {
         if (w.bad) return -1;
         if (w.stack_need <= kWideStackSize) {
             out.swap(w.out);
+            top_levels_first(out);
             return (int)out.size();
         }
     }

@@ -130,7 +130,7 @@ __device__ inline int face_containing(const SceneView& s, int g, const V3<float>
 // so the per-bounce arrays stay in registers.
 // Visibility of the D+1 legs of a reflection chain through its hit points
 // (each leg ignores the groups it starts and ends on).
-template <int D>
+template <int D, bool kTop = false>
 __device__ inline bool chain_legs_clear(const SceneView& s, const int* groups, const V3<double>* hits,
                                         const V3<float>& tx, const V3<float>& rx) {
     V3<float> a = tx;
@@ -144,7 +144,7 @@ __device__ inline bool chain_legs_clear(const SceneView& s, const int* groups, c
         if (len > 2.0f * kSegmentBias) {
             const float eps = kSegmentBias / len;
             const RayPre r = ray_setup(a.x, a.y, a.z, seg.x, seg.y, seg.z);
-            if (occluded(s, r, eps, 1.0f - eps, ga, gb)) return false;
+            if (occluded<kTop>(s, r, eps, 1.0f - eps, ga, gb)) return false;
         }
         a = b;
         ga = gb;
@@ -152,7 +152,7 @@ __device__ inline bool chain_legs_clear(const SceneView& s, const int* groups, c
     return true;
 }
 
-template <int D, bool kLegs = true>
+template <int D, bool kLegs = true, bool kTop = false>
 __device__ inline bool solve_chain(const SceneView& s, const int* groups, const double* images, const V3<float>& tx,
                                    const V3<float>& rx, V3<double>* hits, int* faces) {
     V3<double> next = to_d(rx);
@@ -174,7 +174,7 @@ __device__ inline bool solve_chain(const SceneView& s, const int* groups, const 
         next = h;
     }
     if (!kLegs) return true;  // leg visibility already established (chain masks)
-    return chain_legs_clear<D>(s, groups, hits, tx, rx);
+    return chain_legs_clear<D, kTop>(s, groups, hits, tx, rx);
 }
 
 // Field of a reflection chain with the interaction points fixed: the Jones

@@ -107,6 +107,14 @@ __device__ __forceinline__ void reflect_depth(const SceneView& s, const wc_solve
     }
 }
 
+// Visibility kernels stage the tree's top levels in LDS (stage_top_nodes)
+// and walk with them; 0 walks every node from HBM.
+#ifndef WC_TOP_NODES_LDS
+#define WC_TOP_NODES_LDS 1
+#endif
+constexpr bool kStageTop = WC_TOP_NODES_LDS != 0;
+
+template <bool kTop = false>
 __device__ __forceinline__ bool segment_clear(const SceneView& s, const V3<float>& a, const V3<float>& b, int ig0,
                                               int ig1) {
     const V3<float> d = b - a;
@@ -114,12 +122,13 @@ __device__ __forceinline__ bool segment_clear(const SceneView& s, const V3<float
     if (!(len > 2.0f * kSegmentBias)) return true;
     const float eps = kSegmentBias / len;
     const RayPre r = ray_setup(a.x, a.y, a.z, d.x, d.y, d.z);
-    return !occluded(s, r, eps, 1.0f - eps, ig0, ig1);
+    return !occluded<kTop>(s, r, eps, 1.0f - eps, ig0, ig1);
 }
 
 // Visibility of a diffraction candidate at the stationary point clamped to
 // the finite edge; the receiver leg is tested first (street-level receivers
 // see fewer edges than an elevated transmitter).
+template <bool kTop = false>
 __device__ __forceinline__ bool edge_visible(const SceneView& s, const Edge& e, const DiffGeometry& g,
                                              const V3<float>& tx, const V3<float>& rx) {
     const double tc = fmin(fmax(g.tstar, 0.0), (double)e.length);
@@ -129,7 +138,7 @@ __device__ __forceinline__ bool edge_visible(const SceneView& s, const Edge& e, 
     for (int leg = 0; leg < 2; ++leg) {
         const V3<float> a = leg == 0 ? qc : tx;
         const V3<float> b = leg == 0 ? rx : qc;
-        if (!segment_clear(s, a, b, e.group0, e.group1)) return false;
+        if (!segment_clear<kTop>(s, a, b, e.group0, e.group1)) return false;
     }
     return true;
 }
@@ -884,12 +893,12 @@ __global__ void __launch_bounds__(256) diff_sum_kernel(wc_solve q, wc_radiomap_o
 // surviving chains exactly. Only geometry and traversal live here, so the
 // kernel keeps many wavefronts per SIMD in flight; the reflection field pass
 // then visits the valid chains only.
-template <int D>
+template <int D, bool kTop = false>
 __device__ __forceinline__ bool chain_valid(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
                                             const V3<float>& rx) {
     V3<double> hd[D];
     int faces[D];
-    return solve_chain<D, true>(s, q.chain_groups + c * kMaxDepth, q.chain_images + c * kMaxDepth * 3, tx, rx, hd,
+    return solve_chain<D, true, kTop>(s, q.chain_groups + c * kMaxDepth, q.chain_images + c * kMaxDepth * 3, tx, rx, hd,
                                 faces);
 }
 
@@ -1056,6 +1065,7 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
                                                                int tile_count, unsigned* __restrict__ masks,
                                                                int words, const unsigned* __restrict__ super_masks, long long wave_base,
                                                                unsigned* __restrict__ touched) {
+    if (kStageTop) stage_top_nodes(s);
     const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)tile_count * words) return;
     const int tile = (int)(wave / words);
@@ -1102,11 +1112,11 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
         while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
         bool ok;
         switch (d) {
-            case 1: ok = chain_valid<1>(s, q, c, tx, rx); break;
-            case 2: ok = chain_valid<2>(s, q, c, tx, rx); break;
-            case 3: ok = chain_valid<3>(s, q, c, tx, rx); break;
-            case 4: ok = chain_valid<4>(s, q, c, tx, rx); break;
-            default: ok = chain_valid<5>(s, q, c, tx, rx); break;
+            case 1: ok = chain_valid<1, kStageTop>(s, q, c, tx, rx); break;
+            case 2: ok = chain_valid<2, kStageTop>(s, q, c, tx, rx); break;
+            case 3: ok = chain_valid<3, kStageTop>(s, q, c, tx, rx); break;
+            case 4: ok = chain_valid<4, kStageTop>(s, q, c, tx, rx); break;
+            default: ok = chain_valid<5, kStageTop>(s, q, c, tx, rx); break;
         }
         if (ok) bits |= 1u << b;
     }
@@ -1118,7 +1128,7 @@ __global__ void __launch_bounds__(256) chain_visibility_kernel(SceneView s, wc_s
 template <int D>
 __device__ __forceinline__ bool chain_lands(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
                                             const V3<float>& rx);
-template <int D>
+template <int D, bool kTop = false>
 __device__ __forceinline__ bool chain_legs(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
                                            const V3<float>& rx);
 
@@ -1144,6 +1154,7 @@ chain_visibility_seg_kernel(SceneView s, wc_solve q,
                                                                    const unsigned* __restrict__ super_masks,
                                                                    long long wave_base,
                                                                    unsigned* __restrict__ touched) {
+    if (kStageTop) stage_top_nodes(s);
     const int segs = (words + 63) >> 6;
     const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)tile_count * segs) return;
@@ -1194,11 +1205,11 @@ chain_visibility_seg_kernel(SceneView s, wc_solve q,
             if (!live) continue;
             bool ok;
             switch (d) {
-                case 1: ok = chain_valid<1>(s, q, c, tx, rx); break;
-                case 2: ok = chain_valid<2>(s, q, c, tx, rx); break;
-                case 3: ok = chain_valid<3>(s, q, c, tx, rx); break;
-                case 4: ok = chain_valid<4>(s, q, c, tx, rx); break;
-                default: ok = chain_valid<5>(s, q, c, tx, rx); break;
+                case 1: ok = chain_valid<1, kStageTop>(s, q, c, tx, rx); break;
+                case 2: ok = chain_valid<2, kStageTop>(s, q, c, tx, rx); break;
+                case 3: ok = chain_valid<3, kStageTop>(s, q, c, tx, rx); break;
+                case 4: ok = chain_valid<4, kStageTop>(s, q, c, tx, rx); break;
+                default: ok = chain_valid<5, kStageTop>(s, q, c, tx, rx); break;
             }
             if (ok) bits |= 1u << bb;
         }
@@ -1224,6 +1235,7 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_queue_kernel(SceneVi
                                                                          unsigned* __restrict__ touched) {
     __shared__ unsigned queue[4][128];
     __shared__ float4 lane_rx[4][64];
+    if (kStageTop) stage_top_nodes(s);
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int segs = (words + 63) >> 6;
     const long long wave = wave_base + (long long)blockIdx.x * 4 + wid;
@@ -1276,11 +1288,11 @@ __global__ void __launch_bounds__(256) chain_visibility_seg_queue_kernel(SceneVi
             while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
             bool ok;
             switch (d) {
-                case 1: ok = chain_legs<1>(s, q, c, tx, orx); break;
-                case 2: ok = chain_legs<2>(s, q, c, tx, orx); break;
-                case 3: ok = chain_legs<3>(s, q, c, tx, orx); break;
-                case 4: ok = chain_legs<4>(s, q, c, tx, orx); break;
-                default: ok = chain_legs<5>(s, q, c, tx, orx); break;
+                case 1: ok = chain_legs<1, kStageTop>(s, q, c, tx, orx); break;
+                case 2: ok = chain_legs<2, kStageTop>(s, q, c, tx, orx); break;
+                case 3: ok = chain_legs<3, kStageTop>(s, q, c, tx, orx); break;
+                case 4: ok = chain_legs<4, kStageTop>(s, q, c, tx, orx); break;
+                default: ok = chain_legs<5, kStageTop>(s, q, c, tx, orx); break;
             }
             if (ok) {
                 atomicOr(&masks[(size_t)chunk * q.total_cells + __float_as_int(r4.w)], 1u << bb);
@@ -1350,14 +1362,14 @@ __device__ __forceinline__ bool chain_lands(const SceneView& s, const wc_solve& 
                                  faces);
 }
 
-template <int D>
+template <int D, bool kTop>
 __device__ __forceinline__ bool chain_legs(const SceneView& s, const wc_solve& q, int c, const V3<float>& tx,
                                            const V3<float>& rx) {
     V3<double> hd[D];
     int faces[D];
     const int* groups = q.chain_groups + c * kMaxDepth;
     solve_chain<D, false>(s, groups, q.chain_images + c * kMaxDepth * 3, tx, rx, hd, faces);
-    return chain_legs_clear<D>(s, groups, hd, tx, rx);
+    return chain_legs_clear<D, kTop>(s, groups, hd, tx, rx);
 }
 
 __global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s, wc_solve q,
@@ -1368,6 +1380,7 @@ __global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s
     __shared__ unsigned short queue[4][128];
     __shared__ float4 lane_rx[4][64];
     __shared__ unsigned found[4][64];
+    if (kStageTop) stage_top_nodes(s);
     const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long long wave = wave_base + (long long)blockIdx.x * 4 + wid;
     const bool wave_live = wave < (long long)tile_count * words;
@@ -1458,11 +1471,11 @@ __global__ void __launch_bounds__(256) chain_visibility_queue_kernel(SceneView s
             while (c >= q.chain_offset[t * kMaxDepth + d]) ++d;
             bool ok;
             switch (d) {
-                case 1: ok = chain_legs<1>(s, q, c, tx, orx); break;
-                case 2: ok = chain_legs<2>(s, q, c, tx, orx); break;
-                case 3: ok = chain_legs<3>(s, q, c, tx, orx); break;
-                case 4: ok = chain_legs<4>(s, q, c, tx, orx); break;
-                default: ok = chain_legs<5>(s, q, c, tx, orx); break;
+                case 1: ok = chain_legs<1, kStageTop>(s, q, c, tx, orx); break;
+                case 2: ok = chain_legs<2, kStageTop>(s, q, c, tx, orx); break;
+                case 3: ok = chain_legs<3, kStageTop>(s, q, c, tx, orx); break;
+                case 4: ok = chain_legs<4, kStageTop>(s, q, c, tx, orx); break;
+                default: ok = chain_legs<5, kStageTop>(s, q, c, tx, orx); break;
             }
             if (ok) atomicOr(&found[wid][o], 1u << bit);
         }
@@ -1686,6 +1699,7 @@ __global__ void __launch_bounds__(256) edge_shadow_kernel(SceneView s, wc_solve 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VIS_WAVES_PER_EU)))
 diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, int words, long long waves,
                        const unsigned long long* __restrict__ shadow, long long wave_base) {
+    if (kStageTop) stage_top_nodes(s);
     const long long wave = wave_base + (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= waves) return;
     const int chunk = (int)(wave % words);
@@ -1712,7 +1726,7 @@ diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, in
             const int bin = min((int)(tc * (64.0 / (double)e.length)), 63);
             if ((sh >> bin) & 1ull) continue;
         }
-        if (edge_visible(s, e, g, tx, rx)) bits |= 1u << (ei - e0);
+        if (edge_visible<kStageTop>(s, e, g, tx, rx)) bits |= 1u << (ei - e0);
     }
     masks[(size_t)cell * words + chunk] = bits;
 }
@@ -1727,6 +1741,7 @@ diff_visibility_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, in
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WC_VIS_WAVES_PER_EU)))
 diff_visibility_queue_kernel(SceneView s, wc_solve q, unsigned* __restrict__ masks, int words, long long waves,
                        const unsigned long long* __restrict__ shadow, long long wave_base) {
+    if (kStageTop) stage_top_nodes(s);
     // per wavefront: the candidate queue (lane | edge << 6), each lane's
     // receiver and transmitter, and the visible bits
     __shared__ unsigned short queue[4][128];
@@ -1802,7 +1817,7 @@ diff_visibility_queue_kernel(SceneView s, wc_solve q, unsigned* __restrict__ mas
             const Edge e = s.edges[e0 + bit];
             DiffGeometry g;
             diffraction_geometry(e, otx, orx, g, false);
-            if (edge_visible(s, e, g, otx, orx)) atomicOr(&found[wid][o], 1u << bit);
+            if (edge_visible<kStageTop>(s, e, g, otx, orx)) atomicOr(&found[wid][o], 1u << bit);
         }
     }
     __syncthreads();

@@ -109,6 +109,35 @@ __device__ __forceinline__ float grid_bound(unsigned packed, int k, float step, 
     return fmaf((float)((packed >> (8 * k)) & 0xFFu), step, origin);
 }
 
+// The tree's top levels (wide nodes 0 .. kTopNodes - 1, numbered breadth-first
+// by the host build) staged in LDS. Every walk opens at the root and a
+// wavefront's segments share their first levels, so these fetches repeat in
+// every lane of every walk; from LDS they skip the vector memory path. Only a
+// kernel that calls stage_top_nodes before its first walk may walk with
+// kTop = true: the array holds nothing until then.
+__shared__ uint4 wc_top_words[kTopNodes * 4];
+
+// Cooperative copy of the top nodes into LDS; every thread of the workgroup
+// must reach it (it ends in a barrier), before any early return.
+__device__ __forceinline__ void stage_top_nodes(const SceneView& s) {
+    const int words = (s.node_count < kTopNodes ? s.node_count : kTopNodes) * 4;
+    const uint4* src = reinterpret_cast<const uint4*>(s.nodes);
+    for (int i = (int)threadIdx.x; i < words; i += (int)blockDim.x) wc_top_words[i] = src[i];
+    __syncthreads();
+}
+
+template <bool kTop>
+__device__ __forceinline__ WideNode fetch_node(const SceneView& s, int node) {
+    if (kTop && node < kTopNodes) {
+        WideNode nd;
+        uint4* w = reinterpret_cast<uint4*>(&nd);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) w[i] = wc_top_words[4 * node + i];
+        return nd;
+    }
+    return s.nodes[node];
+}
+
 // Walks the 4-wide tree over the segment (tmin, limit), handing every leaf
 // whose box it crosses to `leaf` (which returns true to end the walk). Of the
 // internal children a node's segment crosses, the nearest is entered next and
@@ -117,14 +146,14 @@ __device__ __forceinline__ float grid_bound(unsigned packed, int k, float step, 
 // triangle per lane per round. `limit` is read at every box test: a
 // closest-hit leaf may shrink it. The stack never overflows: the host build
 // checks the tree's worst-case stack need against kWideStackSize.
-template <class Leaf>
+template <bool kTop = false, class Leaf>
 __device__ __forceinline__ void walk_wide(const SceneView& s, const RayPre& r, float tmin, const float& limit,
                                           Leaf&& leaf) {
     int stack[kWideStackSize];
     int sp = 0;
     int node = 0;
     while (true) {
-        const WideNode nd = s.nodes[node];
+        const WideNode nd = fetch_node<kTop>(s, node);
         const float sx = grid_step(nd.exps, 0), sy = grid_step(nd.exps, 1), sz = grid_step(nd.exps, 2);
         const int count = (int)(nd.exps >> 24);
         int next = -1;
@@ -172,10 +201,11 @@ __device__ __forceinline__ void walk_wide(const SceneView& s, const RayPre& r, f
 
 // Any hit on the segment o + t*d, t in (tmin, tmax), skipping triangles of
 // groups ig0/ig1 (the surfaces the segment starts or ends on).
+template <bool kTop = false>
 __device__ inline bool occluded(const SceneView& s, const RayPre& r, float tmin, float tmax, int ig0, int ig1) {
     if (s.tri_count == 0) return false;
     bool blocked = false;
-    walk_wide(s, r, tmin, tmax, [&](const Tri& tri) {
+    walk_wide<kTop>(s, r, tmin, tmax, [&](const Tri& tri) {
         const int g = bits_i(tri.v0.w);
         blocked = g != ig0 && g != ig1 && tri_hit(r, tri, tmin, tmax) >= 0.0f;
         return blocked;
@@ -209,12 +239,13 @@ __device__ inline void closest_candidate(const RayPre& r, const Tri& tri, float 
     }
 }
 
+template <bool kTop = false>
 __device__ inline Hit closest_hit(const SceneView& s, const RayPre& r, float tmin, float tmax, int ignore) {
     Hit best{-1.0f, -1, -1};
     if (s.tri_count == 0) return best;
     float tbest = tmax;
     float limit = tmax;
-    walk_wide(s, r, tmin, limit, [&](const Tri& tri) {
+    walk_wide<kTop>(s, r, tmin, limit, [&](const Tri& tri) {
         closest_candidate(r, tri, tmin, ignore, tbest, best);
         // once a hit exists, boxes up to a few ulps past it still hold ties
         limit = best.face >= 0 ? tbest * kSlabWiden : tbest;
@@ -233,11 +264,12 @@ __device__ __forceinline__ bool hit_before(const Hit& a, const Hit& b) {
 // coplanar group (a segment crossing a wall through the shared diagonal of
 // its two triangles counts once, with the group's lowest face id). Returns
 // the number of distinct hits, or cap + 1 when there are more than `cap`.
+template <bool kTop = false>
 __device__ inline int gather_hits(const SceneView& s, const RayPre& r, float tmin, float tmax, int cap,
                                   Hit* hits) {
     int count = 0;
     if (s.tri_count == 0) return 0;
-    walk_wide(s, r, tmin, tmax, [&](const Tri& tri) {
+    walk_wide<kTop>(s, r, tmin, tmax, [&](const Tri& tri) {
         const float t = tri_hit(r, tri, tmin, tmax);
         if (t < 0.0f) return false;
         const int g = bits_i(tri.v0.w);
