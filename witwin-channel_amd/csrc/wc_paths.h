@@ -122,22 +122,22 @@ __device__ inline int face_containing(const SceneView& s, int g, const V3<float>
     return -1;
 }
 
-// Validates one image-method chain for receiver rx: back-projects from the
-// receiver through the images, requires each hit strictly between its image
-// and the next point and inside a face of its group, then (kLegs) tests every
-// leg for occlusion (legs skip the groups they start and end on). On success
-// fills hits/faces and returns true. The depth D is a compile-time constant
-// so the per-bounce arrays stay in registers.
 // Visibility of the D+1 legs of a reflection chain through its hit points
-// (each leg ignores the groups it starts and ends on).
+// (each leg ignores the groups it starts and ends on). The legs are tested
+// from the receiver back to the transmitter: the verdict is the AND of the
+// legs, and a receiver near the ground is the end most often blocked, so the
+// first walk most often ends the test (scaled city, chain validity pass per
+// batch: 50.0 ms transmitter leg first, 33.3 ms receiver leg first, 33.9 ms
+// receiver leg, then transmitter leg, then the middle legs).
 template <int D, bool kTop = false>
 __device__ inline bool chain_legs_clear(const SceneView& s, const int* groups, const V3<double>* hits,
                                         const V3<float>& tx, const V3<float>& rx) {
-    V3<float> a = tx;
-    int ga = -1;
 #pragma unroll
-    for (int k = 0; k <= D; ++k) {
+    for (int j = 0; j <= D; ++j) {
+        const int k = D - j;  // leg k runs from point k to point k + 1
+        const V3<float> a = k == 0 ? tx : to_f(hits[k - 1]);
         const V3<float> b = k < D ? to_f(hits[k]) : rx;
+        const int ga = k == 0 ? -1 : groups[k - 1];
         const int gb = k < D ? groups[k] : -1;
         const V3<float> seg = b - a;
         const float len = norm(seg);
@@ -146,12 +146,16 @@ __device__ inline bool chain_legs_clear(const SceneView& s, const int* groups, c
             const RayPre r = ray_setup(a.x, a.y, a.z, seg.x, seg.y, seg.z);
             if (occluded<kTop>(s, r, eps, 1.0f - eps, ga, gb)) return false;
         }
-        a = b;
-        ga = gb;
     }
     return true;
 }
 
+// Validates one image-method chain for receiver rx: back-projects from the
+// receiver through the images, requires each hit strictly between its image
+// and the next point and inside a face of its group, then (kLegs) tests every
+// leg for occlusion (legs skip the groups they start and end on). On success
+// fills hits/faces and returns true. The depth D is a compile-time constant
+// so the per-bounce arrays stay in registers.
 template <int D, bool kLegs = true, bool kTop = false>
 __device__ inline bool solve_chain(const SceneView& s, const int* groups, const double* images, const V3<float>& tx,
                                    const V3<float>& rx, V3<double>* hits, int* faces) {
