@@ -258,3 +258,78 @@ def test_replay_slots_and_wideband_columns(device):
                                             topology=prepared, response="scalar_transport", ad_mode="none")
         ref = consumer.reevaluate(other, req)
         assert torch.equal(cols[:, j].cpu(), ref.paths.transport.coefficient.cpu())
+
+
+def _los_closed_form(src, snk, p_tx, p_rx, power, freq, response):
+    """Free-space row in float64 torch (autograd through positions and
+    frequency): E = (p_tx - (p_tx.k)k) / (2 k0 L) e^{-j k0 L}; the scalar
+    coefficient projects E on p_rx - (p_rx.k)k and both carry sqrt(P)."""
+    off = snk - src
+    length = off.norm(dim=1)
+    k = off / length[:, None]
+    k0 = 2.0 * np.pi * freq / C0
+    phase = torch.polar(torch.ones_like(length), -k0 * length) / (2.0 * k0 * length)
+    e_perp = p_tx - (p_tx * k).sum(1, keepdim=True) * k
+    amp = power.sqrt()
+    if response == "scalar_transport":
+        dip = (p_tx * p_rx).sum(1) - (p_tx * k).sum(1) * (p_rx * k).sum(1)
+        return dip * phase * amp
+    return e_perp * (phase * amp)[:, None]
+
+
+@pytest.mark.parametrize("response", ["scalar_transport", "complex3_transport"])
+def test_replay_los_rows_differentiate_endpoints_and_frequency(device, response):
+    _say(f"case: reevaluate ad_mode vjp / jvp, line-of-sight rows, {response}, canyon 2 sources x 10 sinks")
+    import torch.autograd.forward_ad as fwAD
+
+    rng = np.random.default_rng(23)
+    compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    tx = [workloads.URBAN_CANYON_TX, (52.7, 140.3, 12.0)]
+    ev, sources, sinks = _discover(device, tx, _street_sinks(rng, 10), compiled, comps=frozenset({"los"}), depth=1)
+    prepared = consumer.prepare_fixed_topology(ev.paths.topology)
+    src_pos = sources.positions_m.detach().clone().requires_grad_(True)
+    snk_pos = sinks.positions_m.detach().clone().requires_grad_(True)
+    freq = torch.tensor(28e9, dtype=torch.float64, device=device, requires_grad=True)
+
+    def run(ad_mode, s_pos, r_pos, f):
+        s_b = consumer.EndpointBatch(stable_ids=sources.stable_ids, positions_m=s_pos,
+                                     polarizations=sources.polarizations, powers_w=sources.powers_w)
+        r_b = consumer.EndpointBatch(stable_ids=sinks.stable_ids, positions_m=r_pos, polarizations=sinks.polarizations)
+        req = consumer.FixedTopologyRequest(sources=s_b, sinks=r_b, reference_frequency_hz=f, topology=prepared,
+                                            response=response, ad_mode=ad_mode)
+        out = consumer.reevaluate(compiled, req)
+        t = out.paths.transport
+        return out, (t.coefficient if response == "scalar_transport" else t.field)
+
+    plain, values_plain = run("none", sources.positions_m, sinks.positions_m, 28e9)
+    out, values = run("vjp", src_pos, snk_pos, freq)
+    valid = out.row_valid
+    assert valid is not None and int(valid.sum()) > 5
+    torch.testing.assert_close(values.detach(), values_plain, rtol=2e-5, atol=1e-12)
+    w = torch.as_tensor(rng.normal(size=tuple(values.shape) + (2,)), dtype=torch.float32, device=device)
+    loss = (torch.view_as_real(values) * w).sum()
+    loss.backward()
+    # float64 closed form over the rows that still see each other
+    topo = prepared.topology
+    si, ri = topo.source_index.long(), topo.sink_index.long()
+    s64 = sources.positions_m.detach().double().clone().requires_grad_(True)
+    r64 = sinks.positions_m.detach().double().clone().requires_grad_(True)
+    f64 = torch.tensor(28e9, dtype=torch.float64, device=device, requires_grad=True)
+    ref = _los_closed_form(s64[si], r64[ri], sources.polarizations.double()[si], sinks.polarizations.double()[ri],
+                           sources.powers_w.double()[si], f64, response)
+    mask = valid.reshape((-1,) + (1,) * (ref.ndim - 1)).double()
+    ref_loss = (torch.view_as_real(ref) * w.double() * mask[..., None]).sum()
+    ref_loss.backward()
+    for name, got, want in (("source", src_pos.grad, s64.grad), ("sink", snk_pos.grad, r64.grad),
+                            ("frequency", freq.grad.reshape(1), f64.grad.reshape(1))):
+        err = float((got.double() - want).norm() / want.norm().clamp_min(1e-300))
+        _say(f"  d loss / d {name}: relative error {err:.2e} (|grad| {float(want.norm()):.3e})")
+        assert err < 2e-3, name
+    # forward mode: the tangent of the loss along a source-position direction
+    direction = torch.as_tensor(rng.normal(size=tuple(src_pos.shape)), dtype=torch.float32, device=device)
+    with fwAD.dual_level():
+        dual = fwAD.make_dual(sources.positions_m.detach().clone(), direction)
+        _, v = run("jvp", dual, sinks.positions_m, 28e9)
+        tangent = fwAD.unpack_dual((torch.view_as_real(v) * w).sum()).tangent
+    want = float((s64.grad * direction.double()).sum())
+    assert abs(float(tangent) - want) <= 2e-3 * abs(want), (float(tangent), want)

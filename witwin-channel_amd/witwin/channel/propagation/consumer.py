@@ -19,11 +19,12 @@ the request's endpoints instead of the compiled scene's: rows are pair-major
 with ``pair_index = sink_index * source_count + source_index``. The Jones
 response is line-of-sight only, as in the reference; it excites the native
 free-space row operator with the two source basis vectors and projects onto
-the sink basis. Forward evaluation only: ``ad_mode`` other than ``"none"`` is
-refused (the differentiable route of this build is ``deterministic.solve``
-with ``ad_mode="vjp"``). ``reevaluate`` replays frozen line-of-sight and
-reflection rows at new endpoints in one native launch per frequency column
-(``wc_fixed_rows_evaluate``).
+the sink basis. ``evaluate`` is forward only (its differentiable route is
+``deterministic.solve`` with ``ad_mode="vjp"``). ``reevaluate`` replays
+frozen line-of-sight and reflection rows at new endpoints in one native launch
+per frequency column (``wc_fixed_rows_evaluate``); with ``ad_mode`` ``"vjp"``
+or ``"jvp"`` its line-of-sight rows take their values from the
+differentiable free-space row operator (reference consumer.py:3994-4017).
 """
 
 from __future__ import annotations
@@ -313,6 +314,10 @@ class PropagationCapabilities:
     fixed_topology_row_validity_components: frozenset[str] = frozenset({"los", "reflection"})
     wideband_responses: frozenset[str] = frozenset({"scalar_transport", "complex3_transport"})
     world_motions: frozenset[str] = WORLD_MOTIONS
+    # reevaluate differentiates line-of-sight rows (scalar and complex3
+    # transports) through the free-space row operator
+    fixed_topology_ad_modes: frozenset[str] = frozenset({"none", "jvp", "vjp"})
+    fixed_topology_ad_components: frozenset[str] = frozenset({"los"})
 
     def components_for(self, response: str) -> frozenset[str]:
         return dict(self.response_components)[response]
@@ -805,6 +810,26 @@ def _replay_rows(dscene: Any, topology: PropagationTopology, source: torch.Tenso
     return out
 
 
+def _los_rows_ad(out: dict[str, torch.Tensor], valid: torch.Tensor, source: torch.Tensor, target: torch.Tensor,
+                 tx_power: torch.Tensor, tx_pol: torch.Tensor, rx_pol: torch.Tensor, frequency: Any,
+                 frequency_value: float, offset: float = 0.0) -> dict[str, torch.Tensor]:
+    """Line-of-sight rows through the differentiable free-space operator
+    (reference consumer.py:3994-4017): derivatives reach the endpoint
+    positions and, when the reference frequency is a tensor, the frequency
+    (reverse or forward mode); power and polarizations enter as constants.
+    Rows the native replay found blocked keep zero transport."""
+    from witwin.channel.kernels import fields as field_kernels
+
+    f = frequency + offset if isinstance(frequency, torch.Tensor) else frequency_value + offset
+    rows = field_kernels.field_free_space_ad(source, target, tx_power.detach(), tx_pol.detach(), rx_pol.detach(),
+                                             frequency=f, frequency_value=frequency_value + offset)
+    keep = valid.reshape(-1, 1)
+    zero = torch.zeros((), dtype=rows["path_field"].dtype, device=source.device)
+    return {**out, "path_field": torch.where(valid, rows["path_field"], zero),
+            "field_xyz": torch.where(keep, rows["field_vector"], zero), "path_length": rows["path_length_m"],
+            "delay": rows["delay_s"], "direction": rows["direction"]}
+
+
 def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> FixedTopologyEvaluation:
     """Replay frozen rows at the request's endpoints without discovery or
     compaction (reference consumer.py:3965-4110): line-of-sight rows re-test
@@ -822,9 +847,9 @@ def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> 
     if abs(freq - compiled.frequency_hz) > 1e-6 * compiled.frequency_hz:
         raise ValueError(f"reference_frequency_hz {freq} does not match the compiled scene's "
                          f"{compiled.frequency_hz}")
-    if request.ad_mode not in _CAPABILITIES.ad_modes:
-        raise NotImplementedError(f"ad_mode {request.ad_mode!r} is not supported by this build; supported: "
-                                  f"{sorted(_CAPABILITIES.ad_modes)}")
+    if request.ad_mode not in _CAPABILITIES.fixed_topology_ad_modes:
+        raise NotImplementedError(f"ad_mode {request.ad_mode!r} is not supported by reevaluate; supported: "
+                                  f"{sorted(_CAPABILITIES.fixed_topology_ad_modes)}")
     topo = request.frozen_topology
     width = int(topo.primitive_sequence.shape[1])
     prepared = request.topology if isinstance(request.topology, PreparedFixedTopology) else None
@@ -840,6 +865,13 @@ def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> 
             raise ValueError("polarimetric_transport requires source and sink polarization_basis tensors")
         if components - {"los"}:
             raise NotImplementedError("polarimetric_transport replays line-of-sight rows only")
+    differentiable = request.ad_mode != "none"
+    if differentiable:
+        if request.response == "polarimetric_transport":
+            raise NotImplementedError(f"ad_mode {request.ad_mode!r} covers the scalar and complex3 transports")
+        if components - _CAPABILITIES.fixed_topology_ad_components:
+            raise NotImplementedError(f"ad_mode {request.ad_mode!r} replays line-of-sight rows only; the rows carry "
+                                      f"{sorted(components - _CAPABILITIES.fixed_topology_ad_components)}")
     if topo.primitive_sequence.device != request.sources.positions_m.device:
         raise ValueError("fixed topology and endpoint batches must share a device")
     device = _native.require_gpu()
@@ -866,6 +898,11 @@ def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> 
     rx_pol = sinks.polarizations[snk].to(torch.float32).contiguous()
     out = _replay_rows(dscene, topo, source, target, tx_power, tx_pol, rx_pol, compiled.frequency_hz)
     valid = out["valid"].bool()
+    if differentiable:
+        # the native replay decided which rows still see each other; the
+        # values of those rows come from the differentiable free-space operator
+        out = _los_rows_ad(out, valid, source, target, tx_power, tx_pol, rx_pol, request.reference_frequency_hz,
+                           compiled.frequency_hz)
     amplitude = torch.sqrt(torch.clamp(tx_power, min=0.0)).unsqueeze(-1)
     offsets = request.frequency_offsets_hz
     if slots > 1:
@@ -889,7 +926,11 @@ def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> 
             f = compiled.frequency_hz + df
             if not f > 0.0:
                 raise ValueError(f"frequency offset {df} gives a non-positive frequency")
-            columns.append(_replay_rows(dscene, topo, source, target, tx_power, tx_pol, rx_pol, f))
+            column = _replay_rows(dscene, topo, source, target, tx_power, tx_pol, rx_pol, f)
+            if differentiable:
+                column = _los_rows_ad(column, valid, source, target, tx_power, tx_pol, rx_pol,
+                                      request.reference_frequency_hz, compiled.frequency_hz, df)
+            columns.append(column)
     if request.response == "scalar_transport":
         payload = torch.stack([c["path_field"] for c in columns], dim=1) if columns else None
         transport: Any = ScalarTransport(coefficient=out["path_field"], coefficient_offsets=payload,
