@@ -333,3 +333,115 @@ def test_replay_los_rows_differentiate_endpoints_and_frequency(device, response)
         tangent = fwAD.unpack_dual((torch.view_as_real(v) * w).sum()).tangent
     want = float((s64.grad * direction.double()).sum())
     assert abs(float(tangent) - want) <= 2e-3 * abs(want), (float(tangent), want)
+
+
+def test_replay_reflection_rows_differentiate_endpoints(device):
+    """Reflection rows under ad_mode vjp / jvp: values equal the plain replay;
+    gradients with respect to source and sink positions and the frequency
+    equal central differences of the FP64 oracle's rows (line of sight:
+    oracle free_space; reflection: oracle reflection_sequence at interaction
+    points re-solved by the image method over the frozen faces' planes)."""
+    _say("case: reevaluate ad_mode vjp / jvp with reflection rows, canyon 2 sources x 8 sinks, depth 2")
+    import torch.autograd.forward_ad as fwAD
+
+    import oracle
+
+    rng = np.random.default_rng(24)
+    compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    tx = [workloads.URBAN_CANYON_TX, (52.7, 140.3, 12.0)]
+    ev, sources, sinks = _discover(device, tx, _street_sinks(rng, 8), compiled)
+    prepared = consumer.prepare_fixed_topology(ev.paths.topology)
+    topo = prepared.topology
+
+    def run(ad_mode, s_pos, r_pos, f):
+        s_b = consumer.EndpointBatch(stable_ids=sources.stable_ids, positions_m=s_pos,
+                                     polarizations=sources.polarizations, powers_w=sources.powers_w)
+        r_b = consumer.EndpointBatch(stable_ids=sinks.stable_ids, positions_m=r_pos, polarizations=sinks.polarizations)
+        req = consumer.FixedTopologyRequest(sources=s_b, sinks=r_b, reference_frequency_hz=f, topology=prepared,
+                                            response="scalar_transport", ad_mode=ad_mode)
+        return consumer.reevaluate(compiled, req)
+
+    plain = run("none", sources.positions_m, sinks.positions_m, 28e9)
+    src_pos = sources.positions_m.detach().clone().requires_grad_(True)
+    snk_pos = sinks.positions_m.detach().clone().requires_grad_(True)
+    freq = torch.tensor(28e9, dtype=torch.float64, device=device, requires_grad=True)
+    out = run("vjp", src_pos, snk_pos, freq)
+    coef = out.paths.transport.coefficient
+    valid = out.row_valid.cpu().numpy()
+    comp = topo.component_id.cpu().numpy()
+    assert valid.sum() > 10 and (comp[valid] == 1).sum() > 5
+    torch.testing.assert_close(coef.detach(), plain.paths.transport.coefficient, rtol=2e-5, atol=1e-12)
+    w = rng.normal(size=(len(comp), 2)) * valid[:, None]
+    loss = (torch.view_as_real(coef) * torch.as_tensor(w, dtype=torch.float32, device=device)).sum()
+    loss.backward()
+
+    # FP64 oracle loss over the valid rows
+    si, ri = topo.source_index.cpu().numpy(), topo.sink_index.cpu().numpy()
+    depth, seq = topo.depth.cpu().numpy(), topo.primitive_sequence.cpu().numpy()
+    p_tx, p_rx = sources.polarizations.cpu().double().numpy(), sinks.polarizations.cpu().double().numpy()
+    power = sources.powers_w.cpu().double().numpy()
+    mats = compiled.materials
+
+    def hits_of(a, b, planes):
+        images, img = [], a
+        for n, c in planes:
+            img = img - 2.0 * (img @ n - c) * n
+            images.append(img)
+        out_h, nxt = [None] * len(planes), b
+        for j in reversed(range(len(planes))):
+            n, c = planes[j]
+            s_i, s_n = images[j] @ n - c, nxt @ n - c
+            out_h[j] = images[j] + (nxt - images[j]) * (s_i / (s_i - s_n))
+            nxt = out_h[j]
+        return out_h
+
+    def oracle_loss(s_np, r_np, f):
+        total = 0.0
+        for row in np.nonzero(valid)[0]:
+            a, b = s_np[si[row]], r_np[ri[row]]
+            if comp[row] == 0:
+                pf = oracle.free_space(a, b, power[si[row]], p_tx[si[row]], p_rx[ri[row]], f)["path_field"]
+            else:
+                faces = seq[row, :depth[row]]
+                planes = [(compiled.group_plane[compiled.face_group[fc], :3].astype(np.float64),
+                           float(compiled.group_plane[compiled.face_group[fc], 3])) for fc in faces]
+                m = compiled.face_material[faces]
+                pf = oracle.reflection_sequence(a, b, hits_of(a, b, planes), [n for n, _ in planes], power[si[row]],
+                                                p_tx[si[row]], p_rx[ri[row]], mats.eps_r[m], mats.sigma_e[m],
+                                                mats.mu_r[m], mats.gain[m], mats.thickness_m[m], f)["path_field"]
+            total += w[row, 0] * pf.real + w[row, 1] * pf.imag
+        return total
+
+    s0 = sources.positions_m.cpu().double().numpy()
+    r0 = sinks.positions_m.cpu().double().numpy()
+    h = 1e-4
+
+    def fd_grad(which):
+        base = s0 if which == "source" else r0
+        g = np.zeros_like(base)
+        for idx in np.ndindex(*base.shape):
+            plus, minus = base.copy(), base.copy()
+            plus[idx] += h
+            minus[idx] -= h
+            args = (plus, r0) if which == "source" else (s0, plus)
+            args_m = (minus, r0) if which == "source" else (s0, minus)
+            g[idx] = (oracle_loss(*args, 28e9) - oracle_loss(*args_m, 28e9)) / (2 * h)
+        return g
+
+    g_src, g_snk = fd_grad("source"), fd_grad("sink")
+    g_f = (oracle_loss(s0, r0, 28e9 + 1e3) - oracle_loss(s0, r0, 28e9 - 1e3)) / 2e3
+    for name, got, want in (("source", src_pos.grad.cpu().double().numpy(), g_src),
+                            ("sink", snk_pos.grad.cpu().double().numpy(), g_snk),
+                            ("frequency", np.array([float(freq.grad)]), np.array([g_f]))):
+        err = float(np.linalg.norm(got - want) / max(np.linalg.norm(want), 1e-300))
+        _say(f"  d loss / d {name}: relative error vs FP64 oracle central differences {err:.2e}")
+        assert err < 2e-3, name
+    direction = rng.normal(size=r0.shape)
+    with fwAD.dual_level():
+        dual = fwAD.make_dual(sinks.positions_m.detach().clone(),
+                              torch.as_tensor(direction, dtype=torch.float32, device=device))
+        v = run("jvp", sources.positions_m, dual, 28e9).paths.transport.coefficient
+        tangent = float(fwAD.unpack_dual((torch.view_as_real(v) * torch.as_tensor(
+            w, dtype=torch.float32, device=device)).sum()).tangent)
+    want = float((g_snk * direction).sum())
+    assert abs(tangent - want) <= 2e-3 * abs(want), (tangent, want)

@@ -33,6 +33,7 @@ import ctypes
 from dataclasses import dataclass, field
 from typing import Any
 
+import numpy as np
 import torch
 
 from witwin.channel import _native
@@ -314,10 +315,10 @@ class PropagationCapabilities:
     fixed_topology_row_validity_components: frozenset[str] = frozenset({"los", "reflection"})
     wideband_responses: frozenset[str] = frozenset({"scalar_transport", "complex3_transport"})
     world_motions: frozenset[str] = WORLD_MOTIONS
-    # reevaluate differentiates line-of-sight rows (scalar and complex3
-    # transports) through the free-space row operator
+    # reevaluate differentiates line-of-sight and reflection rows (scalar and
+    # complex3 transports) through the free-space and reflection row operators
     fixed_topology_ad_modes: frozenset[str] = frozenset({"none", "jvp", "vjp"})
-    fixed_topology_ad_components: frozenset[str] = frozenset({"los"})
+    fixed_topology_ad_components: frozenset[str] = frozenset({"los", "reflection"})
 
     def components_for(self, response: str) -> frozenset[str]:
         return dict(self.response_components)[response]
@@ -810,24 +811,89 @@ def _replay_rows(dscene: Any, topology: PropagationTopology, source: torch.Tenso
     return out
 
 
-def _los_rows_ad(out: dict[str, torch.Tensor], valid: torch.Tensor, source: torch.Tensor, target: torch.Tensor,
-                 tx_power: torch.Tensor, tx_pol: torch.Tensor, rx_pol: torch.Tensor, frequency: Any,
-                 frequency_value: float, offset: float = 0.0) -> dict[str, torch.Tensor]:
-    """Line-of-sight rows through the differentiable free-space operator
-    (reference consumer.py:3994-4017): derivatives reach the endpoint
-    positions and, when the reference frequency is a tensor, the frequency
-    (reverse or forward mode); power and polarizations enter as constants.
-    Rows the native replay found blocked keep zero transport."""
+def _image_hits(source: torch.Tensor, target: torch.Tensor, plane: torch.Tensor) -> torch.Tensor:
+    """Interaction points of reflection rows over fixed planes, differentiable
+    in the endpoints (float64): the source mirrored through each plane in turn,
+    then the back-projection from the target through the images. ``plane`` is
+    [n, depth, 4] (unit normal, offset); returns [n, depth, 3]."""
+    normal, offset = plane[..., :3], plane[..., 3]
+    depth = int(plane.shape[1])
+    images = []
+    image = source
+    for j in range(depth):
+        dist = (image * normal[:, j]).sum(-1) - offset[:, j]
+        image = image - 2.0 * dist[:, None] * normal[:, j]
+        images.append(image)
+    hits: list[torch.Tensor] = [source] * depth
+    nxt = target
+    for j in reversed(range(depth)):
+        si = (images[j] * normal[:, j]).sum(-1) - offset[:, j]
+        sn = (nxt * normal[:, j]).sum(-1) - offset[:, j]
+        hits[j] = images[j] + (nxt - images[j]) * (si / (si - sn))[:, None]
+        nxt = hits[j]
+    return torch.stack(hits, dim=1)
+
+
+def _rows_ad(out: dict[str, torch.Tensor], valid: torch.Tensor, topo: PropagationTopology, compiled: CompiledScene,
+             source: torch.Tensor, target: torch.Tensor, tx_power: torch.Tensor, tx_pol: torch.Tensor,
+             rx_pol: torch.Tensor, frequency: Any, frequency_value: float,
+             offset: float = 0.0) -> dict[str, torch.Tensor]:
+    """Frozen rows through the differentiable row operators (reference
+    consumer.py:3994-4017 and the prepared route): line-of-sight rows through
+    ``field_free_space_ad``, reflection rows, grouped by depth, through
+    ``field_reflection_sequence_ad`` at interaction points re-solved by the
+    image method over their frozen faces' planes (``_image_hits``).
+    Derivatives reach the endpoint positions and, when the reference frequency
+    is a tensor, the frequency (reverse or forward mode); power,
+    polarizations, materials and planes enter as constants. Rows the native
+    replay found invalid keep zero transport."""
     from witwin.channel.kernels import fields as field_kernels
 
+    device = source.device
+    k = topo.row_count
     f = frequency + offset if isinstance(frequency, torch.Tensor) else frequency_value + offset
-    rows = field_kernels.field_free_space_ad(source, target, tx_power.detach(), tx_pol.detach(), rx_pol.detach(),
-                                             frequency=f, frequency_value=frequency_value + offset)
-    keep = valid.reshape(-1, 1)
-    zero = torch.zeros((), dtype=rows["path_field"].dtype, device=source.device)
-    return {**out, "path_field": torch.where(valid, rows["path_field"], zero),
-            "field_xyz": torch.where(keep, rows["field_vector"], zero), "path_length": rows["path_length_m"],
-            "delay": rows["delay_s"], "direction": rows["direction"]}
+    fv = frequency_value + offset
+    power, p_tx, p_rx = tx_power.detach(), tx_pol.detach(), rx_pol.detach()
+    component = topo.component_id.to(torch.int64)
+    depth = topo.depth.to(torch.int64)
+    pieces = []  # (row indices, operator outputs)
+    los = torch.nonzero(component == 0).flatten()
+    if los.numel():
+        pieces.append((los, field_kernels.field_free_space_ad(source[los], target[los], power[los], p_tx[los],
+                                                              p_rx[los], frequency=f, frequency_value=fv)))
+    refl = component == 1
+    if bool(refl.any()):
+        mats = compiled.materials
+        table = {name: torch.as_tensor(np.asarray(v, np.float32), device=device) for name, v in
+                 (("eps", mats.eps_r), ("sigma", mats.sigma_e), ("mu", mats.mu_r), ("gain", mats.gain),
+                  ("thickness", mats.thickness_m))}
+        face_group = torch.as_tensor(np.asarray(compiled.face_group, np.int64), device=device)
+        face_material = torch.as_tensor(np.asarray(compiled.face_material, np.int64), device=device)
+        group_plane = torch.as_tensor(np.asarray(compiled.group_plane, np.float64), device=device)
+        for d in sorted(set(depth[refl].tolist())):
+            rows = torch.nonzero(refl & (depth == d)).flatten()
+            faces = topo.primitive_sequence[rows, :d].to(torch.int64)
+            plane = group_plane[face_group[faces]]
+            hits = _image_hits(source[rows].double(), target[rows].double(), plane).float()
+            m = face_material[faces]
+            pieces.append((rows, field_kernels.field_reflection_sequence_ad(
+                source[rows], target[rows], hits.contiguous(), plane[..., :3].float().contiguous(), power[rows],
+                p_tx[rows], p_rx[rows], table["eps"][m], table["sigma"][m], table["mu"][m], table["gain"][m],
+                table["thickness"][m], frequency=f, frequency_value=fv)))
+
+    def assemble(key: str, like: torch.Tensor) -> torch.Tensor:
+        full = torch.zeros_like(like)
+        for rows, values in pieces:
+            full = full.index_copy(0, rows, values[key].to(like.dtype))
+        return full
+
+    path_field = assemble("path_field", out["path_field"])
+    field_xyz = assemble("field_vector", out["field_xyz"])
+    zero = torch.zeros((), dtype=path_field.dtype, device=device)
+    return {**out, "path_field": torch.where(valid, path_field, zero),
+            "field_xyz": torch.where(valid.reshape(-1, 1), field_xyz, zero),
+            "path_length": assemble("path_length_m", out["path_length"]), "delay": assemble("delay_s", out["delay"]),
+            "direction": assemble("direction", out["direction"])} if k else out
 
 
 def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> FixedTopologyEvaluation:
@@ -870,7 +936,7 @@ def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> 
         if request.response == "polarimetric_transport":
             raise NotImplementedError(f"ad_mode {request.ad_mode!r} covers the scalar and complex3 transports")
         if components - _CAPABILITIES.fixed_topology_ad_components:
-            raise NotImplementedError(f"ad_mode {request.ad_mode!r} replays line-of-sight rows only; the rows carry "
+            raise NotImplementedError(f"ad_mode {request.ad_mode!r} replays line-of-sight and reflection rows; the rows carry "
                                       f"{sorted(components - _CAPABILITIES.fixed_topology_ad_components)}")
     if topo.primitive_sequence.device != request.sources.positions_m.device:
         raise ValueError("fixed topology and endpoint batches must share a device")
@@ -901,8 +967,8 @@ def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> 
     if differentiable:
         # the native replay decided which rows still see each other; the
         # values of those rows come from the differentiable free-space operator
-        out = _los_rows_ad(out, valid, source, target, tx_power, tx_pol, rx_pol, request.reference_frequency_hz,
-                           compiled.frequency_hz)
+        out = _rows_ad(out, valid, topo, compiled, source, target, tx_power, tx_pol, rx_pol,
+                       request.reference_frequency_hz, compiled.frequency_hz)
     amplitude = torch.sqrt(torch.clamp(tx_power, min=0.0)).unsqueeze(-1)
     offsets = request.frequency_offsets_hz
     if slots > 1:
@@ -928,8 +994,8 @@ def reevaluate(compiled_scene: CompiledScene, request: FixedTopologyRequest) -> 
                 raise ValueError(f"frequency offset {df} gives a non-positive frequency")
             column = _replay_rows(dscene, topo, source, target, tx_power, tx_pol, rx_pol, f)
             if differentiable:
-                column = _los_rows_ad(column, valid, source, target, tx_power, tx_pol, rx_pol,
-                                      request.reference_frequency_hz, compiled.frequency_hz, df)
+                column = _rows_ad(column, valid, topo, compiled, source, target, tx_power, tx_pol, rx_pol,
+                                  request.reference_frequency_hz, compiled.frequency_hz, df)
             columns.append(column)
     if request.response == "scalar_transport":
         payload = torch.stack([c["path_field"] for c in columns], dim=1) if columns else None
