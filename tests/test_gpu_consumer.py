@@ -445,3 +445,51 @@ def test_replay_reflection_rows_differentiate_endpoints(device):
             w, dtype=torch.float32, device=device)).sum()).tangent)
     want = float((g_snk * direction).sum())
     assert abs(tangent - want) <= 2e-3 * abs(want), (tangent, want)
+
+
+def test_evaluate_ad_matches_replay_ad(device):
+    """evaluate(ad_mode="vjp") discovers natively and differentiates the rows it
+    found: values equal evaluate(ad_mode="none"), gradients equal those of the
+    replay of the same rows (pinned against the FP64 oracle above); a request
+    with diffraction is refused."""
+    _say("case: evaluate ad_mode vjp, canyon 2 sources x 6 sinks, depth 2, los+refl, vs the replay's gradients")
+    rng = np.random.default_rng(25)
+    compiled = compile_scene(workloads.urban_canyon_scene(seed=2, shape=(2, 2)), reference_frequency_hz=28e9)
+    tx = [workloads.URBAN_CANYON_TX, (52.7, 140.3, 12.0)]
+    sources, sinks = _batches(device, tx, [1.0, 2.0], _street_sinks(rng, 6))
+    comps = frozenset({"los", "reflection"})
+
+    def batches(s_pos, r_pos):
+        return (consumer.EndpointBatch(stable_ids=sources.stable_ids, positions_m=s_pos,
+                                       polarizations=sources.polarizations, powers_w=sources.powers_w),
+                consumer.EndpointBatch(stable_ids=sinks.stable_ids, positions_m=r_pos,
+                                       polarizations=sinks.polarizations))
+
+    def evaluate(ad_mode, s_pos, r_pos, components=comps):
+        s_b, r_b = batches(s_pos, r_pos)
+        return consumer.evaluate(compiled, consumer.PropagationRequest(
+            sources=s_b, sinks=r_b, reference_frequency_hz=28e9, components=components, max_depth=2,
+            response="scalar_transport", topology_mode="discover", ad_mode=ad_mode))
+
+    plain = evaluate("none", sources.positions_m, sinks.positions_m)
+    s1 = sources.positions_m.detach().clone().requires_grad_(True)
+    r1 = sinks.positions_m.detach().clone().requires_grad_(True)
+    ev = evaluate("vjp", s1, r1)
+    coef = ev.paths.transport.coefficient
+    assert ev.paths.path_count == plain.paths.path_count > 5
+    torch.testing.assert_close(coef.detach(), plain.paths.transport.coefficient, rtol=2e-5, atol=1e-12)
+    w = torch.as_tensor(rng.normal(size=(ev.paths.path_count, 2)), dtype=torch.float32, device=device)
+    (torch.view_as_real(coef) * w).sum().backward()
+    prepared = consumer.prepare_fixed_topology(plain.paths.topology)
+    s2 = sources.positions_m.detach().clone().requires_grad_(True)
+    r2 = sinks.positions_m.detach().clone().requires_grad_(True)
+    s_b, r_b = batches(s2, r2)
+    out = consumer.reevaluate(compiled, consumer.FixedTopologyRequest(
+        sources=s_b, sinks=r_b, reference_frequency_hz=28e9, topology=prepared, response="scalar_transport",
+        ad_mode="vjp"))
+    assert bool(out.row_valid.all())
+    (torch.view_as_real(out.paths.transport.coefficient) * w).sum().backward()
+    torch.testing.assert_close(s1.grad, s2.grad, rtol=1e-5, atol=1e-9)
+    torch.testing.assert_close(r1.grad, r2.grad, rtol=1e-5, atol=1e-9)
+    with pytest.raises(NotImplementedError, match="line-of-sight and reflection"):
+        evaluate("vjp", s1, r1, frozenset({"los", "reflection", "diffraction"}))

@@ -19,8 +19,9 @@ the request's endpoints instead of the compiled scene's: rows are pair-major
 with ``pair_index = sink_index * source_count + source_index``. The Jones
 response is line-of-sight only, as in the reference; it excites the native
 free-space row operator with the two source basis vectors and projects onto
-the sink basis. ``evaluate`` is forward only (its differentiable route is
-``deterministic.solve`` with ``ad_mode="vjp"``). ``reevaluate`` replays
+the sink basis. With ``ad_mode`` ``"vjp"`` or ``"jvp"``, ``evaluate``
+discovers natively and takes its line-of-sight and reflection rows' values
+from the differentiable row operators (as ``reevaluate`` does). ``reevaluate`` replays
 frozen line-of-sight and reflection rows at new endpoints in one native launch
 per frequency column (``wc_fixed_rows_evaluate``); with ``ad_mode`` ``"vjp"``
 or ``"jvp"`` its line-of-sight rows take their values from the
@@ -332,10 +333,10 @@ _CAPABILITIES = PropagationCapabilities(
     components=COMPONENTS,
     responses=RESPONSES,
     topology_modes=TOPOLOGY_MODES,
-    ad_modes=frozenset({"none"}),
+    ad_modes=frozenset({"none", "jvp", "vjp"}),
     response_components=(("scalar_transport", COMPONENTS), ("complex3_transport", COMPONENTS),
                          ("polarimetric_transport", frozenset({"los"}))),
-    response_ad_modes=(("scalar_transport", frozenset({"none"})), ("complex3_transport", frozenset({"none"})),
+    response_ad_modes=(("scalar_transport", AD_MODES), ("complex3_transport", AD_MODES),
                        ("polarimetric_transport", frozenset({"none"}))),
 )
 
@@ -381,6 +382,10 @@ def _preflight(compiled: object, request: object) -> tuple[CompiledScene, Propag
     if request.ad_mode not in _CAPABILITIES.ad_modes_for(request.response):
         raise NotImplementedError(f"{request.response} does not support AD mode {request.ad_mode!r} in this "
                                   "build (differentiate through deterministic.solve(ad_mode='vjp'))")
+    if request.ad_mode != "none" and not request.components <= _CAPABILITIES.fixed_topology_ad_components:
+        raise NotImplementedError(f"ad_mode {request.ad_mode!r} differentiates line-of-sight and reflection rows; "
+                                  f"the request asks for "
+                                  f"{sorted(request.components - _CAPABILITIES.fixed_topology_ad_components)}")
     if request.response == "polarimetric_transport" and (request.sources.polarization_basis is None
                                                           or request.sinks.polarization_basis is None):
         raise ValueError("polarimetric_transport requires source and sink polarization_basis tensors")
@@ -525,6 +530,23 @@ def evaluate(compiled_scene: CompiledScene, request: PropagationRequest) -> Prop
         primitive_sequence=rows["primitive_sequence"].reshape(k, width).to(torch.int32),
         material_sequence=rows["material_sequence"].reshape(k, width).to(torch.int32),
         interaction_type=interaction_types(comp, rows["depth"], width))
+    if request.ad_mode != "none" and k:
+        # discovery fixed the rows; their values come from the differentiable
+        # row operators at the request's (possibly tangent-carrying) endpoints
+        src_rows, snk_rows = source_index.long(), sink_index.long()
+        values = _rows_ad({"path_field": rows["path_field"].to(torch.complex64),
+                           "field_xyz": rows["field_xyz"].to(torch.complex64),
+                           "path_length": rows["path_length_m"].to(torch.float32),
+                           "delay": rows["delay_s"].to(torch.float32),
+                           "direction": rows["arrival_direction"].to(torch.float32)},
+                          torch.ones(k, dtype=torch.bool, device=device), topology, compiled,
+                          sources.positions_m[src_rows].contiguous(), sinks.positions_m[snk_rows].contiguous(),
+                          sources.powers_w[src_rows].contiguous(), sources.polarizations[src_rows].contiguous(),
+                          sinks.polarizations[snk_rows].contiguous(), request.reference_frequency_hz,
+                          compiled.frequency_hz)
+        rows = {**rows, "path_field": values["path_field"], "field_xyz": values["field_xyz"],
+                "path_length_m": values["path_length"], "delay_s": values["delay"],
+                "arrival_direction": values["direction"]}
     geometry = PropagationGeometry(
         path_length_m=rows["path_length_m"].to(torch.float32), delay_s=rows["delay_s"].to(torch.float32),
         field_direction=rows["arrival_direction"].to(torch.float32),
